@@ -1,0 +1,240 @@
+/*
+ * mole_gpu.h -- C ABI of the MI355X (gfx950) walker hot path for mole.
+ *
+ * This is the boundary a Rust FFI crate binds: plain C types, caller-owned
+ * host buffers valid for the duration of a call, no callbacks, no
+ * exceptions.  A context owns its device buffers and one HIP stream; it is
+ * not internally synchronised (one host thread at a time).
+ *
+ * Each entry point names the reference interface it replaces
+ * (Jvanrhijn/mole, paths relative to the repository root).
+ */
+#ifndef MOLE_GPU_H
+#define MOLE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ------------------------------------------------------
+ * 1..6 are the variants of errors::Error (src/errors/src/lib.rs:7-15) in
+ * declaration order.  The remaining codes have no variant there; a Rust
+ * binding maps them to Error::FuncError. */
+typedef int32_t mole_status;
+#define MOLE_OK 0
+#define MOLE_ERR_LINALG 1                 /* Error::LinalgError */
+#define MOLE_ERR_SHAPE 2                  /* Error::ShapeError */
+#define MOLE_ERR_FUNC 3                   /* Error::FuncError */
+#define MOLE_ERR_OPERATOR_VALUE_ACCESS 4  /* Error::OperatorValueAccessError */
+#define MOLE_ERR_DATA_ACCESS 5            /* Error::DataAccessError */
+#define MOLE_ERR_EMPTY_CACHE 6            /* Error::EmptyCacheError */
+#define MOLE_ERR_INVALID_ARGUMENT 16      /* reference panics (asserts) */
+#define MOLE_ERR_DEVICE 17                /* HIP runtime failure */
+#define MOLE_ERR_UNSUPPORTED 18
+#define MOLE_ERR_OUT_OF_MEMORY 19
+
+/* ---- wavefunctions, Hamiltonians, proposals --------------------------- */
+/* psi = exp(-zeta (r1 + r2) + b r12/(1 + c r12)) about nucleus 0,
+ * params (zeta, b, c); one electron of each spin. */
+#define MOLE_WF_HELIUM_PADE 1
+/* psi = phi(r1) phi(r2) exp(b r12/(1 + c r12)),
+ * phi(r) = sum_I exp(-zeta |r - R_I|), params (zeta, b, c). */
+#define MOLE_WF_LCAO2_PADE 2
+
+#define MOLE_H_ELECTRONIC 0 /* ElectronicHamiltonian, src/operator/src/operator.rs:157-183 */
+#define MOLE_H_IONIC 1      /* IonicHamiltonian, operator.rs:131-149 */
+
+#define MOLE_PROPOSAL_BOX 0 /* MetropolisBox, src/metropolis/src/metrop.rs:21-101 */
+
+/* Convention of the "Parameter gradient" observable O.
+ * LOGDERIV:  O = d ln psi / dp.
+ * REFERENCE: the reference's literal value, O = G/psi with the stored
+ *            G = psi ./ (psi dpsi/dp) produced by Vector/Scalar
+ *            (src/operator/src/traits.rs:150, src/vmc/src/operators.rs:9-13,
+ *            src/optimize/src/util.rs:42). */
+#define MOLE_O_LOGDERIV 0
+#define MOLE_O_REFERENCE 1
+
+/* run flags */
+#define MOLE_FLAG_TRACE 1u   /* record per-sweep accept masks and E_L */
+#define MOLE_FLAG_SAMPLES 2u /* keep per-sample E, psi, G (compat data) */
+
+/* mole_sweep flags */
+#define MOLE_SWEEP_SAMPLE 1u /* sample observables after every sweep */
+
+#define MOLE_MAX_ELECTRONS 40
+#define MOLE_MAX_NUCLEI 64
+#define MOLE_MAX_PARAMS 16
+
+typedef struct mole_ctx mole_ctx;
+
+/* One system: nuclei, electrons, wavefunction and walker count.  The
+ * nuclei and charges are those of IonicPotential::new
+ * (operator.rs:40-55). */
+typedef struct {
+  uint32_t wf_kind;     /* MOLE_WF_* */
+  uint32_t hamiltonian; /* MOLE_H_* */
+  uint32_t n_up, n_down;
+  uint32_t n_nuclei;
+  const double *nuclei_xyz;     /* [n_nuclei][3] */
+  const int32_t *nuclei_charge; /* [n_nuclei] */
+  uint32_t n_params;
+  const double *params; /* Optimize::parameters, src/optimize/src/traits.rs:13 */
+  uint32_t n_walkers;
+} mole_system_desc;
+
+/* How the walkers move and what they record. */
+typedef struct {
+  uint64_t seed;         /* Philox key (lo32, hi32 ^ system index) */
+  uint32_t proposal;     /* MOLE_PROPOSAL_* */
+  double step;           /* MetropolisBox box_side (metrop.rs:26,66) */
+  uint32_t block_size;   /* Runner::run block_size (src/montecarlo/src/montecarlo.rs:24-46) */
+  uint32_t o_convention; /* MOLE_O_* */
+  uint32_t flags;        /* MOLE_FLAG_* */
+  uint32_t record_sweeps;/* capacity, in sweeps, of trace/sample buffers */
+} mole_run_desc;
+
+/* Energy statistics since the last reset, summed over the system's
+ * walkers.  Block means are over block_size consecutive samples of one
+ * walker, as VmcRunner::process_monte_carlo_results chunks the
+ * worker-ordered sample list (src/vmc/src/vmc.rs:150-170). */
+typedef struct {
+  uint64_t n_samples;
+  uint64_t n_blocks;
+  uint64_t n_moves;
+  uint64_t n_accepted;
+  double sum_e, sum_e2;
+  double sum_block_mean, sum_block_mean2;
+} mole_stats;
+
+/* Sums for the energy gradient and the SR matrix
+ * (src/optimize/src/util.rs:6-46, src/optimize/src/optimizers.rs:191-233),
+ * over the same samples as mole_stats.  sum_oo is the full n_params^2
+ * matrix, row-major. */
+typedef struct {
+  uint32_t n_params;
+  uint64_t n;
+  double sum_e;
+  double sum_o[MOLE_MAX_PARAMS];
+  double sum_oe[MOLE_MAX_PARAMS];
+  double sum_oo[MOLE_MAX_PARAMS * MOLE_MAX_PARAMS];
+} mole_sr_sums;
+
+/* ---- library ---------------------------------------------------------- */
+const char *mole_status_string(mole_status s);
+/* Message of the last failing call on this thread ("" if none). */
+const char *mole_last_error(void);
+/* Device name and compute-unit count of the current device. */
+mole_status mole_device_info(char *name, size_t cap, int32_t *cu_count);
+
+/* ---- context -----------------------------------------------------------
+ * Replaces Sampler::new / MetropolisBox::from_rng
+ * (src/montecarlo/src/samplers.rs:40-55, metrop.rs:34-36) for a whole
+ * batch of walkers of n_sys systems. */
+mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys,
+                            const mole_run_desc *run, mole_ctx **out);
+void mole_ctx_destroy(mole_ctx *ctx);
+
+/* Sampler::new's initial configuration: U(-1, 1) per coordinate from draws
+ * 0..3N-1 of each walker's stream (samplers.rs:46); the stream position is
+ * left at 3N. */
+mole_status mole_init_walkers(mole_ctx *ctx);
+/* Sampler::with_initial_configuration (samplers.rs:57-71) and checkpoint:
+ * configurations as [walker][electron][xyz]. */
+mole_status mole_set_configs(mole_ctx *ctx, uint32_t sys, const double *cfg,
+                             size_t n_walkers);
+mole_status mole_get_configs(mole_ctx *ctx, uint32_t sys, double *cfg,
+                             size_t n_walkers);
+/* Every walker of sys takes the configuration cfg [electron][xyz]; the
+ * per-iteration clone of the master sampler (src/vmc/src/vmc.rs:56). */
+mole_status mole_broadcast_config(mole_ctx *ctx, uint32_t sys,
+                                  const double *cfg);
+/* Metropolis::reseed_rng (src/metropolis/src/traits.rs:30-37): selects the
+ * Philox epoch (counter word 3) and the next draw index of every walker. */
+mole_status mole_set_stream(mole_ctx *ctx, uint32_t epoch, uint64_t position);
+mole_status mole_get_stream(mole_ctx *ctx, uint32_t *epoch,
+                            uint64_t *position);
+
+/* Optimize::update_parameters / parameters (src/optimize/src/traits.rs:8-16). */
+mole_status mole_set_params(mole_ctx *ctx, uint32_t sys, const double *p,
+                            uint32_t n_params);
+mole_status mole_get_params(mole_ctx *ctx, uint32_t sys, double *p,
+                            uint32_t n_params);
+
+/* ---- sampling ----------------------------------------------------------
+ * n_sweeps calls of Sampler::move_state (samplers.rs:106-117: electrons
+ * 0..N-1 in order, MetropolisBox::move_state metrop.rs:84-96), each
+ * followed, with MOLE_SWEEP_SAMPLE, by Sampler::sample of the "Energy",
+ * "Wavefunction value" and "Parameter gradient" observables
+ * (samplers.rs:81-104) folded into the statistics. */
+mole_status mole_sweep(mole_ctx *ctx, uint32_t n_sweeps, uint32_t flags);
+/* Runner::run(steps, block_size) (montecarlo.rs:24-46): steps/block_size
+ * blocks of block_size sweeps, observables sampled from the second block
+ * on.  Fails with MOLE_ERR_INVALID_ARGUMENT when steps < 2*block_size
+ * (the assert at montecarlo.rs:29) or block_size differs from the
+ * context's. */
+mole_status mole_run(mole_ctx *ctx, uint32_t steps, uint32_t block_size);
+mole_status mole_reset_stats(mole_ctx *ctx);
+mole_status mole_read_stats(mole_ctx *ctx, uint32_t sys, mole_stats *out);
+mole_status mole_read_sr_sums(mole_ctx *ctx, uint32_t sys, mole_sr_sums *out);
+/* Per-sweep accept masks (bit e: move of electron e accepted) as
+ * [walker][sweep] and local energies of sampled sweeps as [walker][sample],
+ * since the last reset (needs MOLE_FLAG_TRACE). */
+mole_status mole_read_trace(mole_ctx *ctx, uint32_t sys, uint64_t *accept_mask,
+                            double *e_local, size_t walkers, size_t sweeps,
+                            size_t samples);
+/* MonteCarloSampler::data (src/montecarlo/src/traits.rs:27) in compat
+ * form: "Energy", "Wavefunction value" and "Parameter gradient" samples as
+ * [walker][sample] (and [walker][sample][param]), the order in which
+ * VmcRunner::concatenate_worker_data appends worker data (vmc.rs:108-130).
+ * Needs MOLE_FLAG_SAMPLES. */
+mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local,
+                              double *psi, double *pgrad, size_t walkers,
+                              size_t samples);
+
+/* ---- single configurations ---------------------------------------------
+ * Function::value, Differentiate::gradient/laplacian
+ * (src/wavefunction_traits/src/lib.rs:7-24), Optimize::parameter_gradient
+ * (src/optimize/src/traits.rs:9) and the local energy
+ * ElectronicHamiltonian::act_on / psi (operator.rs:177-183) for n
+ * configurations [n][N][3].  grad is d psi (not d ln psi) as
+ * [n][N][3]; pgrad is d psi/dp as [n][n_params].  Output pointers may be
+ * NULL. */
+mole_status mole_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg,
+                      double *value, double *grad, double *lap,
+                      double *e_local, double *pgrad);
+
+/* ---- optimizer step from device-reduced sums -------------------------
+ * Optimizer::compute_parameter_update (src/optimize/src/traits.rs:18-25)
+ * for SteepestDescent (src/optimize/src/optimizers.rs:20-30) and
+ * StochasticReconfiguration (optimizers.rs:181-253), evaluated from the
+ * sums of mole_read_sr_sums instead of per-sample vectors:
+ *   Ebar = sum_e/n, g = 2 (sum_oe - Ebar sum_o)/n   (util.rs:6-46)
+ *   SD: dp = -step g
+ *   SR: S = sum_oo/n - Obar Obar^T (TEXTBOOK) or - (sum_i Obar_i)^2 on
+ *       every element (REFERENCE, optimizers.rs:219-224); diag *= 1 + eps
+ *       (REFERENCE: eps = 1e-2, optimizers.rs:225-231);
+ *       dp = step S^{-1} (-g/2)  (solveh_into, optimizers.rs:251).
+ * g may be NULL.  A singular S returns MOLE_ERR_LINALG. */
+#define MOLE_OPT_SD 0
+#define MOLE_OPT_SR 1
+#define MOLE_SR_TEXTBOOK 0
+#define MOLE_SR_REFERENCE 1
+mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t optimizer,
+                                          double step, uint32_t sr_mode, double sr_eps,
+                                          double *dp, double *g);
+
+/* Waits for all work of the context. */
+mole_status mole_synchronize(mole_ctx *ctx);
+/* Device time of the kernels of the last mole_sweep/mole_run, measured
+ * with HIP events on the context's stream, and the number of sweep-kernel
+ * launches it made. */
+mole_status mole_last_timing(mole_ctx *ctx, double *ms, uint32_t *launches);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MOLE_GPU_H */

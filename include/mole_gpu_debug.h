@@ -1,0 +1,33 @@
+/*
+ * mole_gpu_debug.h -- diagnostic entry points of libmole_amd.so.
+ *
+ * These have no counterpart in the reference.  They exist so that tests can
+ * check, element by element, that the device primitives the walker kernels
+ * use (Philox draws, the u64 -> f64 maps, exp, sqrt, division) round exactly
+ * like the CPU restatement they are compared with.
+ */
+#ifndef MOLE_GPU_DEBUG_H
+#define MOLE_GPU_DEBUG_H
+
+#include "mole_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* draws first .. first+count-1 of the u64 stream of (walker, epoch) */
+mole_status mole_debug_philox(uint64_t seed, uint32_t sys, uint32_t walker, uint32_t epoch,
+                              uint64_t first, size_t count, uint64_t *out);
+
+#define MOLE_DEBUG_EXP 0      /* out = exp(a)            */
+#define MOLE_DEBUG_SQRT 1     /* out = sqrt(a)           */
+#define MOLE_DEBUG_DIV 2      /* out = a / b             */
+#define MOLE_DEBUG_STANDARD 3 /* out = Standard(bits(a)) */
+#define MOLE_DEBUG_RANGE 4    /* out = Range(-b, b)(bits(a)) */
+#define MOLE_DEBUG_NORM3 5    /* out = |(a, b, a*b)|      */
+mole_status mole_debug_math(int32_t op, size_t n, const double *a, const double *b, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

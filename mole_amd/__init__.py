@@ -1,0 +1,370 @@
+"""Python access to libmole_amd.so, the gfx950 walker hot path of mole.
+
+Thin ctypes plumbing over the C ABI in include/mole_gpu.h: no computation
+happens in Python and there is no CPU fallback -- if the library is missing
+or no device is present, calls fail with MoleError.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmole_amd.so")
+
+# status codes (include/mole_gpu.h)
+OK = 0
+ERR_LINALG = 1
+ERR_SHAPE = 2
+ERR_FUNC = 3
+ERR_OPERATOR_VALUE_ACCESS = 4
+ERR_DATA_ACCESS = 5
+ERR_EMPTY_CACHE = 6
+ERR_INVALID_ARGUMENT = 16
+ERR_DEVICE = 17
+ERR_UNSUPPORTED = 18
+ERR_OUT_OF_MEMORY = 19
+
+WF_HELIUM_PADE = 1
+WF_LCAO2_PADE = 2
+H_ELECTRONIC = 0
+H_IONIC = 1
+PROPOSAL_BOX = 0
+O_LOGDERIV = 0
+O_REFERENCE = 1
+FLAG_TRACE = 1
+FLAG_SAMPLES = 2
+SWEEP_SAMPLE = 1
+MAX_PARAMS = 16
+
+OPT_SD = 0
+OPT_SR = 1
+SR_TEXTBOOK = 0
+SR_REFERENCE = 1
+
+DEBUG_EXP = 0
+DEBUG_SQRT = 1
+DEBUG_DIV = 2
+DEBUG_STANDARD = 3
+DEBUG_RANGE = 4
+DEBUG_NORM3 = 5
+
+
+class MoleError(RuntimeError):
+    def __init__(self, status, message):
+        super().__init__(f"[{status}] {message}")
+        self.status = status
+
+
+class SystemDesc(C.Structure):
+    _fields_ = [
+        ("wf_kind", C.c_uint32),
+        ("hamiltonian", C.c_uint32),
+        ("n_up", C.c_uint32),
+        ("n_down", C.c_uint32),
+        ("n_nuclei", C.c_uint32),
+        ("nuclei_xyz", C.POINTER(C.c_double)),
+        ("nuclei_charge", C.POINTER(C.c_int32)),
+        ("n_params", C.c_uint32),
+        ("params", C.POINTER(C.c_double)),
+        ("n_walkers", C.c_uint32),
+    ]
+
+
+class RunDesc(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64),
+        ("proposal", C.c_uint32),
+        ("step", C.c_double),
+        ("block_size", C.c_uint32),
+        ("o_convention", C.c_uint32),
+        ("flags", C.c_uint32),
+        ("record_sweeps", C.c_uint32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("n_samples", C.c_uint64),
+        ("n_blocks", C.c_uint64),
+        ("n_moves", C.c_uint64),
+        ("n_accepted", C.c_uint64),
+        ("sum_e", C.c_double),
+        ("sum_e2", C.c_double),
+        ("sum_block_mean", C.c_double),
+        ("sum_block_mean2", C.c_double),
+    ]
+
+
+class SrSums(C.Structure):
+    _fields_ = [
+        ("n_params", C.c_uint32),
+        ("n", C.c_uint64),
+        ("sum_e", C.c_double),
+        ("sum_o", C.c_double * MAX_PARAMS),
+        ("sum_oe", C.c_double * MAX_PARAMS),
+        ("sum_oo", C.c_double * (MAX_PARAMS * MAX_PARAMS)),
+    ]
+
+
+_lib = None
+
+# (name, restype, argtypes)
+_P = C.c_void_p
+_pd = C.POINTER(C.c_double)
+_pu64 = C.POINTER(C.c_uint64)
+_SIGS = [
+    ("mole_status_string", C.c_char_p, [C.c_int32]),
+    ("mole_last_error", C.c_char_p, []),
+    ("mole_device_info", C.c_int32, [C.c_char_p, C.c_size_t, C.POINTER(C.c_int32)]),
+    ("mole_ctx_create", C.c_int32, [C.POINTER(SystemDesc), C.c_uint32, C.POINTER(RunDesc), C.POINTER(_P)]),
+    ("mole_ctx_destroy", None, [_P]),
+    ("mole_init_walkers", C.c_int32, [_P]),
+    ("mole_set_configs", C.c_int32, [_P, C.c_uint32, _pd, C.c_size_t]),
+    ("mole_get_configs", C.c_int32, [_P, C.c_uint32, _pd, C.c_size_t]),
+    ("mole_broadcast_config", C.c_int32, [_P, C.c_uint32, _pd]),
+    ("mole_set_stream", C.c_int32, [_P, C.c_uint32, C.c_uint64]),
+    ("mole_get_stream", C.c_int32, [_P, C.POINTER(C.c_uint32), _pu64]),
+    ("mole_set_params", C.c_int32, [_P, C.c_uint32, _pd, C.c_uint32]),
+    ("mole_get_params", C.c_int32, [_P, C.c_uint32, _pd, C.c_uint32]),
+    ("mole_sweep", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    ("mole_run", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    ("mole_reset_stats", C.c_int32, [_P]),
+    ("mole_read_stats", C.c_int32, [_P, C.c_uint32, C.POINTER(Stats)]),
+    ("mole_read_sr_sums", C.c_int32, [_P, C.c_uint32, C.POINTER(SrSums)]),
+    ("mole_read_trace", C.c_int32, [_P, C.c_uint32, _pu64, _pd, C.c_size_t, C.c_size_t, C.c_size_t]),
+    ("mole_read_samples", C.c_int32, [_P, C.c_uint32, _pd, _pd, _pd, C.c_size_t, C.c_size_t]),
+    ("mole_eval", C.c_int32, [_P, C.c_uint32, C.c_size_t, _pd, _pd, _pd, _pd, _pd, _pd]),
+    ("mole_synchronize", C.c_int32, [_P]),
+    ("mole_last_timing", C.c_int32, [_P, _pd, C.POINTER(C.c_uint32)]),
+    ("mole_compute_parameter_update", C.c_int32, [C.POINTER(SrSums), C.c_uint32, C.c_double, C.c_uint32, C.c_double, _pd, _pd]),
+    ("mole_debug_philox", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t, _pu64]),
+    ("mole_debug_math", C.c_int32, [C.c_int32, C.c_size_t, _pd, _pd, _pd]),
+]
+
+
+def lib():
+    """Load libmole_amd.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise MoleError(ERR_DEVICE, f"{LIB_PATH} is missing: build it with python -m mole_amd.build")
+    L = C.CDLL(LIB_PATH)
+    for name, res, args in _SIGS:
+        f = getattr(L, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = L
+    return L
+
+
+def exported_symbols():
+    return [s[0] for s in _SIGS]
+
+
+def check(status):
+    if status != OK:
+        L = lib()
+        raise MoleError(status, f"{L.mole_status_string(status).decode()}: {L.mole_last_error().decode()}")
+
+
+def _dp(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pd)
+
+
+def _u64p(a):
+    assert a.dtype == np.uint64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pu64)
+
+
+class System:
+    """Host description of one system (keeps the arrays the descriptor points into)."""
+
+    def __init__(self, wf_kind, nuclei, charges, params, n_walkers, n_up=1, n_down=1,
+                 hamiltonian=H_ELECTRONIC):
+        self.nuclei = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 3)
+        self.charges = np.ascontiguousarray(charges, dtype=np.int32)
+        self.params = np.ascontiguousarray(params, dtype=np.float64)
+        self.desc = SystemDesc(
+            wf_kind=wf_kind, hamiltonian=hamiltonian, n_up=n_up, n_down=n_down,
+            n_nuclei=self.nuclei.shape[0],
+            nuclei_xyz=self.nuclei.ctypes.data_as(_pd),
+            nuclei_charge=self.charges.ctypes.data_as(C.POINTER(C.c_int32)),
+            n_params=self.params.size, params=self.params.ctypes.data_as(_pd),
+            n_walkers=n_walkers)
+        self.n_electrons = n_up + n_down
+        self.n_walkers = n_walkers
+
+
+def helium(n_walkers, zeta=1.6875, b=0.5, c=0.3, hamiltonian=H_ELECTRONIC):
+    return System(WF_HELIUM_PADE, [(0.0, 0.0, 0.0)], [2], [zeta, b, c], n_walkers,
+                  hamiltonian=hamiltonian)
+
+
+def h2(n_walkers, zeta=1.0, b=0.5, c=0.3, half_bond=0.7):
+    return System(WF_LCAO2_PADE, [(0.0, 0.0, -half_bond), (0.0, 0.0, half_bond)], [1, 1],
+                  [zeta, b, c], n_walkers)
+
+
+class Context:
+    """One mole_ctx: a batch of walkers of one or more systems on the device."""
+
+    def __init__(self, systems, *, seed=0, step=0.5, block_size=100, o_convention=O_LOGDERIV,
+                 flags=0, record_sweeps=0, proposal=PROPOSAL_BOX):
+        L = lib()
+        self.systems = list(systems)
+        arr = (SystemDesc * len(self.systems))(*[s.desc for s in self.systems])
+        self.run_desc = RunDesc(seed=seed, proposal=proposal, step=step, block_size=block_size,
+                                o_convention=o_convention, flags=flags,
+                                record_sweeps=record_sweeps)
+        h = C.c_void_p()
+        check(L.mole_ctx_create(arr, len(self.systems), C.byref(self.run_desc), C.byref(h)))
+        self.h = h
+        self.block_size = block_size
+        self.record_sweeps = record_sweeps
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mole_ctx_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def init_walkers(self):
+        check(lib().mole_init_walkers(self.h))
+
+    def set_configs(self, cfg, sys=0):
+        s = self.systems[sys]
+        a = np.ascontiguousarray(cfg, dtype=np.float64).reshape(s.n_walkers, s.n_electrons * 3)
+        check(lib().mole_set_configs(self.h, sys, _dp(a), s.n_walkers))
+
+    def get_configs(self, sys=0):
+        s = self.systems[sys]
+        a = np.zeros((s.n_walkers, s.n_electrons * 3))
+        check(lib().mole_get_configs(self.h, sys, _dp(a), s.n_walkers))
+        return a
+
+    def broadcast_config(self, cfg, sys=0):
+        a = np.ascontiguousarray(cfg, dtype=np.float64).ravel()
+        check(lib().mole_broadcast_config(self.h, sys, _dp(a)))
+
+    def set_stream(self, epoch, position):
+        check(lib().mole_set_stream(self.h, epoch, position))
+
+    def get_stream(self):
+        e, p = C.c_uint32(), C.c_uint64()
+        check(lib().mole_get_stream(self.h, C.byref(e), C.byref(p)))
+        return e.value, p.value
+
+    def set_params(self, p, sys=0):
+        a = np.ascontiguousarray(p, dtype=np.float64)
+        check(lib().mole_set_params(self.h, sys, _dp(a), a.size))
+
+    def get_params(self, sys=0):
+        a = np.zeros(self.systems[sys].params.size)
+        check(lib().mole_get_params(self.h, sys, _dp(a), a.size))
+        return a
+
+    def sweep(self, n, sample=False):
+        check(lib().mole_sweep(self.h, n, SWEEP_SAMPLE if sample else 0))
+
+    def run(self, steps, block_size=None):
+        check(lib().mole_run(self.h, steps, self.block_size if block_size is None else block_size))
+
+    def reset_stats(self):
+        check(lib().mole_reset_stats(self.h))
+
+    def stats(self, sys=0):
+        s = Stats()
+        check(lib().mole_read_stats(self.h, sys, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in Stats._fields_}
+
+    def sr_sums_raw(self, sys=0):
+        s = SrSums()
+        check(lib().mole_read_sr_sums(self.h, sys, C.byref(s)))
+        return s
+
+    def sr_sums(self, sys=0):
+        s = self.sr_sums_raw(sys)
+        np_ = s.n_params
+        return dict(n=s.n, sum_e=s.sum_e, sum_o=np.array(s.sum_o[:np_]),
+                    sum_oe=np.array(s.sum_oe[:np_]),
+                    sum_oo=np.array(s.sum_oo[:np_ * np_]).reshape(np_, np_))
+
+    def trace(self, sweeps, samples, sys=0):
+        W = self.systems[sys].n_walkers
+        m = np.zeros((W, sweeps), dtype=np.uint64)
+        e = np.zeros((W, samples))
+        check(lib().mole_read_trace(self.h, sys, _u64p(m), _dp(e), W, sweeps, samples))
+        return m, e
+
+    def samples(self, samples, sys=0):
+        W = self.systems[sys].n_walkers
+        npar = self.systems[sys].params.size
+        e = np.zeros((W, samples))
+        p = np.zeros((W, samples))
+        g = np.zeros((W, samples, npar))
+        check(lib().mole_read_samples(self.h, sys, _dp(e), _dp(p), _dp(g), W, samples))
+        return e, p, g
+
+    def eval(self, cfg, sys=0):
+        s = self.systems[sys]
+        D = 3 * s.n_electrons
+        a = np.ascontiguousarray(cfg, dtype=np.float64).reshape(-1, D)
+        n = a.shape[0]
+        out = dict(value=np.zeros(n), grad=np.zeros((n, D)), lap=np.zeros(n),
+                   e_local=np.zeros(n), pgrad=np.zeros((n, s.params.size)))
+        check(lib().mole_eval(self.h, sys, n, _dp(a), _dp(out["value"]), _dp(out["grad"]),
+                              _dp(out["lap"]), _dp(out["e_local"]), _dp(out["pgrad"])))
+        return out
+
+    def synchronize(self):
+        check(lib().mole_synchronize(self.h))
+
+    def last_timing(self):
+        ms, n = C.c_double(), C.c_uint32()
+        check(lib().mole_last_timing(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+
+def device_info():
+    buf = C.create_string_buffer(256)
+    cu = C.c_int32()
+    check(lib().mole_device_info(buf, 256, C.byref(cu)))
+    return buf.value.decode(), cu.value
+
+
+def debug_philox(seed, sys, walker, epoch, first, count):
+    out = np.zeros(count, dtype=np.uint64)
+    check(lib().mole_debug_philox(seed, sys, walker, epoch, first, count, _u64p(out)))
+    return out
+
+
+def debug_math(op, a, b=None):
+    a = np.ascontiguousarray(a, dtype=np.float64)
+    out = np.zeros_like(a)
+    bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
+    check(lib().mole_debug_math(op, a.size, _dp(a), None if bb is None else _dp(bb), _dp(out)))
+    return out
+
+
+def parameter_update(sums, optimizer=OPT_SR, step=0.05, sr_mode=SR_TEXTBOOK, sr_eps=1e-3):
+    """mole_compute_parameter_update on an SrSums; returns (dp, g)."""
+    n = sums.n_params
+    dp = np.zeros(n)
+    g = np.zeros(n)
+    check(lib().mole_compute_parameter_update(C.byref(sums), optimizer, step, sr_mode, sr_eps,
+                                              _dp(dp), _dp(g)))
+    return dp, g

@@ -1,0 +1,82 @@
+// Diagnostic kernels: the device primitives of the walker path applied to
+// caller-given inputs, for element-wise comparison with the CPU restatement.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "../../include/mole_gpu_debug.h"
+#include "device_common.hpp"
+
+using namespace mole;
+
+namespace {
+
+__global__ void philox_kernel(uint64_t seed, uint32_t sys, uint32_t walker, uint32_t epoch,
+                              uint64_t first, size_t count, uint64_t *out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  Stream s;
+  s.init(seed, sys, walker, epoch);
+  out[i] = s.draw(first + i);
+}
+
+__global__ void math_kernel(int op, size_t n, const double *a, const double *b, double *out) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double x = a[i];
+  const double y = b ? b[i] : 0.0;
+  double r = 0.0;
+  switch (op) {
+    case MOLE_DEBUG_EXP: r = exp_m(x); break;
+    case MOLE_DEBUG_SQRT: r = sqrt(x); break;
+    case MOLE_DEBUG_DIV: r = x / y; break;
+    case MOLE_DEBUG_STANDARD: r = u64_standard((uint64_t)__double_as_longlong(x)); break;
+    case MOLE_DEBUG_RANGE: {
+      const double lo = -y, hi = y;
+      const double scale = hi - lo;
+      r = u64_range((uint64_t)__double_as_longlong(x), scale, lo - scale);
+      break;
+    }
+    case MOLE_DEBUG_NORM3: r = norm3(x, y, x * y); break;
+    default: r = 0.0;
+  }
+  out[i] = r;
+}
+
+}  // namespace
+
+extern "C" mole_status mole_debug_philox(uint64_t seed, uint32_t sys, uint32_t walker,
+                                         uint32_t epoch, uint64_t first, size_t count,
+                                         uint64_t *out) {
+  if (!out) return MOLE_ERR_INVALID_ARGUMENT;
+  if (count == 0) return MOLE_OK;
+  uint64_t *d = nullptr;
+  if (hipMalloc(&d, count * sizeof(uint64_t)) != hipSuccess) return MOLE_ERR_OUT_OF_MEMORY;
+  hipLaunchKernelGGL(philox_kernel, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, 0, seed,
+                     sys, walker, epoch, first, count, d);
+  hipError_t e = hipGetLastError();
+  if (e == hipSuccess) e = hipMemcpy(out, d, count * sizeof(uint64_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
+}
+
+extern "C" mole_status mole_debug_math(int32_t op, size_t n, const double *a, const double *b,
+                                       double *out) {
+  if (!a || !out || op < 0 || op > MOLE_DEBUG_NORM3) return MOLE_ERR_INVALID_ARGUMENT;
+  if ((op == MOLE_DEBUG_DIV || op == MOLE_DEBUG_RANGE || op == MOLE_DEBUG_NORM3) && !b)
+    return MOLE_ERR_INVALID_ARGUMENT;
+  if (n == 0) return MOLE_OK;
+  double *d = nullptr;
+  if (hipMalloc(&d, 3 * n * sizeof(double)) != hipSuccess) return MOLE_ERR_OUT_OF_MEMORY;
+  hipError_t e = hipMemcpy(d, a, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && b) e = hipMemcpy(d + n, b, n * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(math_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, 0, op, n, d,
+                       b ? d + n : nullptr, d + 2 * n);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
+}

@@ -1,0 +1,78 @@
+// Internal structures shared by the host side of the C ABI and the kernels.
+#pragma once
+
+#include <stddef.h>
+#include <stdint.h>
+
+#include <hip/hip_runtime.h>
+
+namespace mole {
+
+constexpr int kMaxNuc2 = 8;   // nuclei of the two-electron kinds
+constexpr int kNParam2 = 3;   // (zeta, b, c)
+
+// Per-walker accumulator columns, stored [column][walker].
+enum AccColumn : int {
+  kAccSumE = 0,
+  kAccSumE2 = 1,
+  kAccSumBm = 2,
+  kAccSumBm2 = 3,
+  kAccBlockSum = 4,   // partial sum of the block in progress
+  kAccAccepted = 5,   // accepted moves (exact integers in a double)
+  kAccSumO = 6,       // kNParam2 columns
+  kAccSumOE = 6 + kNParam2,
+  kAccSumOO = 6 + 2 * kNParam2,  // kNParam2*(kNParam2+1)/2 columns, upper triangle
+  kAccColumns2 = 6 + 2 * kNParam2 + kNParam2 * (kNParam2 + 1) / 2
+};
+
+struct Sys2 {
+  int kind;         // MOLE_WF_HELIUM_PADE or MOLE_WF_LCAO2_PADE
+  int hamiltonian;  // MOLE_H_*
+  int n_nuc;
+  double nuc[kMaxNuc2][3];
+  double charge[kMaxNuc2];
+  double v_nn;      // nuclear repulsion, summed as IonicPotential::new does
+  double zeta, b, c;
+};
+
+struct Sweep2Args {
+  Sys2 sp;
+  int W;             // walkers of this system
+  double *pos;       // [6][W]
+  double *acc;       // [kAccColumns2][W]
+  // stream
+  uint64_t seed;
+  uint32_t sys, epoch;
+  uint64_t draw0;    // draw index of the first proposal of this launch
+  // box proposal Range(lo, hi) as scale/offset
+  double scale, offset;
+  uint32_t n_sweeps;
+  uint32_t sample_start;  // sweeps s >= sample_start are sampled
+  uint32_t block_size;
+  uint32_t block_fill;    // samples already in the block in progress
+  int o_conv;
+  // trace (optional)
+  uint64_t *trace_mask;   // [W][cap_sweeps]
+  double *trace_e;        // [W][cap_samples]
+  uint32_t trace_sweep0, trace_sample0, cap_sweeps, cap_samples;
+  double *samp_psi;       // [W][cap_samples]
+  double *samp_g;         // [W][cap_samples][3]
+};
+
+struct Eval2Args {
+  Sys2 sp;
+  size_t n;
+  const double *cfg;  // [n][6]
+  double *value, *grad, *lap, *e_local, *pgrad;
+};
+
+hipError_t launch_sweep2(const Sweep2Args &a, int block_threads,
+                         hipStream_t stream);
+hipError_t launch_init2(double *pos, int W, uint64_t seed, uint32_t sys,
+                        uint32_t epoch, hipStream_t stream);
+hipError_t launch_eval2(const Eval2Args &a, hipStream_t stream);
+hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out,
+                                 hipStream_t stream);
+hipError_t launch_fill(double *p, size_t n, double v, hipStream_t stream);
+
+}  // namespace mole

@@ -132,6 +132,9 @@ typedef struct {
                           0: cfg given, moves start at draw pos0 */
   uint64_t pos0;
   int o_convention;    /* ORC_O_* for the "Parameter gradient" samples */
+  int all_observables; /* 1: also evaluate "Wavefunction value" and
+                          "Parameter gradient" every sample, even when they
+                          are not stored (the VMC-optimisation cost) */
 } orc_run_params;
 
 /* Optional per-walker outputs of a run; any pointer may be NULL.
@@ -148,6 +151,7 @@ typedef struct {
   /* Accumulated statistics over all walkers (for rate runs without
    * per-sample storage): sums of E and E^2, of block means and squares. */
   double sum_e, sum_e2, sum_bm, sum_bm2;
+  double sum_g[ORC_NPARAM2]; /* sums of the stored-form parameter gradient */
   uint64_t n_samples, n_blocks, n_accepted, n_moves;
 } orc_run_out;
 

@@ -46,6 +46,7 @@ class RunParams(C.Structure):
         ("init_from_stream", C.c_int),
         ("pos0", C.c_uint64),
         ("o_convention", C.c_int),
+        ("all_observables", C.c_int),
     ]
 
 
@@ -61,6 +62,7 @@ class RunOut(C.Structure):
         ("sum_e2", C.c_double),
         ("sum_bm", C.c_double),
         ("sum_bm2", C.c_double),
+        ("sum_g", C.c_double * 3),
         ("n_samples", C.c_uint64),
         ("n_blocks", C.c_uint64),
         ("n_accepted", C.c_uint64),
@@ -179,12 +181,12 @@ def philox(ctr, key):
 
 def run_box(wf, *, seed, walkers, steps, block_size, box_side, epoch=0, sys=0,
             cfg=None, pos0=0, o_convention=O_LOGDERIV, n_threads=1,
-            record=True, pgrad=False):
+            record=True, pgrad=False, all_observables=False):
     """Run `walkers` independent Sampler+Runner::run chains on the oracle."""
     L = lib()
     rp = RunParams(seed=seed, sys=sys, epoch=epoch, box_side=box_side, steps=steps,
                    block_size=block_size, init_from_stream=0 if cfg is not None else 1,
-                   pos0=pos0, o_convention=o_convention)
+                   pos0=pos0, o_convention=o_convention, all_observables=int(all_observables))
     blocks = steps // block_size
     ns = (blocks - 1) * block_size
     res = {}

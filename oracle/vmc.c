@@ -24,6 +24,7 @@ typedef struct {
   orc_run_out *out; /* shared; only per-walker slots are written */
   /* private accumulators */
   double sum_e, sum_e2, sum_bm, sum_bm2;
+  double sum_g[ORC_NPARAM2];
   uint64_t n_samples, n_blocks, n_accepted, n_moves;
 } run_task;
 
@@ -90,12 +91,12 @@ static void run_walker(run_task *t, uint32_t w) {
         double el = orc_wf2_local_energy(wf, cfg);
         size_t si = (size_t)w * nsamp_w + sidx;
         if (out->e_local) out->e_local[si] = el;
-        if (out->psi || out->pgrad) {
+        if (out->psi || out->pgrad || rp->all_observables) {
           double psi = orc_wf2_value(wf, cfg);
           /* WavefunctionValue: Scalar(psi^2) / Scalar(psi)
            * (src/vmc/src/operators.rs:19-23, samplers.rs:89-90) */
           if (out->psi) out->psi[si] = (psi * psi) / psi;
-          if (out->pgrad) {
+          if (out->pgrad || rp->all_observables) {
             /* ParameterGradient: Vector(dpsi) * Scalar(psi), then
              * / Scalar(psi) (operators.rs:9-13); the reference's
              * Vector / Scalar is scalar ./ vector (traits.rs:150). */
@@ -103,8 +104,9 @@ static void run_walker(run_task *t, uint32_t w) {
             orc_wf2_parameter_gradient(wf, cfg, dp);
             for (int k = 0; k < ORC_NPARAM2; ++k) {
               double v = psi * dp[k];
-              out->pgrad[si * ORC_NPARAM2 + k] =
-                  (rp->o_convention == ORC_O_REFERENCE) ? psi / v : v / psi;
+              double G = (rp->o_convention == ORC_O_REFERENCE) ? psi / v : v / psi;
+              if (out->pgrad) out->pgrad[si * ORC_NPARAM2 + k] = G;
+              t->sum_g[k] += G;
             }
           }
         }
@@ -164,6 +166,7 @@ int orc_run_box(const orc_wf2 *wf, const orc_run_params *rp, uint32_t w0,
     out->n_blocks += tasks[i].n_blocks;
     out->n_accepted += tasks[i].n_accepted;
     out->n_moves += tasks[i].n_moves;
+    for (int k = 0; k < ORC_NPARAM2; ++k) out->sum_g[k] += tasks[i].sum_g[k];
   }
   free(tasks);
   free(th);
