@@ -44,7 +44,10 @@ def flops_per_launch(walkers, sweeps, sampled):
     return walkers * (sweeps * N_ELEC * FLOP_MOVE + sampled * FLOP_SAMPLE)
 
 
-def gpu_run(iters, warmup, seed=0):
+VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
+
+
+def gpu_run(iters, warmup, seed=0, variant="auto"):
     import mole_amd as M
 
     def one_iteration(ctx, p):
@@ -61,6 +64,7 @@ def gpu_run(iters, warmup, seed=0):
     # warmup on its own context: the timed run starts from the initial parameters
     if warmup:
         with M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed + 1, step=BOX, block_size=BLOCK) as w:
+            w.set_launch_config(VARIANTS[variant])
             w.init_walkers()
             p = np.array(PARAMS0)
             for _ in range(warmup):
@@ -68,6 +72,7 @@ def gpu_run(iters, warmup, seed=0):
             w.synchronize()
 
     ctx = M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed, step=BOX, block_size=BLOCK)
+    ctx.set_launch_config(VARIANTS[variant])
     ctx.init_walkers()
     ctx.synchronize()
     p = np.array(PARAMS0)
@@ -133,13 +138,14 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--variant", choices=sorted(VARIANTS), default="auto")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: walkers are independent Markov chains and this "
               "build measures one GPU's share; multi-GPU sharding is not built.", file=sys.stderr)
         sys.exit(2)
 
-    r = gpu_run(args.steps, args.warmup)
+    r = gpu_run(args.steps, args.warmup, variant=args.variant)
     moves = args.steps * WALKERS * SWEEPS * N_ELEC
     value = moves / r["seconds"]
     ms_step = 1e3 * r["seconds"] / args.steps
@@ -150,7 +156,9 @@ def main():
     achieved = fl / avg_kernel_s / 1e12
     roofline = dict(bound="fp64-valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
-                    kernel="sweep2_kernel<LCAO2>", kernel_ms=1e3 * avg_kernel_s,
+                    kernel=("sweep2_lane_kernel<LCAO2>" if args.variant == "lane"
+                            else "sweep2_ws_kernel<LCAO2,2,4>"),
+                    kernel_ms=1e3 * avg_kernel_s,
                     flop_per_launch=fl)
     cpu = None
     if not args.no_cpu_baseline:

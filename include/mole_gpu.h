@@ -227,6 +227,17 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
                                           double step, uint32_t sr_mode, double sr_eps,
                                           double *dp, double *g);
 
+/* ---- launch configuration ----------------------------------------------
+ * No counterpart in the reference (rayon's fan-out, src/vmc/src/vmc.rs:63-76,
+ * is the closest).  Selects the sweep kernel: AUTO picks the
+ * wave-specialised kernel where it applies; LANE runs one lane per walker
+ * with block_threads threads per workgroup (0 keeps the current value).
+ * Every variant produces bit-identical walker trajectories. */
+#define MOLE_VARIANT_AUTO 0
+#define MOLE_VARIANT_LANE 1
+#define MOLE_VARIANT_WAVESPEC 2
+mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t block_threads);
+
 /* Waits for all work of the context. */
 mole_status mole_synchronize(mole_ctx *ctx);
 /* Device time of the kernels of the last mole_sweep/mole_run, measured

@@ -35,6 +35,9 @@ O_REFERENCE = 1
 FLAG_TRACE = 1
 FLAG_SAMPLES = 2
 SWEEP_SAMPLE = 1
+VARIANT_AUTO = 0
+VARIANT_LANE = 1
+VARIANT_WAVESPEC = 2
 MAX_PARAMS = 16
 
 OPT_SD = 0
@@ -135,6 +138,7 @@ _SIGS = [
     ("mole_read_trace", C.c_int32, [_P, C.c_uint32, _pu64, _pd, C.c_size_t, C.c_size_t, C.c_size_t]),
     ("mole_read_samples", C.c_int32, [_P, C.c_uint32, _pd, _pd, _pd, C.c_size_t, C.c_size_t]),
     ("mole_eval", C.c_int32, [_P, C.c_uint32, C.c_size_t, _pd, _pd, _pd, _pd, _pd, _pd]),
+    ("mole_set_launch_config", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_synchronize", C.c_int32, [_P]),
     ("mole_last_timing", C.c_int32, [_P, _pd, C.POINTER(C.c_uint32)]),
     ("mole_compute_parameter_update", C.c_int32, [C.POINTER(SrSums), C.c_uint32, C.c_double, C.c_uint32, C.c_double, _pd, _pd]),
@@ -329,6 +333,9 @@ class Context:
         check(lib().mole_eval(self.h, sys, n, _dp(a), _dp(out["value"]), _dp(out["grad"]),
                               _dp(out["lap"]), _dp(out["e_local"]), _dp(out["pgrad"])))
         return out
+
+    def set_launch_config(self, variant=VARIANT_AUTO, block_threads=0):
+        check(lib().mole_set_launch_config(self.h, variant, block_threads))
 
     def synchronize(self):
         check(lib().mole_synchronize(self.h))

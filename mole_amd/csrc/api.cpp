@@ -76,7 +76,8 @@ struct mole_ctx {
   double *d_red = nullptr;
   uint32_t last_launches = 0;
   bool timed = false;
-  int block_threads = 64;
+  int variant = 0;        // MOLE_VARIANT_*
+  int block_threads = 64; // one-lane-per-walker kernel
   uint32_t max_sweeps_per_launch = 4096;
 };
 
@@ -210,7 +211,7 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
       a.trace_sample0 = ctx->trace_samples;
       a.cap_sweeps = (st.d_trace_mask) ? ctx->run.record_sweeps : 0;
       a.cap_samples = (st.d_trace_e || st.d_samp_psi) ? ctx->run.record_sweeps : 0;
-      HIP_TRY(launch_sweep2(a, ctx->block_threads, ctx->stream));
+      HIP_TRY(launch_sweep2(a, ctx->variant, ctx->block_threads, ctx->stream));
       ++launches;
       const uint64_t filled = (uint64_t)ctx->block_fill + n_samp;
       st.n_samples += (uint64_t)st.W * n_samp;
@@ -663,6 +664,21 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
     dp[i] = s / A[i][i];
   }
   for (int i = 0; i < np; ++i) dp[i] = step * dp[i];
+  return ok();
+}
+
+mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t block_threads) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (variant > MOLE_VARIANT_WAVESPEC) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown variant %u", variant);
+  if (block_threads != 0 && (block_threads % 64 != 0 || block_threads > 256))
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "block_threads must be 0 or a multiple of 64 up to 256");
+  if (variant == MOLE_VARIANT_WAVESPEC)
+    for (const SysState &s : ctx->sys)
+      if (!sweep2_wavespec_eligible(s.sp))
+        return fail(MOLE_ERR_UNSUPPORTED, "wave-specialised kernel needs <= 2 nuclei (1 for helium)");
+  ctx->variant = (int)variant;
+  if (block_threads) ctx->block_threads = (int)block_threads;
   return ok();
 }
 

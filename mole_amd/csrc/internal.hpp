@@ -66,7 +66,9 @@ struct Eval2Args {
   double *value, *grad, *lap, *e_local, *pgrad;
 };
 
-hipError_t launch_sweep2(const Sweep2Args &a, int block_threads,
+// variant: 0 auto, 1 one lane per walker, 2 wave-specialised (when eligible)
+bool sweep2_wavespec_eligible(const Sys2 &sp);
+hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads,
                          hipStream_t stream);
 hipError_t launch_init2(double *pos, int W, uint64_t seed, uint32_t sys,
                         uint32_t epoch, hipStream_t stream);

@@ -123,8 +123,73 @@ __device__ __forceinline__ double value2(const Sys2 &sp, const double *x) {
   return (phi1 * phi2) * exp_m(pade(sp.b, sp.c, r12));
 }
 
-template <int KIND, bool SR>
-__global__ void __launch_bounds__(256) sweep2_kernel(Sweep2Args a) {
+// Per-walker sample accumulators: energy sums, block means, SR sums, and
+// the optional trace / compat-sample writes, in sample order.
+struct SampleAcc {
+  double acc[kAccColumns2];
+  uint32_t fill, sample_idx;
+
+  __device__ __forceinline__ void load(const Sweep2Args &a, int w) {
+#pragma unroll
+    for (int k = 0; k < kAccColumns2; ++k)
+      if (k != kAccAccepted) acc[k] = a.acc[(size_t)k * a.W + w];
+    fill = a.block_fill;
+    sample_idx = a.trace_sample0;
+  }
+  __device__ __forceinline__ void store(const Sweep2Args &a, int w) const {
+#pragma unroll
+    for (int k = 0; k < kAccColumns2; ++k)
+      if (k != kAccAccepted) a.acc[(size_t)k * a.W + w] = acc[k];
+  }
+  __device__ __forceinline__ void add(const Sweep2Args &a, int w, double E,
+                                      const double *dlog, double psi, double inv_block) {
+    acc[kAccSumE] += E;
+    acc[kAccSumE2] = fma(E, E, acc[kAccSumE2]);
+    acc[kAccBlockSum] += E;
+    if (++fill == a.block_size) {
+      const double bm = acc[kAccBlockSum] * inv_block;
+      acc[kAccSumBm] += bm;
+      acc[kAccSumBm2] = fma(bm, bm, acc[kAccSumBm2]);
+      acc[kAccBlockSum] = 0.0;
+      fill = 0;
+    }
+    double O[kNParam2];
+    if (a.o_conv == MOLE_O_REFERENCE) {
+      // G = psi ./ (psi * dpsi), dpsi = psi dlog; O = G / psi_s with
+      // psi_s = (psi*psi)/psi the "Wavefunction value" sample
+      const double psi_s = (psi * psi) / psi;
+#pragma unroll
+      for (int k = 0; k < kNParam2; ++k) O[k] = (psi / (psi * (psi * dlog[k]))) / psi_s;
+    } else {
+#pragma unroll
+      for (int k = 0; k < kNParam2; ++k) O[k] = dlog[k];
+    }
+    int t = 0;
+#pragma unroll
+    for (int k = 0; k < kNParam2; ++k) {
+      acc[kAccSumO + k] += O[k];
+      acc[kAccSumOE + k] = fma(O[k], E, acc[kAccSumOE + k]);
+#pragma unroll
+      for (int l = k; l < kNParam2; ++l, ++t)
+        acc[kAccSumOO + t] = fma(O[k], O[l], acc[kAccSumOO + t]);
+    }
+    if (sample_idx < a.cap_samples) {
+      const size_t si = (size_t)w * a.cap_samples + sample_idx;
+      const double psi_s = (psi * psi) / psi;
+      if (a.trace_e) a.trace_e[si] = E;
+      if (a.samp_psi) a.samp_psi[si] = psi_s;
+      if (a.samp_g)
+#pragma unroll
+        for (int k = 0; k < kNParam2; ++k) a.samp_g[si * kNParam2 + k] = O[k] * psi_s;
+    }
+    ++sample_idx;
+  }
+};
+
+// One lane per walker, every stage in the same wave: the general path
+// (any number of nuclei up to kMaxNuc2).
+template <int KIND>
+__global__ void __launch_bounds__(256) sweep2_lane_kernel(Sweep2Args a) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.W) return;
   const Sys2 &sp = a.sp;
@@ -147,15 +212,12 @@ __global__ void __launch_bounds__(256) sweep2_kernel(Sweep2Args a) {
     psi = (phi1 * phi2) * exp_m(pade(sp.b, sp.c, r12));
   }
 
-  double acc[kAccColumns2];
-#pragma unroll
-  for (int k = 0; k < kAccColumns2; ++k) acc[k] = a.acc[(size_t)k * W + w];
-
+  SampleAcc sa;
+  sa.load(a, w);
+  double accepted = a.acc[(size_t)kAccAccepted * W + w];
   Stream rng;
   rng.init(a.seed, a.sys, (uint32_t)w, a.epoch);
   uint64_t p = a.draw0;
-  uint32_t fill = a.block_fill;
-  uint32_t sample_idx = a.trace_sample0;
   const double inv_block = 1.0 / (double)a.block_size;
 
   for (uint32_t s = 0; s < a.n_sweeps; ++s) {
@@ -181,8 +243,7 @@ __global__ void __launch_bounds__(256) sweep2_kernel(Sweep2Args a) {
         psin = ((e == 0) ? (nphi * phi2) : (phi1 * nphi)) * jas;
       }
       const double ratio = min_one((psin * psin) / (psi * psi));
-      const bool acc_move = ratio > u;
-      if (acc_move) {
+      if (ratio > u) {
         x[3 * e] = nx;
         x[3 * e + 1] = ny;
         x[3 * e + 2] = nz;
@@ -193,69 +254,255 @@ __global__ void __launch_bounds__(256) sweep2_kernel(Sweep2Args a) {
         } else {
           if (e == 0) phi1 = nphi; else phi2 = nphi;
         }
-        acc[kAccAccepted] += 1.0;
+        accepted += 1.0;
         mask |= (uint64_t)1 << e;
       }
     }
     if (a.trace_mask && a.trace_sweep0 + s < a.cap_sweeps)
       a.trace_mask[(size_t)w * a.cap_sweeps + a.trace_sweep0 + s] = mask;
-
     if (s >= a.sample_start) {
       const Local2 L = local2<KIND>(sp, x);
-      const double E = L.e_local;
-      acc[kAccSumE] += E;
-      acc[kAccSumE2] = fma(E, E, acc[kAccSumE2]);
-      acc[kAccBlockSum] += E;
-      if (++fill == a.block_size) {
-        const double bm = acc[kAccBlockSum] * inv_block;
-        acc[kAccSumBm] += bm;
-        acc[kAccSumBm2] = fma(bm, bm, acc[kAccSumBm2]);
-        acc[kAccBlockSum] = 0.0;
-        fill = 0;
-      }
-      double O[kNParam2];
-      if (SR || a.samp_g) {
-        if (a.o_conv == MOLE_O_REFERENCE) {
-          // G = psi ./ (psi * dpsi), dpsi = psi dlog; O = G / psi_s with
-          // psi_s = (psi*psi)/psi the "Wavefunction value" sample
-          const double psi_s = (psi * psi) / psi;
-#pragma unroll
-          for (int k = 0; k < kNParam2; ++k)
-            O[k] = (psi / (psi * (psi * L.dlog[k]))) / psi_s;
-        } else {
-#pragma unroll
-          for (int k = 0; k < kNParam2; ++k) O[k] = L.dlog[k];
-        }
-      }
-      if (SR) {
-        int t = 0;
-#pragma unroll
-        for (int k = 0; k < kNParam2; ++k) {
-          acc[kAccSumO + k] += O[k];
-          acc[kAccSumOE + k] = fma(O[k], E, acc[kAccSumOE + k]);
-#pragma unroll
-          for (int l = k; l < kNParam2; ++l, ++t)
-            acc[kAccSumOO + t] = fma(O[k], O[l], acc[kAccSumOO + t]);
-        }
-      }
-      if (sample_idx < a.cap_samples) {
-        const size_t si = (size_t)w * a.cap_samples + sample_idx;
-        if (a.trace_e) a.trace_e[si] = E;
-        if (a.samp_psi) a.samp_psi[si] = (psi * psi) / psi;
-        if (a.samp_g) {
-          const double psi_s = (psi * psi) / psi;
-#pragma unroll
-          for (int k = 0; k < kNParam2; ++k) a.samp_g[si * kNParam2 + k] = O[k] * psi_s;
-        }
-      }
-      ++sample_idx;
+      sa.add(a, w, L.e_local, L.dlog, psi, inv_block);
     }
   }
-
 #pragma unroll
   for (int k = 0; k < 6; ++k) a.pos[(size_t)k * W + w] = x[k];
+  sa.store(a, w);
+  a.acc[(size_t)kAccAccepted * W + w] = accepted;
+}
+
+// Local energy and d ln psi/dp from a configuration whose distances to the
+// nuclei (and, for the LCAO kind, the orbital exponentials) are known.
+template <int KIND, int NNUC>
+__device__ __forceinline__ void local2_cached(const Sys2 &sp, const double *x,
+                                              const double (*r)[NNUC],
+                                              const double (*ee)[NNUC], double r12,
+                                              double &E, double *dlog) {
+  const double zeta = sp.zeta, b = sp.b, c = sp.c;
+  double V = sp.v_nn;
+  double gi[2][3], li[2], dz = 0.0;
 #pragma unroll
-  for (int k = 0; k < kAccColumns2; ++k) a.acc[(size_t)k * W + w] = acc[k];
+  for (int e = 0; e < 2; ++e) {
+    double phi = 0.0, gs0 = 0.0, gs1 = 0.0, gs2 = 0.0, ls = 0.0, zs = 0.0;
+#pragma unroll
+    for (int I = 0; I < NNUC; ++I) {
+      const double d0 = x[3 * e] - sp.nuc[I][0], d1 = x[3 * e + 1] - sp.nuc[I][1],
+                   d2 = x[3 * e + 2] - sp.nuc[I][2];
+      const double inv = 1.0 / r[e][I];
+      V = V - sp.charge[I] * inv;
+      const double eI = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : ee[e][I];
+      phi += eI;
+      const double f = -zeta * eI * inv;
+      gs0 = fma(f, d0, gs0);
+      gs1 = fma(f, d1, gs1);
+      gs2 = fma(f, d2, gs2);
+      ls = fma(eI, zeta * zeta - 2.0 * zeta * inv, ls);
+      zs = fma(-r[e][I], eI, zs);
+    }
+    const double ip = 1.0 / phi;
+    gi[e][0] = gs0 * ip;
+    gi[e][1] = gs1 * ip;
+    gi[e][2] = gs2 * ip;
+    li[e] = ls * ip;
+    dz += zs * ip;
+  }
+  const double d0 = x[0] - x[3], d1 = x[1] - x[4], d2 = x[2] - x[5];
+  const double i12 = 1.0 / r12;
+  const double iden = 1.0 / (1.0 + c * r12);
+  const double up = b * iden * iden;
+  const double upp = -2.0 * b * c * iden * iden * iden;
+  const double J = upp + 2.0 * up * i12 + up * up;
+  const double h0 = d0 * i12, h1 = d1 * i12, h2 = d2 * i12;
+  const double gd1 = gi[0][0] * h0 + gi[0][1] * h1 + gi[0][2] * h2;
+  const double gd2 = gi[1][0] * h0 + gi[1][1] * h1 + gi[1][2] * h2;
+  const double k1 = li[0] + J + 2.0 * up * gd1;
+  const double k2 = li[1] + J - 2.0 * up * gd2;
+  if (sp.hamiltonian == MOLE_H_ELECTRONIC) V = V + i12;
+  E = -0.5 * (k1 + k2) + V;
+  dlog[0] = dz;
+  dlog[1] = r12 * iden;
+  dlog[2] = -b * r12 * r12 * iden * iden;
+}
+
+// Wave-specialised sweep: a workgroup of four waves owns 64 walkers.
+//   wave 0      the Metropolis chain (proposal, psi', accept, state update);
+//   waves 1, 2  Philox draws of chunk c+1 (electron 0 / electron 1) into LDS;
+//   wave 3      local energy, d ln psi/dp and accumulators of chunk c-1,
+//               from state snapshots the chain leaves in LDS.
+// Chunks are T sweeps; every buffer is double-buffered and the four roles
+// meet at one barrier per chunk, so the chain -- the only serial part --
+// runs alone on its SIMD and its arithmetic is exactly that of the
+// one-wave kernel.
+template <int KIND, int NNUC, int T>
+__global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
+  constexpr int SNAP = 6 + 4 * NNUC + 2;  // x, r[e][I], e[e][I], r12, psi
+  __shared__ double rng_lds[2][T][2][4][64];
+  __shared__ double snap_lds[2][T][SNAP][64];
+  const int lane = threadIdx.x & 63;
+  const int role = threadIdx.x >> 6;
+  const int w = blockIdx.x * 64 + lane;
+  const bool live = w < a.W;
+  const int W = a.W;
+  const Sys2 &sp = a.sp;
+  const uint32_t nchunks = (a.n_sweeps + T - 1) / T;
+  const double inv_block = 1.0 / (double)a.block_size;
+
+  // chain state
+  double x[6];
+  double r[2][NNUC], ee[2][NNUC], phi[2];
+  double r12 = 0.0, psi = 1.0, accepted = 0.0;
+  // producer state
+  Stream rng;
+  // sampler state
+  SampleAcc sa;
+
+  if (role == 0) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) x[k] = live ? a.pos[(size_t)k * W + w] : 0.0;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+      phi[e] = 0.0;
+#pragma unroll
+      for (int I = 0; I < NNUC; ++I) {
+        r[e][I] = norm3(x[3 * e] - sp.nuc[I][0], x[3 * e + 1] - sp.nuc[I][1],
+                        x[3 * e + 2] - sp.nuc[I][2]);
+        ee[e][I] = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : exp_m(-sp.zeta * r[e][I]);
+        phi[e] = (I == 0) ? ee[e][I] : phi[e] + ee[e][I];
+      }
+    }
+    r12 = norm3(x[0] - x[3], x[1] - x[4], x[2] - x[5]);
+    if (KIND == MOLE_WF_HELIUM_PADE) {
+      psi = exp_m(-sp.zeta * (r[0][0] + r[1][0]) + pade(sp.b, sp.c, r12));
+    } else {
+      psi = (phi[0] * phi[1]) * exp_m(pade(sp.b, sp.c, r12));
+    }
+    accepted = live ? a.acc[(size_t)kAccAccepted * W + w] : 0.0;
+  } else if (role == 3) {
+    if (live) sa.load(a, w);
+  } else {
+    rng.init(a.seed, a.sys, (uint32_t)w, a.epoch);
+  }
+
+  // producers: draws of chunk cc for electron e = role - 1
+  auto produce = [&](uint32_t cc) {
+    const int e = role - 1;
+    const int buf = cc & 1;
+    const uint32_t s0 = cc * T;
+    for (int t = 0; t < T; ++t) {
+      if (s0 + t >= a.n_sweeps) break;
+      const uint64_t p = a.draw0 + ((uint64_t)(s0 + t) * 2 + e) * 4;
+      rng_lds[buf][t][e][0][lane] = u64_range(rng.draw(p), a.scale, a.offset);
+      rng_lds[buf][t][e][1][lane] = u64_range(rng.draw(p + 1), a.scale, a.offset);
+      rng_lds[buf][t][e][2][lane] = u64_range(rng.draw(p + 2), a.scale, a.offset);
+      rng_lds[buf][t][e][3][lane] = u64_standard(rng.draw(p + 3));
+    }
+  };
+
+  if (role == 1 || role == 2) {
+    if (nchunks > 0) produce(0);
+  }
+  __syncthreads();
+
+  for (uint32_t c = 0; c <= nchunks; ++c) {
+    const int buf = c & 1;
+    const uint32_t s0 = c * T;
+    if (role == 0 && c < nchunks) {
+      for (int t = 0; t < T; ++t) {
+        const uint32_t s = s0 + t;
+        if (s >= a.n_sweeps) break;
+        uint64_t mask = 0;
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const double dx = rng_lds[buf][t][e][0][lane];
+          const double dy = rng_lds[buf][t][e][1][lane];
+          const double dz = rng_lds[buf][t][e][2][lane];
+          const double u = rng_lds[buf][t][e][3][lane];
+          const double nx = x[3 * e] + dx, ny = x[3 * e + 1] + dy, nz = x[3 * e + 2] + dz;
+          const double n12 = (e == 0) ? norm3(nx - x[3], ny - x[4], nz - x[5])
+                                      : norm3(x[0] - nx, x[1] - ny, x[2] - nz);
+          double nr[NNUC], ne[NNUC], nphi = 0.0, psin;
+#pragma unroll
+          for (int I = 0; I < NNUC; ++I) {
+            nr[I] = norm3(nx - sp.nuc[I][0], ny - sp.nuc[I][1], nz - sp.nuc[I][2]);
+            ne[I] = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : exp_m(-sp.zeta * nr[I]);
+            nphi = (I == 0) ? ne[I] : nphi + ne[I];
+          }
+          if (KIND == MOLE_WF_HELIUM_PADE) {
+            const double s12 = (e == 0) ? (nr[0] + r[1][0]) : (r[0][0] + nr[0]);
+            psin = exp_m(-sp.zeta * s12 + pade(sp.b, sp.c, n12));
+          } else {
+            const double jas = exp_m(pade(sp.b, sp.c, n12));
+            psin = ((e == 0) ? (nphi * phi[1]) : (phi[0] * nphi)) * jas;
+          }
+          const double ratio = min_one((psin * psin) / (psi * psi));
+          if (ratio > u) {
+            x[3 * e] = nx;
+            x[3 * e + 1] = ny;
+            x[3 * e + 2] = nz;
+#pragma unroll
+            for (int I = 0; I < NNUC; ++I) {
+              r[e][I] = nr[I];
+              ee[e][I] = ne[I];
+            }
+            phi[e] = nphi;
+            r12 = n12;
+            psi = psin;
+            accepted += 1.0;
+            mask |= (uint64_t)1 << e;
+          }
+        }
+        if (live && a.trace_mask && a.trace_sweep0 + s < a.cap_sweeps)
+          a.trace_mask[(size_t)w * a.cap_sweeps + a.trace_sweep0 + s] = mask;
+        double *sn = &snap_lds[buf][t][0][0];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) sn[k * 64 + lane] = x[k];
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int I = 0; I < NNUC; ++I) {
+            sn[(6 + e * NNUC + I) * 64 + lane] = r[e][I];
+            sn[(6 + 2 * NNUC + e * NNUC + I) * 64 + lane] = ee[e][I];
+          }
+        sn[(6 + 4 * NNUC) * 64 + lane] = r12;
+        sn[(7 + 4 * NNUC) * 64 + lane] = psi;
+      }
+    } else if ((role == 1 || role == 2) && c + 1 < nchunks) {
+      produce(c + 1);
+    } else if (role == 3 && c >= 1) {
+      const int pb = (c - 1) & 1;
+      const uint32_t p0 = (c - 1) * T;
+      for (int t = 0; t < T; ++t) {
+        const uint32_t s = p0 + t;
+        if (s >= a.n_sweeps) break;
+        if (s < a.sample_start) continue;
+        const double *sn = &snap_lds[pb][t][0][0];
+        double xs[6], rs[2][NNUC], es[2][NNUC];
+#pragma unroll
+        for (int k = 0; k < 6; ++k) xs[k] = sn[k * 64 + lane];
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int I = 0; I < NNUC; ++I) {
+            rs[e][I] = sn[(6 + e * NNUC + I) * 64 + lane];
+            es[e][I] = sn[(6 + 2 * NNUC + e * NNUC + I) * 64 + lane];
+          }
+        const double r12s = sn[(6 + 4 * NNUC) * 64 + lane];
+        double E, dlog[kNParam2];
+        local2_cached<KIND, NNUC>(sp, xs, rs, es, r12s, E, dlog);
+        const double psis = sn[(7 + 4 * NNUC) * 64 + lane];
+        if (live) sa.add(a, w, E, dlog, psis, inv_block);
+      }
+    }
+    __syncthreads();
+  }
+
+  if (role == 0 && live) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.pos[(size_t)k * W + w] = x[k];
+    a.acc[(size_t)kAccAccepted * W + w] = accepted;
+  } else if (role == 3 && live) {
+    sa.store(a, w);
+  }
 }
 
 __global__ void init2_kernel(double *pos, int W, uint64_t seed, uint32_t sys,
@@ -309,14 +556,31 @@ __global__ void fill_kernel(double *p, size_t n, double v) {
   if (i < n) p[i] = v;
 }
 
-hipError_t launch_sweep2(const Sweep2Args &a, int block_threads, hipStream_t stream) {
-  const dim3 grid((a.W + block_threads - 1) / block_threads);
-  const bool sr = true;
-  if (a.sp.kind == MOLE_WF_HELIUM_PADE) {
-    if (sr) hipLaunchKernelGGL((sweep2_kernel<MOLE_WF_HELIUM_PADE, true>), grid, dim3(block_threads), 0, stream, a);
-  } else {
-    if (sr) hipLaunchKernelGGL((sweep2_kernel<MOLE_WF_LCAO2_PADE, true>), grid, dim3(block_threads), 0, stream, a);
+bool sweep2_wavespec_eligible(const Sys2 &sp) {
+  return (sp.kind == MOLE_WF_HELIUM_PADE) ? sp.n_nuc == 1 : sp.n_nuc <= 2;
+}
+
+hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads, hipStream_t stream) {
+  const bool he = a.sp.kind == MOLE_WF_HELIUM_PADE;
+  const int nn = a.sp.n_nuc;
+  if (variant != 1 && sweep2_wavespec_eligible(a.sp)) {
+    // wave-specialised path: 64 walkers per 256-thread workgroup
+    const dim3 grid((a.W + 63) / 64), blk(256);
+    constexpr int T = 4;
+    if (he)
+      hipLaunchKernelGGL((sweep2_ws_kernel<MOLE_WF_HELIUM_PADE, 1, T>), grid, blk, 0, stream, a);
+    else if (nn == 1)
+      hipLaunchKernelGGL((sweep2_ws_kernel<MOLE_WF_LCAO2_PADE, 1, T>), grid, blk, 0, stream, a);
+    else
+      hipLaunchKernelGGL((sweep2_ws_kernel<MOLE_WF_LCAO2_PADE, 2, T>), grid, blk, 0, stream, a);
+    return hipGetLastError();
   }
+  const int bt = block_threads ? block_threads : 64;
+  const dim3 grid((a.W + bt - 1) / bt);
+  if (he)
+    hipLaunchKernelGGL((sweep2_lane_kernel<MOLE_WF_HELIUM_PADE>), grid, dim3(bt), 0, stream, a);
+  else
+    hipLaunchKernelGGL((sweep2_lane_kernel<MOLE_WF_LCAO2_PADE>), grid, dim3(bt), 0, stream, a);
   return hipGetLastError();
 }
 

@@ -74,17 +74,29 @@ def _systems(kind, W, params):
     return M.h2(W, *params), O.h2(*params)
 
 
-@pytest.mark.parametrize("kind,params", [("he", (1.6875, 0.5, 0.3)), ("h2", (1.0, 0.5, 0.3)),
-                                         ("he", (2.0, 0.0, 0.3))])
-def test_trajectory_parity_with_oracle(kind, params):
-    W, steps, B, side, seed = 192, 80, 20, 0.5, 0
+TRAJ_CASES = [
+    # kind, params, walkers, steps, block
+    ("he", (1.6875, 0.5, 0.3), 192, 80, 20),
+    ("h2", (1.0, 0.5, 0.3), 192, 80, 20),
+    ("he", (2.0, 0.0, 0.3), 192, 80, 20),
+    ("h2", (1.2, 0.4, 0.6), 100, 57, 10),   # ragged: 100 walkers, 50 sweeps, sampling from sweep 10
+    ("he", (1.6875, 0.5, 0.3), 1, 23, 7),   # one walker, 21 sweeps
+]
+
+
+@pytest.mark.parametrize("variant", [M.VARIANT_LANE, M.VARIANT_WAVESPEC])
+@pytest.mark.parametrize("kind,params,W,steps,B", TRAJ_CASES)
+def test_trajectory_parity_with_oracle(kind, params, W, steps, B, variant):
+    side, seed = 0.5, 0
     gsys, owf = _systems(kind, W, params)
     with M.Context([gsys], seed=seed, step=side, block_size=B, flags=M.FLAG_TRACE | M.FLAG_SAMPLES,
                    record_sweeps=steps) as ctx:
+        ctx.set_launch_config(variant)
         ctx.init_walkers()
         ctx.run(steps)
-        ns = steps - B
-        masks, el = ctx.trace(steps, ns)
+        total = (steps // B) * B
+        ns = total - B
+        masks, el = ctx.trace(total, ns)
         cfg = ctx.get_configs()
         st = ctx.stats()
         e_s, psi_s, g_s = ctx.samples(ns)
@@ -104,6 +116,45 @@ def test_trajectory_parity_with_oracle(kind, params):
     assert abs(st["sum_e2"] - ref["sum_e2"]) <= 1e-10 * ref["sum_e2"]
     assert abs(st["sum_block_mean"] - ref["sum_bm"]) <= 1e-11 * ref["n_blocks"]
     assert abs(st["sum_block_mean2"] - ref["sum_bm2"]) <= 1e-10 * ref["sum_bm2"]
+
+
+def test_three_nuclei_general_path_parity():
+    # H3+-like LCAO with three centres runs on the one-lane kernel only
+    W, steps, B = 96, 40, 10
+    nuc = [(0.0, 0.0, -1.0), (0.0, 0.9, 0.4), (0.0, -0.9, 0.4)]
+    gsys = M.System(M.WF_LCAO2_PADE, nuc, [1, 1, 1], [1.1, 0.5, 0.3], W)
+    owf = O.make_wf2(O.WF_LCAO2_PADE, (1.1, 0.5, 0.3), nuc, [1, 1, 1])
+    with M.Context([gsys], seed=4, step=0.5, block_size=B, flags=M.FLAG_TRACE, record_sweeps=steps) as ctx:
+        with pytest.raises(M.MoleError) as ei:
+            ctx.set_launch_config(M.VARIANT_WAVESPEC)
+        assert ei.value.status == M.ERR_UNSUPPORTED
+        ctx.init_walkers()
+        ctx.run(steps)
+        masks, el = ctx.trace(steps, steps - B)
+    ref = O.run_box(owf, seed=4, walkers=W, steps=steps, block_size=B, box_side=0.5)
+    assert np.array_equal(masks, ref["accept_mask"])
+    assert np.max(np.abs(el - ref["e_local"]) / np.maximum(1, np.abs(ref["e_local"]))) <= E_TOL
+
+
+def test_variants_agree_on_split_sweeps():
+    # arbitrary sweep splits (not multiples of the chunk), sampling switched
+    # on mid-stream: both kernels leave identical state and statistics
+    res = []
+    for variant in (M.VARIANT_LANE, M.VARIANT_WAVESPEC):
+        with M.Context([M.h2(130)], seed=8, step=0.5, block_size=6) as ctx:
+            ctx.set_launch_config(variant)
+            ctx.init_walkers()
+            ctx.sweep(7)
+            ctx.sweep(13, sample=True)
+            ctx.sweep(5, sample=True)
+            res.append((ctx.get_configs(), ctx.stats(), ctx.sr_sums()))
+    (c0, s0, r0), (c1, s1, r1) = res
+    assert np.array_equal(bits(c0), bits(c1))
+    assert s0["n_accepted"] == s1["n_accepted"]
+    for k in ("sum_e", "sum_e2", "sum_block_mean", "sum_block_mean2"):
+        assert s0[k] == s1[k], k
+    for k in ("sum_o", "sum_oe", "sum_oo"):
+        assert np.array_equal(r0[k], r1[k]), k
 
 
 @pytest.mark.parametrize("conv", [M.O_LOGDERIV, M.O_REFERENCE])
