@@ -19,7 +19,12 @@ extern "C" {
 mole_status mole_debug_philox(uint64_t seed, uint32_t sys, uint32_t walker, uint32_t epoch,
                               uint64_t first, size_t count, uint64_t *out);
 
-#define MOLE_DEBUG_EXP 0      /* out = exp(a)            */
+/* Copies n_doubles doubles between two device buffers `repeats` times with a
+ * plain grid-stride kernel (copy_probe_kernel) and reports the average
+ * device time per copy: the known-size reference for HBM byte counters. */
+mole_status mole_debug_copy(size_t n_doubles, int32_t repeats, double *ms_per_copy);
+
+#define MOLE_DEBUG_EXP 0     /* out = exp(a)            */
 #define MOLE_DEBUG_SQRT 1     /* out = sqrt(a)           */
 #define MOLE_DEBUG_DIV 2      /* out = a / b             */
 #define MOLE_DEBUG_STANDARD 3 /* out = Standard(bits(a)) */

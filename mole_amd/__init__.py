@@ -144,6 +144,7 @@ _SIGS = [
     ("mole_compute_parameter_update", C.c_int32, [C.POINTER(SrSums), C.c_uint32, C.c_double, C.c_uint32, C.c_double, _pd, _pd]),
     ("mole_debug_philox", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t, _pu64]),
     ("mole_debug_math", C.c_int32, [C.c_int32, C.c_size_t, _pd, _pd, _pd]),
+    ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
 ]
 
 
@@ -375,3 +376,10 @@ def parameter_update(sums, optimizer=OPT_SR, step=0.05, sr_mode=SR_TEXTBOOK, sr_
     check(lib().mole_compute_parameter_update(C.byref(sums), optimizer, step, sr_mode, sr_eps,
                                               _dp(dp), _dp(g)))
     return dp, g
+
+
+def debug_copy(n_doubles, repeats=1):
+    """Device copy of n_doubles doubles (calibration of HBM byte counters); ms per copy."""
+    ms = C.c_double()
+    check(lib().mole_debug_copy(n_doubles, repeats, C.byref(ms)))
+    return ms.value

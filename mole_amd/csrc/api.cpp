@@ -59,6 +59,9 @@ struct SysState {
   double *d_samp_psi = nullptr;
   double *d_samp_g = nullptr;
   uint64_t n_samples = 0, n_blocks = 0, n_moves = 0;
+  // column sums of d_acc, valid until the next launch that writes d_acc
+  double cols[kAccColumns2] = {};
+  bool cols_valid = false;
 };
 
 }  // namespace
@@ -211,6 +214,7 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
       a.trace_sample0 = ctx->trace_samples;
       a.cap_sweeps = (st.d_trace_mask) ? ctx->run.record_sweeps : 0;
       a.cap_samples = (st.d_trace_e || st.d_samp_psi) ? ctx->run.record_sweeps : 0;
+      st.cols_valid = false;
       HIP_TRY(launch_sweep2(a, ctx->variant, ctx->block_threads, ctx->stream));
       ++launches;
       const uint64_t filled = (uint64_t)ctx->block_fill + n_samp;
@@ -441,7 +445,10 @@ mole_status mole_run(mole_ctx *ctx, uint32_t steps, uint32_t block_size) {
     // a run starts its own blocks: drop the partial block of earlier calls
     HIP_TRY(hipSetDevice(ctx->device));
     for (SysState &s : ctx->sys)
+    {
+      s.cols_valid = false;
       HIP_TRY(launch_fill(s.d_acc + (size_t)kAccBlockSum * s.W, (size_t)s.W, 0.0, ctx->stream));
+    }
     ctx->block_fill = 0;
   }
   const uint32_t blocks = steps / block_size;
@@ -455,6 +462,7 @@ mole_status mole_reset_stats(mole_ctx *ctx) {
   for (SysState &s : ctx->sys) {
     HIP_TRY(launch_fill(s.d_acc, (size_t)kAccColumns2 * s.W, 0.0, ctx->stream));
     s.n_samples = s.n_blocks = s.n_moves = 0;
+    s.cols_valid = false;
   }
   ctx->block_fill = 0;
   ctx->trace_sweeps = ctx->trace_samples = 0;
@@ -463,11 +471,15 @@ mole_status mole_reset_stats(mole_ctx *ctx) {
 
 static mole_status reduce_sys(mole_ctx *ctx, uint32_t sys, double *cols) {
   SysState &s = ctx->sys[sys];
-  HIP_TRY(hipSetDevice(ctx->device));
-  HIP_TRY(launch_reduce_columns(s.d_acc, s.W, kAccColumns2, ctx->d_red, ctx->stream));
-  HIP_TRY(hipMemcpyAsync(cols, ctx->d_red, sizeof(double) * kAccColumns2, hipMemcpyDeviceToHost,
-                         ctx->stream));
-  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (!s.cols_valid) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, kAccColumns2, ctx->d_red, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s.cols, ctx->d_red, sizeof(double) * kAccColumns2,
+                           hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    s.cols_valid = true;
+  }
+  std::memcpy(cols, s.cols, sizeof s.cols);
   return MOLE_OK;
 }
 

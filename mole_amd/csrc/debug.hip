@@ -44,7 +44,48 @@ __global__ void math_kernel(int op, size_t n, const double *a, const double *b, 
   out[i] = r;
 }
 
+// Plain copy of n doubles: the calibration kernel for HBM byte counters.
+__global__ void copy_probe_kernel(const double *__restrict__ src, double *__restrict__ dst,
+                                  size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    dst[i] = src[i];
+}
+
+__global__ void iota_kernel(double *p, size_t n) {
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (size_t)gridDim.x * blockDim.x)
+    p[i] = (double)i;
+}
+
 }  // namespace
+
+extern "C" mole_status mole_debug_copy(size_t n_doubles, int32_t repeats, double *ms_per_copy) {
+  if (n_doubles == 0 || repeats < 1) return MOLE_ERR_INVALID_ARGUMENT;
+  double *src = nullptr, *dst = nullptr;
+  if (hipMalloc(&src, n_doubles * sizeof(double)) != hipSuccess) return MOLE_ERR_OUT_OF_MEMORY;
+  if (hipMalloc(&dst, n_doubles * sizeof(double)) != hipSuccess) {
+    (void)hipFree(src);
+    return MOLE_ERR_OUT_OF_MEMORY;
+  }
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  hipLaunchKernelGGL(iota_kernel, dim3(2048), dim3(256), 0, 0, src, n_doubles);
+  (void)hipEventRecord(e0, 0);
+  for (int r = 0; r < repeats; ++r)
+    hipLaunchKernelGGL(copy_probe_kernel, dim3(2048), dim3(256), 0, 0, src, dst, n_doubles);
+  (void)hipEventRecord(e1, 0);
+  hipError_t e = hipEventSynchronize(e1);
+  float ms = 0.0f;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (ms_per_copy) *ms_per_copy = (double)ms / repeats;
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(src);
+  (void)hipFree(dst);
+  return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
+}
 
 extern "C" mole_status mole_debug_philox(uint64_t seed, uint32_t sys, uint32_t walker,
                                          uint32_t epoch, uint64_t first, size_t count,
