@@ -1,0 +1,575 @@
+/*
+ * CPU oracle (test infrastructure only): Slater-Jastrow wavefunction, its
+ * local energy, and the two Metropolis runners described in sj.h.
+ */
+#include "sj.h"
+
+#include <math.h>
+#include <pthread.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "oracle.h"
+
+#define MAXE 64
+#define MAXN 32
+#define MAXNUC 64
+#define MAXB 512
+
+static int spin_of(const orc_sj *wf, int i) { return i < wf->n_up ? 0 : 1; }
+static int col_of(const orc_sj *wf, int i) { return i < wf->n_up ? i : i - wf->n_up; }
+static int nspin(const orc_sj *wf, int s) { return s == 0 ? wf->n_up : wf->n_dn; }
+static int electron_of(const orc_sj *wf, int s, int c) { return s == 0 ? c : wf->n_up + c; }
+
+static double rust_min1(double a) { return (a < 1.0) ? a : 1.0; }
+
+/* basis values chi[m] at r (distances to all nuclei first) */
+static void basis_at(const orc_sj *wf, const double *r, double *chi) {
+  double rn[MAXNUC];
+  for (int I = 0; I < wf->n_nuc; ++I)
+    rn[I] = orc_norm3(r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]);
+  for (int m = 0; m < wf->n_basis; ++m) chi[m] = orc_exp(-wf->zeta[m] * rn[wf->center[m]]);
+}
+
+static double orbital(const orc_sj *wf, int k, const double *chi) {
+  const double *c = wf->coef + (size_t)k * wf->n_basis;
+  double phi = 0.0;
+  for (int m = 0; m < wf->n_basis; ++m) phi = fma(c[m], chi[m], phi);
+  return phi;
+}
+
+/* Slater matrix A_s[c][k] (FP64) */
+static void slater_matrix(const orc_sj *wf, const double *cfg, int s, double *A) {
+  int n = nspin(wf, s);
+  double chi[MAXB];
+  for (int c = 0; c < n; ++c) {
+    basis_at(wf, cfg + 3 * electron_of(wf, s, c), chi);
+    for (int k = 0; k < n; ++k) A[c * n + k] = orbital(wf, k, chi);
+  }
+}
+
+double orc_sj_logdet(const orc_sj *wf, const double *cfg, int spin, double *sign) {
+  int n = nspin(wf, spin);
+  double A[MAXN * MAXN];
+  slater_matrix(wf, cfg, spin, A);
+  double sg = 1.0, ld = 0.0;
+  for (int p = 0; p < n; ++p) {
+    int piv = p;
+    for (int r = p + 1; r < n; ++r)
+      if (fabs(A[r * n + p]) > fabs(A[piv * n + p])) piv = r;
+    if (A[piv * n + p] == 0.0) {
+      *sign = 0.0;
+      return -INFINITY;
+    }
+    if (piv != p) {
+      for (int j = 0; j < n; ++j) {
+        double t = A[p * n + j];
+        A[p * n + j] = A[piv * n + j];
+        A[piv * n + j] = t;
+      }
+      sg = -sg;
+    }
+    double pv = A[p * n + p];
+    if (pv < 0) sg = -sg;
+    ld += log(fabs(pv));
+    for (int r = p + 1; r < n; ++r) {
+      double f = A[r * n + p] / pv;
+      for (int j = p; j < n; ++j) A[r * n + j] -= f * A[p * n + j];
+    }
+  }
+  *sign = sg;
+  return ld;
+}
+
+static double lu_det(const orc_sj *wf, const double *cfg, int s) {
+  double sg;
+  double ld = orc_sj_logdet(wf, cfg, s, &sg);
+  return sg == 0.0 ? 0.0 : sg * exp(ld);
+}
+
+long double orc_sj_det_leibniz(const orc_sj *wf, const double *cfg, int spin) {
+  int n = nspin(wf, spin);
+  double A[MAXN * MAXN];
+  slater_matrix(wf, cfg, spin, A);
+  if (n == 0) return 1.0L;
+  /* iterate permutations with Heap's algorithm */
+  int perm[8], cnt[8];
+  for (int i = 0; i < n; ++i) {
+    perm[i] = i;
+    cnt[i] = 0;
+  }
+  int sgn = 1;
+  long double total = 0.0L;
+  {
+    long double t = sgn;
+    for (int r = 0; r < n; ++r) t *= A[r * n + perm[r]];
+    total += t;
+  }
+  int i = 0;
+  while (i < n) {
+    if (cnt[i] < i) {
+      int a = (i % 2 == 0) ? 0 : cnt[i];
+      int tmp = perm[a];
+      perm[a] = perm[i];
+      perm[i] = tmp;
+      sgn = -sgn;
+      long double t = sgn;
+      for (int r = 0; r < n; ++r) t *= A[r * n + perm[r]];
+      total += t;
+      cnt[i]++;
+      i = 0;
+    } else {
+      cnt[i] = 0;
+      i++;
+    }
+  }
+  return total;
+}
+
+int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B) {
+  int n = nspin(wf, spin);
+  double M[MAXN * MAXN], I[MAXN * MAXN];
+  slater_matrix(wf, cfg, spin, M);
+  for (int r = 0; r < n; ++r)
+    for (int j = 0; j < n; ++j) I[r * n + j] = (r == j) ? 1.0 : 0.0;
+  int rc = 0;
+  for (int p = 0; p < n; ++p) {
+    int piv = p;
+    for (int r = p + 1; r < n; ++r)
+      if (fabs(M[r * n + p]) > fabs(M[piv * n + p])) piv = r;
+    if (piv != p)
+      for (int j = 0; j < n; ++j) {
+        double t = M[p * n + j];
+        M[p * n + j] = M[piv * n + j];
+        M[piv * n + j] = t;
+        t = I[p * n + j];
+        I[p * n + j] = I[piv * n + j];
+        I[piv * n + j] = t;
+      }
+    double pv = M[p * n + p];
+    if (pv == 0.0) rc = -1;
+    double ip = 1.0 / pv;
+    for (int j = 0; j < n; ++j) {
+      M[p * n + j] = M[p * n + j] * ip;
+      I[p * n + j] = I[p * n + j] * ip;
+    }
+    for (int r = 0; r < n; ++r) {
+      if (r == p) continue;
+      double f = M[r * n + p];
+      for (int j = 0; j < n; ++j) {
+        M[r * n + j] = fma(-f, M[p * n + j], M[r * n + j]);
+        I[r * n + j] = fma(-f, I[p * n + j], I[r * n + j]);
+      }
+    }
+  }
+  memcpy(B, I, sizeof(double) * n * n);
+  return rc;
+}
+
+/* ---- Jastrow ------------------------------------------------------- */
+
+static double pair_b(const orc_sj *wf, int i, int j) {
+  return spin_of(wf, i) == spin_of(wf, j) ? wf->p[1] : wf->p[0];
+}
+
+static double jas_full(const orc_sj *wf, const double *cfg) {
+  int N = wf->n_up + wf->n_dn;
+  double J = 0.0;
+  for (int i = 0; i < N; ++i)
+    for (int j = i + 1; j < N; ++j) {
+      const double *a = cfg + 3 * i, *b = cfg + 3 * j;
+      double r = orc_norm3(a[0] - b[0], a[1] - b[1], a[2] - b[2]);
+      J += (pair_b(wf, i, j) * r) / (1.0 + wf->p[2] * r);
+    }
+  for (int i = 0; i < N; ++i)
+    for (int I = 0; I < wf->n_nuc; ++I) {
+      const double *a = cfg + 3 * i;
+      double r = orc_norm3(a[0] - wf->nuc[3 * I], a[1] - wf->nuc[3 * I + 1], a[2] - wf->nuc[3 * I + 2]);
+      double aZ = wf->p[3] * (double)wf->charge[I];
+      J += -(aZ * r) / (1.0 + wf->p[4] * r);
+    }
+  return J;
+}
+
+/* J_i(r): sum over j != i of the pair terms with electron i at r, then the
+ * e-n terms; FP64 */
+static double jas_i_d(const orc_sj *wf, const double *cfg, int i, const double *r) {
+  int N = wf->n_up + wf->n_dn;
+  double s = 0.0;
+  for (int j = 0; j < N; ++j) {
+    if (j == i) continue;
+    const double *x = cfg + 3 * j;
+    double rij = orc_norm3(r[0] - x[0], r[1] - x[1], r[2] - x[2]);
+    s = s + (pair_b(wf, i, j) * rij) / (1.0 + wf->p[2] * rij);
+  }
+  for (int I = 0; I < wf->n_nuc; ++I) {
+    double rI = orc_norm3(r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]);
+    double aZ = wf->p[3] * (double)wf->charge[I];
+    s = s + -(aZ * rI) / (1.0 + wf->p[4] * rI);
+  }
+  return s;
+}
+
+/* FP32 version: differences rounded to float, float arithmetic */
+static float jas_i_f(const orc_sj *wf, const double *cfg, int i, const double *r) {
+  int N = wf->n_up + wf->n_dn;
+  const float c = (float)wf->p[2], d = (float)wf->p[4];
+  float s = 0.0f;
+  for (int j = 0; j < N; ++j) {
+    if (j == i) continue;
+    const double *x = cfg + 3 * j;
+    float d0 = (float)(r[0] - x[0]), d1 = (float)(r[1] - x[1]), d2 = (float)(r[2] - x[2]);
+    float q = d0 * d0;
+    q = q + d1 * d1;
+    q = q + d2 * d2;
+    float rij = sqrtf(q);
+    float b = (float)pair_b(wf, i, j);
+    s = s + (b * rij) / (1.0f + c * rij);
+  }
+  for (int I = 0; I < wf->n_nuc; ++I) {
+    float d0 = (float)(r[0] - wf->nuc[3 * I]), d1 = (float)(r[1] - wf->nuc[3 * I + 1]),
+          d2 = (float)(r[2] - wf->nuc[3 * I + 2]);
+    float q = d0 * d0;
+    q = q + d1 * d1;
+    q = q + d2 * d2;
+    float rI = sqrtf(q);
+    float aZ = (float)(wf->p[3] * (double)wf->charge[I]);
+    s = s + -(aZ * rI) / (1.0f + d * rI);
+  }
+  return s;
+}
+
+double orc_sj_value(const orc_sj *wf, const double *cfg) {
+  return (lu_det(wf, cfg, 0) * lu_det(wf, cfg, 1)) * orc_exp(jas_full(wf, cfg));
+}
+
+/* ---- local energy ---------------------------------------------------- */
+
+double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bup,
+                           const double *Bdn, double *dlog) {
+  int N = wf->n_up + wf->n_dn;
+  const double bA = wf->p[0], bP = wf->p[1], c = wf->p[2], a = wf->p[3], d = wf->p[4];
+  double kin = 0.0, V = 0.0;
+  double dl[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
+  for (int i = 0; i < N; ++i) {
+    int s = spin_of(wf, i), cc = col_of(wf, i), n = nspin(wf, s);
+    const double *B = s == 0 ? Bup : Bdn;
+    const double *xi = cfg + 3 * i;
+    double dI[MAXNUC][3], rI[MAXNUC];
+    for (int I = 0; I < wf->n_nuc; ++I) {
+      for (int q = 0; q < 3; ++q) dI[I][q] = xi[q] - wf->nuc[3 * I + q];
+      rI[I] = orc_norm3(dI[I][0], dI[I][1], dI[I][2]);
+    }
+    double chi[MAXB], gch[MAXB][3], lch[MAXB];
+    for (int m = 0; m < wf->n_basis; ++m) {
+      int I = wf->center[m];
+      double z = wf->zeta[m];
+      chi[m] = orc_exp(-z * rI[I]);
+      double gf = -z * chi[m] / rI[I];
+      for (int q = 0; q < 3; ++q) gch[m][q] = gf * dI[I][q];
+      lch[m] = chi[m] * (z * z - 2.0 * z / rI[I]);
+    }
+    double Dg[3] = {0, 0, 0}, Dl = 0.0;
+    for (int k = 0; k < n; ++k) {
+      const double *cf = wf->coef + (size_t)k * wf->n_basis;
+      double g[3] = {0, 0, 0}, l = 0.0;
+      for (int m = 0; m < wf->n_basis; ++m) {
+        for (int q = 0; q < 3; ++q) g[q] = fma(cf[m], gch[m][q], g[q]);
+        l = fma(cf[m], lch[m], l);
+      }
+      double bkc = B[k * n + cc];
+      for (int q = 0; q < 3; ++q) Dg[q] = fma(g[q], bkc, Dg[q]);
+      Dl = fma(l, bkc, Dl);
+    }
+    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0;
+    for (int j = 0; j < N; ++j) {
+      if (j == i) continue;
+      const double *xj = cfg + 3 * j;
+      double dv[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
+      double r = orc_norm3(dv[0], dv[1], dv[2]);
+      double b = (spin_of(wf, j) == s) ? bP : bA;
+      double den = 1.0 + c * r;
+      double up = b / (den * den);
+      double upp = -2.0 * b * c / (den * den * den);
+      for (int q = 0; q < 3; ++q) gJ[q] += up * dv[q] / r;
+      lJ += upp + 2.0 * up / r;
+      if (j > i) {
+        Vi += 1.0 / r;
+        double t = r / den;
+        if (spin_of(wf, j) == s)
+          dl[1] += t;
+        else
+          dl[0] += t;
+        dl[2] += -b * r * r / (den * den);
+      }
+    }
+    for (int I = 0; I < wf->n_nuc; ++I) {
+      double Z = (double)wf->charge[I];
+      double r = rI[I];
+      double den = 1.0 + d * r;
+      double aZ = a * Z;
+      double vp = -aZ / (den * den);
+      double vpp = 2.0 * aZ * d / (den * den * den);
+      for (int q = 0; q < 3; ++q) gJ[q] += vp * dI[I][q] / r;
+      lJ += vpp + 2.0 * vp / r;
+      Vi += -Z / r;
+      dl[3] += -Z * r / den;
+      dl[4] += aZ * r * r / (den * den);
+    }
+    double dot = Dg[0] * gJ[0] + Dg[1] * gJ[1] + Dg[2] * gJ[2];
+    double gg = gJ[0] * gJ[0] + gJ[1] * gJ[1] + gJ[2] * gJ[2];
+    kin += Dl + 2.0 * dot + lJ + gg;
+    V += Vi;
+  }
+  double vnn = 0.0;
+  for (int I = 0; I < wf->n_nuc; ++I)
+    for (int J = I + 1; J < wf->n_nuc; ++J) {
+      double r = orc_norm3(wf->nuc[3 * J] - wf->nuc[3 * I], wf->nuc[3 * J + 1] - wf->nuc[3 * I + 1],
+                           wf->nuc[3 * J + 2] - wf->nuc[3 * I + 2]);
+      vnn += (double)(wf->charge[I] * wf->charge[J]) / r;
+    }
+  if (wf->hamiltonian != ORC_H_ELECTRONIC) {
+    /* IonicHamiltonian: drop the e-e repulsion */
+    V = 0.0;
+    for (int i = 0; i < N; ++i)
+      for (int I = 0; I < wf->n_nuc; ++I) {
+        const double *xi = cfg + 3 * i;
+        V += -(double)wf->charge[I] /
+             orc_norm3(xi[0] - wf->nuc[3 * I], xi[1] - wf->nuc[3 * I + 1], xi[2] - wf->nuc[3 * I + 2]);
+      }
+  }
+  if (dlog) memcpy(dlog, dl, sizeof dl);
+  return -0.5 * kin + V + vnn;
+}
+
+double orc_sj_local_energy_full(const orc_sj *wf, const double *cfg, double *dlog) {
+  double Bu[MAXN * MAXN], Bd[MAXN * MAXN];
+  orc_sj_inverse(wf, cfg, 0, Bu);
+  orc_sj_inverse(wf, cfg, 1, Bd);
+  return orc_sj_local_energy(wf, cfg, Bu, Bd, dlog);
+}
+
+/* ---- runners ---------------------------------------------------------- */
+
+typedef struct {
+  const orc_sj *wf;
+  const orc_sj_run_params *rp;
+  orc_sj_run_out *out;
+  uint32_t w0, w1;
+  int faithful;
+  double sum_e, sum_e2, sum_bm, sum_bm2;
+  uint64_t n_samples, n_blocks, n_accepted, n_moves;
+} sj_task;
+
+static void refresh(const orc_sj *wf, const double *cfg, int prec, double *Bu, double *Bd,
+                    float *Buf, float *Bdf) {
+  orc_sj_inverse(wf, cfg, 0, Bu);
+  orc_sj_inverse(wf, cfg, 1, Bd);
+  if (prec == ORC_PREC_MIXED) {
+    for (int k = 0; k < wf->n_up * wf->n_up; ++k) Buf[k] = (float)Bu[k];
+    for (int k = 0; k < wf->n_dn * wf->n_dn; ++k) Bdf[k] = (float)Bd[k];
+  }
+}
+
+static void sj_walker(sj_task *t, uint32_t w) {
+  const orc_sj *wf = t->wf;
+  const orc_sj_run_params *rp = t->rp;
+  orc_sj_run_out *out = t->out;
+  const int N = wf->n_up + wf->n_dn;
+  const int prec = rp->precision;
+  uint32_t blocks = rp->steps / rp->block_size;
+  uint64_t nsw = (uint64_t)(blocks - 1) * rp->block_size;
+  double cfg[3 * MAXE];
+  uint64_t pos;
+  if (rp->init_from_stream) {
+    for (int k = 0; k < 3 * N; ++k)
+      cfg[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, (uint64_t)k), -1.0, 1.0);
+    pos = 3 * (uint64_t)N;
+  } else {
+    memcpy(cfg, out->cfg + (size_t)w * 3 * N, sizeof(double) * 3 * N);
+    pos = rp->pos0;
+  }
+  double Bu[MAXN * MAXN], Bd[MAXN * MAXN];
+  float Buf[MAXN * MAXN], Bdf[MAXN * MAXN];
+  if (!t->faithful) refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
+  uint64_t m = rp->move0;
+  const double half = 0.5 * rp->box_side;
+  uint64_t sidx = 0;
+  double chi[MAXB];
+  for (uint32_t block = 0; block < blocks; ++block) {
+    double bsum = 0.0;
+    for (uint32_t s = 0; s < rp->block_size; ++s) {
+      uint64_t mask = 0;
+      for (int i = 0; i < N; ++i) {
+        double dd[3];
+        for (int k = 0; k < 3; ++k)
+          dd[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++), -half, half);
+        double u = orc_u64_standard(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++));
+        double xn[3];
+        for (int k = 0; k < 3; ++k) xn[k] = cfg[3 * i + k] + dd[k];
+        int acc;
+        t->n_moves++;
+        if (t->faithful) {
+          double prop[3 * MAXE];
+          memcpy(prop, cfg, sizeof(double) * 3 * N);
+          memcpy(prop + 3 * i, xn, sizeof xn);
+          double pn = orc_sj_value(wf, prop);
+          double po = orc_sj_value(wf, cfg);
+          acc = rust_min1((pn * pn) / (po * po)) > u;
+        } else {
+          int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
+          basis_at(wf, xn, chi);
+          if (prec == ORC_PREC_FP64) {
+            double *B = sp == 0 ? Bu : Bd;
+            double phin[MAXN];
+            for (int k = 0; k < n; ++k) phin[k] = orbital(wf, k, chi);
+            double rd = 0.0;
+            for (int k = 0; k < n; ++k) rd = fma(phin[k], B[k * n + c], rd);
+            double Jn = jas_i_d(wf, cfg, i, xn);
+            double Jo = jas_i_d(wf, cfg, i, cfg + 3 * i);
+            double dJ = Jn - Jo;
+            double R2 = (rd * rd) * orc_exp(2.0 * dJ);
+            acc = rust_min1(R2) > u;
+            if (acc) {
+              double inv = 1.0 / rd;
+              double f[MAXN];
+              for (int l = 0; l < n; ++l) {
+                if (l == c) continue;
+                double v = 0.0;
+                for (int k = 0; k < n; ++k) v = fma(phin[k], B[k * n + l], v);
+                f[l] = v * inv;
+              }
+              for (int k = 0; k < n; ++k)
+                for (int l = 0; l < n; ++l)
+                  if (l != c) B[k * n + l] = fma(-B[k * n + c], f[l], B[k * n + l]);
+              for (int k = 0; k < n; ++k) B[k * n + c] = B[k * n + c] * inv;
+            }
+          } else {
+            float *B = sp == 0 ? Buf : Bdf;
+            float phin[MAXN];
+            for (int k = 0; k < n; ++k) phin[k] = (float)orbital(wf, k, chi);
+            float rd = 0.0f;
+            for (int k = 0; k < n; ++k) rd = fmaf(phin[k], B[k * n + c], rd);
+            float Jn = jas_i_f(wf, cfg, i, xn);
+            float Jo = jas_i_f(wf, cfg, i, cfg + 3 * i);
+            float dJ = Jn - Jo;
+            double rdd = (double)rd;
+            double R2 = (rdd * rdd) * orc_exp(2.0 * (double)dJ);
+            acc = rust_min1(R2) > u;
+            if (acc) {
+              float inv = 1.0f / rd;
+              float f[MAXN];
+              for (int l = 0; l < n; ++l) {
+                if (l == c) continue;
+                float v = 0.0f;
+                for (int k = 0; k < n; ++k) v = fmaf(phin[k], B[k * n + l], v);
+                f[l] = v * inv;
+              }
+              for (int k = 0; k < n; ++k)
+                for (int l = 0; l < n; ++l)
+                  if (l != c) B[k * n + l] = fmaf(-B[k * n + c], f[l], B[k * n + l]);
+              for (int k = 0; k < n; ++k) B[k * n + c] = B[k * n + c] * inv;
+            }
+          }
+        }
+        if (acc) {
+          memcpy(cfg + 3 * i, xn, sizeof xn);
+          mask |= (uint64_t)1 << i;
+          t->n_accepted++;
+        }
+        ++m;
+        if (!t->faithful && rp->lu_period && m % rp->lu_period == 0)
+          refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
+      }
+      if (out->accept_mask)
+        out->accept_mask[(size_t)w * blocks * rp->block_size + (size_t)block * rp->block_size + s] = mask;
+      if (block > 0) {
+        double dl[ORC_SJ_NPARAM];
+        double el;
+        if (t->faithful) {
+          el = orc_sj_local_energy_full(wf, cfg, dl);
+        } else if (prec == ORC_PREC_FP64) {
+          el = orc_sj_local_energy(wf, cfg, Bu, Bd, dl);
+        } else {
+          double bu[MAXN * MAXN], bd[MAXN * MAXN];
+          for (int k = 0; k < wf->n_up * wf->n_up; ++k) bu[k] = (double)Buf[k];
+          for (int k = 0; k < wf->n_dn * wf->n_dn; ++k) bd[k] = (double)Bdf[k];
+          el = orc_sj_local_energy(wf, cfg, bu, bd, dl);
+        }
+        size_t si = (size_t)w * nsw + sidx;
+        if (out->e_local) out->e_local[si] = el;
+        if (out->dlog) memcpy(out->dlog + si * ORC_SJ_NPARAM, dl, sizeof dl);
+        t->sum_e += el;
+        t->sum_e2 += el * el;
+        bsum += el;
+        t->n_samples++;
+        sidx++;
+      }
+    }
+    if (block > 0) {
+      double bm = bsum / (double)rp->block_size;
+      t->sum_bm += bm;
+      t->sum_bm2 += bm * bm;
+      t->n_blocks++;
+    }
+  }
+  if (out->cfg) memcpy(out->cfg + (size_t)w * 3 * N, cfg, sizeof(double) * 3 * N);
+}
+
+static void *sj_task_main(void *arg) {
+  sj_task *t = (sj_task *)arg;
+  for (uint32_t w = t->w0; w < t->w1; ++w) sj_walker(t, w);
+  return NULL;
+}
+
+static int sj_run(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
+                  int n_threads, orc_sj_run_out *out, int faithful) {
+  int N = wf->n_up + wf->n_dn;
+  if (rp->block_size == 0 || rp->steps < 2 * rp->block_size) return -1;
+  if (N > MAXE || wf->n_up > MAXN || wf->n_dn > MAXN || wf->n_nuc > MAXNUC || wf->n_basis > MAXB)
+    return -1;
+  if (out->accept_mask && N > 64) return -1;
+  if (!rp->init_from_stream && !out->cfg) return -1;
+  if (n_threads < 1) n_threads = 1;
+  uint32_t nw = w1 - w0;
+  if ((uint32_t)n_threads > nw) n_threads = nw ? (int)nw : 1;
+  sj_task *tasks = (sj_task *)calloc((size_t)n_threads, sizeof(sj_task));
+  pthread_t *th = (pthread_t *)calloc((size_t)n_threads, sizeof(pthread_t));
+  for (int i = 0; i < n_threads; ++i) {
+    tasks[i].wf = wf;
+    tasks[i].rp = rp;
+    tasks[i].out = out;
+    tasks[i].faithful = faithful;
+    tasks[i].w0 = w0 + (uint32_t)((uint64_t)nw * i / n_threads);
+    tasks[i].w1 = w0 + (uint32_t)((uint64_t)nw * (i + 1) / n_threads);
+  }
+  if (n_threads == 1)
+    sj_task_main(&tasks[0]);
+  else {
+    for (int i = 0; i < n_threads; ++i) pthread_create(&th[i], NULL, sj_task_main, &tasks[i]);
+    for (int i = 0; i < n_threads; ++i) pthread_join(th[i], NULL);
+  }
+  for (int i = 0; i < n_threads; ++i) {
+    out->sum_e += tasks[i].sum_e;
+    out->sum_e2 += tasks[i].sum_e2;
+    out->sum_bm += tasks[i].sum_bm;
+    out->sum_bm2 += tasks[i].sum_bm2;
+    out->n_samples += tasks[i].n_samples;
+    out->n_blocks += tasks[i].n_blocks;
+    out->n_accepted += tasks[i].n_accepted;
+    out->n_moves += tasks[i].n_moves;
+  }
+  free(tasks);
+  free(th);
+  return 0;
+}
+
+int orc_sj_run_parity(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
+                      int n_threads, orc_sj_run_out *out) {
+  return sj_run(wf, rp, w0, w1, n_threads, out, 0);
+}
+
+int orc_sj_run_faithful(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
+                        int n_threads, orc_sj_run_out *out) {
+  return sj_run(wf, rp, w0, w1, n_threads, out, 1);
+}

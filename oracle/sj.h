@@ -1,0 +1,99 @@
+/*
+ * CPU oracle (test infrastructure only): Slater-Jastrow wavefunctions.
+ *
+ * The reference ships no determinant wavefunction (README.md:31-33; the
+ * removed SpinDeterminantProduct survives only as commented calls at
+ * tests/helium_lcao.rs:94-101).  This restates the mathematics the device
+ * path implements:
+ *
+ *   psi = det A_up * det A_dn * exp(J)
+ *   A_s[c][k] = phi_k(r of the c-th electron of spin s),
+ *   phi_k(r)  = sum_m coef[k][m] exp(-zeta_m |r - R_center(m)|)
+ *   J = sum_{i<j} b_ij r_ij/(1 + c r_ij) - sum_{i,I} a Z_I r_iI/(1 + d r_iI)
+ *   b_ij = b_anti (opposite spins) or b_para (same spin)
+ *
+ * (theory/jastrow.tex:17-30 gives the e-e Pade factor; the e-n factor is
+ * the same form.)  Electrons 0..n_up-1 are spin up, the rest spin down.
+ *
+ * Two runners: orc_sj_run_faithful evaluates psi from scratch twice per
+ * move (the reference's accept_move, src/metropolis/src/metrop.rs:73-82)
+ * and is the CPU baseline; orc_sj_run_parity performs the
+ * Sherman-Morrison algorithm the device runs, in the same operation order
+ * and precision, so accept decisions can be compared bit for bit.
+ */
+#ifndef MOLE_ORACLE_SJ_H
+#define MOLE_ORACLE_SJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ORC_SJ_NPARAM 5 /* b_anti, b_para, c, a, d */
+
+typedef struct {
+  int n_up, n_dn, n_nuc, n_basis, n_orb, hamiltonian;
+  const double *nuc;    /* [n_nuc][3] */
+  const int *charge;    /* [n_nuc] */
+  const double *zeta;   /* [n_basis] */
+  const int *center;    /* [n_basis] */
+  const double *coef;   /* [n_orb][n_basis] */
+  double p[ORC_SJ_NPARAM];
+} orc_sj;
+
+enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };
+
+/* Function::value by LU determinants (partial pivoting), FP64. */
+double orc_sj_value(const orc_sj *wf, const double *cfg);
+/* log|det A_s| and sign, by LU, for tests. */
+double orc_sj_logdet(const orc_sj *wf, const double *cfg, int spin, double *sign);
+/* Determinant of A_s by the Leibniz expansion in long double (n <= 6). */
+long double orc_sj_det_leibniz(const orc_sj *wf, const double *cfg, int spin);
+/* Inverse B_s[k][c] = (A_s^-1)[k][c] by Gauss-Jordan with partial
+ * pivoting, FP64, exactly as the device refreshes it.  Returns 0, -1 if
+ * singular. */
+int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B);
+/* Local energy -1/2 lap(psi)/psi + V and d ln psi / dp of the five
+ * Jastrow parameters, FP64, from the inverses B_up, B_dn (as doubles). */
+double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bup,
+                           const double *Bdn, double *dlog);
+/* The same from scratch (inverses by orc_sj_inverse). */
+double orc_sj_local_energy_full(const orc_sj *wf, const double *cfg, double *dlog);
+
+typedef struct {
+  uint64_t seed;
+  uint32_t sys, epoch;
+  double box_side;
+  uint32_t steps, block_size;
+  int init_from_stream; /* 1: U(-1,1) configurations from draws 0..3N-1 */
+  uint64_t pos0;        /* first draw of the moves when !init_from_stream */
+  uint64_t move0;       /* global move counter at start (LU refresh phase) */
+  int precision;        /* ORC_PREC_* (parity runner only) */
+  uint32_t lu_period;   /* refresh both inverses after every lu_period moves */
+} orc_sj_run_params;
+
+typedef struct {
+  double *cfg;           /* in/out [W][3N] */
+  uint64_t *accept_mask; /* [W][blocks*block_size] (N <= 64) */
+  double *e_local;       /* [W][samples] */
+  double *dlog;          /* [W][samples][5] */
+  double sum_e, sum_e2, sum_bm, sum_bm2;
+  uint64_t n_samples, n_blocks, n_accepted, n_moves;
+} orc_sj_run_out;
+
+/* Parity runner: walkers [w0, w1), each its own Runner::run
+ * (src/montecarlo/src/montecarlo.rs:24-46), Sherman-Morrison ratios and
+ * updates, LU refresh every lu_period moves.  Returns 0 or -1. */
+int orc_sj_run_parity(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
+                      int n_threads, orc_sj_run_out *out);
+/* Reference-faithful runner: two full LU evaluations per move and the
+ * local energy from fresh inverses after every sampled sweep. */
+int orc_sj_run_faithful(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
+                        int n_threads, orc_sj_run_out *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

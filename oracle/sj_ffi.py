@@ -1,0 +1,142 @@
+"""ctypes view of the oracle's Slater-Jastrow functions (oracle/sj.h).
+
+Test infrastructure only (tests/, smoke, bench cpu_baseline)."""
+import ctypes as C
+
+import numpy as np
+
+from . import oracle_ffi as OF
+
+PREC_FP64 = 0
+PREC_MIXED = 1
+NPARAM = 5
+
+_pd = C.POINTER(C.c_double)
+_pi = C.POINTER(C.c_int)
+
+
+class Sj(C.Structure):
+    _fields_ = [
+        ("n_up", C.c_int), ("n_dn", C.c_int), ("n_nuc", C.c_int), ("n_basis", C.c_int),
+        ("n_orb", C.c_int), ("hamiltonian", C.c_int),
+        ("nuc", _pd), ("charge", _pi), ("zeta", _pd), ("center", _pi), ("coef", _pd),
+        ("p", C.c_double * NPARAM),
+    ]
+
+
+class SjRunParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("sys", C.c_uint32), ("epoch", C.c_uint32), ("box_side", C.c_double),
+        ("steps", C.c_uint32), ("block_size", C.c_uint32), ("init_from_stream", C.c_int),
+        ("pos0", C.c_uint64), ("move0", C.c_uint64), ("precision", C.c_int),
+        ("lu_period", C.c_uint32),
+    ]
+
+
+class SjRunOut(C.Structure):
+    _fields_ = [
+        ("cfg", _pd), ("accept_mask", C.POINTER(C.c_uint64)), ("e_local", _pd), ("dlog", _pd),
+        ("sum_e", C.c_double), ("sum_e2", C.c_double), ("sum_bm", C.c_double), ("sum_bm2", C.c_double),
+        ("n_samples", C.c_uint64), ("n_blocks", C.c_uint64), ("n_accepted", C.c_uint64),
+        ("n_moves", C.c_uint64),
+    ]
+
+
+_bound = False
+
+
+def lib():
+    global _bound
+    L = OF.lib()
+    if not _bound:
+        ps = C.POINTER(Sj)
+        L.orc_sj_value.restype = C.c_double
+        L.orc_sj_value.argtypes = [ps, _pd]
+        L.orc_sj_logdet.restype = C.c_double
+        L.orc_sj_logdet.argtypes = [ps, _pd, C.c_int, _pd]
+        L.orc_sj_det_leibniz.restype = C.c_longdouble
+        L.orc_sj_det_leibniz.argtypes = [ps, _pd, C.c_int]
+        L.orc_sj_inverse.restype = C.c_int
+        L.orc_sj_inverse.argtypes = [ps, _pd, C.c_int, _pd]
+        L.orc_sj_local_energy.restype = C.c_double
+        L.orc_sj_local_energy.argtypes = [ps, _pd, _pd, _pd, _pd]
+        L.orc_sj_local_energy_full.restype = C.c_double
+        L.orc_sj_local_energy_full.argtypes = [ps, _pd, _pd]
+        for f in (L.orc_sj_run_parity, L.orc_sj_run_faithful):
+            f.restype = C.c_int
+            f.argtypes = [ps, C.POINTER(SjRunParams), C.c_uint32, C.c_uint32, C.c_int,
+                          C.POINTER(SjRunOut)]
+        _bound = True
+    return L
+
+
+class SjWave:
+    """Keeps the arrays of a systems.SJSystem alive behind an orc_sj."""
+
+    def __init__(self, system, hamiltonian=0, params=None):
+        s = system
+        self._keep = [s.nuclei, s.charges.astype(np.intc), s.zeta, s.center.astype(np.intc), s.coef]
+        nuc, ch, ze, ce, co = self._keep
+        self.st = Sj(n_up=s.n_up, n_dn=s.n_dn, n_nuc=nuc.shape[0], n_basis=ze.size, n_orb=co.shape[0],
+                     hamiltonian=hamiltonian, nuc=nuc.ctypes.data_as(_pd), charge=ch.ctypes.data_as(_pi),
+                     zeta=ze.ctypes.data_as(_pd), center=ce.ctypes.data_as(_pi),
+                     coef=co.ctypes.data_as(_pd))
+        p = s.jastrow if params is None else params
+        for k in range(NPARAM):
+            self.st.p[k] = float(p[k])
+        self.n_up, self.n_dn = s.n_up, s.n_dn
+        self.N = s.n_up + s.n_dn
+
+    def ref(self):
+        return C.byref(self.st)
+
+
+def value(wv, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    return lib().orc_sj_value(wv.ref(), OF.dptr(x))
+
+
+def inverse(wv, x, spin):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = wv.n_up if spin == 0 else wv.n_dn
+    B = np.zeros(max(1, n * n))
+    rc = lib().orc_sj_inverse(wv.ref(), OF.dptr(x), spin, OF.dptr(B))
+    return B[:n * n].reshape(n, n), rc
+
+
+def local_energy(wv, x):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    dl = np.zeros(NPARAM)
+    e = lib().orc_sj_local_energy_full(wv.ref(), OF.dptr(x), OF.dptr(dl))
+    return e, dl
+
+
+def run(wv, *, seed, walkers, steps, block_size, box_side, faithful=False, precision=PREC_FP64,
+        lu_period=0, sys=0, epoch=0, cfg=None, pos0=0, move0=0, n_threads=1, record=True):
+    L = lib()
+    N = wv.N
+    rp = SjRunParams(seed=seed, sys=sys, epoch=epoch, box_side=box_side, steps=steps,
+                     block_size=block_size, init_from_stream=0 if cfg is not None else 1, pos0=pos0,
+                     move0=move0, precision=precision, lu_period=lu_period)
+    blocks = steps // block_size
+    ns = (blocks - 1) * block_size
+    out = SjRunOut()
+    c = np.zeros((walkers, 3 * N)) if cfg is None else np.ascontiguousarray(cfg, dtype=np.float64).copy()
+    out.cfg = OF.dptr(c)
+    res = {}
+    if record:
+        masks = np.zeros((walkers, blocks * block_size), dtype=np.uint64)
+        el = np.zeros((walkers, ns))
+        dl = np.zeros((walkers, ns, NPARAM))
+        out.accept_mask = OF.u64ptr(masks)
+        out.e_local = OF.dptr(el)
+        out.dlog = OF.dptr(dl)
+        res.update(accept_mask=masks, e_local=el, dlog=dl)
+    f = L.orc_sj_run_faithful if faithful else L.orc_sj_run_parity
+    rc = f(wv.ref(), C.byref(rp), 0, walkers, n_threads, C.byref(out))
+    if rc != 0:
+        raise ValueError("orc_sj_run failed (steps < 2*block_size or sizes too large)")
+    res.update(cfg=c, sum_e=out.sum_e, sum_e2=out.sum_e2, sum_bm=out.sum_bm, sum_bm2=out.sum_bm2,
+               n_samples=out.n_samples, n_blocks=out.n_blocks, n_accepted=out.n_accepted,
+               n_moves=out.n_moves)
+    return res
