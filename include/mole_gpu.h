@@ -43,6 +43,19 @@ typedef int32_t mole_status;
 /* psi = phi(r1) phi(r2) exp(b r12/(1 + c r12)),
  * phi(r) = sum_I exp(-zeta |r - R_I|), params (zeta, b, c). */
 #define MOLE_WF_LCAO2_PADE 2
+/* psi = det A_up det A_dn exp(J), A_s[c][k] = phi_k(r of the c-th electron
+ * of spin s), phi_k(r) = sum_m coef[k][m] exp(-zeta_m |r - R_center(m)|),
+ * J = sum_{i<j} b_ij r_ij/(1 + c r_ij) - sum_{i,I} a Z_I r_iI/(1 + d r_iI),
+ * b_ij = b_anti (opposite spins) or b_para (same spin); params
+ * (b_anti, b_para, c, a, d).  Electrons 0..n_up-1 are spin up.  A context
+ * holds either two-electron kinds or Slater-Jastrow systems, not both. */
+#define MOLE_WF_SLATER_JASTROW 3
+
+/* Slater-Jastrow precision: FP64 throughout, or MIXED = Sherman-Morrison
+ * inverses, ratios and Jastrow differences in FP32 with FP64 positions,
+ * FP64 Gauss-Jordan re-inversion, FP64 local energy and sums. */
+#define MOLE_PREC_FP64 0
+#define MOLE_PREC_MIXED 1
 
 #define MOLE_H_ELECTRONIC 0 /* ElectronicHamiltonian, src/operator/src/operator.rs:157-183 */
 #define MOLE_H_IONIC 1      /* IonicHamiltonian, operator.rs:131-149 */
@@ -84,6 +97,12 @@ typedef struct {
   uint32_t n_params;
   const double *params; /* Optimize::parameters, src/optimize/src/traits.rs:13 */
   uint32_t n_walkers;
+  /* MOLE_WF_SLATER_JASTROW only (ignored otherwise) */
+  uint32_t n_basis;
+  const double *basis_exponent;  /* [n_basis] zeta_m */
+  const uint32_t *basis_center;  /* [n_basis] nucleus index */
+  uint32_t n_orbitals;           /* >= max(n_up, n_down) */
+  const double *orbital_coeff;   /* [n_orbitals][n_basis] */
 } mole_system_desc;
 
 /* How the walkers move and what they record. */
@@ -95,6 +114,9 @@ typedef struct {
   uint32_t o_convention; /* MOLE_O_* */
   uint32_t flags;        /* MOLE_FLAG_* */
   uint32_t record_sweeps;/* capacity, in sweeps, of trace/sample buffers */
+  uint32_t precision;    /* MOLE_PREC_* (Slater-Jastrow) */
+  uint32_t lu_period;    /* re-invert the Slater matrices after every
+                            lu_period moves; 0 = never (Slater-Jastrow) */
 } mole_run_desc;
 
 /* Energy statistics since the last reset, summed over the system's
@@ -108,6 +130,10 @@ typedef struct {
   uint64_t n_accepted;
   double sum_e, sum_e2;
   double sum_block_mean, sum_block_mean2;
+  /* Slater-Jastrow: largest ||A||_1 ||A^-1||_1 met at a re-inversion since
+   * the last reset (0 otherwise) -- the condition estimate that says
+   * whether FP32 inverses are trustworthy. */
+  double max_condition;
 } mole_stats;
 
 /* Sums for the energy gradient and the SR matrix
@@ -237,6 +263,11 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
 #define MOLE_VARIANT_LANE 1
 #define MOLE_VARIANT_WAVESPEC 2
 mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t block_threads);
+/* Slater-Jastrow: lanes of a wave that share one walker of system sys
+ * (power of two, 1..64; 0 = the library's choice).  Results do not depend
+ * on it; it only trades per-move parallelism against walkers per wave. */
+mole_status mole_set_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t lanes);
+mole_status mole_get_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t *lanes);
 
 /* Waits for all work of the context. */
 mole_status mole_synchronize(mole_ctx *ctx);

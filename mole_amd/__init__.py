@@ -27,6 +27,9 @@ ERR_OUT_OF_MEMORY = 19
 
 WF_HELIUM_PADE = 1
 WF_LCAO2_PADE = 2
+WF_SLATER_JASTROW = 3
+PREC_FP64 = 0
+PREC_MIXED = 1
 H_ELECTRONIC = 0
 H_IONIC = 1
 PROPOSAL_BOX = 0
@@ -71,6 +74,11 @@ class SystemDesc(C.Structure):
         ("n_params", C.c_uint32),
         ("params", C.POINTER(C.c_double)),
         ("n_walkers", C.c_uint32),
+        ("n_basis", C.c_uint32),
+        ("basis_exponent", C.POINTER(C.c_double)),
+        ("basis_center", C.POINTER(C.c_uint32)),
+        ("n_orbitals", C.c_uint32),
+        ("orbital_coeff", C.POINTER(C.c_double)),
     ]
 
 
@@ -83,6 +91,8 @@ class RunDesc(C.Structure):
         ("o_convention", C.c_uint32),
         ("flags", C.c_uint32),
         ("record_sweeps", C.c_uint32),
+        ("precision", C.c_uint32),
+        ("lu_period", C.c_uint32),
     ]
 
 
@@ -96,6 +106,7 @@ class Stats(C.Structure):
         ("sum_e2", C.c_double),
         ("sum_block_mean", C.c_double),
         ("sum_block_mean2", C.c_double),
+        ("max_condition", C.c_double),
     ]
 
 
@@ -139,6 +150,8 @@ _SIGS = [
     ("mole_read_samples", C.c_int32, [_P, C.c_uint32, _pd, _pd, _pd, C.c_size_t, C.c_size_t]),
     ("mole_eval", C.c_int32, [_P, C.c_uint32, C.c_size_t, _pd, _pd, _pd, _pd, _pd, _pd]),
     ("mole_set_launch_config", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    ("mole_set_walker_lanes", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
+    ("mole_get_walker_lanes", C.c_int32, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("mole_synchronize", C.c_int32, [_P]),
     ("mole_last_timing", C.c_int32, [_P, _pd, C.POINTER(C.c_uint32)]),
     ("mole_compute_parameter_update", C.c_int32, [C.POINTER(SrSums), C.c_uint32, C.c_double, C.c_uint32, C.c_double, _pd, _pd]),
@@ -202,6 +215,26 @@ class System:
         self.n_electrons = n_up + n_down
         self.n_walkers = n_walkers
 
+    def with_basis(self, zeta, center, coef):
+        """Attach the s-type STO basis and orbital coefficients (Slater-Jastrow)."""
+        self.zeta = np.ascontiguousarray(zeta, dtype=np.float64)
+        self.center = np.ascontiguousarray(center, dtype=np.uint32)
+        self.coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1, self.zeta.size)
+        self.desc.n_basis = self.zeta.size
+        self.desc.basis_exponent = self.zeta.ctypes.data_as(_pd)
+        self.desc.basis_center = self.center.ctypes.data_as(C.POINTER(C.c_uint32))
+        self.desc.n_orbitals = self.coef.shape[0]
+        self.desc.orbital_coeff = self.coef.ctypes.data_as(_pd)
+        return self
+
+
+def slater_jastrow(sj, n_walkers, hamiltonian=H_ELECTRONIC, params=None):
+    """Device system from a mole_amd.systems.SJSystem."""
+    p = sj.jastrow if params is None else params
+    s = System(WF_SLATER_JASTROW, sj.nuclei, sj.charges, p, n_walkers, n_up=sj.n_up,
+               n_down=sj.n_dn, hamiltonian=hamiltonian)
+    return s.with_basis(sj.zeta, sj.center, sj.coef)
+
 
 def helium(n_walkers, zeta=1.6875, b=0.5, c=0.3, hamiltonian=H_ELECTRONIC):
     return System(WF_HELIUM_PADE, [(0.0, 0.0, 0.0)], [2], [zeta, b, c], n_walkers,
@@ -217,13 +250,15 @@ class Context:
     """One mole_ctx: a batch of walkers of one or more systems on the device."""
 
     def __init__(self, systems, *, seed=0, step=0.5, block_size=100, o_convention=O_LOGDERIV,
-                 flags=0, record_sweeps=0, proposal=PROPOSAL_BOX):
+                 flags=0, record_sweeps=0, proposal=PROPOSAL_BOX, precision=PREC_FP64,
+                 lu_period=0):
         L = lib()
         self.systems = list(systems)
         arr = (SystemDesc * len(self.systems))(*[s.desc for s in self.systems])
         self.run_desc = RunDesc(seed=seed, proposal=proposal, step=step, block_size=block_size,
                                 o_convention=o_convention, flags=flags,
-                                record_sweeps=record_sweeps)
+                                record_sweeps=record_sweeps, precision=precision,
+                                lu_period=lu_period)
         h = C.c_void_p()
         check(L.mole_ctx_create(arr, len(self.systems), C.byref(self.run_desc), C.byref(h)))
         self.h = h
@@ -337,6 +372,14 @@ class Context:
 
     def set_launch_config(self, variant=VARIANT_AUTO, block_threads=0):
         check(lib().mole_set_launch_config(self.h, variant, block_threads))
+
+    def set_walker_lanes(self, lanes, sys=0):
+        check(lib().mole_set_walker_lanes(self.h, sys, lanes))
+
+    def walker_lanes(self, sys=0):
+        v = C.c_uint32()
+        check(lib().mole_get_walker_lanes(self.h, sys, C.byref(v)))
+        return v.value
 
     def synchronize(self):
         check(lib().mole_synchronize(self.h))

@@ -1,11 +1,13 @@
 // Host side of the C ABI declared in include/mole_gpu.h.
 //
 // A context holds, per system, the walker positions as [3N][W] doubles,
-// the per-walker accumulators as [column][W] and the optional trace and
-// sample buffers, plus the state every walker shares: the Philox epoch,
-// the next draw index and the fill of the block in progress.  Arguments
-// are validated before any HIP call, so invalid calls fail the same way
-// with or without a device.
+// the per-walker accumulators as [column][W], optional trace and sample
+// buffers and, for Slater-Jastrow systems, the inverse Slater matrices as
+// [n_up^2 + n_dn^2][W] plus the system's basis/orbital tables.  The state
+// every walker of a system shares -- Philox epoch, stream position, fill of
+// the block in progress -- stays on the host and goes to the kernels as
+// arguments.  Arguments are validated before any HIP call, so invalid calls
+// fail the same way with or without a device.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -13,11 +15,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <numeric>
 #include <string>
 #include <vector>
 
 #include "../../include/mole_gpu.h"
 #include "internal.hpp"
+#include "sj_internal.hpp"
 
 using namespace mole;
 
@@ -40,51 +44,87 @@ mole_status ok() {
   return MOLE_OK;
 }
 
-#define HIP_TRY(expr)                                                              \
-  do {                                                                             \
-    hipError_t e_ = (expr);                                                        \
-    if (e_ != hipSuccess)                                                          \
-      return fail(e_ == hipErrorOutOfMemory ? MOLE_ERR_OUT_OF_MEMORY : MOLE_ERR_DEVICE, \
-                  "%s: %s", #expr, hipGetErrorString(e_));                         \
+#define HIP_TRY(expr)                                                                         \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail(e_ == hipErrorOutOfMemory ? MOLE_ERR_OUT_OF_MEMORY : MOLE_ERR_DEVICE, "%s: %s", \
+                  #expr, hipGetErrorString(e_));                                              \
   } while (0)
 
+enum Family { kFamTwo = 0, kFamSJ = 1 };
+
 struct SysState {
-  Sys2 sp{};
-  uint32_t kind = 0, n_up = 0, n_down = 0, N = 0, n_params = 0;
+  uint32_t kind = 0, hamiltonian = 0, n_up = 0, n_down = 0, N = 0, n_params = 0;
   int W = 0;
   double *d_pos = nullptr;
   double *d_acc = nullptr;
+  int n_cols = 0;
   uint64_t *d_trace_mask = nullptr;
   double *d_trace_e = nullptr;
   double *d_samp_psi = nullptr;
   double *d_samp_g = nullptr;
   uint64_t n_samples = 0, n_blocks = 0, n_moves = 0;
-  // column sums of d_acc, valid until the next launch that writes d_acc
-  double cols[kAccColumns2] = {};
+  std::vector<double> cols;  // column reductions of d_acc
   bool cols_valid = false;
+  // two-electron kinds
+  Sys2 sp{};
+  // Slater-Jastrow
+  std::vector<double> params;
+  int n_nuc = 0, n_basis = 0, n_orb = 0, L = 0;
+  size_t lds_walker = 0;
+  double *d_tab = nullptr;  // nuclei xyz, charges, zeta, coefficients
+  int *d_center = nullptr;
+  void *d_B = nullptr;
+  double v_nn = 0.0;
+  uint64_t draw_base = 0, move_base = 0;
 };
 
 }  // namespace
 
 struct mole_ctx {
+  int family = kFamTwo;
   int device = 0;
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   mole_run_desc run{};
   std::vector<SysState> sys;
   uint32_t epoch = 0;
-  uint64_t position = 0;
+  uint64_t position = 0;     // two-electron family: next draw index
+  uint64_t sweeps_done = 0;  // Slater-Jastrow family: sweeps since the stream origin
   uint32_t block_fill = 0;
   uint32_t trace_sweeps = 0, trace_samples = 0;
   double *d_red = nullptr;
   uint32_t last_launches = 0;
   bool timed = false;
-  int variant = 0;        // MOLE_VARIANT_*
-  int block_threads = 64; // one-lane-per-walker kernel
+  int variant = 0;
+  int block_threads = 64;
   uint32_t max_sweeps_per_launch = 4096;
+  // Slater-Jastrow launch state
+  SJSysDev *d_sysdev = nullptr;
+  SJTile *d_tiles = nullptr;
+  int n_tiles = 0;
+  size_t lds_bytes = 0, lds_limit = 65536;
+  bool sysdev_dirty = true;
 };
 
 namespace {
+
+int family_of(uint32_t kind) { return kind == MOLE_WF_SLATER_JASTROW ? kFamSJ : kFamTwo; }
+
+double nuclear_repulsion(const double *xyz, const int32_t *Z, uint32_t n) {
+  // IonicPotential::new (operator.rs:40-55)
+  double rep = 0.0;
+  for (uint32_t i = 0; i < n; ++i)
+    for (uint32_t j = i + 1; j < n; ++j) {
+      double dx = xyz[3 * j] - xyz[3 * i], dy = xyz[3 * j + 1] - xyz[3 * i + 1], dz = xyz[3 * j + 2] - xyz[3 * i + 2];
+      double s2 = dx * dx;
+      s2 = s2 + dy * dy;
+      s2 = s2 + dz * dz;
+      rep += (double)(Z[i] * Z[j]) / std::sqrt(s2);
+    }
+  return rep;
+}
 
 mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run_desc *run) {
   if (!sys || !run) return fail(MOLE_ERR_INVALID_ARGUMENT, "null descriptor");
@@ -99,33 +139,56 @@ mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run
     return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown O convention %u", run->o_convention);
   if ((run->flags & (MOLE_FLAG_TRACE | MOLE_FLAG_SAMPLES)) && run->record_sweeps == 0)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "record_sweeps must be > 0 with trace/sample flags");
+  if (run->precision > MOLE_PREC_MIXED)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown precision %u", run->precision);
+  const int fam = family_of(sys[0].wf_kind);
   for (uint32_t s = 0; s < n_sys; ++s) {
     const mole_system_desc &d = sys[s];
-    if (d.wf_kind != MOLE_WF_HELIUM_PADE && d.wf_kind != MOLE_WF_LCAO2_PADE)
+    if (d.wf_kind != MOLE_WF_HELIUM_PADE && d.wf_kind != MOLE_WF_LCAO2_PADE &&
+        d.wf_kind != MOLE_WF_SLATER_JASTROW)
       return fail(MOLE_ERR_UNSUPPORTED, "system %u: wavefunction kind %u not supported", s, d.wf_kind);
+    if (family_of(d.wf_kind) != fam)
+      return fail(MOLE_ERR_UNSUPPORTED, "system %u: a context holds either two-electron kinds or "
+                  "Slater-Jastrow systems", s);
     if (d.hamiltonian > MOLE_H_IONIC)
       return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: unknown hamiltonian %u", s, d.hamiltonian);
-    if (d.n_up != 1 || d.n_down != 1)
-      return fail(MOLE_ERR_SHAPE, "system %u: two-electron kinds need n_up = n_down = 1", s);
-    if (d.n_nuclei < 1 || d.n_nuclei > (uint32_t)kMaxNuc2)
-      return fail(MOLE_ERR_SHAPE, "system %u: need 1..%d nuclei", s, kMaxNuc2);
-    if (!d.nuclei_xyz || !d.nuclei_charge)
-      return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: null nuclei", s);
-    if (d.n_params != (uint32_t)kNParam2 || !d.params)
-      return fail(MOLE_ERR_SHAPE, "system %u: expected %d parameters (zeta, b, c)", s, kNParam2);
+    if (!d.nuclei_xyz || !d.nuclei_charge || !d.params)
+      return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: null nuclei or parameters", s);
     if (d.n_walkers == 0 || d.n_walkers > (1u << 26))
       return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: n_walkers must be in 1..2^26", s);
+    if (fam == kFamTwo) {
+      if (d.n_up != 1 || d.n_down != 1)
+        return fail(MOLE_ERR_SHAPE, "system %u: two-electron kinds need n_up = n_down = 1", s);
+      if (d.n_nuclei < 1 || d.n_nuclei > (uint32_t)kMaxNuc2)
+        return fail(MOLE_ERR_SHAPE, "system %u: need 1..%d nuclei", s, kMaxNuc2);
+      if (d.n_params != (uint32_t)kNParam2)
+        return fail(MOLE_ERR_SHAPE, "system %u: expected %d parameters (zeta, b, c)", s, kNParam2);
+    } else {
+      if (d.n_up < 1 || d.n_up > (uint32_t)kSJMaxSpin || d.n_down > (uint32_t)kSJMaxSpin ||
+          d.n_up + d.n_down > (uint32_t)kSJMaxElectrons)
+        return fail(MOLE_ERR_SHAPE, "system %u: need 1 <= n_up <= %d, n_down <= %d, N <= %d", s,
+                    kSJMaxSpin, kSJMaxSpin, kSJMaxElectrons);
+      if (d.n_nuclei < 1 || d.n_nuclei > MOLE_MAX_NUCLEI)
+        return fail(MOLE_ERR_SHAPE, "system %u: need 1..%d nuclei", s, MOLE_MAX_NUCLEI);
+      if (d.n_params != (uint32_t)kSJNParam)
+        return fail(MOLE_ERR_SHAPE, "system %u: expected 5 Jastrow parameters", s);
+      if (d.n_basis < 1 || d.n_basis > 1024 || !d.basis_exponent || !d.basis_center || !d.orbital_coeff)
+        return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: basis/orbital tables missing", s);
+      if (d.n_orbitals < std::max(d.n_up, d.n_down))
+        return fail(MOLE_ERR_SHAPE, "system %u: need n_orbitals >= max(n_up, n_down)", s);
+      for (uint32_t m = 0; m < d.n_basis; ++m)
+        if (d.basis_center[m] >= d.n_nuclei)
+          return fail(MOLE_ERR_SHAPE, "system %u: basis function %u centred on missing nucleus", s, m);
+      if (run->flags & MOLE_FLAG_SAMPLES)
+        return fail(MOLE_ERR_UNSUPPORTED, "per-sample compat data is for the two-electron kinds");
+      if (run->o_convention != MOLE_O_LOGDERIV)
+        return fail(MOLE_ERR_UNSUPPORTED, "Slater-Jastrow systems report O = d ln psi / dp");
+    }
   }
   return MOLE_OK;
 }
 
 void fill_sys2(SysState &st, const mole_system_desc &d) {
-  st.kind = d.wf_kind;
-  st.n_up = d.n_up;
-  st.n_down = d.n_down;
-  st.N = d.n_up + d.n_down;
-  st.n_params = d.n_params;
-  st.W = (int)d.n_walkers;
   Sys2 &sp = st.sp;
   sp.kind = (int)d.wf_kind;
   sp.hamiltonian = (int)d.hamiltonian;
@@ -134,22 +197,13 @@ void fill_sys2(SysState &st, const mole_system_desc &d) {
     for (int k = 0; k < 3; ++k) sp.nuc[I][k] = d.nuclei_xyz[3 * I + k];
     sp.charge[I] = (double)d.nuclei_charge[I];
   }
-  // nuclear repulsion as IonicPotential::new sums it (operator.rs:40-55)
-  double rep = 0.0;
-  for (uint32_t i = 0; i < d.n_nuclei; ++i)
-    for (uint32_t j = i + 1; j < d.n_nuclei; ++j) {
-      double dx = sp.nuc[j][0] - sp.nuc[i][0], dy = sp.nuc[j][1] - sp.nuc[i][1],
-             dz = sp.nuc[j][2] - sp.nuc[i][2];
-      double s2 = dx * dx;
-      s2 = s2 + dy * dy;
-      s2 = s2 + dz * dz;
-      rep += (double)(d.nuclei_charge[i] * d.nuclei_charge[j]) / std::sqrt(s2);
-    }
-  sp.v_nn = rep;
+  sp.v_nn = nuclear_repulsion(d.nuclei_xyz, d.nuclei_charge, d.n_nuclei);
   sp.zeta = d.params[0];
   sp.b = d.params[1];
   sp.c = d.params[2];
 }
+
+int default_lanes(int N) { return N <= 4 ? 4 : N <= 10 ? 8 : N <= 20 ? 16 : 32; }
 
 mole_status check_ctx(mole_ctx *ctx, uint32_t sys) {
   if (!ctx) return fail(MOLE_ERR_INVALID_ARGUMENT, "null context");
@@ -160,24 +214,139 @@ mole_status check_ctx(mole_ctx *ctx, uint32_t sys) {
 void free_ctx(mole_ctx *ctx) {
   if (!ctx) return;
   for (SysState &st : ctx->sys) {
-    if (st.d_pos) (void)hipFree(st.d_pos);
-    if (st.d_acc) (void)hipFree(st.d_acc);
-    if (st.d_trace_mask) (void)hipFree(st.d_trace_mask);
-    if (st.d_trace_e) (void)hipFree(st.d_trace_e);
-    if (st.d_samp_psi) (void)hipFree(st.d_samp_psi);
-    if (st.d_samp_g) (void)hipFree(st.d_samp_g);
+    void *ptrs[] = {st.d_pos, st.d_acc, st.d_trace_mask, st.d_trace_e, st.d_samp_psi,
+                    st.d_samp_g, st.d_tab, st.d_center, st.d_B};
+    for (void *p : ptrs)
+      if (p) (void)hipFree(p);
   }
   if (ctx->d_red) (void)hipFree(ctx->d_red);
+  if (ctx->d_sysdev) (void)hipFree(ctx->d_sysdev);
+  if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
   delete ctx;
 }
 
+// ---- Slater-Jastrow launch plumbing ----------------------------------------
+
+int sj_tsize(const mole_ctx *ctx) { return ctx->run.precision == MOLE_PREC_MIXED ? 4 : 8; }
+
+// Tiles of 64/L walkers, most expensive first (longest-processing-time
+// order, so the tail of the launch is short tiles), and the dynamic LDS of
+// the launch: the largest tile footprint.
+mole_status sj_build_tiles(mole_ctx *ctx) {
+  struct T { SJTile t; double cost; };
+  std::vector<T> tiles;
+  size_t lds = 0;
+  for (size_t s = 0; s < ctx->sys.size(); ++s) {
+    SysState &st = ctx->sys[s];
+    st.lds_walker = sj_lds_per_walker((int)st.N, (int)st.n_up, (int)st.n_down, st.n_nuc, st.n_basis,
+                                      sj_tsize(ctx));
+    const int wpt = 64 / st.L;
+    lds = std::max(lds, (size_t)wpt * st.lds_walker);
+    const int nmax = (int)std::max(st.n_up, st.n_down);
+    const double per_walker = (double)st.N * (st.N + st.n_nuc + st.n_basis + (double)nmax * st.n_basis +
+                                               (double)nmax * nmax);
+    for (int w0 = 0; w0 < st.W; w0 += wpt)
+      tiles.push_back({{(int)s, w0, std::min(wpt, st.W - w0)}, per_walker / st.L});
+  }
+  if (lds > ctx->lds_limit)
+    return fail(MOLE_ERR_UNSUPPORTED, "walker state needs %zu B of LDS per workgroup (limit %zu): use "
+                "more lanes per walker", lds, ctx->lds_limit);
+  std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+  std::vector<SJTile> h(tiles.size());
+  for (size_t i = 0; i < tiles.size(); ++i) h[i] = tiles[i].t;
+  if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
+  ctx->d_tiles = nullptr;
+  HIP_TRY(hipMalloc(&ctx->d_tiles, sizeof(SJTile) * h.size()));
+  HIP_TRY(hipMemcpy(ctx->d_tiles, h.data(), sizeof(SJTile) * h.size(), hipMemcpyHostToDevice));
+  ctx->n_tiles = (int)h.size();
+  ctx->lds_bytes = lds;
+  return MOLE_OK;
+}
+
+mole_status sj_upload_sysdev(mole_ctx *ctx) {
+  std::vector<SJSysDev> h(ctx->sys.size());
+  for (size_t s = 0; s < ctx->sys.size(); ++s) {
+    const SysState &st = ctx->sys[s];
+    SJSysDev &d = h[s];
+    d.N = (int)st.N;
+    d.n_up = (int)st.n_up;
+    d.n_dn = (int)st.n_down;
+    d.n_nuc = st.n_nuc;
+    d.n_basis = st.n_basis;
+    d.L = st.L;
+    d.hamiltonian = (int)st.hamiltonian;
+    d.W = st.W;
+    d.nuc = st.d_tab;
+    d.charge = st.d_tab + 3 * st.n_nuc;
+    d.zeta = st.d_tab + 4 * st.n_nuc;
+    d.center = st.d_center;
+    d.coef = st.d_tab + 4 * st.n_nuc + st.n_basis;
+    for (int k = 0; k < kSJNParam; ++k) d.p[k] = st.params[k];
+    d.v_nn = st.v_nn;
+    d.pos = st.d_pos;
+    d.B = st.d_B;
+    d.acc = st.d_acc;
+    d.trace_mask = st.d_trace_mask;
+    d.trace_e = st.d_trace_e;
+    d.draw_base = st.draw_base;
+    d.move_base = st.move_base;
+    d.sys_index = (uint32_t)s;
+    d.lds_per_walker = (uint32_t)st.lds_walker;
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(ctx->d_sysdev, h.data(), sizeof(SJSysDev) * h.size(), hipMemcpyHostToDevice));
+  ctx->sysdev_dirty = false;
+  return MOLE_OK;
+}
+
+SJArgs sj_args(const mole_ctx *ctx) {
+  SJArgs a{};
+  a.systems = ctx->d_sysdev;
+  a.tiles = ctx->d_tiles;
+  a.seed = ctx->run.seed;
+  a.epoch = ctx->epoch;
+  const double lo = -0.5 * ctx->run.step, hi = 0.5 * ctx->run.step;
+  a.scale = hi - lo;
+  a.offset = lo - a.scale;
+  a.block_size = ctx->run.block_size;
+  a.block_fill = ctx->block_fill;
+  a.lu_period = ctx->run.lu_period;
+  a.sweeps_done = ctx->sweeps_done;
+  a.trace_sweep0 = ctx->trace_sweeps;
+  a.trace_sample0 = ctx->trace_samples;
+  a.cap_sweeps = (ctx->run.flags & MOLE_FLAG_TRACE) ? ctx->run.record_sweeps : 0;
+  a.cap_samples = a.cap_sweeps;
+  return a;
+}
+
+// A zero-sweep launch that draws the initial configurations (init) or
+// rebuilds the inverses from the stored positions (refresh).
+mole_status sj_prepare(mole_ctx *ctx, bool init) {
+  if (ctx->sysdev_dirty) {
+    mole_status st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  SJArgs a = sj_args(ctx);
+  a.n_sweeps = 0;
+  a.sample_start = 0;
+  a.init = init ? 1 : 0;
+  a.refresh = 1;
+  for (SysState &s : ctx->sys) s.cols_valid = false;
+  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, ctx->lds_bytes, ctx->stream));
+  return MOLE_OK;
+}
+
 // Runs n sweeps; sweeps s >= sample_start of this call are sampled.
 mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
   if (n == 0) return ok();
   HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->family == kFamSJ && ctx->sysdev_dirty) {
+    mole_status st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   const double lo = -0.5 * ctx->run.step, hi = 0.5 * ctx->run.step;
   const double scale = hi - lo;
@@ -188,41 +357,51 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
     const uint32_t chunk = std::min(n - done, ctx->max_sweeps_per_launch);
     const uint32_t ss = (sample_start > done) ? std::min(sample_start - done, chunk) : 0;
     const uint32_t n_samp = chunk - ss;
-    for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
-      SysState &st = ctx->sys[s];
-      Sweep2Args a{};
-      a.sp = st.sp;
-      a.W = st.W;
-      a.pos = st.d_pos;
-      a.acc = st.d_acc;
-      a.seed = ctx->run.seed;
-      a.sys = s;
-      a.epoch = ctx->epoch;
-      a.draw0 = ctx->position;
-      a.scale = scale;
-      a.offset = offset;
+    if (ctx->family == kFamSJ) {
+      SJArgs a = sj_args(ctx);
       a.n_sweeps = chunk;
       a.sample_start = ss;
-      a.block_size = ctx->run.block_size;
-      a.block_fill = ctx->block_fill;
-      a.o_conv = (int)ctx->run.o_convention;
-      a.trace_mask = st.d_trace_mask;
-      a.trace_e = st.d_trace_e;
-      a.samp_psi = st.d_samp_psi;
-      a.samp_g = st.d_samp_g;
-      a.trace_sweep0 = ctx->trace_sweeps;
-      a.trace_sample0 = ctx->trace_samples;
-      a.cap_sweeps = (st.d_trace_mask) ? ctx->run.record_sweeps : 0;
-      a.cap_samples = (st.d_trace_e || st.d_samp_psi) ? ctx->run.record_sweeps : 0;
-      st.cols_valid = false;
-      HIP_TRY(launch_sweep2(a, ctx->variant, ctx->block_threads, ctx->stream));
+      HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, ctx->lds_bytes, ctx->stream));
       ++launches;
+      ctx->sweeps_done += chunk;
+    }
+    for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
+      SysState &st = ctx->sys[s];
+      if (ctx->family == kFamTwo) {
+        Sweep2Args a{};
+        a.sp = st.sp;
+        a.W = st.W;
+        a.pos = st.d_pos;
+        a.acc = st.d_acc;
+        a.seed = ctx->run.seed;
+        a.sys = s;
+        a.epoch = ctx->epoch;
+        a.draw0 = ctx->position;
+        a.scale = scale;
+        a.offset = offset;
+        a.n_sweeps = chunk;
+        a.sample_start = ss;
+        a.block_size = ctx->run.block_size;
+        a.block_fill = ctx->block_fill;
+        a.o_conv = (int)ctx->run.o_convention;
+        a.trace_mask = st.d_trace_mask;
+        a.trace_e = st.d_trace_e;
+        a.samp_psi = st.d_samp_psi;
+        a.samp_g = st.d_samp_g;
+        a.trace_sweep0 = ctx->trace_sweeps;
+        a.trace_sample0 = ctx->trace_samples;
+        a.cap_sweeps = (st.d_trace_mask) ? ctx->run.record_sweeps : 0;
+        a.cap_samples = (st.d_trace_e || st.d_samp_psi) ? ctx->run.record_sweeps : 0;
+        HIP_TRY(launch_sweep2(a, ctx->variant, ctx->block_threads, ctx->stream));
+        ++launches;
+      }
+      st.cols_valid = false;
       const uint64_t filled = (uint64_t)ctx->block_fill + n_samp;
       st.n_samples += (uint64_t)st.W * n_samp;
       st.n_blocks += (uint64_t)st.W * (filled / ctx->run.block_size);
       st.n_moves += (uint64_t)st.W * st.N * chunk;
     }
-    ctx->position += (uint64_t)chunk * ctx->sys[0].N * 4;
+    if (ctx->family == kFamTwo) ctx->position += (uint64_t)chunk * ctx->sys[0].N * 4;
     ctx->block_fill = (uint32_t)(((uint64_t)ctx->block_fill + n_samp) % ctx->run.block_size);
     ctx->trace_sweeps += chunk;
     ctx->trace_samples += n_samp;
@@ -232,6 +411,22 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
   ctx->last_launches = launches;
   ctx->timed = true;
   return ok();
+}
+
+mole_status reduce_sys(mole_ctx *ctx, uint32_t sys, const double **cols) {
+  SysState &s = ctx->sys[sys];
+  if (!s.cols_valid) {
+    HIP_TRY(hipSetDevice(ctx->device));
+    s.cols.resize(s.n_cols);
+    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red,
+                                  ctx->family == kFamSJ ? (int)kSJCondMax : -1, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(s.cols.data(), ctx->d_red, sizeof(double) * s.n_cols, hipMemcpyDeviceToHost,
+                           ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    s.cols_valid = true;
+  }
+  *cols = s.cols.data();
+  return MOLE_OK;
 }
 
 }  // namespace
@@ -262,9 +457,7 @@ mole_status mole_device_info(char *name, size_t cap, int32_t *cu_count) {
   HIP_TRY(hipGetDevice(&dev));
   hipDeviceProp_t prop;
   HIP_TRY(hipGetDeviceProperties(&prop, dev));
-  if (name && cap) {
-    std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
-  }
+  if (name && cap) std::snprintf(name, cap, "%s (%s)", prop.name, prop.gcnArchName);
   if (cu_count) *cu_count = prop.multiProcessorCount;
   return ok();
 }
@@ -277,47 +470,98 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
   if (st != MOLE_OK) return st;
   mole_ctx *ctx = new mole_ctx();
   ctx->run = *run;
+  ctx->family = family_of(sys[0].wf_kind);
   ctx->sys.resize(n_sys);
-  for (uint32_t s = 0; s < n_sys; ++s) fill_sys2(ctx->sys[s], sys[s]);
+  for (uint32_t s = 0; s < n_sys; ++s) {
+    SysState &S = ctx->sys[s];
+    const mole_system_desc &d = sys[s];
+    S.kind = d.wf_kind;
+    S.hamiltonian = d.hamiltonian;
+    S.n_up = d.n_up;
+    S.n_down = d.n_down;
+    S.N = d.n_up + d.n_down;
+    S.n_params = d.n_params;
+    S.W = (int)d.n_walkers;
+    S.params.assign(d.params, d.params + d.n_params);
+    if (ctx->family == kFamTwo) {
+      fill_sys2(S, d);
+      S.n_cols = kAccColumns2;
+    } else {
+      S.n_nuc = (int)d.n_nuclei;
+      S.n_basis = (int)d.n_basis;
+      S.n_orb = (int)d.n_orbitals;
+      S.v_nn = nuclear_repulsion(d.nuclei_xyz, d.nuclei_charge, d.n_nuclei);
+      S.L = default_lanes((int)S.N);
+      S.n_cols = kSJColumns;
+    }
+  }
   auto bail = [&](mole_status s) {
     std::string msg = g_last_error;
     free_ctx(ctx);
     g_last_error = msg;
     return s;
   };
-#define CTX_TRY(expr)                                                               \
-  do {                                                                              \
-    hipError_t e_ = (expr);                                                         \
-    if (e_ != hipSuccess)                                                           \
-      return bail(fail(e_ == hipErrorOutOfMemory ? MOLE_ERR_OUT_OF_MEMORY : MOLE_ERR_DEVICE, \
-                       "%s: %s", #expr, hipGetErrorString(e_)));                    \
+#define CTX_TRY(expr)                                                                              \
+  do {                                                                                             \
+    hipError_t e_ = (expr);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return bail(fail(e_ == hipErrorOutOfMemory ? MOLE_ERR_OUT_OF_MEMORY : MOLE_ERR_DEVICE, "%s: %s", \
+                       #expr, hipGetErrorString(e_)));                                             \
   } while (0)
   CTX_TRY(hipGetDevice(&ctx->device));
+  {
+    hipDeviceProp_t prop;
+    CTX_TRY(hipGetDeviceProperties(&prop, ctx->device));
+    ctx->lds_limit = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+  }
   CTX_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CTX_TRY(hipEventCreate(&ctx->ev0));
   CTX_TRY(hipEventCreate(&ctx->ev1));
-  CTX_TRY(hipMalloc(&ctx->d_red, sizeof(double) * kAccColumns2));
-  for (SysState &s : ctx->sys) {
-    const size_t W = (size_t)s.W;
-    CTX_TRY(hipMalloc(&s.d_pos, sizeof(double) * 3 * s.N * W));
-    CTX_TRY(hipMalloc(&s.d_acc, sizeof(double) * kAccColumns2 * W));
-    CTX_TRY(launch_fill(s.d_pos, 3 * s.N * W, 0.0, ctx->stream));
-    CTX_TRY(launch_fill(s.d_acc, kAccColumns2 * W, 0.0, ctx->stream));
+  CTX_TRY(hipMalloc(&ctx->d_red, sizeof(double) * std::max<int>(kAccColumns2, kSJColumns)));
+  for (uint32_t s = 0; s < n_sys; ++s) {
+    SysState &S = ctx->sys[s];
+    const mole_system_desc &d = sys[s];
+    const size_t W = (size_t)S.W;
+    CTX_TRY(hipMalloc(&S.d_pos, sizeof(double) * 3 * S.N * W));
+    CTX_TRY(hipMalloc(&S.d_acc, sizeof(double) * S.n_cols * W));
+    CTX_TRY(launch_fill(S.d_pos, 3 * S.N * W, 0.0, ctx->stream));
+    CTX_TRY(launch_fill(S.d_acc, S.n_cols * W, 0.0, ctx->stream));
     const size_t cap = run->record_sweeps;
     if (run->flags & MOLE_FLAG_TRACE) {
-      CTX_TRY(hipMalloc(&s.d_trace_mask, sizeof(uint64_t) * cap * W));
-      CTX_TRY(hipMalloc(&s.d_trace_e, sizeof(double) * cap * W));
-      CTX_TRY(hipMemsetAsync(s.d_trace_mask, 0, sizeof(uint64_t) * cap * W, ctx->stream));
-      CTX_TRY(launch_fill(s.d_trace_e, cap * W, 0.0, ctx->stream));
+      CTX_TRY(hipMalloc(&S.d_trace_mask, sizeof(uint64_t) * cap * W));
+      CTX_TRY(hipMalloc(&S.d_trace_e, sizeof(double) * cap * W));
+      CTX_TRY(launch_fill((double *)S.d_trace_mask, cap * W, 0.0, ctx->stream));
+      CTX_TRY(launch_fill(S.d_trace_e, cap * W, 0.0, ctx->stream));
     }
     if (run->flags & MOLE_FLAG_SAMPLES) {
-      CTX_TRY(hipMalloc(&s.d_samp_psi, sizeof(double) * cap * W));
-      CTX_TRY(hipMalloc(&s.d_samp_g, sizeof(double) * cap * W * kNParam2));
-      if (!s.d_trace_e) {
-        CTX_TRY(hipMalloc(&s.d_trace_e, sizeof(double) * cap * W));
-        CTX_TRY(launch_fill(s.d_trace_e, cap * W, 0.0, ctx->stream));
+      CTX_TRY(hipMalloc(&S.d_samp_psi, sizeof(double) * cap * W));
+      CTX_TRY(hipMalloc(&S.d_samp_g, sizeof(double) * cap * W * kNParam2));
+      if (!S.d_trace_e) {
+        CTX_TRY(hipMalloc(&S.d_trace_e, sizeof(double) * cap * W));
+        CTX_TRY(launch_fill(S.d_trace_e, cap * W, 0.0, ctx->stream));
       }
     }
+    if (ctx->family == kFamSJ) {
+      // tables: nuclei xyz (3 n_nuc), charges (n_nuc), zeta (n_basis), coef (n_orb n_basis)
+      std::vector<double> tab;
+      tab.insert(tab.end(), d.nuclei_xyz, d.nuclei_xyz + 3 * d.n_nuclei);
+      for (uint32_t I = 0; I < d.n_nuclei; ++I) tab.push_back((double)d.nuclei_charge[I]);
+      tab.insert(tab.end(), d.basis_exponent, d.basis_exponent + d.n_basis);
+      tab.insert(tab.end(), d.orbital_coeff, d.orbital_coeff + (size_t)d.n_orbitals * d.n_basis);
+      std::vector<int> cen(d.basis_center, d.basis_center + d.n_basis);
+      CTX_TRY(hipMalloc(&S.d_tab, sizeof(double) * tab.size()));
+      CTX_TRY(hipMemcpy(S.d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+      CTX_TRY(hipMalloc(&S.d_center, sizeof(int) * cen.size()));
+      CTX_TRY(hipMemcpy(S.d_center, cen.data(), sizeof(int) * cen.size(), hipMemcpyHostToDevice));
+      const size_t nB = (size_t)S.n_up * S.n_up + (size_t)S.n_down * S.n_down;
+      CTX_TRY(hipMalloc(&S.d_B, (size_t)sj_tsize(ctx) * nB * W + 8));
+      CTX_TRY(launch_fill((double *)S.d_B, (sj_tsize(ctx) * nB * W) / 8, 0.0, ctx->stream));
+    }
+  }
+  if (ctx->family == kFamSJ) {
+    CTX_TRY(hipMalloc(&ctx->d_sysdev, sizeof(SJSysDev) * n_sys));
+    mole_status ts = sj_build_tiles(ctx);
+    if (ts != MOLE_OK) return bail(ts);
   }
   CTX_TRY(hipStreamSynchronize(ctx->stream));
 #undef CTX_TRY
@@ -336,12 +580,22 @@ mole_status mole_init_walkers(mole_ctx *ctx) {
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
   HIP_TRY(hipSetDevice(ctx->device));
-  for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
-    SysState &ss = ctx->sys[s];
-    HIP_TRY(launch_init2(ss.d_pos, ss.W, ctx->run.seed, s, ctx->epoch, ctx->stream));
+  if (ctx->family == kFamTwo) {
+    for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
+      SysState &ss = ctx->sys[s];
+      HIP_TRY(launch_init2(ss.d_pos, ss.W, ctx->run.seed, s, ctx->epoch, ctx->stream));
+    }
+    ctx->position = 3ull * ctx->sys[0].N;
+    return ok();
   }
-  ctx->position = 3ull * ctx->sys[0].N;
-  return ok();
+  for (SysState &s : ctx->sys) {
+    s.draw_base = 3ull * s.N;  // moves start after Sampler::new's 3N draws
+    s.move_base = 0;
+  }
+  ctx->sweeps_done = 0;
+  ctx->sysdev_dirty = true;
+  st = sj_prepare(ctx, true);
+  return st == MOLE_OK ? ok() : st;
 }
 
 mole_status mole_set_configs(mole_ctx *ctx, uint32_t sys, const double *cfg, size_t n_walkers) {
@@ -358,6 +612,10 @@ mole_status mole_set_configs(mole_ctx *ctx, uint32_t sys, const double *cfg, siz
   HIP_TRY(hipMemcpyAsync(s.d_pos, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice,
                          ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (ctx->family == kFamSJ) {
+    st = sj_prepare(ctx, false);
+    if (st != MOLE_OK) return st;
+  }
   return ok();
 }
 
@@ -386,6 +644,10 @@ mole_status mole_broadcast_config(mole_ctx *ctx, uint32_t sys, const double *cfg
   HIP_TRY(hipSetDevice(ctx->device));
   for (uint32_t k = 0; k < 3 * s.N; ++k)
     HIP_TRY(launch_fill(s.d_pos + (size_t)k * s.W, (size_t)s.W, cfg[k], ctx->stream));
+  if (ctx->family == kFamSJ) {
+    st = sj_prepare(ctx, false);
+    if (st != MOLE_OK) return st;
+  }
   return ok();
 }
 
@@ -394,6 +656,14 @@ mole_status mole_set_stream(mole_ctx *ctx, uint32_t epoch, uint64_t position) {
   if (st != MOLE_OK) return st;
   ctx->epoch = epoch;
   ctx->position = position;
+  if (ctx->family == kFamSJ) {
+    for (SysState &s : ctx->sys) {
+      s.draw_base = position;
+      s.move_base = 0;
+    }
+    ctx->sweeps_done = 0;
+    ctx->sysdev_dirty = true;
+  }
   return ok();
 }
 
@@ -401,7 +671,10 @@ mole_status mole_get_stream(mole_ctx *ctx, uint32_t *epoch, uint64_t *position) 
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
   if (epoch) *epoch = ctx->epoch;
-  if (position) *position = ctx->position;
+  if (position)
+    *position = ctx->family == kFamTwo
+                    ? ctx->position
+                    : ctx->sys[0].draw_base + 4ull * ctx->sys[0].N * ctx->sweeps_done;
   return ok();
 }
 
@@ -410,9 +683,14 @@ mole_status mole_set_params(mole_ctx *ctx, uint32_t sys, const double *p, uint32
   if (st != MOLE_OK) return st;
   SysState &s = ctx->sys[sys];
   if (!p || n_params != s.n_params) return fail(MOLE_ERR_SHAPE, "expected %u parameters", s.n_params);
-  s.sp.zeta = p[0];
-  s.sp.b = p[1];
-  s.sp.c = p[2];
+  s.params.assign(p, p + n_params);
+  if (ctx->family == kFamTwo) {
+    s.sp.zeta = p[0];
+    s.sp.b = p[1];
+    s.sp.c = p[2];
+  } else {
+    ctx->sysdev_dirty = true;
+  }
   return ok();
 }
 
@@ -421,9 +699,7 @@ mole_status mole_get_params(mole_ctx *ctx, uint32_t sys, double *p, uint32_t n_p
   if (st != MOLE_OK) return st;
   SysState &s = ctx->sys[sys];
   if (!p || n_params != s.n_params) return fail(MOLE_ERR_SHAPE, "expected %u parameters", s.n_params);
-  p[0] = s.sp.zeta;
-  p[1] = s.sp.b;
-  p[2] = s.sp.c;
+  std::copy(s.params.begin(), s.params.end(), p);
   return ok();
 }
 
@@ -444,10 +720,10 @@ mole_status mole_run(mole_ctx *ctx, uint32_t steps, uint32_t block_size) {
   if (ctx->block_fill != 0) {
     // a run starts its own blocks: drop the partial block of earlier calls
     HIP_TRY(hipSetDevice(ctx->device));
-    for (SysState &s : ctx->sys)
-    {
+    for (SysState &s : ctx->sys) {
       s.cols_valid = false;
-      HIP_TRY(launch_fill(s.d_acc + (size_t)kAccBlockSum * s.W, (size_t)s.W, 0.0, ctx->stream));
+      const int col = ctx->family == kFamTwo ? (int)kAccBlockSum : (int)kSJBlockSum;
+      HIP_TRY(launch_fill(s.d_acc + (size_t)col * s.W, (size_t)s.W, 0.0, ctx->stream));
     }
     ctx->block_fill = 0;
   }
@@ -460,7 +736,7 @@ mole_status mole_reset_stats(mole_ctx *ctx) {
   if (st != MOLE_OK) return st;
   HIP_TRY(hipSetDevice(ctx->device));
   for (SysState &s : ctx->sys) {
-    HIP_TRY(launch_fill(s.d_acc, (size_t)kAccColumns2 * s.W, 0.0, ctx->stream));
+    HIP_TRY(launch_fill(s.d_acc, (size_t)s.n_cols * s.W, 0.0, ctx->stream));
     s.n_samples = s.n_blocks = s.n_moves = 0;
     s.cols_valid = false;
   }
@@ -469,36 +745,24 @@ mole_status mole_reset_stats(mole_ctx *ctx) {
   return ok();
 }
 
-static mole_status reduce_sys(mole_ctx *ctx, uint32_t sys, double *cols) {
-  SysState &s = ctx->sys[sys];
-  if (!s.cols_valid) {
-    HIP_TRY(hipSetDevice(ctx->device));
-    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, kAccColumns2, ctx->d_red, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(s.cols, ctx->d_red, sizeof(double) * kAccColumns2,
-                           hipMemcpyDeviceToHost, ctx->stream));
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    s.cols_valid = true;
-  }
-  std::memcpy(cols, s.cols, sizeof s.cols);
-  return MOLE_OK;
-}
-
 mole_status mole_read_stats(mole_ctx *ctx, uint32_t sys, mole_stats *out) {
   mole_status st = check_ctx(ctx, sys);
   if (st != MOLE_OK) return st;
   if (!out) return fail(MOLE_ERR_INVALID_ARGUMENT, "null output");
-  double cols[kAccColumns2];
-  st = reduce_sys(ctx, sys, cols);
+  const double *cols = nullptr;
+  st = reduce_sys(ctx, sys, &cols);
   if (st != MOLE_OK) return st;
   const SysState &s = ctx->sys[sys];
+  const bool sj = ctx->family == kFamSJ;
   out->n_samples = s.n_samples;
   out->n_blocks = s.n_blocks;
   out->n_moves = s.n_moves;
-  out->n_accepted = (uint64_t)cols[kAccAccepted];
-  out->sum_e = cols[kAccSumE];
-  out->sum_e2 = cols[kAccSumE2];
-  out->sum_block_mean = cols[kAccSumBm];
-  out->sum_block_mean2 = cols[kAccSumBm2];
+  out->n_accepted = (uint64_t)cols[sj ? kSJAccepted : kAccAccepted];
+  out->sum_e = cols[sj ? kSJSumE : kAccSumE];
+  out->sum_e2 = cols[sj ? kSJSumE2 : kAccSumE2];
+  out->sum_block_mean = cols[sj ? kSJSumBm : kAccSumBm];
+  out->sum_block_mean2 = cols[sj ? kSJSumBm2 : kAccSumBm2];
+  out->max_condition = sj ? cols[kSJCondMax] : 0.0;
   return ok();
 }
 
@@ -506,21 +770,23 @@ mole_status mole_read_sr_sums(mole_ctx *ctx, uint32_t sys, mole_sr_sums *out) {
   mole_status st = check_ctx(ctx, sys);
   if (st != MOLE_OK) return st;
   if (!out) return fail(MOLE_ERR_INVALID_ARGUMENT, "null output");
-  double cols[kAccColumns2];
-  st = reduce_sys(ctx, sys, cols);
+  const double *cols = nullptr;
+  st = reduce_sys(ctx, sys, &cols);
   if (st != MOLE_OK) return st;
   std::memset(out, 0, sizeof *out);
-  const int np = kNParam2;
+  const bool sj = ctx->family == kFamSJ;
+  const int np = sj ? kSJNParam : kNParam2;
+  const int cO = sj ? kSJSumO : kAccSumO, cOE = sj ? kSJSumOE : kAccSumOE, cOO = sj ? kSJSumOO : kAccSumOO;
   out->n_params = np;
   out->n = ctx->sys[sys].n_samples;
-  out->sum_e = cols[kAccSumE];
+  out->sum_e = cols[sj ? kSJSumE : kAccSumE];
   int t = 0;
   for (int k = 0; k < np; ++k) {
-    out->sum_o[k] = cols[kAccSumO + k];
-    out->sum_oe[k] = cols[kAccSumOE + k];
+    out->sum_o[k] = cols[cO + k];
+    out->sum_oe[k] = cols[cOE + k];
     for (int l = k; l < np; ++l, ++t) {
-      out->sum_oo[k * np + l] = cols[kAccSumOO + t];
-      out->sum_oo[l * np + k] = cols[kAccSumOO + t];
+      out->sum_oo[k * np + l] = cols[cOO + t];
+      out->sum_oo[l * np + k] = cols[cOO + t];
     }
   }
   return ok();
@@ -552,8 +818,8 @@ mole_status mole_read_trace(mole_ctx *ctx, uint32_t sys, uint64_t *accept_mask, 
   return ok();
 }
 
-mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, double *psi,
-                              double *pgrad, size_t walkers, size_t samples) {
+mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, double *psi, double *pgrad,
+                              size_t walkers, size_t samples) {
   mole_status st = check_ctx(ctx, sys);
   if (st != MOLE_OK) return st;
   SysState &s = ctx->sys[sys];
@@ -575,8 +841,7 @@ mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, doub
       std::memcpy(psi + w * samples, h.data() + w * cap, samples * sizeof(double));
   }
   if (pgrad) {
-    HIP_TRY(hipMemcpy(h.data(), s.d_samp_g, cap * s.W * kNParam2 * sizeof(double),
-                      hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(h.data(), s.d_samp_g, cap * s.W * kNParam2 * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t w = 0; w < walkers; ++w)
       std::memcpy(pgrad + w * samples * kNParam2, h.data() + w * cap * kNParam2,
                   samples * kNParam2 * sizeof(double));
@@ -589,6 +854,8 @@ mole_status mole_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg, 
   mole_status st = check_ctx(ctx, sys);
   if (st != MOLE_OK) return st;
   if (!cfg) return fail(MOLE_ERR_INVALID_ARGUMENT, "null cfg");
+  if (ctx->family != kFamTwo)
+    return fail(MOLE_ERR_UNSUPPORTED, "mole_eval covers the two-electron kinds");
   if (n == 0) return ok();
   SysState &s = ctx->sys[sys];
   const size_t D = 3 * s.N;
@@ -609,8 +876,7 @@ mole_status mole_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg, 
   hipError_t e = hipMemcpyAsync(d_cfg, cfg, sizeof(double) * D * n, hipMemcpyHostToDevice, ctx->stream);
   if (e == hipSuccess) e = launch_eval2(a, ctx->stream);
   auto back = [&](double *host, const double *dev, size_t cnt) {
-    if (e == hipSuccess && host)
-      e = hipMemcpyAsync(host, dev, sizeof(double) * cnt, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess && host) e = hipMemcpyAsync(host, dev, sizeof(double) * cnt, hipMemcpyDeviceToHost, ctx->stream);
   };
   back(value, a.value, n);
   back(grad, a.grad, D * n);
@@ -623,9 +889,8 @@ mole_status mole_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg, 
   return ok();
 }
 
-mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t optimizer,
-                                          double step, uint32_t sr_mode, double sr_eps,
-                                          double *dp, double *g_out) {
+mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t optimizer, double step,
+                                          uint32_t sr_mode, double sr_eps, double *dp, double *g_out) {
   if (!sums || !dp) return fail(MOLE_ERR_INVALID_ARGUMENT, "null argument");
   const int np = (int)sums->n_params;
   if (np < 1 || np > MOLE_MAX_PARAMS) return fail(MOLE_ERR_SHAPE, "n_params out of range");
@@ -685,12 +950,41 @@ mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t blo
   if (variant > MOLE_VARIANT_WAVESPEC) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown variant %u", variant);
   if (block_threads != 0 && (block_threads % 64 != 0 || block_threads > 256))
     return fail(MOLE_ERR_INVALID_ARGUMENT, "block_threads must be 0 or a multiple of 64 up to 256");
-  if (variant == MOLE_VARIANT_WAVESPEC)
+  if (ctx->family == kFamTwo && variant == MOLE_VARIANT_WAVESPEC)
     for (const SysState &s : ctx->sys)
       if (!sweep2_wavespec_eligible(s.sp))
         return fail(MOLE_ERR_UNSUPPORTED, "wave-specialised kernel needs <= 2 nuclei (1 for helium)");
   ctx->variant = (int)variant;
   if (block_threads) ctx->block_threads = (int)block_threads;
+  return ok();
+}
+
+mole_status mole_set_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t lanes) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "lanes per walker apply to Slater-Jastrow systems");
+  if (lanes > 64 || (lanes & (lanes - 1)) != 0)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "lanes must be a power of two up to 64 (or 0)");
+  SysState &s = ctx->sys[sys];
+  const int old = s.L;
+  s.L = lanes ? (int)lanes : default_lanes((int)s.N);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  st = sj_build_tiles(ctx);
+  if (st != MOLE_OK) {
+    s.L = old;
+    (void)sj_build_tiles(ctx);
+    return st;
+  }
+  ctx->sysdev_dirty = true;
+  return ok();
+}
+
+mole_status mole_get_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t *lanes) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!lanes) return fail(MOLE_ERR_INVALID_ARGUMENT, "null output");
+  *lanes = (uint32_t)ctx->sys[sys].L;
   return ok();
 }
 

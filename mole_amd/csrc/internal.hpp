@@ -74,7 +74,7 @@ hipError_t launch_init2(double *pos, int W, uint64_t seed, uint32_t sys,
                         uint32_t epoch, hipStream_t stream);
 hipError_t launch_eval2(const Eval2Args &a, hipStream_t stream);
 hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out,
-                                 hipStream_t stream);
+                                 int max_col, hipStream_t stream);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t stream);
 
 }  // namespace mole

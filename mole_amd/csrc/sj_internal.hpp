@@ -1,0 +1,80 @@
+// Internal structures of the Slater-Jastrow walker path.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace mole {
+
+constexpr int kSJNParam = 5;  // b_anti, b_para, c, a, d
+constexpr int kSJMaxElectrons = 40;
+constexpr int kSJMaxSpin = 20;
+
+// Per-walker accumulator columns of Slater-Jastrow systems, [column][W].
+enum SJAccColumn : int {
+  kSJSumE = 0,
+  kSJSumE2 = 1,
+  kSJSumBm = 2,
+  kSJSumBm2 = 3,
+  kSJBlockSum = 4,
+  kSJAccepted = 5,
+  kSJCondMax = 6,  // largest ||A||_1 ||A^-1||_1 seen at an LU refresh
+  kSJSumO = 7,
+  kSJSumOE = 7 + kSJNParam,
+  kSJSumOO = 7 + 2 * kSJNParam,  // upper triangle, 15 columns
+  kSJColumns = 7 + 2 * kSJNParam + kSJNParam * (kSJNParam + 1) / 2
+};
+
+// One system as the kernels see it (device memory).
+struct SJSysDev {
+  int N, n_up, n_dn, n_nuc, n_basis;
+  int L;  // lanes per walker (power of two, <= 64)
+  int hamiltonian;
+  int W;
+  const double *nuc;     // [n_nuc][3]
+  const double *charge;  // [n_nuc]
+  const double *zeta;    // [n_basis]
+  const int *center;     // [n_basis]
+  const double *coef;    // [n_orb][n_basis]
+  double p[kSJNParam];
+  double v_nn;
+  double *pos;          // [3N][W]
+  void *B;              // [n_up^2 + n_dn^2][W], element type of the launch
+  double *acc;          // [kSJColumns][W]
+  uint64_t *trace_mask; // [W][cap_sweeps] or null
+  double *trace_e;      // [W][cap_samples] or null
+  uint64_t draw_base;   // draw index of the first move after the stream origin
+  uint64_t move_base;   // moves made before the stream origin (LU refresh phase)
+  uint32_t sys_index;   // mixed into the Philox key
+  uint32_t lds_per_walker;  // bytes
+};
+
+// A workgroup's work: walkers [w0, w0 + nw) of one system.
+struct SJTile {
+  int sys, w0, nw;
+};
+
+struct SJArgs {
+  const SJSysDev *systems;
+  const SJTile *tiles;
+  uint64_t seed;
+  uint32_t epoch;
+  double scale, offset;    // Range(-side/2, side/2)
+  uint64_t sweeps_done;    // sweeps since the stream origin: draws start at
+                           // draw_base + 4N sweeps_done, moves at
+                           // move_base + N sweeps_done
+  uint32_t n_sweeps;
+  uint32_t sample_start;   // sweeps s >= sample_start are sampled
+  uint32_t block_size, block_fill;
+  uint32_t lu_period;      // 0: never
+  uint32_t trace_sweep0, trace_sample0, cap_sweeps, cap_samples;
+  int init;                // 1: draw U(-1,1) configurations first (draws 0..3N-1)
+  int refresh;             // 1: rebuild the inverses from the positions at entry
+};
+
+size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);
+
+hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream);
+
+}  // namespace mole

@@ -1,0 +1,566 @@
+// Slater-Jastrow walker kernel: single-electron box moves with
+// Sherman-Morrison determinant ratios and updates, periodic Gauss-Jordan
+// re-inversion in FP64, and the local energy after sampled sweeps.
+//
+// A workgroup is one wave.  It owns 64/L walkers of one system (a tile);
+// the L lanes of a walker split every stage of a move -- distances and
+// Jastrow terms over electrons and nuclei, basis functions, orbitals,
+// Sherman-Morrison columns and matrix elements -- and the walker's state
+// (positions, inverse Slater matrices, scratch) lives in LDS for the whole
+// launch, read from and written to HBM once.  Every sum is formed by the
+// walker's sub-lane 0 in a fixed sequential order, so results do not depend
+// on L and equal those of a sequential evaluation bit for bit.
+//
+// Math (per move of electron i of spin s, column c; B = A_s^-1 with
+// A_s[c][k] = phi_k(r_c)):
+//   ratio  R_D = sum_k phi_k(r') B[k][c]
+//   dJ     = J_i(r') - J_i(r_i),  J_i = sum_{j != i} u_ij + sum_I v_I
+//   accept min(R_D^2 exp(2 dJ), 1) > U           (metrop.rs:73-82)
+//   update B[k][l] -= B[k][c] f_l (l != c), B[k][c] /= R_D,
+//          f_l = (sum_k phi_k(r') B[k][l]) / R_D  (Sherman-Morrison)
+#include "../../include/mole_gpu.h"
+#include "device_common.hpp"
+#include "sj_internal.hpp"
+
+namespace mole {
+
+// LDS layout of one walker (bytes); identical on host and device.
+struct SJLayout {
+  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, bytes;
+  __host__ __device__ SJLayout(int N, int nu, int nd, int nn, int K, int tsize) {
+    const int nmax = nu > nd ? nu : nd;
+    uint32_t o = 0;
+    auto take = [&](uint32_t n_bytes) {
+      const uint32_t at = o;
+      o += (n_bytes + 7u) & ~7u;
+      return at;
+    };
+    pos = take(24u * N);
+    B = take((uint32_t)tsize * (nu * nu + nd * nd));
+    rng = take(32u * N);
+    rn = take(8u * (nn > 0 ? nn : 1));
+    chi = take(8u * K);
+    phin = take(8u * nmax);
+    jn = take(8u * (N + nn));
+    jo = take(8u * (N + nn));
+    f = take(8u * nmax);
+    const uint32_t lu = 8u * (2u * nmax * nmax + nmax);
+    const uint32_t el = 8u * (5u * K + 4u * nmax + 8u * (N + nn));
+    scr = take(lu > el ? lu : el);
+    flag = take(16u);
+    bytes = o;
+  }
+};
+
+__host__ size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize) {
+  return SJLayout(N, nu, nd, nn, K, tsize).bytes;
+}
+
+namespace {
+
+template <typename T>
+struct SJWalker {
+  double *pos, *rng, *rn, *chi, *scr, *flag;
+  T *Bup, *Bdn, *phin, *jn, *jo, *f;
+};
+
+__device__ __forceinline__ double pade_pair(double b, double c, double r) {
+  return (b * r) / (1.0 + c * r);
+}
+
+// e-e term between electron i at r and electron j at x (T precision)
+template <typename T>
+__device__ __forceinline__ T pair_term(double b, double c, const double *r, const double *x);
+template <>
+__device__ __forceinline__ double pair_term<double>(double b, double c, const double *r, const double *x) {
+  const double rij = norm3(r[0] - x[0], r[1] - x[1], r[2] - x[2]);
+  return pade_pair(b, c, rij);
+}
+template <>
+__device__ __forceinline__ float pair_term<float>(double b, double c, const double *r, const double *x) {
+  const float d0 = (float)(r[0] - x[0]), d1 = (float)(r[1] - x[1]), d2 = (float)(r[2] - x[2]);
+  float q = d0 * d0;
+  q = q + d1 * d1;
+  q = q + d2 * d2;
+  const float rij = sqrtf(q);
+  return ((float)b * rij) / (1.0f + (float)c * rij);
+}
+
+// e-n term of an electron at r and nucleus R with aZ = a Z_I (T precision)
+template <typename T>
+__device__ __forceinline__ T en_term(double aZ, double d, const double *r, const double *R);
+template <>
+__device__ __forceinline__ double en_term<double>(double aZ, double d, const double *r, const double *R) {
+  const double rI = norm3(r[0] - R[0], r[1] - R[1], r[2] - R[2]);
+  return -(aZ * rI) / (1.0 + d * rI);
+}
+template <>
+__device__ __forceinline__ float en_term<float>(double aZ, double d, const double *r, const double *R) {
+  const float d0 = (float)(r[0] - R[0]), d1 = (float)(r[1] - R[1]), d2 = (float)(r[2] - R[2]);
+  float q = d0 * d0;
+  q = q + d1 * d1;
+  q = q + d2 * d2;
+  const float rI = sqrtf(q);
+  return -((float)aZ * rI) / (1.0f + (float)d * rI);
+}
+
+__device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
+__device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+
+// Basis values chi[m] of the point r into LDS (lanes over nuclei, then over
+// basis functions).  Ends with a barrier.
+__device__ __forceinline__ void basis_to_lds(const SJSysDev &S, const double *r, double *rn, double *chi,
+                                             int t, int L) {
+  for (int I = t; I < S.n_nuc; I += L)
+    rn[I] = norm3(r[0] - S.nuc[3 * I], r[1] - S.nuc[3 * I + 1], r[2] - S.nuc[3 * I + 2]);
+  __syncthreads();
+  for (int m = t; m < S.n_basis; m += L) chi[m] = exp_m(-S.zeta[m] * rn[S.center[m]]);
+  __syncthreads();
+}
+
+__device__ __forceinline__ double orbital_value(const SJSysDev &S, int k, const double *chi) {
+  const double *c = S.coef + (size_t)k * S.n_basis;
+  double phi = 0.0;
+  for (int m = 0; m < S.n_basis; ++m) phi = fma(c[m], chi[m], phi);
+  return phi;
+}
+
+// Gauss-Jordan inverse of the spin-s Slater matrix from the positions, FP64,
+// into B (as T); returns ||A||_1 ||A^-1||_1 on sub-lane 0.  All lanes of the
+// workgroup must call it.
+template <typename T>
+__device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t, int L) {
+  const int n = s == 0 ? S.n_up : S.n_dn;
+  if (n == 0) return 1.0;
+  const int first = s == 0 ? 0 : S.n_up;
+  T *B = s == 0 ? wk.Bup : wk.Bdn;
+  double *M = wk.scr, *I = wk.scr + n * n, *colp = wk.scr + 2 * n * n;
+  for (int c = 0; c < n; ++c) {
+    basis_to_lds(S, wk.pos + 3 * (first + c), wk.rn, wk.chi, t, L);
+    for (int k = t; k < n; k += L) M[c * n + k] = orbital_value(S, k, wk.chi);
+    __syncthreads();
+  }
+  double anorm = 0.0;
+  if (t == 0)
+    for (int k = 0; k < n; ++k) {
+      double cs = 0.0;
+      for (int c = 0; c < n; ++c) cs += fabs(M[c * n + k]);
+      anorm = cs > anorm ? cs : anorm;
+    }
+  for (int e = t; e < n * n; e += L) I[e] = (e / n == e % n) ? 1.0 : 0.0;
+  __syncthreads();
+  for (int p = 0; p < n; ++p) {
+    if (t == 0) {
+      int piv = p;
+      for (int r = p + 1; r < n; ++r)
+        if (fabs(M[r * n + p]) > fabs(M[piv * n + p])) piv = r;
+      wk.flag[0] = (double)piv;
+    }
+    __syncthreads();
+    const int piv = (int)wk.flag[0];
+    if (piv != p)
+      for (int j = t; j < n; j += L) {
+        double tm = M[p * n + j];
+        M[p * n + j] = M[piv * n + j];
+        M[piv * n + j] = tm;
+        tm = I[p * n + j];
+        I[p * n + j] = I[piv * n + j];
+        I[piv * n + j] = tm;
+      }
+    __syncthreads();
+    const double ip = 1.0 / M[p * n + p];
+    __syncthreads();
+    for (int j = t; j < n; j += L) {
+      M[p * n + j] = M[p * n + j] * ip;
+      I[p * n + j] = I[p * n + j] * ip;
+    }
+    for (int r = t; r < n; r += L) colp[r] = M[r * n + p];
+    __syncthreads();
+    // row p's own column entry was just scaled; colp[p] is unused
+    for (int e = t; e < n * n; e += L) {
+      const int r = e / n, j = e % n;
+      if (r == p) continue;
+      M[e] = fma(-colp[r], M[p * n + j], M[e]);
+      I[e] = fma(-colp[r], I[p * n + j], I[e]);
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < n * n; e += L) B[e] = (T)I[e];
+  double bnorm = 0.0;
+  if (t == 0)
+    for (int c = 0; c < n; ++c) {
+      double cs = 0.0;
+      for (int k = 0; k < n; ++k) cs += fabs(I[k * n + c]);
+      bnorm = cs > bnorm ? cs : bnorm;
+    }
+  __syncthreads();
+  return anorm * bnorm;
+}
+
+// Local energy and d ln psi/dp (five Jastrow parameters) of the walker's
+// configuration; returns on sub-lane 0.  All lanes must call it.
+template <typename T>
+__device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl) {
+  const int N = S.N, K = S.n_basis, nn = S.n_nuc;
+  const int nmax = S.n_up > S.n_dn ? S.n_up : S.n_dn;
+  const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
+  double *chi = wk.scr, *gch = chi + K, *lch = gch + 3 * K, *orb = lch + K, *pt = orb + 4 * nmax;
+  double kin = 0.0, V = 0.0;
+  for (int q = 0; q < kSJNParam; ++q) dl[q] = 0.0;
+  for (int i = 0; i < N; ++i) {
+    const int s = i < S.n_up ? 0 : 1;
+    const int cc = s == 0 ? i : i - S.n_up;
+    const int n = s == 0 ? S.n_up : S.n_dn;
+    const T *B = s == 0 ? wk.Bup : wk.Bdn;
+    const double *xi = wk.pos + 3 * i;
+    // basis values and derivatives (lanes over basis functions)
+    for (int m = t; m < K; m += L) {
+      const int I = S.center[m];
+      const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
+      const double r = norm3(d0, d1, d2);
+      const double z = S.zeta[m];
+      const double ch = exp_m(-z * r);
+      const double gf = -z * ch / r;
+      chi[m] = ch;
+      gch[3 * m] = gf * d0;
+      gch[3 * m + 1] = gf * d1;
+      gch[3 * m + 2] = gf * d2;
+      lch[m] = ch * (z * z - 2.0 * z / r);
+    }
+    // Jastrow pair and nuclear terms (lanes over partners)
+    for (int j = t; j < N + nn; j += L) {
+      double *o = pt + 8 * j;
+      if (j < N) {
+        if (j == i) continue;
+        const double *xj = wk.pos + 3 * j;
+        const double dv0 = xi[0] - xj[0], dv1 = xi[1] - xj[1], dv2 = xi[2] - xj[2];
+        const double r = norm3(dv0, dv1, dv2);
+        const bool same = (j < S.n_up ? 0 : 1) == s;
+        const double b = same ? bP : bA;
+        const double den = 1.0 + c * r;
+        const double up = b / (den * den);
+        const double upp = -2.0 * b * c / (den * den * den);
+        o[0] = up * dv0 / r;
+        o[1] = up * dv1 / r;
+        o[2] = up * dv2 / r;
+        o[3] = upp + 2.0 * up / r;
+        o[4] = 1.0 / r;
+        o[5] = r / den;
+        o[6] = -b * r * r / (den * den);
+        o[7] = same ? 1.0 : 0.0;
+      } else {
+        const int I = j - N;
+        const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
+        const double r = norm3(d0, d1, d2);
+        const double Z = S.charge[I];
+        const double den = 1.0 + d * r;
+        const double aZ = a * Z;
+        const double vp = -aZ / (den * den);
+        const double vpp = 2.0 * aZ * d / (den * den * den);
+        o[0] = vp * d0 / r;
+        o[1] = vp * d1 / r;
+        o[2] = vp * d2 / r;
+        o[3] = vpp + 2.0 * vp / r;
+        o[4] = -Z / r;
+        o[5] = -Z * r / den;
+        o[6] = aZ * r * r / (den * den);
+        o[7] = 0.0;
+      }
+    }
+    __syncthreads();
+    // orbital gradients and laplacians (lanes over orbitals)
+    for (int k = t; k < n; k += L) {
+      const double *cf = S.coef + (size_t)k * K;
+      double g0 = 0.0, g1 = 0.0, g2 = 0.0, l = 0.0;
+      for (int m = 0; m < K; ++m) {
+        g0 = fma(cf[m], gch[3 * m], g0);
+        g1 = fma(cf[m], gch[3 * m + 1], g1);
+        g2 = fma(cf[m], gch[3 * m + 2], g2);
+        l = fma(cf[m], lch[m], l);
+      }
+      orb[4 * k] = g0;
+      orb[4 * k + 1] = g1;
+      orb[4 * k + 2] = g2;
+      orb[4 * k + 3] = l;
+    }
+    __syncthreads();
+    if (t == 0) {
+      double Dg0 = 0.0, Dg1 = 0.0, Dg2 = 0.0, Dl = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double bkc = (double)B[k * n + cc];
+        Dg0 = fma(orb[4 * k], bkc, Dg0);
+        Dg1 = fma(orb[4 * k + 1], bkc, Dg1);
+        Dg2 = fma(orb[4 * k + 2], bkc, Dg2);
+        Dl = fma(orb[4 * k + 3], bkc, Dl);
+      }
+      double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0, Vi = 0.0;
+      for (int j = 0; j < N; ++j) {
+        if (j == i) continue;
+        const double *o = pt + 8 * j;
+        gJ0 += o[0];
+        gJ1 += o[1];
+        gJ2 += o[2];
+        lJ += o[3];
+        if (j > i) {
+          Vi += o[4];
+          if (o[7] != 0.0)
+            dl[1] += o[5];
+          else
+            dl[0] += o[5];
+          dl[2] += o[6];
+        }
+      }
+      for (int I = 0; I < nn; ++I) {
+        const double *o = pt + 8 * (N + I);
+        gJ0 += o[0];
+        gJ1 += o[1];
+        gJ2 += o[2];
+        lJ += o[3];
+        Vi += o[4];
+        dl[3] += o[5];
+        dl[4] += o[6];
+      }
+      const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
+      const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
+      kin += Dl + 2.0 * dot + lJ + gg;
+      V += Vi;
+    }
+    __syncthreads();
+  }
+  if (t == 0 && S.hamiltonian != MOLE_H_ELECTRONIC) {
+    V = 0.0;
+    for (int i = 0; i < N; ++i)
+      for (int I = 0; I < nn; ++I) {
+        const double *xi = wk.pos + 3 * i;
+        V += -S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
+      }
+  }
+  return -0.5 * kin + V + S.v_nn;
+}
+
+}  // namespace
+
+template <typename T>
+__global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  const SJTile tile = a.tiles[blockIdx.x];
+  const SJSysDev &S = a.systems[tile.sys];
+  const int N = S.N, nu = S.n_up, nd = S.n_dn, L = S.L, W = S.W;
+  const int lane = threadIdx.x;
+  const int slot = lane / L, t = lane % L;
+  const int nslots = 64 / L;
+  const bool live = slot < tile.nw;
+  const int w = tile.w0 + slot;
+  const SJLayout lay(N, nu, nd, S.n_nuc, S.n_basis, (int)sizeof(T));
+  unsigned char *base = lds + (size_t)slot * lay.bytes;
+  SJWalker<T> wk;
+  wk.pos = (double *)(base + lay.pos);
+  wk.Bup = (T *)(base + lay.B);
+  wk.Bdn = wk.Bup + nu * nu;
+  wk.rng = (double *)(base + lay.rng);
+  wk.rn = (double *)(base + lay.rn);
+  wk.chi = (double *)(base + lay.chi);
+  wk.phin = (T *)(base + lay.phin);
+  wk.jn = (T *)(base + lay.jn);
+  wk.jo = (T *)(base + lay.jo);
+  wk.f = (T *)(base + lay.f);
+  wk.scr = (double *)(base + lay.scr);
+  wk.flag = (double *)(base + lay.flag);
+  const int nB = nu * nu + nd * nd;
+  const T *Bg = (const T *)S.B;
+
+  // ---- state in ----
+  for (int e = lane; e < 3 * N * nslots; e += 64) {
+    const int k = e / nslots, sl = e % nslots;
+    double *pp = (double *)(lds + (size_t)sl * lay.bytes + lay.pos);
+    pp[k] = (sl < tile.nw) ? S.pos[(size_t)k * W + tile.w0 + sl] : 0.0;
+  }
+  for (int e = lane; e < nB * nslots; e += 64) {
+    const int k = e / nslots, sl = e % nslots;
+    T *bb = (T *)(lds + (size_t)sl * lay.bytes + lay.B);
+    bb[k] = (sl < tile.nw) ? Bg[(size_t)k * W + tile.w0 + sl] : (T)0;
+  }
+  double acc[kSJColumns];
+  if (t == 0 && live)
+    for (int k = 0; k < kSJColumns; ++k) acc[k] = S.acc[(size_t)k * W + w];
+  else
+    for (int k = 0; k < kSJColumns; ++k) acc[k] = 0.0;
+  __syncthreads();
+
+  const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32) ^ S.sys_index;
+  if (a.init) {
+    // Sampler::new: U(-1, 1) from draws 0..3N-1 (samplers.rs:46)
+    for (int b = t; 2 * b < 3 * N; b += L) {
+      const u32x4 o = philox4x32_10({(uint32_t)b, 0u, (uint32_t)w, a.epoch}, k0, k1);
+      const uint64_t v0 = (uint64_t)o.x | ((uint64_t)o.y << 32);
+      const uint64_t v1 = (uint64_t)o.z | ((uint64_t)o.w << 32);
+      wk.pos[2 * b] = u64_range(v0, 2.0, -3.0);
+      if (2 * b + 1 < 3 * N) wk.pos[2 * b + 1] = u64_range(v1, 2.0, -3.0);
+    }
+    __syncthreads();
+  }
+  if (a.init || a.refresh) {
+    const double c0 = refresh_spin<T>(S, wk, 0, t, L);
+    const double c1 = refresh_spin<T>(S, wk, 1, t, L);
+    if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
+  }
+  const uint32_t lu_period = a.lu_period;
+
+  uint64_t p = S.draw_base + 4 * (uint64_t)N * a.sweeps_done;
+  uint64_t m = S.move_base + (uint64_t)N * a.sweeps_done;
+  uint32_t fill = a.block_fill;
+  uint32_t sidx = a.trace_sample0;
+  const double inv_block = 1.0 / (double)a.block_size;
+
+  for (uint32_t sw = 0; sw < a.n_sweeps; ++sw) {
+    // ---- the sweep's draws: [p, p + 4N), lanes over Philox blocks ----
+    {
+      const uint64_t b0 = p >> 1, b1 = (p + 4 * (uint64_t)N - 1) >> 1;
+      for (uint64_t b = b0 + t; b <= b1; b += L) {
+        const u32x4 o = philox4x32_10({(uint32_t)b, (uint32_t)(b >> 32), (uint32_t)w, a.epoch}, k0, k1);
+        const uint64_t v[2] = {(uint64_t)o.x | ((uint64_t)o.y << 32), (uint64_t)o.z | ((uint64_t)o.w << 32)};
+        for (int h = 0; h < 2; ++h) {
+          const uint64_t q = 2 * b + h;
+          if (q < p || q >= p + 4 * (uint64_t)N) continue;
+          const int r = (int)(q - p);
+          wk.rng[r] = ((r & 3) == 3) ? u64_standard(v[h]) : u64_range(v[h], a.scale, a.offset);
+        }
+      }
+    }
+    __syncthreads();
+    uint64_t mask = 0;
+    for (int i = 0; i < N; ++i) {
+      const int s = i < nu ? 0 : 1;
+      const int c = s == 0 ? i : i - nu;
+      const int n = s == 0 ? nu : nd;
+      T *B = s == 0 ? wk.Bup : wk.Bdn;
+      const double *xi = wk.pos + 3 * i;
+      const double xn[3] = {xi[0] + wk.rng[4 * i], xi[1] + wk.rng[4 * i + 1], xi[2] + wk.rng[4 * i + 2]};
+      // Jastrow terms of electron i at r' and at r_i (lanes over partners)
+      for (int j = t; j < N + S.n_nuc; j += L) {
+        if (j < N) {
+          if (j == i) continue;
+          const double b = ((j < nu ? 0 : 1) == s) ? S.p[1] : S.p[0];
+          wk.jn[j] = pair_term<T>(b, S.p[2], xn, wk.pos + 3 * j);
+          wk.jo[j] = pair_term<T>(b, S.p[2], xi, wk.pos + 3 * j);
+        } else {
+          const int I = j - N;
+          const double aZ = S.p[3] * S.charge[I];
+          wk.jn[j] = en_term<T>(aZ, S.p[4], xn, S.nuc + 3 * I);
+          wk.jo[j] = en_term<T>(aZ, S.p[4], xi, S.nuc + 3 * I);
+        }
+      }
+      basis_to_lds(S, xn, wk.rn, wk.chi, t, L);
+      for (int k = t; k < n; k += L) wk.phin[k] = (T)orbital_value(S, k, wk.chi);
+      __syncthreads();
+      if (t == 0) {
+        T rd = (T)0;
+        for (int k = 0; k < n; ++k) rd = fma_t(wk.phin[k], B[k * n + c], rd);
+        T Jn = (T)0, Jo = (T)0;
+        for (int j = 0; j < N + S.n_nuc; ++j) {
+          if (j == i) continue;
+          Jn = Jn + wk.jn[j];
+        }
+        for (int j = 0; j < N + S.n_nuc; ++j) {
+          if (j == i) continue;
+          Jo = Jo + wk.jo[j];
+        }
+        const T dJ = Jn - Jo;
+        const double rdd = (double)rd;
+        const double R2 = (rdd * rdd) * exp_m(2.0 * (double)dJ);
+        const bool accept = min_one(R2) > wk.rng[4 * i + 3];
+        wk.flag[0] = accept ? 1.0 : 0.0;
+        wk.flag[1] = (double)((T)1 / rd);
+      }
+      __syncthreads();
+      const bool accept = wk.flag[0] != 0.0;
+      const T inv = (T)wk.flag[1];
+      if (accept)
+        for (int l = t; l < n; l += L) {
+          if (l == c) continue;
+          T v = (T)0;
+          for (int k = 0; k < n; ++k) v = fma_t(wk.phin[k], B[k * n + l], v);
+          wk.f[l] = v * inv;
+        }
+      __syncthreads();
+      if (accept)
+        for (int e = t; e < n * n; e += L) {
+          const int k = e / n, l = e % n;
+          if (l == c) continue;
+          B[e] = fma_t(-B[k * n + c], wk.f[l], B[e]);
+        }
+      __syncthreads();
+      if (accept) {
+        for (int k = t; k < n; k += L) B[k * n + c] = B[k * n + c] * inv;
+        if (t == 0) {
+          wk.pos[3 * i] = xn[0];
+          wk.pos[3 * i + 1] = xn[1];
+          wk.pos[3 * i + 2] = xn[2];
+          mask |= (uint64_t)1 << i;
+          acc[kSJAccepted] += 1.0;
+        }
+      }
+      __syncthreads();
+      ++m;
+      if (lu_period && m % lu_period == 0) {
+        const double c0 = refresh_spin<T>(S, wk, 0, t, L);
+        const double c1 = refresh_spin<T>(S, wk, 1, t, L);
+        if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
+      }
+    }
+    p += 4 * (uint64_t)N;
+    if (t == 0 && live && S.trace_mask && a.trace_sweep0 + sw < a.cap_sweeps)
+      S.trace_mask[(size_t)w * a.cap_sweeps + a.trace_sweep0 + sw] = mask;
+    if (sw >= a.sample_start) {
+      double dlg[kSJNParam];
+      const double E = local_energy<T>(S, wk, t, L, dlg);
+      if (t == 0) {
+        acc[kSJSumE] += E;
+        acc[kSJSumE2] = fma(E, E, acc[kSJSumE2]);
+        acc[kSJBlockSum] += E;
+        int q = 0;
+        for (int k = 0; k < kSJNParam; ++k) {
+          acc[kSJSumO + k] += dlg[k];
+          acc[kSJSumOE + k] = fma(dlg[k], E, acc[kSJSumOE + k]);
+          for (int l = k; l < kSJNParam; ++l, ++q) acc[kSJSumOO + q] = fma(dlg[k], dlg[l], acc[kSJSumOO + q]);
+        }
+        if (live && S.trace_e && sidx < a.cap_samples) S.trace_e[(size_t)w * a.cap_samples + sidx] = E;
+      }
+      if (++fill == a.block_size) {
+        if (t == 0) {
+          const double bm = acc[kSJBlockSum] * inv_block;
+          acc[kSJSumBm] += bm;
+          acc[kSJSumBm2] = fma(bm, bm, acc[kSJSumBm2]);
+          acc[kSJBlockSum] = 0.0;
+        }
+        fill = 0;
+      }
+      ++sidx;
+    }
+  }
+
+  // ---- state out ----
+  __syncthreads();
+  for (int e = lane; e < 3 * N * nslots; e += 64) {
+    const int k = e / nslots, sl = e % nslots;
+    if (sl < tile.nw) S.pos[(size_t)k * W + tile.w0 + sl] = ((double *)(lds + (size_t)sl * lay.bytes + lay.pos))[k];
+  }
+  T *Bo = (T *)S.B;
+  for (int e = lane; e < nB * nslots; e += 64) {
+    const int k = e / nslots, sl = e % nslots;
+    if (sl < tile.nw) Bo[(size_t)k * W + tile.w0 + sl] = ((T *)(lds + (size_t)sl * lay.bytes + lay.B))[k];
+  }
+  if (t == 0 && live)
+    for (int k = 0; k < kSJColumns; ++k) S.acc[(size_t)k * W + w] = acc[k];
+}
+
+hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream) {
+  if (n_tiles <= 0) return hipSuccess;
+  if (fp32)
+    hipLaunchKernelGGL(sj_sweep_kernel<float>, dim3(n_tiles), dim3(64), lds_bytes, stream, a);
+  else
+    hipLaunchKernelGGL(sj_sweep_kernel<double>, dim3(n_tiles), dim3(64), lds_bytes, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace mole

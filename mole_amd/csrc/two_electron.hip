@@ -535,17 +535,20 @@ __global__ void eval2_kernel(Eval2Args a) {
 }
 
 // Column sums of a [K][W] array, one workgroup per column, fixed order.
+// Column max_col, if in range, is reduced with max instead of +.
 __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *acc, int W,
-                                                             double *out) {
+                                                             double *out, int max_col) {
   __shared__ double sh[256];
   const int col = blockIdx.x;
+  const bool mx = col == max_col;
   const double *p = acc + (size_t)col * W;
   double s = 0.0;
-  for (int i = threadIdx.x; i < W; i += 256) s += p[i];
+  for (int i = threadIdx.x; i < W; i += 256) s = mx ? fmax(s, p[i]) : s + p[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
-    if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    if (threadIdx.x < off)
+      sh[threadIdx.x] = mx ? fmax(sh[threadIdx.x], sh[threadIdx.x + off]) : sh[threadIdx.x] + sh[threadIdx.x + off];
     __syncthreads();
   }
   if (threadIdx.x == 0) out[col] = sh[0];
@@ -600,8 +603,9 @@ hipError_t launch_eval2(const Eval2Args &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out, hipStream_t stream) {
-  hipLaunchKernelGGL(reduce_columns_kernel, dim3(K), dim3(256), 0, stream, acc, W, out);
+hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out, int max_col,
+                                 hipStream_t stream) {
+  hipLaunchKernelGGL(reduce_columns_kernel, dim3(K), dim3(256), 0, stream, acc, W, out, max_col);
   return hipGetLastError();
 }
 

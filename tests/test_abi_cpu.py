@@ -111,3 +111,26 @@ def test_parameter_update_matches_oracle_formulas():
     with pytest.raises(M.MoleError) as ei:
         M.parameter_update(z, M.OPT_SR, 0.1, M.SR_TEXTBOOK, 0.0)
     assert ei.value.status == M.ERR_LINALG
+
+
+def test_slater_jastrow_validation_before_device():
+    from mole_amd import systems as S
+    ne = S.atom(10)
+    good = M.slater_jastrow(ne, 8)
+    # mixing families in one context
+    assert _create([good, M.helium(8)]) == M.ERR_UNSUPPORTED
+    # compat samples and the reference O convention are two-electron only
+    assert _create([good], flags=M.FLAG_SAMPLES, record_sweeps=4) == M.ERR_UNSUPPORTED
+    assert _create([good], o_convention=M.O_REFERENCE) == M.ERR_UNSUPPORTED
+    assert _create([good], precision=7) == M.ERR_INVALID_ARGUMENT
+    bad = M.slater_jastrow(ne, 8)
+    bad.desc.n_params = 3
+    assert _create([bad]) == M.ERR_SHAPE
+    bad = M.slater_jastrow(ne, 8)
+    bad.desc.n_orbitals = 4
+    assert _create([bad]) == M.ERR_SHAPE
+    bad = M.slater_jastrow(ne, 8)
+    bad.center[0] = 5
+    assert _create([bad]) == M.ERR_SHAPE
+    big = M.slater_jastrow(S.random_molecule(44, seed=1), 8)
+    assert _create([big]) == M.ERR_SHAPE

@@ -1,0 +1,121 @@
+"""Parity of the gfx950 Slater-Jastrow path (through the C ABI) with the CPU
+oracle's Sherman-Morrison runner on the same Philox streams, in FP64 and in
+mixed precision; independence of the result from lanes-per-walker;
+heterogeneous systems in one launch; zero-variance anchors."""
+import numpy as np
+import pytest
+
+import mole_amd as M
+from mole_amd import systems as S
+from oracle import sj_ffi as SJ
+
+pytestmark = pytest.mark.gpu
+
+E_TOL = 1e-10
+
+
+def bits(a):
+    return np.asarray(a, dtype=np.float64).view(np.uint64)
+
+
+def run_device(systems, W, steps, B, *, precision=M.PREC_FP64, lu=16, lanes=None, seed=0, side=0.6,
+               hamiltonian=M.H_ELECTRONIC):
+    dev = [M.slater_jastrow(s, W, hamiltonian=hamiltonian) for s in systems]
+    with M.Context(dev, seed=seed, step=side, block_size=B, flags=M.FLAG_TRACE, record_sweeps=steps,
+                   precision=precision, lu_period=lu) as ctx:
+        if lanes is not None:
+            for k in range(len(systems)):
+                ctx.set_walker_lanes(lanes, sys=k)
+        ctx.init_walkers()
+        ctx.run(steps)
+        total = (steps // B) * B
+        out = []
+        for k in range(len(systems)):
+            masks, el = ctx.trace(total, total - B, sys=k)
+            out.append(dict(masks=masks, el=el, cfg=ctx.get_configs(sys=k), stats=ctx.stats(sys=k),
+                            sr=ctx.sr_sums(sys=k), lanes=ctx.walker_lanes(sys=k)))
+    return out
+
+
+def run_oracle(system, W, steps, B, *, precision=SJ.PREC_FP64, lu=16, seed=0, side=0.6, sys=0,
+               hamiltonian=0):
+    wv = SJ.SjWave(system, hamiltonian=hamiltonian)
+    return SJ.run(wv, seed=seed, walkers=W, steps=steps, block_size=B, box_side=side,
+                  precision=precision, lu_period=lu, sys=sys, n_threads=8)
+
+
+@pytest.mark.parametrize("Z", [3, 6, 10])
+def test_fp64_parity_with_oracle(Z):
+    W, steps, B = 70, 30, 10
+    system = S.atom(Z)
+    d = run_device([system], W, steps, B)[0]
+    o = run_oracle(system, W, steps, B)
+    assert np.array_equal(d["masks"], o["accept_mask"]), "accept/reject decisions differ"
+    assert np.array_equal(bits(d["cfg"]), bits(o["cfg"]))
+    rel = np.abs(d["el"] - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+    assert np.max(rel) <= E_TOL
+    st = d["stats"]
+    assert st["n_accepted"] == o["n_accepted"] and st["n_moves"] == o["n_moves"]
+    assert abs(st["sum_e"] - o["sum_e"]) <= 1e-10 * o["n_samples"] * max(1.0, abs(o["sum_e"] / o["n_samples"]))
+    assert st["max_condition"] > 1.0
+
+
+@pytest.mark.parametrize("Z", [3, 6])
+def test_mixed_precision_parity_with_oracle(Z):
+    W, steps, B = 64, 30, 10
+    system = S.atom(Z)
+    d = run_device([system], W, steps, B, precision=M.PREC_MIXED, lu=64)[0]
+    o = run_oracle(system, W, steps, B, precision=SJ.PREC_MIXED, lu=64)
+    assert np.array_equal(d["masks"], o["accept_mask"])
+    assert np.array_equal(bits(d["cfg"]), bits(o["cfg"]))
+    rel = np.abs(d["el"] - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+    assert np.max(rel) <= E_TOL
+
+
+def test_results_independent_of_lanes_per_walker():
+    system = S.atom(7)
+    ref = None
+    for L in (2, 4, 8, 16, 32, 64):
+        d = run_device([system], 40, 20, 5, lanes=L)[0]
+        assert d["lanes"] == L
+        if ref is None:
+            ref = d
+            continue
+        assert np.array_equal(d["masks"], ref["masks"]), L
+        assert np.array_equal(bits(d["cfg"]), bits(ref["cfg"])), L
+        assert np.array_equal(bits(d["el"]), bits(ref["el"])), L
+        for k in ("sum_o", "sum_oe", "sum_oo"):
+            assert np.array_equal(d["sr"][k], ref["sr"][k]), (L, k)
+
+
+def test_heterogeneous_systems_in_one_launch():
+    systems = [S.helium_product(1.7, 0.5), S.atom(4), S.hydrogen_chain(4), S.random_molecule(9, seed=2),
+               S.atom(10)]
+    W, steps, B = 33, 20, 5
+    dev = run_device(systems, W, steps, B)
+    for k, system in enumerate(systems):
+        o = run_oracle(system, W, steps, B, sys=k)
+        assert np.array_equal(dev[k]["masks"], o["accept_mask"]), k
+        assert np.array_equal(bits(dev[k]["cfg"]), bits(o["cfg"])), k
+        rel = np.abs(dev[k]["el"] - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+        assert np.max(rel) <= E_TOL, k
+
+
+def test_zero_variance_anchors():
+    d = run_device([S.hydrogenic(1)], 128, 20, 5)[0]
+    assert np.max(np.abs(d["el"] + 0.5)) <= 1e-12
+    d = run_device([S.helium_product(2.0, 0.0)], 128, 20, 5, hamiltonian=M.H_IONIC)[0]
+    assert np.max(np.abs(d["el"] + 4.0)) <= 1e-12
+
+
+def test_helium_product_anchor_through_determinants():
+    zeta = 1.5
+    sysd = M.slater_jastrow(S.helium_product(zeta, 0.0), 4096)
+    with M.Context([sysd], seed=9, step=1.0, block_size=50, lu_period=32) as ctx:
+        ctx.init_walkers()
+        ctx.run(1000)
+        st = ctx.stats()
+    mean = st["sum_e"] / st["n_samples"]
+    nb = st["n_blocks"]
+    err = np.sqrt((st["sum_block_mean2"] / nb - (st["sum_block_mean"] / nb) ** 2) / (nb - 1))
+    assert abs(mean - (zeta**2 - 27 * zeta / 8)) < 4 * err
