@@ -131,8 +131,37 @@ def cpu_baseline(threads, budget_s=20.0):
                        f"{t1:.1f} s; host {platform.processor() or platform.machine()}")
 
 
+def cpu_atom_sample(threads, walkers=1024, sweeps=40, block=10):
+    """Reference-faithful oracle (two full LU evaluations per move, fresh-inverse
+    local energy) on a bounded sample of an atom's workload."""
+    from oracle import sj_ffi as SJ
+
+    def sample(sj):
+        wv = SJ.SjWave(sj)
+        t0 = time.perf_counter()
+        SJ.run(wv, seed=1, walkers=walkers, steps=sweeps, block_size=block, box_side=0.6,
+               faithful=True, n_threads=threads, record=False)
+        dt = time.perf_counter() - t0
+        moves = walkers * sweeps * sj.n_electrons
+        return dict(value=moves / dt, unit="walker-moves/s", cores=threads, kind="port",
+                    sample=f"{walkers} walkers x {sweeps} sweeps, oracle reference-faithful mode, {dt:.2f} s")
+    return sample
+
+
+def atoms_main(args):
+    from mole_amd import workloads
+
+    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    cpu = None if args.no_cpu_baseline else cpu_atom_sample(threads)
+    workloads.atoms(walkers=args.walkers or 65536, sweeps=args.sweeps or 1000, cpu_sample=cpu,
+                    log=lambda r: print(json.dumps(r), flush=True))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["h2-sr", "atoms"], default="h2-sr")
+    ap.add_argument("--walkers", type=int, default=0)
+    ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50, help="timed SR iterations")
     ap.add_argument("--warmup", type=int, default=3)
@@ -144,6 +173,9 @@ def main():
         print("bench.py runs on exactly one GPU: walkers are independent Markov chains and this "
               "build measures one GPU's share; multi-GPU sharding is not built.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "atoms":
+        atoms_main(args)
+        return
 
     r = gpu_run(args.steps, args.warmup, variant=args.variant)
     moves = args.steps * WALKERS * SWEEPS * N_ELEC

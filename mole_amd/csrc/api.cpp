@@ -512,7 +512,10 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
   {
     hipDeviceProp_t prop;
     CTX_TRY(hipGetDeviceProperties(&prop, ctx->device));
-    ctx->lds_limit = std::min<size_t>(prop.sharedMemPerBlock, 65536);
+    // opt-in LDS per workgroup (160 KiB on gfx950), granted to the
+    // Slater-Jastrow kernel through hipFuncAttributeMaxDynamicSharedMemorySize
+    ctx->lds_limit = std::max<size_t>(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+    if (ctx->lds_limit == 0) ctx->lds_limit = 65536;
   }
   CTX_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CTX_TRY(hipEventCreate(&ctx->ev0));

@@ -26,7 +26,7 @@ namespace mole {
 
 // LDS layout of one walker (bytes); identical on host and device.
 struct SJLayout {
-  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, bytes;
+  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, acc, bytes;
   __host__ __device__ SJLayout(int N, int nu, int nd, int nn, int K, int tsize) {
     const int nmax = nu > nd ? nu : nd;
     uint32_t o = 0;
@@ -48,6 +48,7 @@ struct SJLayout {
     const uint32_t el = 8u * (5u * K + 4u * nmax + 8u * (N + nn));
     scr = take(lu > el ? lu : el);
     flag = take(16u);
+    acc = take(8u * kSJColumns);
     bytes = o;
   }
 };
@@ -380,11 +381,10 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
     T *bb = (T *)(lds + (size_t)sl * lay.bytes + lay.B);
     bb[k] = (sl < tile.nw) ? Bg[(size_t)k * W + tile.w0 + sl] : (T)0;
   }
-  double acc[kSJColumns];
-  if (t == 0 && live)
-    for (int k = 0; k < kSJColumns; ++k) acc[k] = S.acc[(size_t)k * W + w];
-  else
-    for (int k = 0; k < kSJColumns; ++k) acc[k] = 0.0;
+  // accumulators: LDS, touched by sub-lane 0 only
+  double *acc = (double *)(base + lay.acc);
+  if (t == 0)
+    for (int k = 0; k < kSJColumns; ++k) acc[k] = live ? S.acc[(size_t)k * W + w] : 0.0;
   __syncthreads();
 
   const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32) ^ S.sys_index;
@@ -556,6 +556,12 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
 
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream) {
   if (n_tiles <= 0) return hipSuccess;
+  if (lds_bytes > 65536) {
+    // more than the default 64 KiB of dynamic LDS needs an opt-in per kernel
+    const void *fn = fp32 ? (const void *)sj_sweep_kernel<float> : (const void *)sj_sweep_kernel<double>;
+    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
   if (fp32)
     hipLaunchKernelGGL(sj_sweep_kernel<float>, dim3(n_tiles), dim3(64), lds_bytes, stream, a);
   else

@@ -1,0 +1,85 @@
+"""Drivers of the benchmark workloads beyond the default bench line.
+
+atoms():   Li..Ne Slater-Jastrow atoms, 65,536 walkers each, Sherman-Morrison
+           in FP32 with an FP64 Gauss-Jordan refresh every 64 moves, against
+           an all-FP64 run on the same Philox streams.
+hetero():  256 systems of 2..40 electrons (hydrogen chains and Poisson-disk
+           molecules), walkers proportional to electron count, one launch per
+           call, autotuned lanes per walker, sweeps replayed from a HIP graph.
+"""
+import time
+
+import numpy as np
+
+import mole_amd as M
+from mole_amd import autotune, systems
+
+
+def energy_and_error(st):
+    n = st["n_samples"]
+    mean = st["sum_e"] / n
+    nb = st["n_blocks"]
+    err = float(np.sqrt(max(st["sum_block_mean2"] / nb - (st["sum_block_mean"] / nb) ** 2, 0.0) / (nb - 1)))
+    return mean, err
+
+
+def run_atom(sj, walkers, sweeps, block, precision, lu_period, lanes, seed):
+    dev = M.slater_jastrow(sj, walkers)
+    with M.Context([dev], seed=seed, step=0.6, block_size=block, precision=precision,
+                   lu_period=lu_period) as ctx:
+        if lanes:
+            ctx.set_walker_lanes(lanes)
+        ctx.init_walkers()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(sweeps)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        kernel_ms, launches = ctx.last_timing()
+        st = ctx.stats()
+        lanes = ctx.walker_lanes()
+    e, err = energy_and_error(st)
+    moves = walkers * ((sweeps // block) * block) * sj.n_electrons
+    return dict(energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
+                max_condition=st["max_condition"], seconds=wall, kernel_ms=kernel_ms,
+                moves_per_s=moves / wall, lanes=lanes)
+
+
+def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, seed=1, tune=True,
+          cpu_sample=None, log=print):
+    out = []
+    for Z in Zs:
+        sj = systems.atom(Z)
+        L = 0
+        if tune:
+            table, _ = autotune.tune([sj], lambda s: min(walkers, 16384), candidates=(1, 2, 4, 8, 16),
+                                     sweeps=4, precision=M.PREC_MIXED, lu_period=lu_period)
+            L = table[sj.n_electrons]
+        mixed = run_atom(sj, walkers, sweeps, block, M.PREC_MIXED, lu_period, L, seed)
+        fp64 = run_atom(sj, walkers, sweeps, block, M.PREC_FP64, lu_period, L, seed)
+        diff = mixed["energy"] - fp64["energy"]
+        sig = float(np.hypot(mixed["error"], fp64["error"]))
+        rec = dict(workload="atom", Z=Z, element=sj.name, n_up=sj.n_up, n_dn=sj.n_dn,
+                   walkers=walkers, sweeps=sweeps, block=block, lu_period=lu_period,
+                   mixed=mixed, fp64=fp64, mixed_minus_fp64=diff, sigma=sig,
+                   within_1sigma=bool(abs(diff) <= sig),
+                   fp32_trusted=bool(fp64["max_condition"] < 1e5))
+        if cpu_sample:
+            rec["cpu_baseline"] = cpu_sample(sj)
+        log(rec)
+        out.append(rec)
+    return out
+
+
+def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2):
+    """Hydrogen chains and Poisson-disk random molecules with 2..40 electrons."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for k in range(n_systems):
+        N = int(rng.integers(2, max_electrons + 1))
+        if k % 4 == 0:
+            out.append(systems.hydrogen_chain(N, spacing=1.4, basis_per_nucleus=basis_per_nucleus, seed=seed + k))
+        else:
+            out.append(systems.random_molecule(N, seed=seed * 1000 + k, min_sep=1.2,
+                                               basis_per_nucleus=basis_per_nucleus))
+    return out
