@@ -157,9 +157,20 @@ def atoms_main(args):
                     log=lambda r: print(json.dumps(r), flush=True))
 
 
+def hetero_main(args):
+    from mole_amd import workloads
+
+    def log(r):
+        per = r.pop("per_system")
+        print(json.dumps(r), flush=True)
+        print(json.dumps({"workload": "hetero-per-system", "systems": per}), flush=True)
+
+    workloads.hetero(sweeps=args.sweeps or 100, total_walkers=args.walkers or (1 << 20), log=log)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["h2-sr", "atoms"], default="h2-sr")
+    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero"], default="h2-sr")
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +186,9 @@ def main():
         sys.exit(2)
     if args.workload == "atoms":
         atoms_main(args)
+        return
+    if args.workload == "hetero":
+        hetero_main(args)
         return
 
     r = gpu_run(args.steps, args.warmup, variant=args.variant)

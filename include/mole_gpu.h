@@ -268,6 +268,23 @@ mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t blo
  * on it; it only trades per-move parallelism against walkers per wave. */
 mole_status mole_set_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t lanes);
 mole_status mole_get_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t *lanes);
+/* Slater-Jastrow launch shape: workgroups per launch and dynamic LDS bytes
+ * per workgroup (the largest tile footprint of the context). */
+mole_status mole_get_launch_info(mole_ctx *ctx, uint32_t *workgroups, size_t *lds_bytes);
+
+/* ---- captured sweeps ---------------------------------------------------
+ * No counterpart in the reference (its sweeps are a host loop,
+ * src/montecarlo/src/montecarlo.rs:32-33).  Captures one launch of n_sweeps
+ * sweeps of every system of a Slater-Jastrow context (sampled with
+ * MOLE_SWEEP_SAMPLE), plus the advance of the context's device-resident
+ * stream/block state, into a HIP graph.  mole_graph_launch replays it:
+ * `replays` replays are exactly `replays` calls of mole_sweep(ctx, n_sweeps,
+ * flags).  Parameter changes are seen by replays; a change of lanes per
+ * walker makes the graph stale (MOLE_ERR_INVALID_ARGUMENT on launch). */
+typedef struct mole_graph mole_graph;
+mole_status mole_graph_capture(mole_ctx *ctx, uint32_t n_sweeps, uint32_t flags, mole_graph **out);
+mole_status mole_graph_launch(mole_ctx *ctx, mole_graph *graph, uint32_t replays);
+void mole_graph_destroy(mole_graph *graph);
 
 /* Waits for all work of the context. */
 mole_status mole_synchronize(mole_ctx *ctx);

@@ -152,6 +152,10 @@ _SIGS = [
     ("mole_set_launch_config", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_set_walker_lanes", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_get_walker_lanes", C.c_int32, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    ("mole_get_launch_info", C.c_int32, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_size_t)]),
+    ("mole_graph_capture", C.c_int32, [_P, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
+    ("mole_graph_launch", C.c_int32, [_P, _P, C.c_uint32]),
+    ("mole_graph_destroy", None, [_P]),
     ("mole_synchronize", C.c_int32, [_P]),
     ("mole_last_timing", C.c_int32, [_P, _pd, C.POINTER(C.c_uint32)]),
     ("mole_compute_parameter_update", C.c_int32, [C.POINTER(SrSums), C.c_uint32, C.c_double, C.c_uint32, C.c_double, _pd, _pd]),
@@ -381,6 +385,17 @@ class Context:
         check(lib().mole_get_walker_lanes(self.h, sys, C.byref(v)))
         return v.value
 
+    def launch_info(self):
+        wg, lds = C.c_uint32(), C.c_size_t()
+        check(lib().mole_get_launch_info(self.h, C.byref(wg), C.byref(lds)))
+        return wg.value, lds.value
+
+    def capture(self, n_sweeps, sample=False):
+        """HIP graph of one launch of n_sweeps sweeps (see mole_graph_capture)."""
+        g = C.c_void_p()
+        check(lib().mole_graph_capture(self.h, n_sweeps, SWEEP_SAMPLE if sample else 0, C.byref(g)))
+        return Graph(self, g)
+
     def synchronize(self):
         check(lib().mole_synchronize(self.h))
 
@@ -388,6 +403,26 @@ class Context:
         ms, n = C.c_double(), C.c_uint32()
         check(lib().mole_last_timing(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+
+class Graph:
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.h = handle
+
+    def launch(self, replays=1):
+        check(lib().mole_graph_launch(self.ctx.h, self.h, replays))
+
+    def close(self):
+        if self.h:
+            lib().mole_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def device_info():

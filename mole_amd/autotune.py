@@ -15,9 +15,10 @@ CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
 
 
 def time_lanes(system, walkers, lanes, *, sweeps=6, precision=M.PREC_MIXED, lu_period=64, seed=7,
-               sample=True):
+               sample=True, lds_budget=None):
     """Device ms per sweep of `walkers` walkers of one system at L = lanes
-    (None if the walker state does not fit the LDS budget at that L)."""
+    (None if the walker state does not fit the device, or lds_budget, at
+    that L)."""
     dev = M.slater_jastrow(system, walkers)
     with M.Context([dev], seed=seed, step=0.6, block_size=2, precision=precision,
                    lu_period=lu_period) as ctx:
@@ -27,6 +28,8 @@ def time_lanes(system, walkers, lanes, *, sweeps=6, precision=M.PREC_MIXED, lu_p
             if e.status == M.ERR_UNSUPPORTED:
                 return None
             raise
+        if lds_budget is not None and ctx.launch_info()[1] > lds_budget:
+            return None
         ctx.init_walkers()
         ctx.sweep(2, sample=sample)  # warm
         ctx.synchronize()
@@ -36,9 +39,11 @@ def time_lanes(system, walkers, lanes, *, sweeps=6, precision=M.PREC_MIXED, lu_p
 
 
 def tune(systems, walkers_of, *, candidates=CANDIDATES, sweeps=6, precision=M.PREC_MIXED,
-         lu_period=64, verbose=False):
+         lu_period=64, lds_budget=None, verbose=False):
     """Best L per electron count.  walkers_of(system) gives the walker count
-    to time with (capped so a probe stays short)."""
+    to time with; lds_budget caps the LDS of a workgroup (one launch shares
+    the largest footprint of its tiles, so a budget keeps large systems from
+    lowering the occupancy of small ones)."""
     table = {}
     timings = {}
     reps = {}
@@ -48,10 +53,14 @@ def tune(systems, walkers_of, *, candidates=CANDIDATES, sweeps=6, precision=M.PR
         W = walkers_of(s)
         best, best_ms = None, None
         for L in candidates:
-            ms = time_lanes(s, W, L, sweeps=sweeps, precision=precision, lu_period=lu_period)
+            ms = time_lanes(s, W, L, sweeps=sweeps, precision=precision, lu_period=lu_period,
+                            lds_budget=lds_budget)
             timings[f"{N}:{L}"] = ms
             if ms is not None and (best_ms is None or ms < best_ms):
                 best, best_ms = L, ms
+        if best is None:  # nothing under the budget: the smallest footprint
+            best = max(c for c in candidates if time_lanes(s, W, c, sweeps=1, precision=precision,
+                                                           lu_period=lu_period) is not None)
         table[N] = best
         if verbose:
             print(f"N={N:2d}: L={best} ({best_ms:.3f} ms/sweep for {W} walkers)", flush=True)

@@ -102,6 +102,8 @@ struct mole_ctx {
   uint32_t max_sweeps_per_launch = 4096;
   // Slater-Jastrow launch state
   SJSysDev *d_sysdev = nullptr;
+  SJLaunchState *d_state = nullptr;
+  uint32_t generation = 0;  // bumps when tiles/launch shape change (graphs go stale)
   SJTile *d_tiles = nullptr;
   int n_tiles = 0;
   size_t lds_bytes = 0, lds_limit = 65536;
@@ -221,6 +223,7 @@ void free_ctx(mole_ctx *ctx) {
   }
   if (ctx->d_red) (void)hipFree(ctx->d_red);
   if (ctx->d_sysdev) (void)hipFree(ctx->d_sysdev);
+  if (ctx->d_state) (void)hipFree(ctx->d_state);
   if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -312,14 +315,49 @@ SJArgs sj_args(const mole_ctx *ctx) {
   a.scale = hi - lo;
   a.offset = lo - a.scale;
   a.block_size = ctx->run.block_size;
-  a.block_fill = ctx->block_fill;
   a.lu_period = ctx->run.lu_period;
-  a.sweeps_done = ctx->sweeps_done;
-  a.trace_sweep0 = ctx->trace_sweeps;
-  a.trace_sample0 = ctx->trace_samples;
+  a.state = ctx->d_state;
   a.cap_sweeps = (ctx->run.flags & MOLE_FLAG_TRACE) ? ctx->run.record_sweeps : 0;
   a.cap_samples = a.cap_sweeps;
   return a;
+}
+
+// Writes the host's view of the launch state to the device copy.
+mole_status sj_push_state(mole_ctx *ctx) {
+  SJLaunchState v{};
+  v.sweeps_done = ctx->sweeps_done;
+  v.block_fill = ctx->block_fill;
+  v.trace_sweeps = ctx->trace_sweeps;
+  v.trace_samples = ctx->trace_samples;
+  HIP_TRY(launch_sj_state_set(ctx->d_state, v, ctx->stream));
+  return MOLE_OK;
+}
+
+// Enqueues one sweep launch of n sweeps (sampled from sweep ss on) and the
+// state advance; shared by eager calls and graph capture.
+mole_status sj_enqueue(mole_ctx *ctx, uint32_t n, uint32_t ss) {
+  SJArgs a = sj_args(ctx);
+  a.n_sweeps = n;
+  a.sample_start = ss;
+  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, ctx->lds_bytes, ctx->stream));
+  HIP_TRY(launch_sj_advance(ctx->d_state, n, n - std::min(ss, n), ctx->run.block_size, ctx->stream));
+  return MOLE_OK;
+}
+
+// Host bookkeeping of n sweeps with n_samp sampled, for every system.
+void account(mole_ctx *ctx, uint32_t chunk, uint32_t n_samp) {
+  for (SysState &st : ctx->sys) {
+    st.cols_valid = false;
+    const uint64_t filled = (uint64_t)ctx->block_fill + n_samp;
+    st.n_samples += (uint64_t)st.W * n_samp;
+    st.n_blocks += (uint64_t)st.W * (filled / ctx->run.block_size);
+    st.n_moves += (uint64_t)st.W * st.N * chunk;
+  }
+  if (ctx->family == kFamTwo) ctx->position += (uint64_t)chunk * ctx->sys[0].N * 4;
+  else ctx->sweeps_done += chunk;
+  ctx->block_fill = (uint32_t)(((uint64_t)ctx->block_fill + n_samp) % ctx->run.block_size);
+  ctx->trace_sweeps += chunk;
+  ctx->trace_samples += n_samp;
 }
 
 // A zero-sweep launch that draws the initial configurations (init) or
@@ -358,12 +396,9 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
     const uint32_t ss = (sample_start > done) ? std::min(sample_start - done, chunk) : 0;
     const uint32_t n_samp = chunk - ss;
     if (ctx->family == kFamSJ) {
-      SJArgs a = sj_args(ctx);
-      a.n_sweeps = chunk;
-      a.sample_start = ss;
-      HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, ctx->lds_bytes, ctx->stream));
+      mole_status es = sj_enqueue(ctx, chunk, ss);
+      if (es != MOLE_OK) return es;
       ++launches;
-      ctx->sweeps_done += chunk;
     }
     for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
       SysState &st = ctx->sys[s];
@@ -395,16 +430,8 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
         HIP_TRY(launch_sweep2(a, ctx->variant, ctx->block_threads, ctx->stream));
         ++launches;
       }
-      st.cols_valid = false;
-      const uint64_t filled = (uint64_t)ctx->block_fill + n_samp;
-      st.n_samples += (uint64_t)st.W * n_samp;
-      st.n_blocks += (uint64_t)st.W * (filled / ctx->run.block_size);
-      st.n_moves += (uint64_t)st.W * st.N * chunk;
     }
-    if (ctx->family == kFamTwo) ctx->position += (uint64_t)chunk * ctx->sys[0].N * 4;
-    ctx->block_fill = (uint32_t)(((uint64_t)ctx->block_fill + n_samp) % ctx->run.block_size);
-    ctx->trace_sweeps += chunk;
-    ctx->trace_samples += n_samp;
+    account(ctx, chunk, n_samp);
     done += chunk;
   }
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -563,6 +590,8 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
   }
   if (ctx->family == kFamSJ) {
     CTX_TRY(hipMalloc(&ctx->d_sysdev, sizeof(SJSysDev) * n_sys));
+    CTX_TRY(hipMalloc(&ctx->d_state, sizeof(SJLaunchState)));
+    if (sj_push_state(ctx) != MOLE_OK) return bail(MOLE_ERR_DEVICE);
     mole_status ts = sj_build_tiles(ctx);
     if (ts != MOLE_OK) return bail(ts);
   }
@@ -597,6 +626,8 @@ mole_status mole_init_walkers(mole_ctx *ctx) {
   }
   ctx->sweeps_done = 0;
   ctx->sysdev_dirty = true;
+  st = sj_push_state(ctx);
+  if (st != MOLE_OK) return st;
   st = sj_prepare(ctx, true);
   return st == MOLE_OK ? ok() : st;
 }
@@ -666,6 +697,9 @@ mole_status mole_set_stream(mole_ctx *ctx, uint32_t epoch, uint64_t position) {
     }
     ctx->sweeps_done = 0;
     ctx->sysdev_dirty = true;
+    HIP_TRY(hipSetDevice(ctx->device));
+    st = sj_push_state(ctx);
+    if (st != MOLE_OK) return st;
   }
   return ok();
 }
@@ -729,6 +763,10 @@ mole_status mole_run(mole_ctx *ctx, uint32_t steps, uint32_t block_size) {
       HIP_TRY(launch_fill(s.d_acc + (size_t)col * s.W, (size_t)s.W, 0.0, ctx->stream));
     }
     ctx->block_fill = 0;
+    if (ctx->family == kFamSJ) {
+      st = sj_push_state(ctx);
+      if (st != MOLE_OK) return st;
+    }
   }
   const uint32_t blocks = steps / block_size;
   return do_sweeps(ctx, blocks * block_size, block_size);
@@ -745,6 +783,10 @@ mole_status mole_reset_stats(mole_ctx *ctx) {
   }
   ctx->block_fill = 0;
   ctx->trace_sweeps = ctx->trace_samples = 0;
+  if (ctx->family == kFamSJ) {
+    st = sj_push_state(ctx);
+    if (st != MOLE_OK) return st;
+  }
   return ok();
 }
 
@@ -980,6 +1022,7 @@ mole_status mole_set_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t lanes) {
     return st;
   }
   ctx->sysdev_dirty = true;
+  ++ctx->generation;
   return ok();
 }
 
@@ -988,6 +1031,89 @@ mole_status mole_get_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t *lanes) 
   if (st != MOLE_OK) return st;
   if (!lanes) return fail(MOLE_ERR_INVALID_ARGUMENT, "null output");
   *lanes = (uint32_t)ctx->sys[sys].L;
+  return ok();
+}
+
+struct mole_graph {
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  uint32_t n_sweeps = 0, n_samp = 0, generation = 0;
+  const mole_ctx *ctx = nullptr;
+};
+
+void mole_graph_destroy(mole_graph *g) {
+  if (!g) return;
+  if (g->exec) (void)hipGraphExecDestroy(g->exec);
+  if (g->graph) (void)hipGraphDestroy(g->graph);
+  delete g;
+}
+
+mole_status mole_graph_capture(mole_ctx *ctx, uint32_t n_sweeps, uint32_t flags, mole_graph **out) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (!out) return fail(MOLE_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "graph capture covers Slater-Jastrow contexts");
+  if (n_sweeps == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "n_sweeps must be >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->sysdev_dirty) {
+    st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  const uint32_t ss = (flags & MOLE_SWEEP_SAMPLE) ? 0u : n_sweeps;
+  mole_graph *g = new mole_graph();
+  g->n_sweeps = n_sweeps;
+  g->n_samp = n_sweeps - ss;
+  g->generation = ctx->generation;
+  g->ctx = ctx;
+  hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
+    delete g;
+    return fail(MOLE_ERR_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+  }
+  st = sj_enqueue(ctx, n_sweeps, ss);
+  e = hipStreamEndCapture(ctx->stream, &g->graph);
+  if (st != MOLE_OK || e != hipSuccess) {
+    mole_graph_destroy(g);
+    return st != MOLE_OK ? st : fail(MOLE_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+  }
+  e = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+  if (e != hipSuccess) {
+    mole_graph_destroy(g);
+    return fail(MOLE_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+  }
+  *out = g;
+  return ok();
+}
+
+mole_status mole_graph_launch(mole_ctx *ctx, mole_graph *g, uint32_t replays) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (!g || g->ctx != ctx) return fail(MOLE_ERR_INVALID_ARGUMENT, "graph does not belong to this context");
+  if (g->generation != ctx->generation)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "graph is stale: the launch shape changed after capture");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->sysdev_dirty) {
+    st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  for (uint32_t r = 0; r < replays; ++r) {
+    HIP_TRY(hipGraphLaunch(g->exec, ctx->stream));
+    account(ctx, g->n_sweeps, g->n_samp);
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->last_launches = replays;
+  ctx->timed = true;
+  return ok();
+}
+
+mole_status mole_get_launch_info(mole_ctx *ctx, uint32_t *workgroups, size_t *lds_bytes) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "launch shape applies to Slater-Jastrow contexts");
+  if (workgroups) *workgroups = (uint32_t)ctx->n_tiles;
+  if (lds_bytes) *lds_bytes = ctx->lds_bytes;
   return ok();
 }
 

@@ -55,26 +55,44 @@ struct SJTile {
   int sys, w0, nw;
 };
 
+// Launch state shared by every walker of a context, in device memory so a
+// captured sequence [sweep kernel, advance kernel] can be replayed: the
+// sweep kernel reads it, the advance kernel moves it on.
+struct SJLaunchState {
+  uint64_t sweeps_done;   // sweeps since the stream origin
+  uint32_t block_fill;    // samples in the block in progress
+  uint32_t trace_sweeps;  // trace cursors since the last reset
+  uint32_t trace_samples;
+  uint32_t pad;
+};
+
 struct SJArgs {
+  SJLaunchState *state;
   const SJSysDev *systems;
   const SJTile *tiles;
   uint64_t seed;
   uint32_t epoch;
   double scale, offset;    // Range(-side/2, side/2)
-  uint64_t sweeps_done;    // sweeps since the stream origin: draws start at
-                           // draw_base + 4N sweeps_done, moves at
-                           // move_base + N sweeps_done
   uint32_t n_sweeps;
   uint32_t sample_start;   // sweeps s >= sample_start are sampled
-  uint32_t block_size, block_fill;
+  uint32_t block_size;
   uint32_t lu_period;      // 0: never
-  uint32_t trace_sweep0, trace_sample0, cap_sweeps, cap_samples;
+  uint32_t cap_sweeps, cap_samples;
   int init;                // 1: draw U(-1,1) configurations first (draws 0..3N-1)
   int refresh;             // 1: rebuild the inverses from the positions at entry
 };
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);
 
+// Draws start at draw_base + 4N sweeps_done and moves at move_base +
+// N sweeps_done for every system; sample s of a launch is sampled when
+// s >= sample_start.
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream);
+// state->sweeps_done += n_sweeps; block fill and trace cursors advance by the
+// launch's samples.
+hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n_samples,
+                             uint32_t block_size, hipStream_t stream);
+// *state = v, by a kernel (no memset nodes if the stream is being captured)
+hipError_t launch_sj_state_set(SJLaunchState *state, const SJLaunchState &v, hipStream_t stream);
 
 }  // namespace mole

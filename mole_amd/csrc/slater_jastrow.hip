@@ -406,10 +406,11 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
   }
   const uint32_t lu_period = a.lu_period;
 
-  uint64_t p = S.draw_base + 4 * (uint64_t)N * a.sweeps_done;
-  uint64_t m = S.move_base + (uint64_t)N * a.sweeps_done;
-  uint32_t fill = a.block_fill;
-  uint32_t sidx = a.trace_sample0;
+  const SJLaunchState ls = *a.state;
+  uint64_t p = S.draw_base + 4 * (uint64_t)N * ls.sweeps_done;
+  uint64_t m = S.move_base + (uint64_t)N * ls.sweeps_done;
+  uint32_t fill = ls.block_fill;
+  uint32_t sidx = ls.trace_samples;
   const double inv_block = 1.0 / (double)a.block_size;
 
   for (uint32_t sw = 0; sw < a.n_sweeps; ++sw) {
@@ -509,8 +510,8 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
       }
     }
     p += 4 * (uint64_t)N;
-    if (t == 0 && live && S.trace_mask && a.trace_sweep0 + sw < a.cap_sweeps)
-      S.trace_mask[(size_t)w * a.cap_sweeps + a.trace_sweep0 + sw] = mask;
+    if (t == 0 && live && S.trace_mask && ls.trace_sweeps + sw < a.cap_sweeps)
+      S.trace_mask[(size_t)w * a.cap_sweeps + ls.trace_sweeps + sw] = mask;
     if (sw >= a.sample_start) {
       double dlg[kSJNParam];
       const double E = local_energy<T>(S, wk, t, L, dlg);
@@ -552,6 +553,31 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
   }
   if (t == 0 && live)
     for (int k = 0; k < kSJColumns; ++k) S.acc[(size_t)k * W + w] = acc[k];
+}
+
+__global__ void sj_advance_kernel(SJLaunchState *st, uint32_t n_sweeps, uint32_t n_samples,
+                                  uint32_t block_size) {
+  if (threadIdx.x != 0) return;
+  st->sweeps_done += n_sweeps;
+  st->block_fill = (uint32_t)(((uint64_t)st->block_fill + n_samples) % block_size);
+  st->trace_sweeps += n_sweeps;
+  st->trace_samples += n_samples;
+}
+
+__global__ void sj_state_set_kernel(SJLaunchState *st, SJLaunchState v) {
+  if (threadIdx.x == 0) *st = v;
+}
+
+hipError_t launch_sj_state_set(SJLaunchState *state, const SJLaunchState &v, hipStream_t stream) {
+  hipLaunchKernelGGL(sj_state_set_kernel, dim3(1), dim3(64), 0, stream, state, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n_samples,
+                             uint32_t block_size, hipStream_t stream) {
+  hipLaunchKernelGGL(sj_advance_kernel, dim3(1), dim3(64), 0, stream, state, n_sweeps, n_samples,
+                     block_size);
+  return hipGetLastError();
 }
 
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream) {

@@ -71,6 +71,69 @@ def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, s
     return out
 
 
+def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, chunk=10, block=10,
+           precision=M.PREC_MIXED, lu_period=64, seed=3, tune=True, table=None, lds_budget=20 << 10,
+           log=print):
+    """All systems in one context: one launch per `chunk` sweeps (every
+    system's tiles, most expensive first), replayed from a HIP graph."""
+    sys_list = hetero_systems(n_systems, seed)
+    Ns = np.array([s.n_electrons for s in sys_list])
+    W = np.maximum(1, np.floor(Ns / Ns.sum() * total_walkers)).astype(int)
+    W[np.argmax(Ns)] += total_walkers - W.sum()
+    timings = {}
+    if table is None:
+        table = {}
+        if tune:
+            table, timings = autotune.tune(sys_list, lambda s: 16384, sweeps=4, precision=precision,
+                                           lu_period=lu_period, lds_budget=lds_budget)
+    dev = [M.slater_jastrow(s, int(w)) for s, w in zip(sys_list, W)]
+    ctx = M.Context(dev, seed=seed, step=0.6, block_size=block, precision=precision, lu_period=lu_period)
+    autotune.apply(ctx, sys_list, table)
+    t0 = time.perf_counter()
+    ctx.init_walkers()
+    ctx.synchronize()
+    t_init = time.perf_counter() - t0
+    geq = ctx.capture(chunk, sample=False)
+    geq.launch(max(1, equil_sweeps // chunk))
+    ctx.synchronize()
+    ctx.reset_stats()
+    g = ctx.capture(chunk, sample=True)
+    reps = max(1, sweeps // chunk)
+    t0 = time.perf_counter()
+    g.launch(reps)
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    graph_ms, _ = ctx.last_timing()
+    done = reps * chunk
+    moves = int(np.sum(W * Ns)) * done
+    n_wg, lds = ctx.launch_info()
+    per_sys = []
+    for k, s in enumerate(sys_list):
+        st = ctx.stats(k)
+        e, err = energy_and_error(st)
+        per_sys.append(dict(name=s.name, N=int(Ns[k]), walkers=int(W[k]), lanes=ctx.walker_lanes(k),
+                            energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
+                            max_condition=st["max_condition"]))
+    # the same sweeps launched eagerly, for the graph's overhead saving
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.sweep(chunk, sample=True)
+    ctx.synchronize()
+    wall_eager = time.perf_counter() - t0
+    g.close()
+    geq.close()
+    ctx.close()
+    rec = dict(workload="hetero", systems=n_systems, walkers=int(W.sum()),
+               electrons_min=int(Ns.min()), electrons_max=int(Ns.max()), sweeps=done, chunk=chunk,
+               precision="mixed" if precision == M.PREC_MIXED else "fp64", lu_period=lu_period,
+               seconds=wall, graph_device_ms=graph_ms, moves_per_s=moves / wall,
+               eager_seconds=wall_eager, eager_moves_per_s=moves / wall_eager, init_seconds=t_init,
+               workgroups_per_launch=n_wg, lds_per_workgroup=lds,
+               lanes_table={int(k): int(v) for k, v in table.items()}, per_system=per_sys)
+    log(rec)
+    return rec
+
+
 def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2):
     """Hydrogen chains and Poisson-disk random molecules with 2..40 electrons."""
     rng = np.random.default_rng(seed)

@@ -119,3 +119,59 @@ def test_helium_product_anchor_through_determinants():
     nb = st["n_blocks"]
     err = np.sqrt((st["sum_block_mean2"] / nb - (st["sum_block_mean"] / nb) ** 2) / (nb - 1))
     assert abs(mean - (zeta**2 - 27 * zeta / 8)) < 4 * err
+
+
+def test_graph_replays_match_eager_sweeps():
+    # a captured [sweep, advance] launch replayed several times, with eager
+    # work and synchronisations between replays, equals the eager sequence
+    systems = [S.atom(4), S.random_molecule(6, seed=4)]
+
+    def make():
+        dev = [M.slater_jastrow(s, 50) for s in systems]
+        ctx = M.Context(dev, seed=5, step=0.6, block_size=4, precision=M.PREC_MIXED, lu_period=16)
+        ctx.init_walkers()
+        return ctx
+
+    snaps = []
+    for use_graph in (False, True):
+        ctx = make()
+        g = ctx.capture(5, sample=True) if use_graph else None
+        out = []
+        for r in range(4):
+            if use_graph:
+                g.launch(1)
+            else:
+                ctx.sweep(5, sample=True)
+            ctx.synchronize()
+            ctx.sweep(2)  # eager work between replays
+            out.append((ctx.get_configs(0), ctx.get_configs(1), ctx.stats(0), ctx.sr_sums(1), ctx.get_stream()))
+        if g is not None:
+            g.launch(2)
+            out.append((ctx.get_configs(0), ctx.get_configs(1), ctx.stats(0), ctx.sr_sums(1), ctx.get_stream()))
+            g.close()
+        else:
+            ctx.sweep(5, sample=True)
+            ctx.sweep(5, sample=True)
+            out.append((ctx.get_configs(0), ctx.get_configs(1), ctx.stats(0), ctx.sr_sums(1), ctx.get_stream()))
+        ctx.close()
+        snaps.append(out)
+    for a, b in zip(*snaps):
+        assert np.array_equal(bits(a[0]), bits(b[0]))
+        assert np.array_equal(bits(a[1]), bits(b[1]))
+        assert a[2] == b[2]
+        for k in ("sum_o", "sum_oe", "sum_oo"):
+            assert np.array_equal(a[3][k], b[3][k])
+        assert a[4] == b[4]
+
+
+def test_graph_goes_stale_when_lanes_change():
+    dev = [M.slater_jastrow(S.atom(3), 32)]
+    with M.Context(dev, seed=1, step=0.6, block_size=4, lu_period=16) as ctx:
+        ctx.init_walkers()
+        g = ctx.capture(2)
+        g.launch(1)
+        ctx.set_walker_lanes(2)
+        with pytest.raises(M.MoleError) as ei:
+            g.launch(1)
+        assert ei.value.status == M.ERR_INVALID_ARGUMENT
+        g.close()
