@@ -52,7 +52,7 @@ def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, s
         sj = systems.atom(Z)
         L = 0
         if tune:
-            table, _ = autotune.tune([sj], lambda s: min(walkers, 16384), candidates=(1, 2, 4, 8, 16),
+            table, _ = autotune.tune([sj], lambda s: walkers, candidates=(1, 2, 4, 8, 16),
                                      sweeps=4, precision=M.PREC_MIXED, lu_period=lu_period)
             L = table[sj.n_electrons]
         mixed = run_atom(sj, walkers, sweeps, block, M.PREC_MIXED, lu_period, L, seed)
