@@ -61,6 +61,12 @@ typedef int32_t mole_status;
 #define MOLE_H_IONIC 1      /* IonicHamiltonian, operator.rs:131-149 */
 
 #define MOLE_PROPOSAL_BOX 0 /* MetropolisBox, src/metropolis/src/metrop.rs:21-101 */
+/* MetropolisDiffuse, metrop.rs:103-212: x' = (x_i + tau v) + sqrt(tau) z with
+ * v = grad_i psi / psi, a sign change rejects, Green's functions over the
+ * moved electron's row (the reference's norms run over every row).  step is
+ * tau.  Slater-Jastrow systems only; 6 draws per move (Box-Muller normals
+ * from draws 0-3, accept uniform from draw 4). */
+#define MOLE_PROPOSAL_DIFFUSE 1
 
 /* Convention of the "Parameter gradient" observable O.
  * LOGDERIV:  O = d ln psi / dp.
@@ -109,7 +115,8 @@ typedef struct {
 typedef struct {
   uint64_t seed;         /* Philox key (lo32, hi32 ^ system index) */
   uint32_t proposal;     /* MOLE_PROPOSAL_* */
-  double step;           /* MetropolisBox box_side (metrop.rs:26,66) */
+  double step;           /* MetropolisBox box_side (metrop.rs:26,66), or the
+                            MetropolisDiffuse time step (metrop.rs:104-108) */
   uint32_t block_size;   /* Runner::run block_size (src/montecarlo/src/montecarlo.rs:24-46) */
   uint32_t o_convention; /* MOLE_O_* */
   uint32_t flags;        /* MOLE_FLAG_* */

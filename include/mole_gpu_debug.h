@@ -30,7 +30,18 @@ mole_status mole_debug_copy(size_t n_doubles, int32_t repeats, double *ms_per_co
 #define MOLE_DEBUG_STANDARD 3 /* out = Standard(bits(a)) */
 #define MOLE_DEBUG_RANGE 4    /* out = Range(-b, b)(bits(a)) */
 #define MOLE_DEBUG_NORM3 5    /* out = |(a, b, a*b)|      */
+#define MOLE_DEBUG_LOG 6      /* out = ln a               */
+#define MOLE_DEBUG_SINPI 7    /* out = sin(pi a), 0 <= a < 2 */
+#define MOLE_DEBUG_COSPI 8    /* out = cos(pi a), 0 <= a < 2 */
+#define MOLE_DEBUG_BOXMULLER0 9  /* out = first normal of Box-Muller(bits(a), bits(b)) */
+#define MOLE_DEBUG_BOXMULLER1 10 /* out = second normal */
+#define MOLE_DEBUG_BF16 11    /* out = bf16 round-to-nearest-even of (float)a */
 mole_status mole_debug_math(int32_t op, size_t n, const double *a, const double *b, double *out);
+
+/* D = A B for 16x16 row-major FP32 matrices rounded to bf16, through one
+ * v_mfma_f32_16x16x16_bf16 with the lane placement the walker MLP kernels
+ * use: pins that placement against a host product. */
+mole_status mole_debug_mfma_bf16(const float *A, const float *B, float *D);
 
 #ifdef __cplusplus
 }

@@ -33,6 +33,7 @@ PREC_MIXED = 1
 H_ELECTRONIC = 0
 H_IONIC = 1
 PROPOSAL_BOX = 0
+PROPOSAL_DIFFUSE = 1
 O_LOGDERIV = 0
 O_REFERENCE = 1
 FLAG_TRACE = 1
@@ -54,6 +55,12 @@ DEBUG_DIV = 2
 DEBUG_STANDARD = 3
 DEBUG_RANGE = 4
 DEBUG_NORM3 = 5
+DEBUG_LOG = 6
+DEBUG_SINPI = 7
+DEBUG_COSPI = 8
+DEBUG_BOXMULLER0 = 9
+DEBUG_BOXMULLER1 = 10
+DEBUG_BF16 = 11
 
 
 class MoleError(RuntimeError):
@@ -127,6 +134,7 @@ _lib = None
 _P = C.c_void_p
 _pd = C.POINTER(C.c_double)
 _pu64 = C.POINTER(C.c_uint64)
+_pf = C.POINTER(C.c_float)
 _SIGS = [
     ("mole_status_string", C.c_char_p, [C.c_int32]),
     ("mole_last_error", C.c_char_p, []),
@@ -162,6 +170,7 @@ _SIGS = [
     ("mole_debug_philox", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t, _pu64]),
     ("mole_debug_math", C.c_int32, [C.c_int32, C.c_size_t, _pd, _pd, _pd]),
     ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
+    ("mole_debug_mfma_bf16", C.c_int32, [_pf, _pf, _pf]),
 ]
 
 
@@ -194,6 +203,11 @@ def check(status):
 def _dp(a):
     assert a.dtype == np.float64 and a.flags.c_contiguous
     return a.ctypes.data_as(_pd)
+
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(_pf)
 
 
 def _u64p(a):
@@ -444,6 +458,16 @@ def debug_math(op, a, b=None):
     bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
     check(lib().mole_debug_math(op, a.size, _dp(a), None if bb is None else _dp(bb), _dp(out)))
     return out
+
+
+def debug_mfma_bf16(A, B):
+    """One 16x16x16 bf16 MFMA of FP32 16x16 operands (rounded to bf16 on the device)."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    assert A.shape == (16, 16) and B.shape == (16, 16)
+    D = np.zeros((16, 16), np.float32)
+    check(lib().mole_debug_mfma_bf16(_fp(A), _fp(B), _fp(D)))
+    return D
 
 
 def parameter_update(sums, optimizer=OPT_SR, step=0.05, sr_mode=SR_TEXTBOOK, sr_eps=1e-3):

@@ -131,7 +131,7 @@ double nuclear_repulsion(const double *xyz, const int32_t *Z, uint32_t n) {
 mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run_desc *run) {
   if (!sys || !run) return fail(MOLE_ERR_INVALID_ARGUMENT, "null descriptor");
   if (n_sys == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "n_sys must be >= 1");
-  if (run->proposal != MOLE_PROPOSAL_BOX)
+  if (run->proposal != MOLE_PROPOSAL_BOX && run->proposal != MOLE_PROPOSAL_DIFFUSE)
     return fail(MOLE_ERR_UNSUPPORTED, "proposal %u not supported", run->proposal);
   // rand's Range::new asserts low < high (metrop.rs:66)
   if (!(run->step > 0.0) || !std::isfinite(run->step))
@@ -186,6 +186,9 @@ mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run
       if (run->o_convention != MOLE_O_LOGDERIV)
         return fail(MOLE_ERR_UNSUPPORTED, "Slater-Jastrow systems report O = d ln psi / dp");
     }
+    if (fam == kFamTwo && run->proposal != MOLE_PROPOSAL_BOX) {
+      return fail(MOLE_ERR_UNSUPPORTED, "system %u: the two-electron kinds move with the box proposal", s);
+    }
   }
   return MOLE_OK;
 }
@@ -234,6 +237,10 @@ void free_ctx(mole_ctx *ctx) {
 // ---- Slater-Jastrow launch plumbing ----------------------------------------
 
 int sj_tsize(const mole_ctx *ctx) { return ctx->run.precision == MOLE_PREC_MIXED ? 4 : 8; }
+
+int sj_prop(const mole_ctx *ctx) {
+  return ctx->run.proposal == MOLE_PROPOSAL_DIFFUSE ? kSJPropDiffuse : kSJPropBox;
+}
 
 // Tiles of 64/L walkers, most expensive first (longest-processing-time
 // order, so the tail of the launch is short tiles), and the dynamic LDS of
@@ -286,7 +293,8 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.charge = st.d_tab + 3 * st.n_nuc;
     d.zeta = st.d_tab + 4 * st.n_nuc;
     d.center = st.d_center;
-    d.coef = st.d_tab + 4 * st.n_nuc + st.n_basis;
+    d.coef_t = st.d_tab + 4 * st.n_nuc + st.n_basis;
+    d.n_orb = st.n_orb;
     for (int k = 0; k < kSJNParam; ++k) d.p[k] = st.params[k];
     d.v_nn = st.v_nn;
     d.pos = st.d_pos;
@@ -314,6 +322,8 @@ SJArgs sj_args(const mole_ctx *ctx) {
   const double lo = -0.5 * ctx->run.step, hi = 0.5 * ctx->run.step;
   a.scale = hi - lo;
   a.offset = lo - a.scale;
+  a.tau = ctx->run.step;
+  a.sqrt_tau = std::sqrt(ctx->run.step);
   a.block_size = ctx->run.block_size;
   a.lu_period = ctx->run.lu_period;
   a.state = ctx->d_state;
@@ -339,7 +349,7 @@ mole_status sj_enqueue(mole_ctx *ctx, uint32_t n, uint32_t ss) {
   SJArgs a = sj_args(ctx);
   a.n_sweeps = n;
   a.sample_start = ss;
-  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, ctx->lds_bytes, ctx->stream));
+  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
   HIP_TRY(launch_sj_advance(ctx->d_state, n, n - std::min(ss, n), ctx->run.block_size, ctx->stream));
   return MOLE_OK;
 }
@@ -373,7 +383,7 @@ mole_status sj_prepare(mole_ctx *ctx, bool init) {
   a.init = init ? 1 : 0;
   a.refresh = 1;
   for (SysState &s : ctx->sys) s.cols_valid = false;
-  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, ctx->lds_bytes, ctx->stream));
+  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
   return MOLE_OK;
 }
 
@@ -572,12 +582,14 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       }
     }
     if (ctx->family == kFamSJ) {
-      // tables: nuclei xyz (3 n_nuc), charges (n_nuc), zeta (n_basis), coef (n_orb n_basis)
+      // tables: nuclei xyz (3 n_nuc), charges (n_nuc), zeta (n_basis), and the
+      // orbital coefficients transposed to [n_basis][n_orb]
       std::vector<double> tab;
       tab.insert(tab.end(), d.nuclei_xyz, d.nuclei_xyz + 3 * d.n_nuclei);
       for (uint32_t I = 0; I < d.n_nuclei; ++I) tab.push_back((double)d.nuclei_charge[I]);
       tab.insert(tab.end(), d.basis_exponent, d.basis_exponent + d.n_basis);
-      tab.insert(tab.end(), d.orbital_coeff, d.orbital_coeff + (size_t)d.n_orbitals * d.n_basis);
+      for (uint32_t m = 0; m < d.n_basis; ++m)
+        for (uint32_t k = 0; k < d.n_orbitals; ++k) tab.push_back(d.orbital_coeff[(size_t)k * d.n_basis + m]);
       std::vector<int> cen(d.basis_center, d.basis_center + d.n_basis);
       CTX_TRY(hipMalloc(&S.d_tab, sizeof(double) * tab.size()));
       CTX_TRY(hipMemcpy(S.d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
@@ -711,7 +723,8 @@ mole_status mole_get_stream(mole_ctx *ctx, uint32_t *epoch, uint64_t *position) 
   if (position)
     *position = ctx->family == kFamTwo
                     ? ctx->position
-                    : ctx->sys[0].draw_base + 4ull * ctx->sys[0].N * ctx->sweeps_done;
+                    : ctx->sys[0].draw_base + (uint64_t)sj_draws_per_move(sj_prop(ctx)) *
+                                                    ctx->sys[0].N * ctx->sweeps_done;
   return ok();
 }
 

@@ -39,9 +39,43 @@ __global__ void math_kernel(int op, size_t n, const double *a, const double *b, 
       break;
     }
     case MOLE_DEBUG_NORM3: r = norm3(x, y, x * y); break;
+    case MOLE_DEBUG_LOG: r = log_m(x); break;
+    case MOLE_DEBUG_SINPI:
+    case MOLE_DEBUG_COSPI: {
+      double sn, cs;
+      sincospi_m(x, &sn, &cs);
+      r = op == MOLE_DEBUG_SINPI ? sn : cs;
+      break;
+    }
+    case MOLE_DEBUG_BOXMULLER0:
+    case MOLE_DEBUG_BOXMULLER1: {
+      double z0, z1;
+      box_muller((uint64_t)__double_as_longlong(x), (uint64_t)__double_as_longlong(y), &z0, &z1);
+      r = op == MOLE_DEBUG_BOXMULLER0 ? z0 : z1;
+      break;
+    }
+    case MOLE_DEBUG_BF16: r = (double)bf16_float(bf16_bits((float)x)); break;
     default: r = 0.0;
   }
   out[i] = r;
+}
+
+// One v_mfma_f32_16x16x16_bf16 on 16x16 operands given row-major in FP32
+// (rounded to bf16 here), placed in lanes as the walker MLP assumes:
+//   A: lane l holds A[l % 16][4 (l / 16) + j], j = 0..3
+//   B: lane l holds B[4 (l / 16) + j][l % 16]
+//   D: lane l holds D[4 (l / 16) + j][l % 16]
+typedef short bf16x4_t __attribute__((ext_vector_type(4)));
+typedef float f32x4_t __attribute__((ext_vector_type(4)));
+__global__ void __launch_bounds__(64) mfma_probe_kernel(const float *A, const float *B, float *D) {
+  const int l = threadIdx.x, r = l % 16, q = l / 16;
+  bf16x4_t a, b;
+  for (int j = 0; j < 4; ++j) {
+    a[j] = (short)bf16_bits(A[r * 16 + 4 * q + j]);
+    b[j] = (short)bf16_bits(B[(4 * q + j) * 16 + r]);
+  }
+  const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, (f32x4_t){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  for (int j = 0; j < 4; ++j) D[(4 * q + j) * 16 + r] = d[j];
 }
 
 // Plain copy of n doubles: the calibration kernel for HBM byte counters.
@@ -104,8 +138,9 @@ extern "C" mole_status mole_debug_philox(uint64_t seed, uint32_t sys, uint32_t w
 
 extern "C" mole_status mole_debug_math(int32_t op, size_t n, const double *a, const double *b,
                                        double *out) {
-  if (!a || !out || op < 0 || op > MOLE_DEBUG_NORM3) return MOLE_ERR_INVALID_ARGUMENT;
-  if ((op == MOLE_DEBUG_DIV || op == MOLE_DEBUG_RANGE || op == MOLE_DEBUG_NORM3) && !b)
+  if (!a || !out || op < 0 || op > MOLE_DEBUG_BF16) return MOLE_ERR_INVALID_ARGUMENT;
+  if ((op == MOLE_DEBUG_DIV || op == MOLE_DEBUG_RANGE || op == MOLE_DEBUG_NORM3 ||
+       op == MOLE_DEBUG_BOXMULLER0 || op == MOLE_DEBUG_BOXMULLER1) && !b)
     return MOLE_ERR_INVALID_ARGUMENT;
   if (n == 0) return MOLE_OK;
   double *d = nullptr;
@@ -118,6 +153,21 @@ extern "C" mole_status mole_debug_math(int32_t op, size_t n, const double *a, co
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(out, d + 2 * n, n * sizeof(double), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
+}
+
+extern "C" mole_status mole_debug_mfma_bf16(const float *A, const float *B, float *D) {
+  if (!A || !B || !D) return MOLE_ERR_INVALID_ARGUMENT;
+  float *d = nullptr;
+  if (hipMalloc(&d, 3 * 256 * sizeof(float)) != hipSuccess) return MOLE_ERR_OUT_OF_MEMORY;
+  hipError_t e = hipMemcpy(d, A, 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + 256, B, 256 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, 0, d, d + 256, d + 512);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(D, d + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
 }

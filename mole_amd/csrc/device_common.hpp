@@ -138,4 +138,88 @@ __device__ __forceinline__ double min_one(double a) {
   return (a < 1.0) ? a : 1.0;
 }
 
+// ln x for finite x > 0 with a normal exponent: x = 2^k m, m in
+// [sqrt(1/2), sqrt(2)), ln m = 2 atanh((m-1)/(m+1)) by its series to s^20
+// (Horner with fma), ln x = k ln2_hi + (ln m + k ln2_lo), fdlibm split.
+__device__ __forceinline__ double log_m(double x) {
+  const double kLn2Hi = 6.93147180369123816490e-01;
+  const double kLn2Lo = 1.90821492927058770002e-10;
+  const unsigned long long bits = (unsigned long long)__double_as_longlong(x);
+  int k = (int)(bits >> 52) - 1023;
+  double m = __longlong_as_double((long long)((bits & 0x000FFFFFFFFFFFFFull) | 0x3FF0000000000000ull));
+  if (m > 1.41421356237309504880) {
+    m = m * 0.5;
+    k = k + 1;
+  }
+  const double f = m - 1.0;
+  const double s = f / (2.0 + f);
+  const double s2 = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  p = fma(p, s2, 1.0);
+  const double lnm = (2.0 * s) * p;
+  const double kd = (double)k;
+  return fma(kd, kLn2Hi, fma(kd, kLn2Lo, lnm));
+}
+
+// sin(pi x), cos(pi x) for 0 <= x < 2: n = rint(2x), y = x - n/2 (exact),
+// Taylor polynomials of sin (to t^17) and cos (to t^18) at t = pi y, then
+// the quadrant n.
+__device__ __forceinline__ void sincospi_m(double x, double *sn, double *cs) {
+  const double nd = rint(2.0 * x);
+  const double y = x - 0.5 * nd;
+  const double t = 3.14159265358979323846 * y;
+  const double t2 = t * t;
+  double ps = -1.0 / 355687428096000.0;
+  ps = fma(ps, t2, 1.0 / 1307674368000.0);
+  ps = fma(ps, t2, -1.0 / 6227020800.0);
+  ps = fma(ps, t2, 1.0 / 39916800.0);
+  ps = fma(ps, t2, -1.0 / 362880.0);
+  ps = fma(ps, t2, 1.0 / 5040.0);
+  ps = fma(ps, t2, -1.0 / 120.0);
+  ps = fma(ps, t2, 1.0 / 6.0);
+  const double s = fma(-(t * t2), ps, t);
+  double pc = 1.0 / 6402373705728000.0;
+  pc = fma(pc, t2, -1.0 / 20922789888000.0);
+  pc = fma(pc, t2, 1.0 / 87178291200.0);
+  pc = fma(pc, t2, -1.0 / 479001600.0);
+  pc = fma(pc, t2, 1.0 / 3628800.0);
+  pc = fma(pc, t2, -1.0 / 40320.0);
+  pc = fma(pc, t2, 1.0 / 720.0);
+  pc = fma(pc, t2, -1.0 / 24.0);
+  pc = fma(pc, t2, 0.5);
+  const double c = fma(-t2, pc, 1.0);
+  const int q = (int)nd & 3;
+  *sn = q == 0 ? s : q == 1 ? c : q == 2 ? -s : -c;
+  *cs = q == 0 ? c : q == 1 ? -s : q == 2 ? -c : s;
+}
+
+// Two N(0,1) values from two u64 draws: u1 = 1 - Standard(a) in (0, 1],
+// u2 = Standard(b), r = sqrt(-2 ln u1), (r cos 2 pi u2, r sin 2 pi u2).
+__device__ __forceinline__ void box_muller(uint64_t a, uint64_t b, double *z0, double *z1) {
+  const double u1 = 1.0 - u64_standard(a);
+  const double u2 = u64_standard(b);
+  const double r = sqrt(-2.0 * log_m(u1));
+  double sn, cs;
+  sincospi_m(2.0 * u2, &sn, &cs);
+  *z0 = r * cs;
+  *z1 = r * sn;
+}
+
+// float -> bfloat16 bits, round to nearest even (NaN stays a quiet NaN)
+__device__ __forceinline__ uint16_t bf16_bits(float f) {
+  const uint32_t u = __float_as_uint(f);
+  if ((u & 0x7FFFFFFFu) > 0x7F800000u) return (uint16_t)((u >> 16) | 0x40u);
+  return (uint16_t)((u + 0x7FFFu + ((u >> 16) & 1u)) >> 16);
+}
+__device__ __forceinline__ float bf16_float(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
 }  // namespace mole

@@ -8,6 +8,10 @@
 namespace mole {
 
 constexpr int kSJNParam = 5;  // b_anti, b_para, c, a, d
+
+// Proposals of the Slater-Jastrow kernel and their draws per move.
+enum SJProposal : int { kSJPropBox = 0, kSJPropDiffuse = 1, kSJPropLearned = 2 };
+__host__ __device__ constexpr int sj_draws_per_move(int prop) { return prop == kSJPropBox ? 4 : 6; }
 constexpr int kSJMaxElectrons = 40;
 constexpr int kSJMaxSpin = 20;
 
@@ -28,7 +32,7 @@ enum SJAccColumn : int {
 
 // One system as the kernels see it (device memory).
 struct SJSysDev {
-  int N, n_up, n_dn, n_nuc, n_basis;
+  int N, n_up, n_dn, n_nuc, n_basis, n_orb;
   int L;  // lanes per walker (power of two, <= 64)
   int hamiltonian;
   int W;
@@ -36,7 +40,7 @@ struct SJSysDev {
   const double *charge;  // [n_nuc]
   const double *zeta;    // [n_basis]
   const int *center;     // [n_basis]
-  const double *coef;    // [n_orb][n_basis]
+  const double *coef_t;  // [n_basis][n_orb]: lanes over orbitals read consecutive words
   double p[kSJNParam];
   double v_nn;
   double *pos;          // [3N][W]
@@ -73,6 +77,7 @@ struct SJArgs {
   uint64_t seed;
   uint32_t epoch;
   double scale, offset;    // Range(-side/2, side/2)
+  double tau, sqrt_tau;    // drift-diffusion time step
   uint32_t n_sweeps;
   uint32_t sample_start;   // sweeps s >= sample_start are sampled
   uint32_t block_size;
@@ -84,10 +89,11 @@ struct SJArgs {
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);
 
-// Draws start at draw_base + 4N sweeps_done and moves at move_base +
-// N sweeps_done for every system; sample s of a launch is sampled when
-// s >= sample_start.
-hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream);
+// Draws start at draw_base + D N sweeps_done (D draws per move of the
+// proposal) and moves at move_base + N sweeps_done for every system; sample
+// s of a launch is sampled when s >= sample_start.
+hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_t lds_bytes,
+                     hipStream_t stream);
 // state->sweeps_done += n_sweeps; block fill and trace cursors advance by the
 // launch's samples.
 hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n_samples,

@@ -26,7 +26,7 @@ namespace mole {
 
 // LDS layout of one walker (bytes); identical on host and device.
 struct SJLayout {
-  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, acc, bytes;
+  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, mv, acc, bytes;
   __host__ __device__ SJLayout(int N, int nu, int nd, int nn, int K, int tsize) {
     const int nmax = nu > nd ? nu : nd;
     uint32_t o = 0;
@@ -48,6 +48,7 @@ struct SJLayout {
     const uint32_t el = 8u * (5u * K + 4u * nmax + 8u * (N + nn));
     scr = take(lu > el ? lu : el);
     flag = take(16u);
+    mv = take(64u);  // proposed position and drift (Gaussian proposals)
     acc = take(8u * kSJColumns);
     bytes = o;
   }
@@ -61,7 +62,7 @@ namespace {
 
 template <typename T>
 struct SJWalker {
-  double *pos, *rng, *rn, *chi, *scr, *flag;
+  double *pos, *rng, *rn, *chi, *scr, *flag, *mv;
   T *Bup, *Bdn, *phin, *jn, *jo, *f;
 };
 
@@ -108,6 +109,10 @@ __device__ __forceinline__ float en_term<float>(double aZ, double d, const doubl
 __device__ __forceinline__ double fma_t(double a, double b, double c) { return fma(a, b, c); }
 __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 
+// e / n for 0 <= e < 4096, 1 <= n <= 64 via a float reciprocal: (e + 1/2)/n
+// lies at least 1/(2n) from an integer, far beyond the rounding error.
+__device__ __forceinline__ int small_div(int e, float inv_n) { return (int)(((float)e + 0.5f) * inv_n); }
+
 // Basis values chi[m] of the point r into LDS (lanes over nuclei, then over
 // basis functions).  Ends with a barrier.
 __device__ __forceinline__ void basis_to_lds(const SJSysDev &S, const double *r, double *rn, double *chi,
@@ -119,10 +124,61 @@ __device__ __forceinline__ void basis_to_lds(const SJSysDev &S, const double *r,
   __syncthreads();
 }
 
+// Sequential sums whose loads are issued U at a time ahead of the
+// arithmetic: the same operations in the same order as a plain loop, with
+// the LDS/cache latency paid once per group instead of once per term.
+template <int U, typename T>
+__device__ __forceinline__ T seq_sum(const T *x, int n) {
+  T s = (T)0;
+  int j = 0;
+  for (; j + U <= n; j += U) {
+    T v[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) v[u] = x[j + u];
+#pragma unroll
+    for (int u = 0; u < U; ++u) s = s + v[u];
+  }
+  for (; j < n; ++j) s = s + x[j];
+  return s;
+}
+
+// sum_k a[k] b[k stride] as an fma chain in k order
+template <int U, typename T>
+__device__ __forceinline__ T seq_dot(const T *a, const T *b, int stride, int n) {
+  T s = (T)0;
+  int k = 0;
+  for (; k + U <= n; k += U) {
+    T av[U], bv[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      av[u] = a[k + u];
+      bv[u] = b[(k + u) * stride];
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) s = fma_t(av[u], bv[u], s);
+  }
+  for (; k < n; ++k) s = fma_t(a[k], b[k * stride], s);
+  return s;
+}
+
+// phi_k = sum_m c_km chi_m, fma chain in m order; coefficients from the
+// [n_basis][n_orb] table (consecutive k on consecutive lanes)
 __device__ __forceinline__ double orbital_value(const SJSysDev &S, int k, const double *chi) {
-  const double *c = S.coef + (size_t)k * S.n_basis;
+  const double *c = S.coef_t + k;
+  const int K = S.n_basis, no = S.n_orb;
   double phi = 0.0;
-  for (int m = 0; m < S.n_basis; ++m) phi = fma(c[m], chi[m], phi);
+  int m = 0;
+  for (; m + 8 <= K; m += 8) {
+    double cv[8], xv[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      cv[u] = c[(size_t)(m + u) * no];
+      xv[u] = chi[m + u];
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) phi = fma(cv[u], xv[u], phi);
+  }
+  for (; m < K; ++m) phi = fma(c[(size_t)m * no], chi[m], phi);
   return phi;
 }
 
@@ -148,7 +204,8 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
       for (int c = 0; c < n; ++c) cs += fabs(M[c * n + k]);
       anorm = cs > anorm ? cs : anorm;
     }
-  for (int e = t; e < n * n; e += L) I[e] = (e / n == e % n) ? 1.0 : 0.0;
+  const float inv_n = 1.0f / (float)n;
+  for (int e = t; e < n * n; e += L) I[e] = (e == small_div(e, inv_n) * (n + 1)) ? 1.0 : 0.0;
   __syncthreads();
   for (int p = 0; p < n; ++p) {
     if (t == 0) {
@@ -179,7 +236,7 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
     __syncthreads();
     // row p's own column entry was just scaled; colp[p] is unused
     for (int e = t; e < n * n; e += L) {
-      const int r = e / n, j = e % n;
+      const int r = small_div(e, inv_n), j = e - r * n;
       if (r == p) continue;
       M[e] = fma(-colp[r], M[p * n + j], M[e]);
       I[e] = fma(-colp[r], I[p * n + j], I[e]);
@@ -202,7 +259,7 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
 // configuration; returns on sub-lane 0.  All lanes must call it.
 template <typename T>
 __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl) {
-  const int N = S.N, K = S.n_basis, nn = S.n_nuc;
+  const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
   const int nmax = S.n_up > S.n_dn ? S.n_up : S.n_dn;
   const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
   double *chi = wk.scr, *gch = chi + K, *lch = gch + 3 * K, *orb = lch + K, *pt = orb + 4 * nmax;
@@ -228,11 +285,15 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
       gch[3 * m + 2] = gf * d2;
       lch[m] = ch * (z * z - 2.0 * z / r);
     }
-    // Jastrow pair and nuclear terms (lanes over partners)
+    // Jastrow pair and nuclear terms (lanes over partners); i's own slot is
+    // zero, which leaves every sum below unchanged (they start at +0)
     for (int j = t; j < N + nn; j += L) {
       double *o = pt + 8 * j;
       if (j < N) {
-        if (j == i) continue;
+        if (j == i) {
+          for (int q = 0; q < 7; ++q) o[q] = 0.0;
+          continue;
+        }
         const double *xj = wk.pos + 3 * j;
         const double dv0 = xi[0] - xj[0], dv1 = xi[1] - xj[1], dv2 = xi[2] - xj[2];
         const double r = norm3(dv0, dv1, dv2);
@@ -248,7 +309,6 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         o[4] = 1.0 / r;
         o[5] = r / den;
         o[6] = -b * r * r / (den * den);
-        o[7] = same ? 1.0 : 0.0;
       } else {
         const int I = j - N;
         const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
@@ -265,19 +325,32 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         o[4] = -Z / r;
         o[5] = -Z * r / den;
         o[6] = aZ * r * r / (den * den);
-        o[7] = 0.0;
       }
     }
     __syncthreads();
-    // orbital gradients and laplacians (lanes over orbitals)
+    // orbital gradients and laplacians (lanes over orbitals), chains in m order
     for (int k = t; k < n; k += L) {
-      const double *cf = S.coef + (size_t)k * K;
+      const double *cf = S.coef_t + k;
       double g0 = 0.0, g1 = 0.0, g2 = 0.0, l = 0.0;
-      for (int m = 0; m < K; ++m) {
-        g0 = fma(cf[m], gch[3 * m], g0);
-        g1 = fma(cf[m], gch[3 * m + 1], g1);
-        g2 = fma(cf[m], gch[3 * m + 2], g2);
-        l = fma(cf[m], lch[m], l);
+      int m = 0;
+      for (; m + 4 <= K; m += 4) {
+        double cv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) cv[u] = cf[(size_t)(m + u) * no];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+          g0 = fma(cv[u], gch[3 * (m + u)], g0);
+          g1 = fma(cv[u], gch[3 * (m + u) + 1], g1);
+          g2 = fma(cv[u], gch[3 * (m + u) + 2], g2);
+          l = fma(cv[u], lch[m + u], l);
+        }
+      }
+      for (; m < K; ++m) {
+        const double cm = cf[(size_t)m * no];
+        g0 = fma(cm, gch[3 * m], g0);
+        g1 = fma(cm, gch[3 * m + 1], g1);
+        g2 = fma(cm, gch[3 * m + 2], g2);
+        l = fma(cm, lch[m], l);
       }
       orb[4 * k] = g0;
       orb[4 * k + 1] = g1;
@@ -286,37 +359,78 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     }
     __syncthreads();
     if (t == 0) {
+      // determinant part, four chains in k order
       double Dg0 = 0.0, Dg1 = 0.0, Dg2 = 0.0, Dl = 0.0;
-      for (int k = 0; k < n; ++k) {
-        const double bkc = (double)B[k * n + cc];
-        Dg0 = fma(orb[4 * k], bkc, Dg0);
-        Dg1 = fma(orb[4 * k + 1], bkc, Dg1);
-        Dg2 = fma(orb[4 * k + 2], bkc, Dg2);
-        Dl = fma(orb[4 * k + 3], bkc, Dl);
-      }
-      double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0, Vi = 0.0;
-      for (int j = 0; j < N; ++j) {
-        if (j == i) continue;
-        const double *o = pt + 8 * j;
-        gJ0 += o[0];
-        gJ1 += o[1];
-        gJ2 += o[2];
-        lJ += o[3];
-        if (j > i) {
-          Vi += o[4];
-          if (o[7] != 0.0)
-            dl[1] += o[5];
-          else
-            dl[0] += o[5];
-          dl[2] += o[6];
+      {
+        int k = 0;
+        for (; k + 2 <= n; k += 2) {
+          double bv[2], ov[2][4];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            bv[u] = (double)B[(k + u) * n + cc];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) ov[u][q] = orb[4 * (k + u) + q];
+          }
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            Dg0 = fma(ov[u][0], bv[u], Dg0);
+            Dg1 = fma(ov[u][1], bv[u], Dg1);
+            Dg2 = fma(ov[u][2], bv[u], Dg2);
+            Dl = fma(ov[u][3], bv[u], Dl);
+          }
+        }
+        for (; k < n; ++k) {
+          const double bkc = (double)B[k * n + cc];
+          Dg0 = fma(orb[4 * k], bkc, Dg0);
+          Dg1 = fma(orb[4 * k + 1], bkc, Dg1);
+          Dg2 = fma(orb[4 * k + 2], bkc, Dg2);
+          Dl = fma(orb[4 * k + 3], bkc, Dl);
         }
       }
+      // Jastrow gradient and laplacian: electrons (i's slot zero), then nuclei
+      double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0;
+      {
+        const int M = N + nn;
+        int j = 0;
+        for (; j + 2 <= M; j += 2) {
+          double v[2][4];
+#pragma unroll
+          for (int u = 0; u < 2; ++u)
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[u][q] = pt[8 * (j + u) + q];
+#pragma unroll
+          for (int u = 0; u < 2; ++u) {
+            gJ0 += v[u][0];
+            gJ1 += v[u][1];
+            gJ2 += v[u][2];
+            lJ += v[u][3];
+          }
+        }
+        for (; j < M; ++j) {
+          gJ0 += pt[8 * j];
+          gJ1 += pt[8 * j + 1];
+          gJ2 += pt[8 * j + 2];
+          lJ += pt[8 * j + 3];
+        }
+      }
+      // pairs j > i: same spin up to same_end, opposite spin after; then nuclei
+      double Vi = 0.0;
+      const int same_end = i < S.n_up ? S.n_up : N;
+#pragma unroll 4
+      for (int j = i + 1; j < same_end; ++j) {
+        Vi += pt[8 * j + 4];
+        dl[1] += pt[8 * j + 5];
+        dl[2] += pt[8 * j + 6];
+      }
+#pragma unroll 4
+      for (int j = same_end; j < N; ++j) {
+        Vi += pt[8 * j + 4];
+        dl[0] += pt[8 * j + 5];
+        dl[2] += pt[8 * j + 6];
+      }
+#pragma unroll 4
       for (int I = 0; I < nn; ++I) {
         const double *o = pt + 8 * (N + I);
-        gJ0 += o[0];
-        gJ1 += o[1];
-        gJ2 += o[2];
-        lJ += o[3];
         Vi += o[4];
         dl[3] += o[5];
         dl[4] += o[6];
@@ -339,10 +453,107 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
   return -0.5 * kin + V + S.v_nn;
 }
 
+
+// Pieces of grad_i ln psi at r for the drift-diffusion proposal, FP64
+// (metrop.rs:153-155).  Basis gradients into gch[3m + q] (lanes over basis
+// functions; chi and rn of r must be in LDS) and the Jastrow gradient terms
+// of electron i at r into jg[3j + q] (lanes over partners, i's slot zero).
+// No barrier.
+__device__ __forceinline__ void drift_terms(const SJSysDev &S, const double *pos, int i, const double *r,
+                                            const double *rn, const double *chi, double *gch, double *jg,
+                                            int t, int L) {
+  for (int m = t; m < S.n_basis; m += L) {
+    const int I = S.center[m];
+    const double d0 = r[0] - S.nuc[3 * I], d1 = r[1] - S.nuc[3 * I + 1], d2 = r[2] - S.nuc[3 * I + 2];
+    const double gf = -S.zeta[m] * chi[m] / rn[I];
+    gch[3 * m] = gf * d0;
+    gch[3 * m + 1] = gf * d1;
+    gch[3 * m + 2] = gf * d2;
+  }
+  const int N = S.N, si = i < S.n_up ? 0 : 1;
+  for (int j = t; j < N + S.n_nuc; j += L) {
+    double *o = jg + 3 * j;
+    if (j < N) {
+      if (j == i) {
+        o[0] = o[1] = o[2] = 0.0;
+        continue;
+      }
+      const double *xj = pos + 3 * j;
+      const double dv0 = r[0] - xj[0], dv1 = r[1] - xj[1], dv2 = r[2] - xj[2];
+      const double rr = norm3(dv0, dv1, dv2);
+      const double b = ((j < S.n_up ? 0 : 1) == si) ? S.p[1] : S.p[0];
+      const double den = 1.0 + S.p[2] * rr;
+      const double up = b / (den * den);
+      o[0] = up * dv0 / rr;
+      o[1] = up * dv1 / rr;
+      o[2] = up * dv2 / rr;
+    } else {
+      const int I = j - N;
+      const double d0 = r[0] - S.nuc[3 * I], d1 = r[1] - S.nuc[3 * I + 1], d2 = r[2] - S.nuc[3 * I + 2];
+      const double rr = norm3(d0, d1, d2);
+      const double den = 1.0 + S.p[4] * rr;
+      const double vp = -(S.p[3] * S.charge[I]) / (den * den);
+      o[0] = vp * d0 / rr;
+      o[1] = vp * d1 / rr;
+      o[2] = vp * d2 / rr;
+    }
+  }
+}
+
+// Orbital gradients og[3k + q] = sum_m c_km gch[3m + q] (lanes over
+// orbitals, chains in m order).  No barrier.
+__device__ __forceinline__ void orbital_gradients(const SJSysDev &S, int n, const double *gch, double *og,
+                                                  int t, int L) {
+  const int K = S.n_basis, no = S.n_orb;
+  for (int k = t; k < n; k += L) {
+    const double *cf = S.coef_t + k;
+    double g0 = 0.0, g1 = 0.0, g2 = 0.0;
+    for (int m = 0; m < K; ++m) {
+      const double cm = cf[(size_t)m * no];
+      g0 = fma(cm, gch[3 * m], g0);
+      g1 = fma(cm, gch[3 * m + 1], g1);
+      g2 = fma(cm, gch[3 * m + 2], g2);
+    }
+    og[3 * k] = g0;
+    og[3 * k + 1] = g1;
+    og[3 * k + 2] = g2;
+  }
+}
+
+// Sub-lane 0: g = sum_k og_k B[k][c] (chains in k order) + sum_j jg_j (in j
+// order), each as in the CPU restatement.
+template <typename T>
+__device__ __forceinline__ void drift_sum(const double *og, const T *B, int n, int c, const double *jg, int M,
+                                          double det_scale, double g[3]) {
+  double d0 = 0.0, d1 = 0.0, d2 = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double bkc = (double)B[k * n + c];
+    d0 = fma(og[3 * k], bkc, d0);
+    d1 = fma(og[3 * k + 1], bkc, d1);
+    d2 = fma(og[3 * k + 2], bkc, d2);
+  }
+  double j0 = 0.0, j1 = 0.0, j2 = 0.0;
+  for (int j = 0; j < M; ++j) {
+    j0 += jg[3 * j];
+    j1 += jg[3 * j + 1];
+    j2 += jg[3 * j + 2];
+  }
+  // det_scale = 1: plain sum; otherwise the determinant part is divided by it
+  if (det_scale != 1.0) {
+    d0 = d0 / det_scale;
+    d1 = d1 / det_scale;
+    d2 = d2 / det_scale;
+  }
+  g[0] = d0 + j0;
+  g[1] = d1 + j1;
+  g[2] = d2 + j2;
+}
+
 }  // namespace
 
-template <typename T>
+template <typename T, int PROP>
 __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
+  constexpr int DPM = sj_draws_per_move(PROP);
   extern __shared__ __align__(16) unsigned char lds[];
   const SJTile tile = a.tiles[blockIdx.x];
   const SJSysDev &S = a.systems[tile.sys];
@@ -350,6 +561,7 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
   const int lane = threadIdx.x;
   const int slot = lane / L, t = lane % L;
   const int nslots = 64 / L;
+  const int lg_slots = __ffs(nslots) - 1;  // nslots is a power of two
   const bool live = slot < tile.nw;
   const int w = tile.w0 + slot;
   const SJLayout lay(N, nu, nd, S.n_nuc, S.n_basis, (int)sizeof(T));
@@ -367,17 +579,18 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
   wk.f = (T *)(base + lay.f);
   wk.scr = (double *)(base + lay.scr);
   wk.flag = (double *)(base + lay.flag);
+  wk.mv = (double *)(base + lay.mv);
   const int nB = nu * nu + nd * nd;
   const T *Bg = (const T *)S.B;
 
   // ---- state in ----
   for (int e = lane; e < 3 * N * nslots; e += 64) {
-    const int k = e / nslots, sl = e % nslots;
+    const int k = e >> lg_slots, sl = e & (nslots - 1);
     double *pp = (double *)(lds + (size_t)sl * lay.bytes + lay.pos);
     pp[k] = (sl < tile.nw) ? S.pos[(size_t)k * W + tile.w0 + sl] : 0.0;
   }
   for (int e = lane; e < nB * nslots; e += 64) {
-    const int k = e / nslots, sl = e % nslots;
+    const int k = e >> lg_slots, sl = e & (nslots - 1);
     T *bb = (T *)(lds + (size_t)sl * lay.bytes + lay.B);
     bb[k] = (sl < tile.nw) ? Bg[(size_t)k * W + tile.w0 + sl] : (T)0;
   }
@@ -407,15 +620,15 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
   const uint32_t lu_period = a.lu_period;
 
   const SJLaunchState ls = *a.state;
-  uint64_t p = S.draw_base + 4 * (uint64_t)N * ls.sweeps_done;
+  uint64_t p = S.draw_base + DPM * (uint64_t)N * ls.sweeps_done;
   uint64_t m = S.move_base + (uint64_t)N * ls.sweeps_done;
   uint32_t fill = ls.block_fill;
   uint32_t sidx = ls.trace_samples;
   const double inv_block = 1.0 / (double)a.block_size;
 
   for (uint32_t sw = 0; sw < a.n_sweeps; ++sw) {
-    // ---- the sweep's draws: [p, p + 4N), lanes over Philox blocks ----
-    {
+    if constexpr (PROP == kSJPropBox) {
+      // ---- the sweep's draws: [p, p + 4N), lanes over Philox blocks ----
       const uint64_t b0 = p >> 1, b1 = (p + 4 * (uint64_t)N - 1) >> 1;
       for (uint64_t b = b0 + t; b <= b1; b += L) {
         const u32x4 o = philox4x32_10({(uint32_t)b, (uint32_t)(b >> 32), (uint32_t)w, a.epoch}, k0, k1);
@@ -427,6 +640,23 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
           wk.rng[r] = ((r & 3) == 3) ? u64_standard(v[h]) : u64_range(v[h], a.scale, a.offset);
         }
       }
+    } else {
+      // ---- Gaussian proposals: 6 draws per move, lanes over moves; Box-Muller
+      // of draws (0,1) and (2,3) -> z0, z1, z2; draw 4 -> the accept uniform
+      Stream st;
+      st.init(a.seed, S.sys_index, (uint32_t)w, a.epoch);
+      for (int i = t; i < N; i += L) {
+        const uint64_t q = p + 6 * (uint64_t)i;
+        const uint64_t d0 = st.draw(q), d1 = st.draw(q + 1), d2 = st.draw(q + 2), d3 = st.draw(q + 3);
+        const uint64_t d4 = st.draw(q + 4);
+        double z0, z1, z2, z3;
+        box_muller(d0, d1, &z0, &z1);
+        box_muller(d2, d3, &z2, &z3);
+        wk.rng[4 * i] = z0;
+        wk.rng[4 * i + 1] = z1;
+        wk.rng[4 * i + 2] = z2;
+        wk.rng[4 * i + 3] = u64_standard(d4);
+      }
     }
     __syncthreads();
     uint64_t mask = 0;
@@ -436,11 +666,40 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
       const int n = s == 0 ? nu : nd;
       T *B = s == 0 ? wk.Bup : wk.Bdn;
       const double *xi = wk.pos + 3 * i;
-      const double xn[3] = {xi[0] + wk.rng[4 * i], xi[1] + wk.rng[4 * i + 1], xi[2] + wk.rng[4 * i + 2]};
+      double xn[3];
+      double *gch = wk.scr, *og = wk.scr + 3 * S.n_basis, *jg = og + 3 * (nu > nd ? nu : nd);
+      if constexpr (PROP == kSJPropBox) {
+        xn[0] = xi[0] + wk.rng[4 * i];
+        xn[1] = xi[1] + wk.rng[4 * i + 1];
+        xn[2] = xi[2] + wk.rng[4 * i + 2];
+      } else {
+        // drift at x (metrop.rs:150-163): x' = (x_i + tau v) + sqrt(tau) z
+        basis_to_lds(S, xi, wk.rn, wk.chi, t, L);
+        drift_terms(S, wk.pos, i, xi, wk.rn, wk.chi, gch, jg, t, L);
+        __syncthreads();
+        orbital_gradients(S, n, gch, og, t, L);
+        __syncthreads();
+        if (t == 0) {
+          double v[3];
+          drift_sum<T>(og, B, n, c, jg, N + S.n_nuc, 1.0, v);
+          for (int q = 0; q < 3; ++q) {
+            wk.mv[3 + q] = v[q];
+            wk.mv[q] = (xi[q] + v[q] * a.tau) + a.sqrt_tau * wk.rng[4 * i + q];
+          }
+        }
+        __syncthreads();
+        xn[0] = wk.mv[0];
+        xn[1] = wk.mv[1];
+        xn[2] = wk.mv[2];
+      }
       // Jastrow terms of electron i at r' and at r_i (lanes over partners)
       for (int j = t; j < N + S.n_nuc; j += L) {
         if (j < N) {
-          if (j == i) continue;
+          if (j == i) {  // zero: the sums below start at +0, so it adds nothing
+            wk.jn[j] = (T)0;
+            wk.jo[j] = (T)0;
+            continue;
+          }
           const double b = ((j < nu ? 0 : 1) == s) ? S.p[1] : S.p[0];
           wk.jn[j] = pair_term<T>(b, S.p[2], xn, wk.pos + 3 * j);
           wk.jo[j] = pair_term<T>(b, S.p[2], xi, wk.pos + 3 * j);
@@ -453,23 +712,38 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
       }
       basis_to_lds(S, xn, wk.rn, wk.chi, t, L);
       for (int k = t; k < n; k += L) wk.phin[k] = (T)orbital_value(S, k, wk.chi);
+      if constexpr (PROP == kSJPropDiffuse) {
+        drift_terms(S, wk.pos, i, xn, wk.rn, wk.chi, gch, jg, t, L);
+        __syncthreads();
+        orbital_gradients(S, n, gch, og, t, L);
+      }
       __syncthreads();
       if (t == 0) {
-        T rd = (T)0;
-        for (int k = 0; k < n; ++k) rd = fma_t(wk.phin[k], B[k * n + c], rd);
-        T Jn = (T)0, Jo = (T)0;
-        for (int j = 0; j < N + S.n_nuc; ++j) {
-          if (j == i) continue;
-          Jn = Jn + wk.jn[j];
-        }
-        for (int j = 0; j < N + S.n_nuc; ++j) {
-          if (j == i) continue;
-          Jo = Jo + wk.jo[j];
-        }
+        const T rd = seq_dot<8>(wk.phin, B + c, n, n);
+        const T Jn = seq_sum<8>(wk.jn, N + S.n_nuc);
+        const T Jo = seq_sum<8>(wk.jo, N + S.n_nuc);
         const T dJ = Jn - Jo;
         const double rdd = (double)rd;
         const double R2 = (rdd * rdd) * exp_m(2.0 * (double)dJ);
-        const bool accept = min_one(R2) > wk.rng[4 * i + 3];
+        bool accept;
+        if constexpr (PROP == kSJPropBox) {
+          accept = min_one(R2) > wk.rng[4 * i + 3];
+        } else {
+          // metrop.rs:178-197 over electron i's row: reject a sign change,
+          // else min(T(x'->x) R^2 / T(x->x'), 1) > u
+          double vn[3];
+          drift_sum<T>(og, B, n, c, jg, N + S.n_nuc, rdd, vn);
+          double eh[3], el[3];
+          for (int q = 0; q < 3; ++q) {
+            eh[q] = (xi[q] - xn[q]) - vn[q] * a.tau;
+            el[q] = (xn[q] - xi[q]) - wk.mv[3 + q] * a.tau;
+          }
+          const double nh = norm3(eh[0], eh[1], eh[2]);
+          const double nl = norm3(el[0], el[1], el[2]);
+          const double t_high = exp_m(-(nh * nh) / (2.0 * a.tau));
+          const double t_low = exp_m(-(nl * nl) / (2.0 * a.tau));
+          accept = !(rdd < 0.0) && min_one((t_high * R2) / t_low) > wk.rng[4 * i + 3];
+        }
         wk.flag[0] = accept ? 1.0 : 0.0;
         wk.flag[1] = (double)((T)1 / rd);
       }
@@ -479,17 +753,32 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
       if (accept)
         for (int l = t; l < n; l += L) {
           if (l == c) continue;
-          T v = (T)0;
-          for (int k = 0; k < n; ++k) v = fma_t(wk.phin[k], B[k * n + l], v);
-          wk.f[l] = v * inv;
+          wk.f[l] = seq_dot<8>(wk.phin, B + l, n, n) * inv;
         }
       __syncthreads();
-      if (accept)
-        for (int e = t; e < n * n; e += L) {
-          const int k = e / n, l = e % n;
-          if (l == c) continue;
-          B[e] = fma_t(-B[k * n + c], wk.f[l], B[e]);
+      if (accept) {
+        // B[k][l] -= B[k][c] f_l for l != c; loads of a group ahead of its
+        // stores (column c is only read here)
+        const float inv_n = 1.0f / (float)n;
+        for (int e0 = t; e0 < n * n; e0 += 4 * L) {
+          T bkc[4], fl[4], be[4];
+          int ee[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            const int e = e0 + u * L;
+            const int k = small_div(e, inv_n), l = e - k * n;
+            ee[u] = (e < n * n && l != c) ? e : -1;
+            if (ee[u] >= 0) {
+              bkc[u] = B[k * n + c];
+              fl[u] = wk.f[l];
+              be[u] = B[e];
+            }
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u)
+            if (ee[u] >= 0) B[ee[u]] = fma_t(-bkc[u], fl[u], be[u]);
         }
+      }
       __syncthreads();
       if (accept) {
         for (int k = t; k < n; k += L) B[k * n + c] = B[k * n + c] * inv;
@@ -509,7 +798,7 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
         if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
       }
     }
-    p += 4 * (uint64_t)N;
+    p += DPM * (uint64_t)N;
     if (t == 0 && live && S.trace_mask && ls.trace_sweeps + sw < a.cap_sweeps)
       S.trace_mask[(size_t)w * a.cap_sweeps + ls.trace_sweeps + sw] = mask;
     if (sw >= a.sample_start) {
@@ -543,12 +832,12 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
   // ---- state out ----
   __syncthreads();
   for (int e = lane; e < 3 * N * nslots; e += 64) {
-    const int k = e / nslots, sl = e % nslots;
+    const int k = e >> lg_slots, sl = e & (nslots - 1);
     if (sl < tile.nw) S.pos[(size_t)k * W + tile.w0 + sl] = ((double *)(lds + (size_t)sl * lay.bytes + lay.pos))[k];
   }
   T *Bo = (T *)S.B;
   for (int e = lane; e < nB * nslots; e += 64) {
-    const int k = e / nslots, sl = e % nslots;
+    const int k = e >> lg_slots, sl = e & (nslots - 1);
     if (sl < tile.nw) Bo[(size_t)k * W + tile.w0 + sl] = ((T *)(lds + (size_t)sl * lay.bytes + lay.B))[k];
   }
   if (t == 0 && live)
@@ -580,19 +869,26 @@ hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n
   return hipGetLastError();
 }
 
-hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, size_t lds_bytes, hipStream_t stream) {
-  if (n_tiles <= 0) return hipSuccess;
+template <typename T, int PROP>
+static hipError_t launch_sj_t(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
   if (lds_bytes > 65536) {
     // more than the default 64 KiB of dynamic LDS needs an opt-in per kernel
-    const void *fn = fp32 ? (const void *)sj_sweep_kernel<float> : (const void *)sj_sweep_kernel<double>;
-    const hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    const hipError_t e = hipFuncSetAttribute((const void *)sj_sweep_kernel<T, PROP>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
   }
-  if (fp32)
-    hipLaunchKernelGGL(sj_sweep_kernel<float>, dim3(n_tiles), dim3(64), lds_bytes, stream, a);
-  else
-    hipLaunchKernelGGL(sj_sweep_kernel<double>, dim3(n_tiles), dim3(64), lds_bytes, stream, a);
+  hipLaunchKernelGGL((sj_sweep_kernel<T, PROP>), dim3(n_tiles), dim3(64), lds_bytes, stream, a);
   return hipGetLastError();
+}
+
+hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_t lds_bytes,
+                     hipStream_t stream) {
+  if (n_tiles <= 0) return hipSuccess;
+  if (proposal == kSJPropDiffuse)
+    return fp32 ? launch_sj_t<float, kSJPropDiffuse>(a, n_tiles, lds_bytes, stream)
+                : launch_sj_t<double, kSJPropDiffuse>(a, n_tiles, lds_bytes, stream);
+  return fp32 ? launch_sj_t<float, kSJPropBox>(a, n_tiles, lds_bytes, stream)
+              : launch_sj_t<double, kSJPropBox>(a, n_tiles, lds_bytes, stream);
 }
 
 }  // namespace mole

@@ -68,6 +68,19 @@ double orc_exp(double x);
 /* |v|_2 as ndarray-linalg's norm_l2 spells it: sqrt((x*x + y*y) + z*z). */
 double orc_norm3(double x, double y, double z);
 
+/* ln x (x > 0, normal): atanh series of (m-1)/(m+1) after the exponent is
+ * split off; within a few ulp of the correctly rounded log. */
+double orc_log(double x);
+
+/* sin(pi x), cos(pi x) for 0 <= x < 2 by quadrant reduction and Taylor
+ * polynomials on [-pi/4, pi/4]. */
+void orc_sincospi(double x, double *sn, double *cs);
+
+/* Two N(0,1) draws from two u64 draws (Box-Muller with u1 = 1 - Standard,
+ * so u1 > 0).  Stands in for rand 0.5's ziggurat Normal (metrop.rs:160),
+ * whose stream cannot be reproduced here. */
+void orc_box_muller(uint64_t a, uint64_t b, double *z0, double *z1);
+
 /* ------------------------------------------------------------------ */
 /* Two-electron wavefunctions (configs: He singlet, H2 LCAO)           */
 /* ------------------------------------------------------------------ */

@@ -105,6 +105,10 @@ def lib():
     L.orc_exp.argtypes = [d]
     L.orc_norm3.restype = d
     L.orc_norm3.argtypes = [d, d, d]
+    L.orc_log.restype = d
+    L.orc_log.argtypes = [d]
+    L.orc_sincospi.argtypes = [d, pd, pd]
+    L.orc_box_muller.argtypes = [u64, u64, pd, pd]
     pw = C.POINTER(Wf2)
     L.orc_wf2_value.restype = d
     L.orc_wf2_value.argtypes = [pw, pd]
@@ -168,6 +172,22 @@ def helium(zeta=1.6875, b=0.5, c=0.3, hamiltonian=H_ELECTRONIC):
 def h2(zeta=1.0, b=0.5, c=0.3, half_bond=0.7):
     return make_wf2(WF_LCAO2_PADE, (zeta, b, c),
                     [(0.0, 0.0, -half_bond), (0.0, 0.0, half_bond)], [1, 1])
+
+
+def log(x):
+    return lib().orc_log(float(x))
+
+
+def sincospi(x):
+    s, c = C.c_double(), C.c_double()
+    lib().orc_sincospi(float(x), C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def box_muller(a, b):
+    z0, z1 = C.c_double(), C.c_double()
+    lib().orc_box_muller(int(a), int(b), C.byref(z0), C.byref(z1))
+    return z0.value, z1.value
 
 
 def philox(ctr, key):

@@ -116,3 +116,85 @@ double orc_norm3(double x, double y, double z) {
   s = s + z * z;
   return sqrt(s);
 }
+
+/* ln x for finite x > 0 with a normal exponent: x = 2^k m with m in
+ * [sqrt(1/2), sqrt(2)), s = (m - 1)/(m + 1), ln m = 2 atanh(s) =
+ * 2 s sum_{j=0..10} s^(2j)/(2j+1) (|s| <= 0.1716, so the series is exhausted
+ * at s^20), ln x = k ln2_hi + (ln m + k ln2_lo) with the fdlibm split of ln2.
+ * Every step is an IEEE operation, so the device gets the same bits. */
+double orc_log(double x) {
+  uint64_t bits;
+  memcpy(&bits, &x, sizeof bits);
+  int64_t k = (int64_t)(bits >> 52) - 1023;
+  uint64_t mb = (bits & UINT64_C(0x000FFFFFFFFFFFFF)) | UINT64_C(0x3FF0000000000000);
+  double m;
+  memcpy(&m, &mb, sizeof m);
+  if (m > 1.41421356237309504880) {
+    m = m * 0.5;
+    k = k + 1;
+  }
+  double f = m - 1.0;
+  double s = f / (2.0 + f);
+  double s2 = s * s;
+  double p = 1.0 / 21.0;
+  p = fma(p, s2, 1.0 / 19.0);
+  p = fma(p, s2, 1.0 / 17.0);
+  p = fma(p, s2, 1.0 / 15.0);
+  p = fma(p, s2, 1.0 / 13.0);
+  p = fma(p, s2, 1.0 / 11.0);
+  p = fma(p, s2, 1.0 / 9.0);
+  p = fma(p, s2, 1.0 / 7.0);
+  p = fma(p, s2, 1.0 / 5.0);
+  p = fma(p, s2, 1.0 / 3.0);
+  p = fma(p, s2, 1.0);
+  double lnm = (2.0 * s) * p;
+  double kd = (double)k;
+  return fma(kd, LN2_HI, fma(kd, LN2_LO, lnm));
+}
+
+/* sin(pi x), cos(pi x) for 0 <= x < 2: n = rint(2x), y = x - n/2 in
+ * [-1/4, 1/4] (exact), t = pi y, Taylor polynomials of sin t (to t^17) and
+ * cos t (to t^18) by Horner with fma, then the quadrant n. */
+void orc_sincospi(double x, double *sn, double *cs) {
+  double nd = rint(2.0 * x);
+  double y = x - 0.5 * nd;
+  double t = 3.14159265358979323846 * y;
+  double t2 = t * t;
+  double ps = -1.0 / 355687428096000.0;      /* -1/17! */
+  ps = fma(ps, t2, 1.0 / 1307674368000.0);  /* 1/15! */
+  ps = fma(ps, t2, -1.0 / 6227020800.0);    /* -1/13! */
+  ps = fma(ps, t2, 1.0 / 39916800.0);       /* 1/11! */
+  ps = fma(ps, t2, -1.0 / 362880.0);
+  ps = fma(ps, t2, 1.0 / 5040.0);
+  ps = fma(ps, t2, -1.0 / 120.0);
+  ps = fma(ps, t2, 1.0 / 6.0);
+  double s = fma(-(t * t2), ps, t); /* t - t^3 (1/6 - t^2/120 + ...) */
+  double pc = 1.0 / 6402373705728000.0;     /* 1/18! */
+  pc = fma(pc, t2, -1.0 / 20922789888000.0); /* -1/16! */
+  pc = fma(pc, t2, 1.0 / 87178291200.0);     /* 1/14! */
+  pc = fma(pc, t2, -1.0 / 479001600.0);      /* -1/12! */
+  pc = fma(pc, t2, 1.0 / 3628800.0);
+  pc = fma(pc, t2, -1.0 / 40320.0);
+  pc = fma(pc, t2, 1.0 / 720.0);
+  pc = fma(pc, t2, -1.0 / 24.0);
+  pc = fma(pc, t2, 0.5);
+  double c = fma(-t2, pc, 1.0);
+  switch ((int)nd & 3) {
+    case 0: *sn = s; *cs = c; break;
+    case 1: *sn = c; *cs = -s; break;
+    case 2: *sn = -s; *cs = -c; break;
+    default: *sn = -c; *cs = s; break;
+  }
+}
+
+/* Box-Muller pair from two u64 draws: u1 = 1 - Standard(a) in (0, 1],
+ * u2 = Standard(b); r = sqrt(-2 ln u1); (r cos 2 pi u2, r sin 2 pi u2). */
+void orc_box_muller(uint64_t a, uint64_t b, double *z0, double *z1) {
+  double u1 = 1.0 - orc_u64_standard(a);
+  double u2 = orc_u64_standard(b);
+  double r = sqrt(-2.0 * orc_log(u1));
+  double sn, cs;
+  orc_sincospi(2.0 * u2, &sn, &cs);
+  *z0 = r * cs;
+  *z1 = r * sn;
+}

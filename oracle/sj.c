@@ -349,6 +349,92 @@ double orc_sj_local_energy_full(const orc_sj *wf, const double *cfg, double *dlo
   return orc_sj_local_energy(wf, cfg, Bu, Bd, dlog);
 }
 
+/* ---- drift (grad_i ln psi) pieces for the diffusion proposal ---------- */
+
+/* sum_k grad phi_k(r) Bcol[k]: basis derivatives at r, orbital gradients as
+ * fma chains over the basis, then fma chains over k (FP64). */
+static void det_grad(const orc_sj *wf, const double *r, int n, const double *Bcol, double g[3]) {
+  double gch[MAXB][3];
+  for (int m = 0; m < wf->n_basis; ++m) {
+    int I = wf->center[m];
+    double d[3] = {r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]};
+    double rr = orc_norm3(d[0], d[1], d[2]);
+    double z = wf->zeta[m];
+    double ch = orc_exp(-z * rr);
+    double gf = -z * ch / rr;
+    for (int q = 0; q < 3; ++q) gch[m][q] = gf * d[q];
+  }
+  g[0] = g[1] = g[2] = 0.0;
+  for (int k = 0; k < n; ++k) {
+    const double *cf = wf->coef + (size_t)k * wf->n_basis;
+    double gk[3] = {0, 0, 0};
+    for (int m = 0; m < wf->n_basis; ++m)
+      for (int q = 0; q < 3; ++q) gk[q] = fma(cf[m], gch[m][q], gk[q]);
+    for (int q = 0; q < 3; ++q) g[q] = fma(gk[q], Bcol[k], g[q]);
+  }
+}
+
+/* grad of J_i at electron i placed at r: pair terms j != i in order, then
+ * the nuclei (FP64). */
+static void jas_grad(const orc_sj *wf, const double *cfg, int i, const double *r, double g[3]) {
+  int N = wf->n_up + wf->n_dn;
+  const double c = wf->p[2], a = wf->p[3], d = wf->p[4];
+  g[0] = g[1] = g[2] = 0.0;
+  for (int j = 0; j < N; ++j) {
+    if (j == i) continue;
+    const double *xj = cfg + 3 * j;
+    double dv[3] = {r[0] - xj[0], r[1] - xj[1], r[2] - xj[2]};
+    double rr = orc_norm3(dv[0], dv[1], dv[2]);
+    double b = pair_b(wf, i, j);
+    double den = 1.0 + c * rr;
+    double up = b / (den * den);
+    for (int q = 0; q < 3; ++q) g[q] += up * dv[q] / rr;
+  }
+  for (int I = 0; I < wf->n_nuc; ++I) {
+    double dI[3] = {r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]};
+    double rr = orc_norm3(dI[0], dI[1], dI[2]);
+    double den = 1.0 + d * rr;
+    double vp = -(a * (double)wf->charge[I]) / (den * den);
+    for (int q = 0; q < 3; ++q) g[q] += vp * dI[q] / rr;
+  }
+}
+
+void orc_sj_drift(const orc_sj *wf, const double *cfg, int i, double *v) {
+  int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
+  double B[MAXN * MAXN], col[MAXN], gd[3], gj[3];
+  orc_sj_inverse(wf, cfg, sp, B);
+  for (int k = 0; k < n; ++k) col[k] = B[k * n + c];
+  det_grad(wf, cfg + 3 * i, n, col, gd);
+  jas_grad(wf, cfg, i, cfg + 3 * i, gj);
+  for (int q = 0; q < 3; ++q) v[q] = gd[q] + gj[q];
+}
+
+/* Accept rule of the diffusion move (metrop.rs:178-197, electron i's row):
+ * Bcol[k * stride] is column c of the inverse before the move, rd the
+ * determinant ratio, R2 = rd^2 exp(2 dJ), v the drift at x. */
+static int diffuse_accept(const orc_sj *wf, const double *cfg, int i, const double *xn, int n,
+                          const double *Bcol, int stride, double rd, double R2, const double *v,
+                          double tau, double u) {
+  double col[MAXN];
+  for (int k = 0; k < n; ++k) col[k] = Bcol[k * stride];
+  double gd[3], gj[3], vn[3];
+  det_grad(wf, xn, n, col, gd);
+  jas_grad(wf, cfg, i, xn, gj);
+  for (int q = 0; q < 3; ++q) vn[q] = gd[q] / rd + gj[q];
+  const double *x = cfg + 3 * i;
+  double eh[3], el[3];
+  for (int q = 0; q < 3; ++q) {
+    eh[q] = (x[q] - xn[q]) - vn[q] * tau;
+    el[q] = (xn[q] - x[q]) - v[q] * tau;
+  }
+  double nh = orc_norm3(eh[0], eh[1], eh[2]);
+  double nl = orc_norm3(el[0], el[1], el[2]);
+  double t_high = orc_exp(-(nh * nh) / (2.0 * tau));
+  double t_low = orc_exp(-(nl * nl) / (2.0 * tau));
+  if (rd < 0.0) return 0;
+  return rust_min1((t_high * R2) / t_low) > u;
+}
+
 /* ---- runners ---------------------------------------------------------- */
 
 typedef struct {
@@ -401,12 +487,35 @@ static void sj_walker(sj_task *t, uint32_t w) {
     for (uint32_t s = 0; s < rp->block_size; ++s) {
       uint64_t mask = 0;
       for (int i = 0; i < N; ++i) {
-        double dd[3];
-        for (int k = 0; k < 3; ++k)
-          dd[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++), -half, half);
-        double u = orc_u64_standard(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++));
-        double xn[3];
-        for (int k = 0; k < 3; ++k) xn[k] = cfg[3 * i + k] + dd[k];
+        double xn[3], u;
+        double vold[3] = {0, 0, 0}; /* drift at x (diffusion) */
+        const int diffuse = !t->faithful && rp->proposal == ORC_SJ_DIFFUSE;
+        if (!diffuse) {
+          double dd[3];
+          for (int k = 0; k < 3; ++k)
+            dd[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++), -half, half);
+          u = orc_u64_standard(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++));
+          for (int k = 0; k < 3; ++k) xn[k] = cfg[3 * i + k] + dd[k];
+        } else {
+          uint64_t dr[6];
+          for (int k = 0; k < 6; ++k) dr[k] = orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++);
+          double z[4];
+          orc_box_muller(dr[0], dr[1], &z[0], &z[1]);
+          orc_box_muller(dr[2], dr[3], &z[2], &z[3]);
+          u = orc_u64_standard(dr[4]);
+          int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
+          double Bcol[MAXN];
+          for (int k = 0; k < n; ++k)
+            Bcol[k] = prec == ORC_PREC_FP64 ? (sp == 0 ? Bu : Bd)[k * n + c] : (double)(sp == 0 ? Buf : Bdf)[k * n + c];
+          double gd[3], gj[3];
+          det_grad(wf, cfg + 3 * i, n, Bcol, gd);
+          jas_grad(wf, cfg, i, cfg + 3 * i, gj);
+          const double st = sqrt(rp->tau);
+          for (int q = 0; q < 3; ++q) {
+            vold[q] = gd[q] + gj[q];
+            xn[q] = (cfg[3 * i + q] + vold[q] * rp->tau) + st * z[q];
+          }
+        }
         int acc;
         t->n_moves++;
         if (t->faithful) {
@@ -429,7 +538,8 @@ static void sj_walker(sj_task *t, uint32_t w) {
             double Jo = jas_i_d(wf, cfg, i, cfg + 3 * i);
             double dJ = Jn - Jo;
             double R2 = (rd * rd) * orc_exp(2.0 * dJ);
-            acc = rust_min1(R2) > u;
+            acc = diffuse ? diffuse_accept(wf, cfg, i, xn, n, B + c, n, rd, R2, vold, rp->tau, u)
+                          : rust_min1(R2) > u;
             if (acc) {
               double inv = 1.0 / rd;
               double f[MAXN];
@@ -455,7 +565,13 @@ static void sj_walker(sj_task *t, uint32_t w) {
             float dJ = Jn - Jo;
             double rdd = (double)rd;
             double R2 = (rdd * rdd) * orc_exp(2.0 * (double)dJ);
-            acc = rust_min1(R2) > u;
+            if (diffuse) {
+              double Bcol[MAXN];
+              for (int k = 0; k < n; ++k) Bcol[k] = (double)B[k * n + c];
+              acc = diffuse_accept(wf, cfg, i, xn, n, Bcol, 1, rdd, R2, vold, rp->tau, u);
+            } else {
+              acc = rust_min1(R2) > u;
+            }
             if (acc) {
               float inv = 1.0f / rd;
               float f[MAXN];

@@ -62,6 +62,10 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
 /* The same from scratch (inverses by orc_sj_inverse). */
 double orc_sj_local_energy_full(const orc_sj *wf, const double *cfg, double *dlog);
 
+/* drift v = grad_i ln|psi| of electron i (fresh inverse; FP64): the
+ * drift_velocity of metrop.rs:153-155 for one row */
+void orc_sj_drift(const orc_sj *wf, const double *cfg, int i, double *v);
+
 typedef struct {
   uint64_t seed;
   uint32_t sys, epoch;
@@ -72,7 +76,20 @@ typedef struct {
   uint64_t move0;       /* global move counter at start (LU refresh phase) */
   int precision;        /* ORC_PREC_* (parity runner only) */
   uint32_t lu_period;   /* refresh both inverses after every lu_period moves */
+  int proposal;         /* ORC_SJ_BOX or ORC_SJ_DIFFUSE (parity runner only) */
+  double tau;           /* drift-diffusion time step */
 } orc_sj_run_params;
+
+/* Proposals of the parity runner.  BOX: 4 draws per move (3 Range, 1
+ * Standard; metrop.rs:60-82).  DIFFUSE (metrop.rs:150-197 with the
+ * single-electron Green's function): 6 draws per move -- Box-Muller of draws
+ * (0,1) and (2,3) gives z0, z1, z2 (the fourth normal is unused), draw 4 is
+ * the accept uniform, draw 5 is unused -- and
+ *   x' = (x_i + tau v(x)) + sqrt(tau) z,  v = grad_i ln psi,
+ *   reject if the determinant ratio is negative (psi changes sign),
+ *   accept iff min(T(x'->x) R^2 / T(x->x'), 1) > u,
+ *   T(a->b) = exp(-|b - a - tau v(a)|^2 / (2 tau)) over electron i's row. */
+enum { ORC_SJ_BOX = 0, ORC_SJ_DIFFUSE = 1 };
 
 typedef struct {
   double *cfg;           /* in/out [W][3N] */

@@ -29,7 +29,7 @@ class SjRunParams(C.Structure):
         ("seed", C.c_uint64), ("sys", C.c_uint32), ("epoch", C.c_uint32), ("box_side", C.c_double),
         ("steps", C.c_uint32), ("block_size", C.c_uint32), ("init_from_stream", C.c_int),
         ("pos0", C.c_uint64), ("move0", C.c_uint64), ("precision", C.c_int),
-        ("lu_period", C.c_uint32),
+        ("lu_period", C.c_uint32), ("proposal", C.c_int), ("tau", C.c_double),
     ]
 
 
@@ -62,6 +62,7 @@ def lib():
         L.orc_sj_local_energy.argtypes = [ps, _pd, _pd, _pd, _pd]
         L.orc_sj_local_energy_full.restype = C.c_double
         L.orc_sj_local_energy_full.argtypes = [ps, _pd, _pd]
+        L.orc_sj_drift.argtypes = [ps, _pd, C.c_int, _pd]
         for f in (L.orc_sj_run_parity, L.orc_sj_run_faithful):
             f.restype = C.c_int
             f.argtypes = [ps, C.POINTER(SjRunParams), C.c_uint32, C.c_uint32, C.c_int,
@@ -104,6 +105,13 @@ def inverse(wv, x, spin):
     return B[:n * n].reshape(n, n), rc
 
 
+def drift(wv, x, i):
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    v = np.zeros(3)
+    lib().orc_sj_drift(wv.ref(), x.ctypes.data_as(_pd), int(i), v.ctypes.data_as(_pd))
+    return v
+
+
 def local_energy(wv, x):
     x = np.ascontiguousarray(x, dtype=np.float64)
     dl = np.zeros(NPARAM)
@@ -111,13 +119,18 @@ def local_energy(wv, x):
     return e, dl
 
 
-def run(wv, *, seed, walkers, steps, block_size, box_side, faithful=False, precision=PREC_FP64,
-        lu_period=0, sys=0, epoch=0, cfg=None, pos0=0, move0=0, n_threads=1, record=True):
+BOX, DIFFUSE = 0, 1
+
+
+def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, precision=PREC_FP64,
+        lu_period=0, sys=0, epoch=0, cfg=None, pos0=0, move0=0, n_threads=1, record=True,
+        proposal=BOX, tau=0.0):
     L = lib()
     N = wv.N
     rp = SjRunParams(seed=seed, sys=sys, epoch=epoch, box_side=box_side, steps=steps,
                      block_size=block_size, init_from_stream=0 if cfg is not None else 1, pos0=pos0,
-                     move0=move0, precision=precision, lu_period=lu_period)
+                     move0=move0, precision=precision, lu_period=lu_period,
+                     proposal=proposal, tau=tau)
     blocks = steps // block_size
     ns = (blocks - 1) * block_size
     out = SjRunOut()

@@ -19,10 +19,10 @@ def bits(a):
 
 
 def run_device(systems, W, steps, B, *, precision=M.PREC_FP64, lu=16, lanes=None, seed=0, side=0.6,
-               hamiltonian=M.H_ELECTRONIC):
+               hamiltonian=M.H_ELECTRONIC, proposal=M.PROPOSAL_BOX):
     dev = [M.slater_jastrow(s, W, hamiltonian=hamiltonian) for s in systems]
     with M.Context(dev, seed=seed, step=side, block_size=B, flags=M.FLAG_TRACE, record_sweeps=steps,
-                   precision=precision, lu_period=lu) as ctx:
+                   precision=precision, lu_period=lu, proposal=proposal) as ctx:
         if lanes is not None:
             for k in range(len(systems)):
                 ctx.set_walker_lanes(lanes, sys=k)
@@ -38,10 +38,10 @@ def run_device(systems, W, steps, B, *, precision=M.PREC_FP64, lu=16, lanes=None
 
 
 def run_oracle(system, W, steps, B, *, precision=SJ.PREC_FP64, lu=16, seed=0, side=0.6, sys=0,
-               hamiltonian=0):
+               hamiltonian=0, proposal=SJ.BOX):
     wv = SJ.SjWave(system, hamiltonian=hamiltonian)
     return SJ.run(wv, seed=seed, walkers=W, steps=steps, block_size=B, box_side=side,
-                  precision=precision, lu_period=lu, sys=sys, n_threads=8)
+                  precision=precision, lu_period=lu, sys=sys, n_threads=8, proposal=proposal, tau=side)
 
 
 @pytest.mark.parametrize("Z", [3, 6, 10])
@@ -175,3 +175,51 @@ def test_graph_goes_stale_when_lanes_change():
             g.launch(1)
         assert ei.value.status == M.ERR_INVALID_ARGUMENT
         g.close()
+
+
+# ------------------------------------------------ drift-diffusion proposal
+
+@pytest.mark.parametrize("Z,precision", [(3, M.PREC_FP64), (7, M.PREC_FP64), (4, M.PREC_MIXED)])
+def test_diffusion_parity_with_oracle(Z, precision):
+    # MetropolisDiffuse (metrop.rs:150-197, single-row Green's functions):
+    # same Box-Muller normals, drift and accept decisions as the oracle
+    W, steps, B, tau = 48, 24, 8, 0.05
+    system = S.atom(Z)
+    d = run_device([system], W, steps, B, precision=precision, lu=16, side=tau,
+                   proposal=M.PROPOSAL_DIFFUSE)[0]
+    o = run_oracle(system, W, steps, B, precision=precision, lu=16, side=tau, proposal=SJ.DIFFUSE)
+    assert np.array_equal(d["masks"], o["accept_mask"]), "accept/reject decisions differ"
+    assert np.array_equal(bits(d["cfg"]), bits(o["cfg"]))
+    rel = np.abs(d["el"] - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+    assert np.max(rel) <= E_TOL
+
+
+def test_diffusion_heterogeneous_and_lane_independent():
+    systems = [S.helium_product(1.7, 0.5), S.random_molecule(6, seed=8)]
+    ref = None
+    for L in (2, 8, 32):
+        dev = run_device(systems, 24, 12, 4, lanes=L, side=0.1, proposal=M.PROPOSAL_DIFFUSE)
+        if ref is None:
+            ref = dev
+            for k, system in enumerate(systems):
+                o = run_oracle(system, 24, 12, 4, side=0.1, sys=k, proposal=SJ.DIFFUSE)
+                assert np.array_equal(dev[k]["masks"], o["accept_mask"]), k
+                assert np.array_equal(bits(dev[k]["cfg"]), bits(o["cfg"])), k
+        else:
+            for k in range(len(systems)):
+                assert np.array_equal(bits(dev[k]["cfg"]), bits(ref[k]["cfg"])), (L, k)
+                assert np.array_equal(bits(dev[k]["el"]), bits(ref[k]["el"])), (L, k)
+
+
+def test_diffusion_helium_product_anchor():
+    zeta = 1.5
+    sysd = M.slater_jastrow(S.helium_product(zeta, 0.0), 4096)
+    with M.Context([sysd], seed=3, step=0.3, block_size=50, lu_period=32,
+                   proposal=M.PROPOSAL_DIFFUSE) as ctx:
+        ctx.init_walkers()
+        ctx.run(1000)
+        st = ctx.stats()
+    mean = st["sum_e"] / st["n_samples"]
+    nb = st["n_blocks"]
+    err = np.sqrt((st["sum_block_mean2"] / nb - (st["sum_block_mean"] / nb) ** 2) / (nb - 1))
+    assert abs(mean - (zeta**2 - 27 * zeta / 8)) < 4 * err

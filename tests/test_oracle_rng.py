@@ -134,3 +134,58 @@ def test_norm3_is_left_to_right_sum_of_squares():
         s = s + np.float64(y) * y
         s = s + np.float64(z) * z
         assert L.orc_norm3(x, y, z) == float(np.sqrt(s))
+
+
+def _ulps(a, b):
+    return abs(a - b) / math.ulp(abs(b)) if b != 0 else abs(a)
+
+
+def test_log_within_two_ulp_of_libm():
+    rng = np.random.default_rng(11)
+    xs = list(rng.uniform(0.0, 1.0, 20000)) + [2.0 ** -k for k in range(54)]
+    xs += [1.0 - 2.0 ** -53, 0.7071067811865476, 1.4142135623730951, 3.0, 1e300, 1e-300]
+    worst = 0.0
+    for x in xs:
+        r = math.log(x)
+        v = O.log(x)
+        worst = max(worst, _ulps(v, r) if r != 0 else abs(v))
+    assert worst <= 2.0
+    assert O.log(1.0) == 0.0
+
+
+def test_sincospi_against_libm_and_exact_points():
+    rng = np.random.default_rng(12)
+    for x in rng.uniform(0.0, 2.0, 20000):
+        s, c = O.sincospi(x)
+        # pi*x rounds once in libm's argument; allow that and a few ulp
+        assert abs(s - math.sin(math.pi * x)) <= 2e-15
+        assert abs(c - math.cos(math.pi * x)) <= 2e-15
+    exact = {0.0: (0.0, 1.0), 0.5: (1.0, 0.0), 1.0: (0.0, -1.0), 1.5: (-1.0, 0.0)}
+    for x, (s0, c0) in exact.items():
+        s, c = O.sincospi(x)
+        assert abs(s - s0) <= 1e-300 and abs(c - c0) <= 1e-300, x
+    s, c = O.sincospi(0.25)
+    assert abs(s - math.sqrt(0.5)) <= 1.2e-16 and abs(c - math.sqrt(0.5)) <= 1.2e-16
+
+
+def test_box_muller_is_standard_normal_and_follows_its_recipe():
+    from scipy import stats
+    n = 100000
+    a = [O.lib().orc_stream_u64(5, 0, w, 0, 0) for w in range(n)]
+    b = [O.lib().orc_stream_u64(5, 0, w, 0, 1) for w in range(n)]
+    z = np.array([O.box_muller(x, y) for x, y in zip(a, b)])
+    for col in (z[:, 0], z[:, 1], z.ravel()):
+        assert stats.kstest(col, "norm").pvalue > 1e-3
+    # pairs are uncorrelated
+    assert abs(np.corrcoef(z[:, 0], z[:, 1])[0, 1]) < 0.02
+    # recipe: r = sqrt(-2 ln(1 - U(a))), angle 2 pi U(b)
+    for x, y in list(zip(a, b))[:200]:
+        u1 = 1.0 - O.lib().orc_u64_standard(x)
+        u2 = O.lib().orc_u64_standard(y)
+        r = math.sqrt(-2.0 * math.log(u1))
+        z0, z1 = O.box_muller(x, y)
+        assert abs(z0 - r * math.cos(2 * math.pi * u2)) <= 1e-13 * max(1.0, r)
+        assert abs(z1 - r * math.sin(2 * math.pi * u2)) <= 1e-13 * max(1.0, r)
+    # u1 is never 0: the largest draw still gives a finite radius
+    z0, z1 = O.box_muller(0xFFFFFFFFFFFFFFFF, 0)
+    assert math.isfinite(z0) and z0 > 8.0

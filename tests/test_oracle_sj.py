@@ -152,3 +152,58 @@ def test_helium_product_energy_anchor_through_determinants():
     nb = r["n_blocks"]
     err = np.sqrt((r["sum_bm2"] / nb - (r["sum_bm"] / nb) ** 2) / (nb - 1))
     assert abs(mean - (zeta**2 - 27 * zeta / 8)) < 4 * err
+
+
+@pytest.mark.parametrize("system", [S.atom(4), S.random_molecule(7, seed=3), S.helium_product(1.7, 0.5)])
+def test_drift_matches_finite_differences_of_log_psi(system):
+    # metrop.rs:153-155: drift = grad_i psi / psi
+    wv = SJ.SjWave(system)
+    h = 1e-5
+    for x in cfgs(wv.N, 4, 13):
+        for i in range(wv.N):
+            want = np.zeros(3)
+            for q in range(3):
+                xp, xm = x.copy(), x.copy()
+                xp[3 * i + q] += h
+                xm[3 * i + q] -= h
+                want[q] = (np.log(abs(SJ.value(wv, xp))) - np.log(abs(SJ.value(wv, xm)))) / (2 * h)
+            got = SJ.drift(wv, x, i)
+            assert np.allclose(got, want, rtol=1e-6, atol=1e-6 * max(1.0, np.abs(want).max())), (i, got, want)
+
+
+def test_diffusion_proposal_samples_psi_squared():
+    # He product STO with the drift-diffusion proposal: the exact mean
+    # zeta^2 - 27 zeta/8 (tests/helium_lcao.rs:134) within 4 sigma
+    zeta = 1.5
+    wv = SJ.SjWave(S.helium_product(zeta, 0.0))
+    r = SJ.run(wv, seed=5, walkers=256, steps=1200, block_size=100, lu_period=32, record=False,
+               n_threads=8, proposal=SJ.DIFFUSE, tau=0.3)
+    mean = r["sum_e"] / r["n_samples"]
+    nb = r["n_blocks"]
+    err = np.sqrt((r["sum_bm2"] / nb - (r["sum_bm"] / nb) ** 2) / (nb - 1))
+    assert abs(mean - (zeta**2 - 27 * zeta / 8)) < 4 * err
+    acc = r["n_accepted"] / r["n_moves"]
+    assert 0.5 < acc < 1.0
+
+
+def test_diffusion_acceptance_tends_to_one_for_small_steps():
+    # the Green's-function ratio cancels psi'^2/psi^2 to O(tau^(3/2)) for a
+    # nodeless psi (near a node the unlimited drift of metrop.rs:155 can
+    # overshoot, as in the reference)
+    wv = SJ.SjWave(S.helium_product(1.7, 0.5))
+    accs = []
+    for tau in (1e-2, 1e-4):
+        r = SJ.run(wv, seed=6, walkers=64, steps=40, block_size=20, lu_period=16, record=False,
+                   n_threads=8, proposal=SJ.DIFFUSE, tau=tau)
+        accs.append(r["n_accepted"] / r["n_moves"])
+    assert accs[1] > 0.999 and accs[1] > accs[0]
+
+
+def test_diffusion_mixed_precision_runner_stays_close():
+    wv = SJ.SjWave(S.atom(5))
+    kw = dict(seed=2, walkers=16, steps=20, block_size=10, lu_period=16, n_threads=4,
+              proposal=SJ.DIFFUSE, tau=0.05)
+    a = SJ.run(wv, precision=SJ.PREC_FP64, **kw)
+    b = SJ.run(wv, precision=SJ.PREC_MIXED, **kw)
+    # FP32 Sherman-Morrison and Jastrow flip only rare near-threshold decisions
+    assert np.mean(a["accept_mask"] == b["accept_mask"]) > 0.95
