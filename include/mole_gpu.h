@@ -67,6 +67,14 @@ typedef int32_t mole_status;
  * tau.  Slater-Jastrow systems only; 6 draws per move (Box-Muller normals
  * from draws 0-3, accept uniform from draw 4). */
 #define MOLE_PROPOSAL_DIFFUSE 1
+/* Learned Metropolis-Hastings proposal (no counterpart in the reference; it
+ * plugs into the propose/accept contract of metropolis/src/traits.rs:10-38):
+ * an MLP 13 -> 64 -> 64 -> 64 -> 4 (tanh) maps electron i's neighbourhood to
+ * a shift mu and ln sigma, x' = (x_i + mu) + sigma z, and the move is
+ * accepted with the exact ratio min(R^2 q(x|x')/q(x'|x), 1).  step is
+ * ignored; the network comes from mole_mlp_attach.  Slater-Jastrow systems,
+ * 4 lanes per walker, 6 draws per move as MOLE_PROPOSAL_DIFFUSE. */
+#define MOLE_PROPOSAL_LEARNED 2
 
 /* Convention of the "Parameter gradient" observable O.
  * LOGDERIV:  O = d ln psi / dp.
@@ -299,6 +307,56 @@ mole_status mole_synchronize(mole_ctx *ctx);
  * with HIP events on the context's stream, and the number of sweep-kernel
  * launches it made. */
 mole_status mole_last_timing(mole_ctx *ctx, double *ms, uint32_t *launches);
+
+/* ---- learned proposal: network, training data, training ------------------
+ * No counterpart in the reference.  The network's inputs for electron i at
+ * r: distance to the nearest nucleus, x_j - r for the three nearest other
+ * electrons (nearest first; zeros when missing), r.  Outputs: mu = out[0..2],
+ * ln sigma = ln 0.25 + clamp(out[3], -7, 1).  Parameters are FP32 in the
+ * order W1[13][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][4]
+ * b4[4] (z = b + x W); the kernels run bf16 operands with FP32 accumulation.
+ * Training maximises the mean of A |x' - x_i|^2 (expected squared jump
+ * distance times acceptance) over (configuration, electron) samples with
+ * Adam; the reverse evaluation's inputs are held constant in the gradient. */
+#define MOLE_MLP_PARAMS 9476
+typedef struct mole_mlp mole_mlp;
+
+typedef struct {
+  uint32_t steps;  /* Adam steps */
+  uint32_t batch;  /* samples per step, a multiple of 16 */
+  float lr, beta1, beta2, eps;
+  uint64_t seed;   /* Philox key of the training proposals' normals */
+} mole_train_desc;
+
+/* Xavier-uniform weights from the Philox stream of seed, zero biases. */
+mole_status mole_mlp_create(uint64_t seed, mole_mlp **out);
+void mole_mlp_destroy(mole_mlp *m);
+mole_status mole_mlp_set_weights(mole_mlp *m, const float *w, size_t n);
+mole_status mole_mlp_get_weights(mole_mlp *m, float *w, size_t n);
+/* the network a MOLE_PROPOSAL_LEARNED context proposes with (not owned;
+ * keep it alive while the context sweeps) */
+mole_status mole_mlp_attach(mole_ctx *ctx, mole_mlp *m);
+/* Runs n_snapshots x sweeps_between unsampled sweeps of ctx and appends the
+ * walkers' configurations of system sys after each group to the training
+ * set (N <= 16, spin blocks <= 8). */
+mole_status mole_mlp_collect(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t n_snapshots,
+                             uint32_t sweeps_between);
+mole_status mole_mlp_clear_data(mole_mlp *m);
+mole_status mole_mlp_data_size(mole_mlp *m, size_t *n_cfg);
+/* Adam steps on the training set; sample g (g = step * batch + b) moves
+ * electron g mod N of configuration (g / N) mod n_cfg with normals from draws
+ * 0..3 of the Philox stream (seed, system 0x6D6C70, lo32(g), hi32(g)).
+ * objective (optional, [steps]) receives the batch mean of A |dx|^2. */
+mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_train_desc *d,
+                           double *objective);
+/* One batch without an update: grad[MOLE_MLP_PARAMS] = sum over the samples
+ * first .. first+batch-1 of d(A |dx|^2)/dtheta; objective = their sum. */
+mole_status mole_mlp_gradient(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
+                              uint64_t seed, float *grad, double *objective);
+/* Network outputs out[4 e .. 4 e + 3] for electron electron[e] of
+ * configuration e of cfg[n][3N] (nuclei nuc_xyz[n_nuc][3]). */
+mole_status mole_mlp_eval(mole_mlp *m, const double *cfg, uint32_t N, const double *nuc_xyz, uint32_t n_nuc,
+                          const int32_t *electron, uint32_t n, float *out);
 
 #ifdef __cplusplus
 }

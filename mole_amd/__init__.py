@@ -34,6 +34,8 @@ H_ELECTRONIC = 0
 H_IONIC = 1
 PROPOSAL_BOX = 0
 PROPOSAL_DIFFUSE = 1
+PROPOSAL_LEARNED = 2
+MLP_PARAMS = 9476
 O_LOGDERIV = 0
 O_REFERENCE = 1
 FLAG_TRACE = 1
@@ -67,6 +69,11 @@ class MoleError(RuntimeError):
     def __init__(self, status, message):
         super().__init__(f"[{status}] {message}")
         self.status = status
+
+
+class TrainDesc(C.Structure):
+    _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
 
 
 class SystemDesc(C.Structure):
@@ -171,6 +178,17 @@ _SIGS = [
     ("mole_debug_math", C.c_int32, [C.c_int32, C.c_size_t, _pd, _pd, _pd]),
     ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
     ("mole_debug_mfma_bf16", C.c_int32, [_pf, _pf, _pf]),
+    ("mole_mlp_create", C.c_int32, [C.c_uint64, C.POINTER(_P)]),
+    ("mole_mlp_destroy", None, [_P]),
+    ("mole_mlp_set_weights", C.c_int32, [_P, _pf, C.c_size_t]),
+    ("mole_mlp_get_weights", C.c_int32, [_P, _pf, C.c_size_t]),
+    ("mole_mlp_attach", C.c_int32, [_P, _P]),
+    ("mole_mlp_collect", C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("mole_mlp_clear_data", C.c_int32, [_P]),
+    ("mole_mlp_data_size", C.c_int32, [_P, C.POINTER(C.c_size_t)]),
+    ("mole_mlp_train", C.c_int32, [_P, _P, C.c_uint32, C.POINTER(TrainDesc), _pd]),
+    ("mole_mlp_gradient", C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, _pf, _pd]),
+    ("mole_mlp_eval", C.c_int32, [_P, _pd, C.c_uint32, _pd, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _pf]),
 ]
 
 
@@ -458,6 +476,76 @@ def debug_math(op, a, b=None):
     bb = None if b is None else np.ascontiguousarray(b, dtype=np.float64)
     check(lib().mole_debug_math(op, a.size, _dp(a), None if bb is None else _dp(bb), _dp(out)))
     return out
+
+
+class Mlp:
+    """The learned proposal's network (device weights, Adam state, training set)."""
+
+    def __init__(self, seed=2):
+        self.h = _P()
+        check(lib().mole_mlp_create(seed, C.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().mole_mlp_destroy(self.h)
+            self.h = _P()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def weights(self):
+        w = np.zeros(MLP_PARAMS, np.float32)
+        check(lib().mole_mlp_get_weights(self.h, _fp(w), w.size))
+        return w
+
+    def set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        check(lib().mole_mlp_set_weights(self.h, _fp(w), w.size))
+
+    def attach(self, ctx):
+        check(lib().mole_mlp_attach(ctx.h, self.h))
+
+    def collect(self, ctx, n_snapshots, sweeps_between, sys=0):
+        check(lib().mole_mlp_collect(self.h, ctx.h, sys, n_snapshots, sweeps_between))
+
+    def clear_data(self):
+        check(lib().mole_mlp_clear_data(self.h))
+
+    def data_size(self):
+        n = C.c_size_t()
+        check(lib().mole_mlp_data_size(self.h, C.byref(n)))
+        return n.value
+
+    def train(self, ctx, steps, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=11, sys=0):
+        d = TrainDesc(steps=steps, batch=batch, lr=lr, beta1=beta1, beta2=beta2, eps=eps, seed=seed)
+        obj = np.zeros(max(1, steps))
+        check(lib().mole_mlp_train(self.h, ctx.h, sys, C.byref(d), _dp(obj)))
+        return obj[:steps]
+
+    def gradient(self, ctx, batch, first=0, seed=11, sys=0):
+        g = np.zeros(MLP_PARAMS, np.float32)
+        o = C.c_double()
+        check(lib().mole_mlp_gradient(self.h, ctx.h, sys, batch, first, seed, _fp(g), C.byref(o)))
+        return g, o.value
+
+    def eval(self, cfg, nuclei, electron):
+        cfg = np.ascontiguousarray(cfg, dtype=np.float64)
+        n, threeN = cfg.shape
+        nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 3)
+        el = np.ascontiguousarray(electron, dtype=np.int32)
+        out = np.zeros((n, 4), np.float32)
+        check(lib().mole_mlp_eval(self.h, _dp(cfg), threeN // 3, _dp(nuc), nuc.shape[0],
+                                  el.ctypes.data_as(C.POINTER(C.c_int32)), n, _fp(out)))
+        return out
 
 
 def debug_mfma_bf16(A, B):

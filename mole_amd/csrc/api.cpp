@@ -21,6 +21,7 @@
 
 #include "../../include/mole_gpu.h"
 #include "internal.hpp"
+#include "mlp_internal.hpp"
 #include "sj_internal.hpp"
 
 using namespace mole;
@@ -108,6 +109,26 @@ struct mole_ctx {
   int n_tiles = 0;
   size_t lds_bytes = 0, lds_limit = 65536;
   bool sysdev_dirty = true;
+  mole_mlp *mlp = nullptr;  // network of the learned proposal (not owned)
+};
+
+// The learned proposal's network, its Adam state and its training data.
+struct mole_mlp {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr;
+  bf16x4 *d_fwd = nullptr, *d_bwd = nullptr;
+  float *d_bias = nullptr;
+  int adam_step = 0;
+  // training configurations [n_cfg][3N] of one electron count
+  double *d_cfg = nullptr;
+  size_t n_cfg = 0, cap_cfg = 0;
+  uint32_t data_N = 0;
+  // per-step scratch
+  float *d_partial = nullptr;
+  double *d_obj = nullptr, *d_obj_steps = nullptr;
+  int partial_groups = 0, obj_steps_cap = 0;
+  MlpDev net() const { return MlpDev{d_fwd, d_bwd, d_bias}; }
 };
 
 namespace {
@@ -131,11 +152,12 @@ double nuclear_repulsion(const double *xyz, const int32_t *Z, uint32_t n) {
 mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run_desc *run) {
   if (!sys || !run) return fail(MOLE_ERR_INVALID_ARGUMENT, "null descriptor");
   if (n_sys == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "n_sys must be >= 1");
-  if (run->proposal != MOLE_PROPOSAL_BOX && run->proposal != MOLE_PROPOSAL_DIFFUSE)
+  if (run->proposal != MOLE_PROPOSAL_BOX && run->proposal != MOLE_PROPOSAL_DIFFUSE &&
+      run->proposal != MOLE_PROPOSAL_LEARNED)
     return fail(MOLE_ERR_UNSUPPORTED, "proposal %u not supported", run->proposal);
   // rand's Range::new asserts low < high (metrop.rs:66)
-  if (!(run->step > 0.0) || !std::isfinite(run->step))
-    return fail(MOLE_ERR_INVALID_ARGUMENT, "box side must be a positive finite number");
+  if (run->proposal != MOLE_PROPOSAL_LEARNED && (!(run->step > 0.0) || !std::isfinite(run->step)))
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "box side / time step must be a positive finite number");
   if (run->block_size == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "block_size must be >= 1");
   if (run->o_convention > MOLE_O_REFERENCE)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown O convention %u", run->o_convention);
@@ -239,7 +261,9 @@ void free_ctx(mole_ctx *ctx) {
 int sj_tsize(const mole_ctx *ctx) { return ctx->run.precision == MOLE_PREC_MIXED ? 4 : 8; }
 
 int sj_prop(const mole_ctx *ctx) {
-  return ctx->run.proposal == MOLE_PROPOSAL_DIFFUSE ? kSJPropDiffuse : kSJPropBox;
+  return ctx->run.proposal == MOLE_PROPOSAL_DIFFUSE   ? kSJPropDiffuse
+         : ctx->run.proposal == MOLE_PROPOSAL_LEARNED ? kSJPropLearned
+                                                      : kSJPropBox;
 }
 
 // Tiles of 64/L walkers, most expensive first (longest-processing-time
@@ -329,6 +353,7 @@ SJArgs sj_args(const mole_ctx *ctx) {
   a.state = ctx->d_state;
   a.cap_sweeps = (ctx->run.flags & MOLE_FLAG_TRACE) ? ctx->run.record_sweeps : 0;
   a.cap_samples = a.cap_sweeps;
+  if (ctx->mlp) a.net = ctx->mlp->net();
   return a;
 }
 
@@ -390,6 +415,8 @@ mole_status sj_prepare(mole_ctx *ctx, bool init) {
 // Runs n sweeps; sweeps s >= sample_start of this call are sampled.
 mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
   if (n == 0) return ok();
+  if (ctx->run.proposal == MOLE_PROPOSAL_LEARNED && !ctx->mlp)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "learned proposal: attach a network (mole_mlp_attach) first");
   HIP_TRY(hipSetDevice(ctx->device));
   if (ctx->family == kFamSJ && ctx->sysdev_dirty) {
     mole_status st = sj_upload_sysdev(ctx);
@@ -528,7 +555,7 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       S.n_basis = (int)d.n_basis;
       S.n_orb = (int)d.n_orbitals;
       S.v_nn = nuclear_repulsion(d.nuclei_xyz, d.nuclei_charge, d.n_nuclei);
-      S.L = default_lanes((int)S.N);
+      S.L = run->proposal == MOLE_PROPOSAL_LEARNED ? 4 : default_lanes((int)S.N);
       S.n_cols = kSJColumns;
     }
   }
@@ -1023,9 +1050,11 @@ mole_status mole_set_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t lanes) {
   if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "lanes per walker apply to Slater-Jastrow systems");
   if (lanes > 64 || (lanes & (lanes - 1)) != 0)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "lanes must be a power of two up to 64 (or 0)");
+  if (ctx->run.proposal == MOLE_PROPOSAL_LEARNED && lanes != 0 && lanes != 4)
+    return fail(MOLE_ERR_UNSUPPORTED, "the learned proposal runs 16 walkers of 4 lanes per wave");
   SysState &s = ctx->sys[sys];
   const int old = s.L;
-  s.L = lanes ? (int)lanes : default_lanes((int)s.N);
+  s.L = lanes ? (int)lanes : (ctx->run.proposal == MOLE_PROPOSAL_LEARNED ? 4 : default_lanes((int)s.N));
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   st = sj_build_tiles(ctx);
@@ -1068,6 +1097,8 @@ mole_status mole_graph_capture(mole_ctx *ctx, uint32_t n_sweeps, uint32_t flags,
   *out = nullptr;
   if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "graph capture covers Slater-Jastrow contexts");
   if (n_sweeps == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "n_sweeps must be >= 1");
+  if (ctx->run.proposal == MOLE_PROPOSAL_LEARNED && !ctx->mlp)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "learned proposal: attach a network (mole_mlp_attach) first");
   HIP_TRY(hipSetDevice(ctx->device));
   if (ctx->sysdev_dirty) {
     st = sj_upload_sysdev(ctx);
@@ -1152,3 +1183,304 @@ mole_status mole_last_timing(mole_ctx *ctx, double *ms, uint32_t *launches) {
 }
 
 }  // extern "C"
+
+// ---- learned proposal: the network, its data and its training -------------
+
+namespace {
+
+void free_mlp(mole_mlp *m) {
+  if (!m) return;
+  void *ptrs[] = {m->d_w, m->d_m, m->d_v, m->d_grad, m->d_fwd, m->d_bwd, m->d_bias,
+                  m->d_cfg, m->d_partial, m->d_obj, m->d_obj_steps};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (m->stream) (void)hipStreamDestroy(m->stream);
+  delete m;
+}
+
+mole_status mlp_pack(mole_mlp *m) {
+  HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return MOLE_OK;
+}
+
+// The training kernel keeps a sample's configuration and one spin block in
+// fixed-size LDS arrays.
+mole_status mlp_check_system(const SysState &s) {
+  if (s.N > 16 || s.n_up > 8 || s.n_down > 8)
+    return fail(MOLE_ERR_UNSUPPORTED, "learned-proposal training supports N <= 16 and spin blocks <= 8");
+  return MOLE_OK;
+}
+
+}  // namespace
+
+mole_status mole_mlp_create(uint64_t seed, mole_mlp **out) {
+  if (!out) return fail(MOLE_ERR_INVALID_ARGUMENT, "null output");
+  *out = nullptr;
+  mole_mlp *m = new mole_mlp();
+  auto bail = [&](mole_status s) {
+    std::string msg = g_last_error;
+    free_mlp(m);
+    g_last_error = msg;
+    return s;
+  };
+#define MLP_TRY(expr)                                                                       \
+  do {                                                                                      \
+    hipError_t e_ = (expr);                                                                 \
+    if (e_ != hipSuccess) {                                                                 \
+      fail(e_ == hipErrorOutOfMemory ? MOLE_ERR_OUT_OF_MEMORY : MOLE_ERR_DEVICE, "%s: %s", #expr, \
+           hipGetErrorString(e_));                                                          \
+      return bail(e_ == hipErrorOutOfMemory ? MOLE_ERR_OUT_OF_MEMORY : MOLE_ERR_DEVICE);    \
+    }                                                                                       \
+  } while (0)
+  MLP_TRY(hipGetDevice(&m->device));
+  MLP_TRY(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking));
+  MLP_TRY(hipMalloc(&m->d_w, sizeof(float) * kMlpParams));
+  MLP_TRY(hipMalloc(&m->d_m, sizeof(float) * kMlpParams));
+  MLP_TRY(hipMalloc(&m->d_v, sizeof(float) * kMlpParams));
+  MLP_TRY(hipMalloc(&m->d_grad, sizeof(float) * kMlpParams));
+  MLP_TRY(hipMalloc(&m->d_fwd, sizeof(bf16x4) * kFwdFrags * 64));
+  MLP_TRY(hipMalloc(&m->d_bwd, sizeof(bf16x4) * kBwdFrags * 64));
+  MLP_TRY(hipMalloc(&m->d_bias, sizeof(float) * kMlpBiasPad));
+  MLP_TRY(launch_fill((double *)m->d_m, kMlpParams / 2, 0.0, m->stream));
+  MLP_TRY(launch_fill((double *)m->d_v, kMlpParams / 2, 0.0, m->stream));
+  MLP_TRY(launch_mlp_init(m->d_w, seed, m->stream));
+#undef MLP_TRY
+  mole_status st = mlp_pack(m);
+  if (st != MOLE_OK) return bail(st);
+  *out = m;
+  return ok();
+}
+
+void mole_mlp_destroy(mole_mlp *m) {
+  if (!m) return;
+  (void)hipSetDevice(m->device);
+  if (m->stream) (void)hipStreamSynchronize(m->stream);
+  free_mlp(m);
+}
+
+mole_status mole_mlp_set_weights(mole_mlp *m, const float *w, size_t n) {
+  if (!m || !w) return fail(MOLE_ERR_INVALID_ARGUMENT, "null argument");
+  if (n != (size_t)kMlpParams) return fail(MOLE_ERR_SHAPE, "expected %d parameters", kMlpParams);
+  HIP_TRY(hipSetDevice(m->device));
+  HIP_TRY(hipMemcpyAsync(m->d_w, w, sizeof(float) * n, hipMemcpyHostToDevice, m->stream));
+  HIP_TRY(launch_fill((double *)m->d_m, kMlpParams / 2, 0.0, m->stream));
+  HIP_TRY(launch_fill((double *)m->d_v, kMlpParams / 2, 0.0, m->stream));
+  m->adam_step = 0;
+  mole_status st = mlp_pack(m);
+  return st == MOLE_OK ? ok() : st;
+}
+
+mole_status mole_mlp_get_weights(mole_mlp *m, float *w, size_t n) {
+  if (!m || !w) return fail(MOLE_ERR_INVALID_ARGUMENT, "null argument");
+  if (n != (size_t)kMlpParams) return fail(MOLE_ERR_SHAPE, "expected %d parameters", kMlpParams);
+  HIP_TRY(hipSetDevice(m->device));
+  HIP_TRY(hipMemcpyAsync(w, m->d_w, sizeof(float) * n, hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return ok();
+}
+
+mole_status mole_mlp_attach(mole_ctx *ctx, mole_mlp *m) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "the learned proposal moves Slater-Jastrow systems");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->mlp = m;
+  ++ctx->generation;  // captured graphs hold the old network's addresses
+  return ok();
+}
+
+mole_status mole_mlp_collect(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t n_snapshots,
+                             uint32_t sweeps_between) {
+  if (!m) return fail(MOLE_ERR_INVALID_ARGUMENT, "null network");
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "training data come from Slater-Jastrow systems");
+  SysState &s = ctx->sys[sys];
+  st = mlp_check_system(s);
+  if (st != MOLE_OK) return st;
+  if (m->n_cfg && m->data_N != s.N)
+    return fail(MOLE_ERR_SHAPE, "the training set holds %u-electron configurations", m->data_N);
+  if (sweeps_between == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "sweeps_between must be >= 1");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const size_t need = m->n_cfg + (size_t)n_snapshots * s.W;
+  if (need > m->cap_cfg) {
+    size_t cap = std::max(need, 2 * m->cap_cfg);
+    double *nb = nullptr;
+    HIP_TRY(hipMalloc(&nb, sizeof(double) * 3 * s.N * cap));
+    if (m->d_cfg) {
+      HIP_TRY(hipMemcpyAsync(nb, m->d_cfg, sizeof(double) * 3 * s.N * m->n_cfg, hipMemcpyDeviceToDevice,
+                             m->stream));
+      HIP_TRY(hipStreamSynchronize(m->stream));
+      (void)hipFree(m->d_cfg);
+    }
+    m->d_cfg = nb;
+    m->cap_cfg = cap;
+  }
+  m->data_N = s.N;
+  for (uint32_t k = 0; k < n_snapshots; ++k) {
+    st = do_sweeps(ctx, sweeps_between, sweeps_between);
+    if (st != MOLE_OK) return st;
+    // [3N][W] -> [W][3N]: a strided 2D copy
+    for (uint32_t c = 0; c < 3 * s.N; ++c)
+      HIP_TRY(hipMemcpy2DAsync(m->d_cfg + m->n_cfg * 3 * s.N + c, sizeof(double) * 3 * s.N,
+                               s.d_pos + (size_t)c * s.W, sizeof(double), sizeof(double), (size_t)s.W,
+                               hipMemcpyDeviceToDevice, ctx->stream));
+    m->n_cfg += s.W;
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return ok();
+}
+
+mole_status mole_mlp_clear_data(mole_mlp *m) {
+  if (!m) return fail(MOLE_ERR_INVALID_ARGUMENT, "null network");
+  m->n_cfg = 0;
+  m->data_N = 0;
+  return ok();
+}
+
+mole_status mole_mlp_data_size(mole_mlp *m, size_t *n_cfg) {
+  if (!m || !n_cfg) return fail(MOLE_ERR_INVALID_ARGUMENT, "null argument");
+  *n_cfg = m->n_cfg;
+  return ok();
+}
+
+namespace {
+
+// One gradient evaluation of `batch` samples starting at global sample
+// `first` into m->d_grad and *obj_dev (device); returns after enqueueing.
+mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
+                                 uint64_t seed, double *obj_dev) {
+  const int n_mb = (int)(batch / 16);
+  const int G = std::min(n_mb, 1024);
+  if (G > m->partial_groups) {
+    if (m->d_partial) (void)hipFree(m->d_partial);
+    if (m->d_obj) (void)hipFree(m->d_obj);
+    m->d_partial = nullptr;
+    m->d_obj = nullptr;
+    HIP_TRY(hipMalloc(&m->d_partial, sizeof(float) * kMlpParams * (size_t)G));
+    HIP_TRY(hipMalloc(&m->d_obj, sizeof(double) * (size_t)G));
+    m->partial_groups = G;
+  }
+  HIP_TRY(launch_fill((double *)m->d_partial, (size_t)kMlpParams * G / 2, 0.0, m->stream));
+  MlpTrainArgs a{};
+  a.net = m->net();
+  a.sys = ctx->d_sysdev + sys;
+  a.cfg = m->d_cfg;
+  a.n_cfg = (uint32_t)m->n_cfg;
+  a.first = first;
+  a.batch = batch;
+  a.seed = seed;
+  a.partial = m->d_partial;
+  a.obj = m->d_obj;
+  HIP_TRY(launch_mlp_train(a, G, m->stream));
+  HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_grad, obj_dev, m->stream));
+  return MOLE_OK;
+}
+
+mole_status mlp_train_prepare(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!m) return fail(MOLE_ERR_INVALID_ARGUMENT, "null network");
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "training needs a Slater-Jastrow system");
+  st = mlp_check_system(ctx->sys[sys]);
+  if (st != MOLE_OK) return st;
+  if (m->n_cfg == 0 || m->data_N != ctx->sys[sys].N)
+    return fail(MOLE_ERR_SHAPE, "no training configurations of this system's electron count");
+  if (batch == 0 || batch % 16 != 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "batch must be a positive multiple of 16");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->sysdev_dirty) {
+    st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  return MOLE_OK;
+}
+
+}  // namespace
+
+mole_status mole_mlp_gradient(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
+                              uint64_t seed, float *grad, double *objective) {
+  mole_status st = mlp_train_prepare(m, ctx, sys, batch);
+  if (st != MOLE_OK) return st;
+  if (!m->d_obj_steps) {
+    HIP_TRY(hipMalloc(&m->d_obj_steps, sizeof(double) * 1024));
+    m->obj_steps_cap = 1024;
+  }
+  st = mlp_gradient_enqueue(m, ctx, sys, batch, first, seed, m->d_obj_steps);
+  if (st != MOLE_OK) return st;
+  if (grad) HIP_TRY(hipMemcpyAsync(grad, m->d_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
+  double o = 0.0;
+  HIP_TRY(hipMemcpyAsync(&o, m->d_obj_steps, sizeof(double), hipMemcpyDeviceToHost, m->stream));
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  if (objective) *objective = o;
+  return ok();
+}
+
+mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_train_desc *d,
+                           double *objective) {
+  if (!d) return fail(MOLE_ERR_INVALID_ARGUMENT, "null training descriptor");
+  mole_status st = mlp_train_prepare(m, ctx, sys, d->batch);
+  if (st != MOLE_OK) return st;
+  if (!(d->lr > 0.0f) || !(d->beta1 >= 0.0f && d->beta1 < 1.0f) || !(d->beta2 >= 0.0f && d->beta2 < 1.0f) ||
+      !(d->eps > 0.0f))
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "Adam needs lr > 0, 0 <= beta < 1, eps > 0");
+  if (d->steps == 0) return ok();
+  if ((int)d->steps > m->obj_steps_cap) {
+    if (m->d_obj_steps) (void)hipFree(m->d_obj_steps);
+    m->d_obj_steps = nullptr;
+    HIP_TRY(hipMalloc(&m->d_obj_steps, sizeof(double) * std::max<uint32_t>(d->steps, 1024)));
+    m->obj_steps_cap = (int)std::max<uint32_t>(d->steps, 1024);
+  }
+  for (uint32_t k = 0; k < d->steps; ++k) {
+    const uint64_t first = (uint64_t)m->adam_step * d->batch;
+    st = mlp_gradient_enqueue(m, ctx, sys, d->batch, first, d->seed, m->d_obj_steps + k);
+    if (st != MOLE_OK) return st;
+    ++m->adam_step;
+    HIP_TRY(launch_mlp_adam(m->d_w, m->d_m, m->d_v, m->d_grad, 1.0f / (float)d->batch, d->lr, d->beta1,
+                            d->beta2, d->eps, m->adam_step, m->stream));
+    HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
+  }
+  if (objective) {
+    std::vector<double> o(d->steps);
+    HIP_TRY(hipMemcpyAsync(o.data(), m->d_obj_steps, sizeof(double) * d->steps, hipMemcpyDeviceToHost,
+                           m->stream));
+    HIP_TRY(hipStreamSynchronize(m->stream));
+    for (uint32_t k = 0; k < d->steps; ++k) objective[k] = o[k] / (double)d->batch;
+  }
+  HIP_TRY(hipStreamSynchronize(m->stream));
+  return ok();
+}
+
+mole_status mole_mlp_eval(mole_mlp *m, const double *cfg, uint32_t N, const double *nuc_xyz, uint32_t n_nuc,
+                          const int32_t *electron, uint32_t n, float *out) {
+  if (!m || !cfg || !nuc_xyz || !electron || !out) return fail(MOLE_ERR_INVALID_ARGUMENT, "null argument");
+  if (N == 0 || N > 64 || n_nuc == 0) return fail(MOLE_ERR_SHAPE, "need 1 <= N <= 64 and nuclei");
+  for (uint32_t e = 0; e < n; ++e)
+    if (electron[e] < 0 || (uint32_t)electron[e] >= N) return fail(MOLE_ERR_SHAPE, "electron index out of range");
+  if (n == 0) return ok();
+  HIP_TRY(hipSetDevice(m->device));
+  double *dc = nullptr, *dn = nullptr;
+  int *de = nullptr;
+  float *dout = nullptr;
+  auto cleanup = [&]() {
+    if (dc) (void)hipFree(dc);
+    if (dn) (void)hipFree(dn);
+    if (de) (void)hipFree(de);
+    if (dout) (void)hipFree(dout);
+  };
+  hipError_t e = hipMalloc(&dc, sizeof(double) * 3 * N * n);
+  if (e == hipSuccess) e = hipMalloc(&dn, sizeof(double) * 3 * n_nuc);
+  if (e == hipSuccess) e = hipMalloc(&de, sizeof(int) * n);
+  if (e == hipSuccess) e = hipMalloc(&dout, sizeof(float) * 4 * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(dc, cfg, sizeof(double) * 3 * N * n, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(dn, nuc_xyz, sizeof(double) * 3 * n_nuc, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(de, electron, sizeof(int) * n, hipMemcpyHostToDevice, m->stream);
+  if (e == hipSuccess) e = launch_mlp_eval(m->net(), dc, (int)N, dn, (int)n_nuc, de, (int)n, dout, m->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(float) * 4 * n, hipMemcpyDeviceToHost, m->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
+  cleanup();
+  if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_mlp_eval: %s", hipGetErrorString(e));
+  return ok();
+}

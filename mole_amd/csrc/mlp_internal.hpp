@@ -1,0 +1,43 @@
+// Host-visible launch interface of the walker-MLP kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "mlp.hpp"
+#include "sj_internal.hpp"
+
+namespace mole {
+
+struct MlpTrainArgs {
+  MlpDev net;
+  const SJSysDev *sys;  // the system the configurations belong to
+  const double *cfg;    // [n_cfg][3N] positions
+  uint32_t n_cfg;
+  uint64_t first;       // global index of the batch's first sample
+  uint32_t batch;       // samples, a multiple of 16
+  uint64_t seed;        // Philox key of the proposal normals
+  float *partial;       // [gridDim][kMlpParams], zero on entry
+  double *obj;          // [gridDim] sum of A |dx|^2, zero on entry
+};
+
+// Philox system tag of the training normals: sample g draws 0..3 of the
+// stream (seed, kMlpTrainSys, lo32(g), hi32(g)).
+constexpr uint32_t kMlpTrainSys = 0x6D6C70u;
+// Philox epoch of the Xavier initialiser (walker 0, draw p = parameter p).
+constexpr uint32_t kMlpInitEpoch = 0x58415631u;
+
+hipError_t launch_mlp_init(float *w, uint64_t seed, hipStream_t stream);
+hipError_t launch_mlp_pack(const float *w, bf16x4 *fwd, bf16x4 *bwd, float *bias, hipStream_t stream);
+hipError_t launch_mlp_train(const MlpTrainArgs &a, int n_groups, hipStream_t stream);
+// grad[p] = sum_g partial[g][p], obj_sum = sum_g obj[g] (fixed order)
+hipError_t launch_mlp_reduce(const float *partial, const double *obj, int n_groups, float *grad,
+                             double *obj_sum, hipStream_t stream);
+// Adam on the loss -mean(A |dx|^2): g = -grad / batch
+hipError_t launch_mlp_adam(float *w, float *m, float *v, const float *grad, float inv_batch, float lr,
+                           float beta1, float beta2, float eps, int step, hipStream_t stream);
+// features and network outputs of given configurations (debug / tests)
+hipError_t launch_mlp_eval(const MlpDev &net, const double *cfg, int N, const double *nuc, int n_nuc,
+                           const int *electron, int n, float *out, hipStream_t stream);
+
+}  // namespace mole

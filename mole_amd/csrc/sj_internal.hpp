@@ -5,6 +5,8 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "mlp.hpp"
+
 namespace mole {
 
 constexpr int kSJNParam = 5;  // b_anti, b_para, c, a, d
@@ -85,6 +87,7 @@ struct SJArgs {
   uint32_t cap_sweeps, cap_samples;
   int init;                // 1: draw U(-1,1) configurations first (draws 0..3N-1)
   int refresh;             // 1: rebuild the inverses from the positions at entry
+  MlpDev net;              // learned proposal: the packed network
 };
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);

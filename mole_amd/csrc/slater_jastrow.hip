@@ -20,6 +20,7 @@
 //          f_l = (sum_k phi_k(r') B[k][l]) / R_D  (Sherman-Morrison)
 #include "../../include/mole_gpu.h"
 #include "device_common.hpp"
+#include "mlp.hpp"
 #include "sj_internal.hpp"
 
 namespace mole {
@@ -552,14 +553,18 @@ __device__ __forceinline__ void drift_sum(const double *og, const T *B, int n, i
 }  // namespace
 
 template <typename T, int PROP>
-__global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
+__global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
   constexpr int DPM = sj_draws_per_move(PROP);
   extern __shared__ __align__(16) unsigned char lds[];
   const SJTile tile = a.tiles[blockIdx.x];
   const SJSysDev &S = a.systems[tile.sys];
   const int N = S.N, nu = S.n_up, nd = S.n_dn, L = S.L, W = S.W;
   const int lane = threadIdx.x;
-  const int slot = lane / L, t = lane % L;
+  // learned proposal: 16 walkers of 4 lanes, walker s on lanes s, s+16,
+  // s+32, s+48 (the sample columns of the MFMA tiles); otherwise L
+  // consecutive lanes per walker
+  const int slot = PROP == kSJPropLearned ? (lane & 15) : lane / L;
+  const int t = PROP == kSJPropLearned ? (lane >> 4) : lane % L;
   const int nslots = 64 / L;
   const int lg_slots = __ffs(nslots) - 1;  // nslots is a power of two
   const bool live = slot < tile.nw;
@@ -672,6 +677,41 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
         xn[0] = xi[0] + wk.rng[4 * i];
         xn[1] = xi[1] + wk.rng[4 * i + 1];
         xn[2] = xi[2] + wk.rng[4 * i + 2];
+      } else if constexpr (PROP == kSJPropLearned) {
+        // learned proposal: network at x -> (mu, ln sigma), x' = (x_i + mu) +
+        // sigma z; network at x' for the reverse density (all lanes run the
+        // MFMA passes)
+        float fv[4];
+        mlp_features(wk.pos, N, S.nuc, S.n_nuc, i, xi, t, fv);
+        const f32x4 o1 = mlp_forward(a.net, features_bf16(fv), nullptr);
+        if (t == 0) {
+          const double ls = mlp_log_sigma((double)o1[3]);
+          const double sg = exp_m(ls);
+          double z2 = 0.0;
+          for (int q = 0; q < 3; ++q) {
+            const double z = wk.rng[4 * i + q];
+            wk.mv[q] = (xi[q] + (double)o1[q]) + sg * z;
+            z2 += z * z;
+          }
+          wk.mv[3] = -3.0 * ls - 0.5 * z2;  // log q(x'|x) + const
+        }
+        __syncthreads();
+        xn[0] = wk.mv[0];
+        xn[1] = wk.mv[1];
+        xn[2] = wk.mv[2];
+        mlp_features(wk.pos, N, S.nuc, S.n_nuc, i, xn, t, fv);
+        const f32x4 o2 = mlp_forward(a.net, features_bf16(fv), nullptr);
+        if (t == 0) {
+          const double ls2 = mlp_log_sigma((double)o2[3]);
+          const double sg2 = exp_m(ls2);
+          double e2 = 0.0;
+          for (int q = 0; q < 3; ++q) {
+            const double e = (xi[q] - xn[q]) - (double)o2[q];
+            e2 += e * e;
+          }
+          wk.mv[4] = (-3.0 * ls2 - e2 / (2.0 * sg2 * sg2)) - wk.mv[3];  // log q(x|x') - log q(x'|x)
+        }
+        __syncthreads();
       } else {
         // drift at x (metrop.rs:150-163): x' = (x_i + tau v) + sqrt(tau) z
         basis_to_lds(S, xi, wk.rn, wk.chi, t, L);
@@ -728,6 +768,9 @@ __global__ void __launch_bounds__(64) sj_sweep_kernel(SJArgs a) {
         bool accept;
         if constexpr (PROP == kSJPropBox) {
           accept = min_one(R2) > wk.rng[4 * i + 3];
+        } else if constexpr (PROP == kSJPropLearned) {
+          // exact MH ratio of the learned proposal
+          accept = min_one(R2 * exp_m(wk.mv[4])) > wk.rng[4 * i + 3];
         } else {
           // metrop.rs:178-197 over electron i's row: reject a sign change,
           // else min(T(x'->x) R^2 / T(x->x'), 1) > u
@@ -887,6 +930,9 @@ hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_
   if (proposal == kSJPropDiffuse)
     return fp32 ? launch_sj_t<float, kSJPropDiffuse>(a, n_tiles, lds_bytes, stream)
                 : launch_sj_t<double, kSJPropDiffuse>(a, n_tiles, lds_bytes, stream);
+  if (proposal == kSJPropLearned)
+    return fp32 ? launch_sj_t<float, kSJPropLearned>(a, n_tiles, lds_bytes, stream)
+                : launch_sj_t<double, kSJPropLearned>(a, n_tiles, lds_bytes, stream);
   return fp32 ? launch_sj_t<float, kSJPropBox>(a, n_tiles, lds_bytes, stream)
               : launch_sj_t<double, kSJPropBox>(a, n_tiles, lds_bytes, stream);
 }

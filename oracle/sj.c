@@ -3,6 +3,7 @@
  * local energy, and the two Metropolis runners described in sj.h.
  */
 #include "sj.h"
+#include "mlp.h"
 
 #include <math.h>
 #include <pthread.h>
@@ -409,6 +410,22 @@ void orc_sj_drift(const orc_sj *wf, const double *cfg, int i, double *v) {
   for (int q = 0; q < 3; ++q) v[q] = gd[q] + gj[q];
 }
 
+void orc_sj_move_terms(const orc_sj *wf, const double *cfg, int i, const double *xn, double *lR2,
+                       double *g) {
+  int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
+  double B[MAXN * MAXN], col[MAXN], chi[MAXB], gd[3], gj[3];
+  orc_sj_inverse(wf, cfg, sp, B);
+  for (int k = 0; k < n; ++k) col[k] = B[k * n + c];
+  basis_at(wf, xn, chi);
+  double rd = 0.0;
+  for (int k = 0; k < n; ++k) rd = fma(orbital(wf, k, chi), col[k], rd);
+  double dJ = jas_i_d(wf, cfg, i, xn) - jas_i_d(wf, cfg, i, cfg + 3 * i);
+  *lR2 = 2.0 * log(fabs(rd)) + 2.0 * dJ;
+  det_grad(wf, xn, n, col, gd);
+  jas_grad(wf, cfg, i, xn, gj);
+  for (int q = 0; q < 3; ++q) g[q] = gd[q] / rd + gj[q];
+}
+
 /* Accept rule of the diffusion move (metrop.rs:178-197, electron i's row):
  * Bcol[k * stride] is column c of the inverse before the move, rd the
  * determinant ratio, R2 = rd^2 exp(2 dJ), v the drift at x. */
@@ -490,7 +507,36 @@ static void sj_walker(sj_task *t, uint32_t w) {
         double xn[3], u;
         double vold[3] = {0, 0, 0}; /* drift at x (diffusion) */
         const int diffuse = !t->faithful && rp->proposal == ORC_SJ_DIFFUSE;
-        if (!diffuse) {
+        const int learned = !t->faithful && rp->proposal == ORC_SJ_LEARNED;
+        double lq_ratio = 0.0;
+        if (learned) {
+          uint64_t dr[6];
+          for (int k = 0; k < 6; ++k) dr[k] = orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++);
+          double z[4];
+          orc_box_muller(dr[0], dr[1], &z[0], &z[1]);
+          orc_box_muller(dr[2], dr[3], &z[2], &z[3]);
+          u = orc_u64_standard(dr[4]);
+          double f1[ORC_MLP_NIN], f2[ORC_MLP_NIN], o1[ORC_MLP_NOUT], o2[ORC_MLP_NOUT];
+          orc_mlp_features(wf, cfg, i, cfg + 3 * i, f1);
+          orc_mlp_forward(rp->mlp_w, f1, rp->mlp_emulate_bf16, o1, NULL);
+          double ls = orc_mlp_log_sigma(o1[3]), sg = exp(ls), z2 = 0.0;
+          for (int q = 0; q < 3; ++q) {
+            xn[q] = (cfg[3 * i + q] + o1[q]) + sg * z[q];
+            z2 += z[q] * z[q];
+          }
+          double cfg2[3 * MAXE];
+          memcpy(cfg2, cfg, sizeof(double) * 3 * N);
+          memcpy(cfg2 + 3 * i, xn, sizeof xn);
+          orc_mlp_features(wf, cfg2, i, xn, f2);
+          orc_mlp_forward(rp->mlp_w, f2, rp->mlp_emulate_bf16, o2, NULL);
+          double ls2 = orc_mlp_log_sigma(o2[3]), sg2 = exp(ls2), e2 = 0.0;
+          for (int q = 0; q < 3; ++q) {
+            double e = (cfg[3 * i + q] - xn[q]) - o2[q];
+            e2 += e * e;
+          }
+          lq_ratio = (-3.0 * ls2 - e2 / (2.0 * sg2 * sg2)) - (-3.0 * ls - 0.5 * z2);
+          if (rp->mlp_drop_q_ratio) lq_ratio = 0.0;
+        } else if (!diffuse) {
           double dd[3];
           for (int k = 0; k < 3; ++k)
             dd[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++), -half, half);
@@ -538,8 +584,9 @@ static void sj_walker(sj_task *t, uint32_t w) {
             double Jo = jas_i_d(wf, cfg, i, cfg + 3 * i);
             double dJ = Jn - Jo;
             double R2 = (rd * rd) * orc_exp(2.0 * dJ);
-            acc = diffuse ? diffuse_accept(wf, cfg, i, xn, n, B + c, n, rd, R2, vold, rp->tau, u)
-                          : rust_min1(R2) > u;
+            acc = diffuse   ? diffuse_accept(wf, cfg, i, xn, n, B + c, n, rd, R2, vold, rp->tau, u)
+                  : learned ? rust_min1(R2 * exp(lq_ratio)) > u
+                            : rust_min1(R2) > u;
             if (acc) {
               double inv = 1.0 / rd;
               double f[MAXN];
@@ -569,6 +616,8 @@ static void sj_walker(sj_task *t, uint32_t w) {
               double Bcol[MAXN];
               for (int k = 0; k < n; ++k) Bcol[k] = (double)B[k * n + c];
               acc = diffuse_accept(wf, cfg, i, xn, n, Bcol, 1, rdd, R2, vold, rp->tau, u);
+            } else if (learned) {
+              acc = rust_min1(R2 * exp(lq_ratio)) > u;
             } else {
               acc = rust_min1(R2) > u;
             }

@@ -66,6 +66,12 @@ double orc_sj_local_energy_full(const orc_sj *wf, const double *cfg, double *dlo
  * drift_velocity of metrop.rs:153-155 for one row */
 void orc_sj_drift(const orc_sj *wf, const double *cfg, int i, double *v);
 
+/* Single-electron move of electron i to xn (fresh FP64 inverse):
+ * lR2 = ln (psi(x')/psi(x))^2 and g = grad ln|psi| at x' with respect to
+ * electron i. */
+void orc_sj_move_terms(const orc_sj *wf, const double *cfg, int i, const double *xn, double *lR2,
+                       double *g);
+
 typedef struct {
   uint64_t seed;
   uint32_t sys, epoch;
@@ -76,8 +82,11 @@ typedef struct {
   uint64_t move0;       /* global move counter at start (LU refresh phase) */
   int precision;        /* ORC_PREC_* (parity runner only) */
   uint32_t lu_period;   /* refresh both inverses after every lu_period moves */
-  int proposal;         /* ORC_SJ_BOX or ORC_SJ_DIFFUSE (parity runner only) */
+  int proposal;         /* ORC_SJ_BOX, ORC_SJ_DIFFUSE or ORC_SJ_LEARNED (parity runner only) */
   double tau;           /* drift-diffusion time step */
+  const float *mlp_w;   /* learned proposal: network parameters (mlp.h) */
+  int mlp_emulate_bf16; /* learned proposal: round operands to bf16 */
+  int mlp_drop_q_ratio; /* tests only: a planted mistake (omit q(x|x')/q(x'|x)) */
 } orc_sj_run_params;
 
 /* Proposals of the parity runner.  BOX: 4 draws per move (3 Range, 1
@@ -89,7 +98,9 @@ typedef struct {
  *   reject if the determinant ratio is negative (psi changes sign),
  *   accept iff min(T(x'->x) R^2 / T(x->x'), 1) > u,
  *   T(a->b) = exp(-|b - a - tau v(a)|^2 / (2 tau)) over electron i's row. */
-enum { ORC_SJ_BOX = 0, ORC_SJ_DIFFUSE = 1 };
+enum { ORC_SJ_BOX = 0, ORC_SJ_DIFFUSE = 1, ORC_SJ_LEARNED = 2 };
+/* LEARNED: the draws of DIFFUSE; x' = (x_i + mu) + sigma z with (mu, ln sigma)
+ * from the network (mlp.h); accept iff min(R^2 q(x|x')/q(x'|x), 1) > u. */
 
 typedef struct {
   double *cfg;           /* in/out [W][3N] */

@@ -30,6 +30,7 @@ class SjRunParams(C.Structure):
         ("steps", C.c_uint32), ("block_size", C.c_uint32), ("init_from_stream", C.c_int),
         ("pos0", C.c_uint64), ("move0", C.c_uint64), ("precision", C.c_int),
         ("lu_period", C.c_uint32), ("proposal", C.c_int), ("tau", C.c_double),
+        ("mlp_w", C.POINTER(C.c_float)), ("mlp_emulate_bf16", C.c_int), ("mlp_drop_q_ratio", C.c_int),
     ]
 
 
@@ -119,18 +120,22 @@ def local_energy(wv, x):
     return e, dl
 
 
-BOX, DIFFUSE = 0, 1
+BOX, DIFFUSE, LEARNED = 0, 1, 2
 
 
 def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, precision=PREC_FP64,
         lu_period=0, sys=0, epoch=0, cfg=None, pos0=0, move0=0, n_threads=1, record=True,
-        proposal=BOX, tau=0.0):
+        proposal=BOX, tau=0.0, mlp_w=None, emulate_bf16=False, drop_q_ratio=False):
     L = lib()
     N = wv.N
     rp = SjRunParams(seed=seed, sys=sys, epoch=epoch, box_side=box_side, steps=steps,
                      block_size=block_size, init_from_stream=0 if cfg is not None else 1, pos0=pos0,
                      move0=move0, precision=precision, lu_period=lu_period,
-                     proposal=proposal, tau=tau)
+                     proposal=proposal, tau=tau, mlp_emulate_bf16=int(emulate_bf16),
+                     mlp_drop_q_ratio=int(drop_q_ratio))
+    if mlp_w is not None:
+        mlp_w = np.ascontiguousarray(mlp_w, dtype=np.float32)
+        rp.mlp_w = mlp_w.ctypes.data_as(C.POINTER(C.c_float))
     blocks = steps // block_size
     ns = (blocks - 1) * block_size
     out = SjRunOut()
