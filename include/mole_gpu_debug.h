@@ -43,6 +43,11 @@ mole_status mole_debug_math(int32_t op, size_t n, const double *a, const double 
  * use: pins that placement against a host product. */
 mole_status mole_debug_mfma_bf16(const float *A, const float *B, float *D);
 
+/* D (16x16) = A (16x32) B (32x16) through one v_mfma_f32_16x16x32_bf16 with
+ * the walker MLP's placement: lane l holds A[l%16][8(l/16)+j] and
+ * B[8(l/16)+j][l%16], j = 0..7. */
+mole_status mole_debug_mfma_bf16_k32(const float *A, const float *B, float *D);
+
 #ifdef __cplusplus
 }
 #endif

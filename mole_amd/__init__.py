@@ -178,6 +178,7 @@ _SIGS = [
     ("mole_debug_math", C.c_int32, [C.c_int32, C.c_size_t, _pd, _pd, _pd]),
     ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
     ("mole_debug_mfma_bf16", C.c_int32, [_pf, _pf, _pf]),
+    ("mole_debug_mfma_bf16_k32", C.c_int32, [_pf, _pf, _pf]),
     ("mole_mlp_create", C.c_int32, [C.c_uint64, C.POINTER(_P)]),
     ("mole_mlp_destroy", None, [_P]),
     ("mole_mlp_set_weights", C.c_int32, [_P, _pf, C.c_size_t]),
@@ -555,6 +556,16 @@ def debug_mfma_bf16(A, B):
     assert A.shape == (16, 16) and B.shape == (16, 16)
     D = np.zeros((16, 16), np.float32)
     check(lib().mole_debug_mfma_bf16(_fp(A), _fp(B), _fp(D)))
+    return D
+
+
+def debug_mfma_bf16_k32(A, B):
+    """One 16x16x32 bf16 MFMA: A (16x32) times B (32x16), FP32 in, rounded to bf16."""
+    A = np.ascontiguousarray(A, dtype=np.float32)
+    B = np.ascontiguousarray(B, dtype=np.float32)
+    assert A.shape == (16, 32) and B.shape == (32, 16)
+    D = np.zeros((16, 16), np.float32)
+    check(lib().mole_debug_mfma_bf16_k32(_fp(A), _fp(B), _fp(D)))
     return D
 
 

@@ -117,7 +117,8 @@ struct mole_mlp {
   int device = 0;
   hipStream_t stream = nullptr;
   float *d_w = nullptr, *d_m = nullptr, *d_v = nullptr, *d_grad = nullptr;
-  bf16x4 *d_fwd = nullptr, *d_bwd = nullptr;
+  bf16x4 *d_fwd1 = nullptr, *d_bwd = nullptr;
+  bf16x8 *d_fwd = nullptr;
   float *d_bias = nullptr;
   int adam_step = 0;
   // training configurations [n_cfg][3N] of one electron count
@@ -128,7 +129,7 @@ struct mole_mlp {
   float *d_partial = nullptr;
   double *d_obj = nullptr, *d_obj_steps = nullptr;
   int partial_groups = 0, obj_steps_cap = 0;
-  MlpDev net() const { return MlpDev{d_fwd, d_bwd, d_bias}; }
+  MlpDev net() const { return MlpDev{d_fwd1, d_fwd, d_bwd, d_bias}; }
 };
 
 namespace {
@@ -1190,7 +1191,7 @@ namespace {
 
 void free_mlp(mole_mlp *m) {
   if (!m) return;
-  void *ptrs[] = {m->d_w, m->d_m, m->d_v, m->d_grad, m->d_fwd, m->d_bwd, m->d_bias,
+  void *ptrs[] = {m->d_w, m->d_m, m->d_v, m->d_grad, m->d_fwd1, m->d_fwd, m->d_bwd, m->d_bias,
                   m->d_cfg, m->d_partial, m->d_obj, m->d_obj_steps};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
@@ -1199,7 +1200,7 @@ void free_mlp(mole_mlp *m) {
 }
 
 mole_status mlp_pack(mole_mlp *m) {
-  HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
+  HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd1, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
   HIP_TRY(hipStreamSynchronize(m->stream));
   return MOLE_OK;
 }
@@ -1239,7 +1240,8 @@ mole_status mole_mlp_create(uint64_t seed, mole_mlp **out) {
   MLP_TRY(hipMalloc(&m->d_m, sizeof(float) * kMlpParams));
   MLP_TRY(hipMalloc(&m->d_v, sizeof(float) * kMlpParams));
   MLP_TRY(hipMalloc(&m->d_grad, sizeof(float) * kMlpParams));
-  MLP_TRY(hipMalloc(&m->d_fwd, sizeof(bf16x4) * kFwdFrags * 64));
+  MLP_TRY(hipMalloc(&m->d_fwd1, sizeof(bf16x4) * kFwd1Frags * 64));
+  MLP_TRY(hipMalloc(&m->d_fwd, sizeof(bf16x8) * kFwdFrags * 64));
   MLP_TRY(hipMalloc(&m->d_bwd, sizeof(bf16x4) * kBwdFrags * 64));
   MLP_TRY(hipMalloc(&m->d_bias, sizeof(float) * kMlpBiasPad));
   MLP_TRY(launch_fill((double *)m->d_m, kMlpParams / 2, 0.0, m->stream));
@@ -1440,7 +1442,7 @@ mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_
     ++m->adam_step;
     HIP_TRY(launch_mlp_adam(m->d_w, m->d_m, m->d_v, m->d_grad, 1.0f / (float)d->batch, d->lr, d->beta1,
                             d->beta2, d->eps, m->adam_step, m->stream));
-    HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
+    HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd1, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
   }
   if (objective) {
     std::vector<double> o(d->steps);

@@ -78,6 +78,21 @@ __global__ void __launch_bounds__(64) mfma_probe_kernel(const float *A, const fl
   for (int j = 0; j < 4; ++j) D[(4 * q + j) * 16 + r] = d[j];
 }
 
+// One v_mfma_f32_16x16x32_bf16: A 16x32, B 32x16 row-major FP32 (rounded
+// to bf16), lane l holding A[l % 16][8 (l / 16) + j] and B[8 (l / 16) + j][l % 16],
+// j = 0..7; D as for 16x16x16.
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+__global__ void __launch_bounds__(64) mfma32_probe_kernel(const float *A, const float *B, float *D) {
+  const int l = threadIdx.x, r = l % 16, q = l / 16;
+  bf16x8_t a, b;
+  for (int j = 0; j < 8; ++j) {
+    a[j] = (__bf16)A[r * 32 + 8 * q + j];
+    b[j] = (__bf16)B[(8 * q + j) * 16 + r];
+  }
+  const f32x4_t d = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, (f32x4_t){0.f, 0.f, 0.f, 0.f}, 0, 0, 0);
+  for (int j = 0; j < 4; ++j) D[(4 * q + j) * 16 + r] = d[j];
+}
+
 // Plain copy of n doubles: the calibration kernel for HBM byte counters.
 __global__ void copy_probe_kernel(const double *__restrict__ src, double *__restrict__ dst,
                                   size_t n) {
@@ -168,6 +183,21 @@ extern "C" mole_status mole_debug_mfma_bf16(const float *A, const float *B, floa
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(D, d + 512, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
+}
+
+extern "C" mole_status mole_debug_mfma_bf16_k32(const float *A, const float *B, float *D) {
+  if (!A || !B || !D) return MOLE_ERR_INVALID_ARGUMENT;
+  float *d = nullptr;
+  if (hipMalloc(&d, (512 + 512 + 256) * sizeof(float)) != hipSuccess) return MOLE_ERR_OUT_OF_MEMORY;
+  hipError_t e = hipMemcpy(d, A, 512 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(d + 512, B, 512 * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(mfma32_probe_kernel, dim3(1), dim3(64), 0, 0, d, d + 512, d + 1024);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(D, d + 1024, 256 * sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
 }

@@ -46,27 +46,36 @@ __global__ void mlp_init_kernel(float *w, uint64_t seed) {
   w[p] = (float)((2.0 * u - 1.0) * a);
 }
 
-// One thread per (fragment, lane) of the forward and backward sets, then the
-// biases.
-__global__ void mlp_pack_kernel(const float *w, bf16x4 *fwd, bf16x4 *bwd, float *bias) {
+// One thread per (fragment, lane) of the forward sets (layer 1 with K = 16,
+// layers 2-4 with K = 32 in the order of pack_k32) and the backward set,
+// then the biases.
+__global__ void mlp_pack_kernel(const float *w, bf16x4 *fwd1, bf16x8 *fwd, bf16x4 *bwd, float *bias) {
   const int id = blockIdx.x * blockDim.x + threadIdx.x;
-  const int nf = kFwdFrags * 64, nb = kBwdFrags * 64;
-  if (id < nf) {
-    const int frag = id / 64, l = id % 64, r = l % 16, q = l / 16;
-    int layer, mt, kc;
-    if (frag < kFwdL2) { layer = 0; mt = frag; kc = 0; }
-    else if (frag < kFwdL3) { layer = 1; mt = (frag - kFwdL2) / 4; kc = (frag - kFwdL2) % 4; }
-    else if (frag < kFwdL4) { layer = 2; mt = (frag - kFwdL3) / 4; kc = (frag - kFwdL3) % 4; }
-    else { layer = 3; mt = 0; kc = frag - kFwdL4; }
-    const LayerDims d = layer_dims(layer);
+  const int n1 = kFwd1Frags * 64, nf = kFwdFrags * 64, nb = kBwdFrags * 64;
+  if (id < n1) {
+    const int mt = id / 64, l = id % 64, r = l % 16, q = l / 16;
     bf16x4 v;
     for (int j = 0; j < 4; ++j) {
-      const int m = 16 * mt + r, k = 16 * kc + 4 * q + j;  // A[m = out][k = in] = W[in][out]
-      v[j] = (m < d.nout && k < d.nin) ? (short)bf16_bits(w[d.woff + k * d.nout + m]) : (short)0;
+      const int m = 16 * mt + r, k = 4 * q + j;  // A[m = out][k = in] = W1[in][out]
+      v[j] = k < kMlpIn ? bf16_short(w[kMlpW1 + k * kMlpHid + m]) : (short)0;
     }
-    fwd[id] = v;
-  } else if (id < nf + nb) {
-    const int e = id - nf, frag = e / 64, l = e % 64, r = l % 16, q = l / 16;
+    fwd1[id] = v;
+  } else if (id < n1 + nf) {
+    const int e = id - n1, frag = e / 64, l = e % 64, r = l % 16, q = l / 16;
+    int layer, mt, c;
+    if (frag < kFwdL3) { layer = 1; mt = frag / 2; c = frag % 2; }
+    else if (frag < kFwdL4) { layer = 2; mt = (frag - kFwdL3) / 2; c = (frag - kFwdL3) % 2; }
+    else { layer = 3; mt = 0; c = frag - kFwdL4; }
+    const LayerDims d = layer_dims(layer);
+    bf16x8 v;
+    for (int j = 0; j < 8; ++j) {
+      const int m = 16 * mt + r;                                          // output feature
+      const int k = 32 * c + (j < 4 ? 4 * q + j : 16 + 4 * q + (j - 4));  // input feature
+      v[j] = (__bf16)((m < d.nout && k < d.nin) ? w[d.woff + k * d.nout + m] : 0.0f);
+    }
+    fwd[e] = v;
+  } else if (id < n1 + nf + nb) {
+    const int e = id - n1 - nf, frag = e / 64, l = e % 64, r = l % 16, q = l / 16;
     int layer, mt, kc;
     if (frag < kBwdL3) { layer = 1; mt = frag / 4; kc = frag % 4; }
     else if (frag < kBwdL4) { layer = 2; mt = (frag - kBwdL3) / 4; kc = (frag - kBwdL3) % 4; }
@@ -75,11 +84,11 @@ __global__ void mlp_pack_kernel(const float *w, bf16x4 *fwd, bf16x4 *bwd, float 
     bf16x4 v;
     for (int j = 0; j < 4; ++j) {
       const int m = 16 * mt + r, k = 16 * kc + 4 * q + j;  // A[m = in][k = out] = W[in][out]
-      v[j] = (m < d.nin && k < d.nout) ? (short)bf16_bits(w[d.woff + m * d.nout + k]) : (short)0;
+      v[j] = (m < d.nin && k < d.nout) ? bf16_short(w[d.woff + m * d.nout + k]) : (short)0;
     }
     bwd[e] = v;
-  } else if (id < nf + nb + kMlpBiasPad) {
-    const int b = id - nf - nb;
+  } else if (id < n1 + nf + nb + kMlpBiasPad) {
+    const int b = id - n1 - nf - nb;
     float v = 0.0f;
     if (b < 64) v = w[kMlpB1 + b];
     else if (b < 128) v = w[kMlpB2 + b - 64];
@@ -438,9 +447,10 @@ hipError_t launch_mlp_init(float *w, uint64_t seed, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_mlp_pack(const float *w, bf16x4 *fwd, bf16x4 *bwd, float *bias, hipStream_t stream) {
-  const int n = (kFwdFrags + kBwdFrags) * 64 + kMlpBiasPad;
-  hipLaunchKernelGGL(mlp_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, w, fwd, bwd, bias);
+hipError_t launch_mlp_pack(const float *w, bf16x4 *fwd1, bf16x8 *fwd, bf16x4 *bwd, float *bias,
+                           hipStream_t stream) {
+  const int n = (kFwd1Frags + kFwdFrags + kBwdFrags) * 64 + kMlpBiasPad;
+  hipLaunchKernelGGL(mlp_pack_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, w, fwd1, fwd, bwd, bias);
   return hipGetLastError();
 }
 

@@ -28,40 +28,78 @@ constexpr int kMlpW1 = 0, kMlpB1 = 832, kMlpW2 = 896, kMlpB2 = 4992, kMlpW3 = 50
               kMlpW4 = 9216, kMlpB4 = 9472;
 constexpr double kMlpLS0 = -1.3862943611198906;  // ln 0.25
 constexpr double kMlpLSLo = -7.0, kMlpLSHi = 1.0;
-// forward fragments: layer 1 [4 m-tiles], layers 2-3 [4 m][4 k], layer 4 [4 k]
-constexpr int kFwdL1 = 0, kFwdL2 = 4, kFwdL3 = 20, kFwdL4 = 36, kFwdFrags = 40;
+// forward fragments: layer 1 on 16x16x16 [4 m-tiles]; layers 2-4 on
+// 16x16x32: layers 2-3 [4 m][2 k], layer 4 [2 k]
+constexpr int kFwd1Frags = 4;
+constexpr int kFwdL2 = 0, kFwdL3 = 8, kFwdL4 = 16, kFwdFrags = 18;
 // backward fragments (A = W, M = inputs): layers 2-3 [4 m][4 k], layer 4 [4 m]
 constexpr int kBwdL2 = 0, kBwdL3 = 16, kBwdL4 = 32, kBwdFrags = 36;
 constexpr int kMlpBiasPad = 64 + 64 + 64 + 16;
 
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 struct MlpDev {
-  const bf16x4 *fwd;  // [kFwdFrags][64]
-  const bf16x4 *bwd;  // [kBwdFrags][64]
-  const float *bias;  // b1[64] b2[64] b3[64] b4[16] (zero padded)
+  const bf16x4 *fwd1;  // [kFwd1Frags][64]   layer 1, K = 16
+  const bf16x8 *fwd;   // [kFwdFrags][64]    layers 2-4, K = 32
+  const bf16x4 *bwd;   // [kBwdFrags][64]
+  const float *bias;   // b1[64] b2[64] b3[64] b4[16] (zero padded)
 };
 
 __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// v_mfma_f32_16x16x32_bf16: lane l holds A[l%16][8(l/16) + j] and
+// B[8(l/16) + j][l%16], j = 0..7; the accumulator as for 16x16x16
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+
+// round to nearest even (v_cvt_pk_bf16_f32 on gfx950)
+__device__ __forceinline__ short bf16_short(float x) { return __builtin_bit_cast(short, (__bf16)x); }
+
 __device__ __forceinline__ bf16x4 to_bf16x4(f32x4 v) {
   bf16x4 r;
-  r[0] = (short)bf16_bits(v[0]);
-  r[1] = (short)bf16_bits(v[1]);
-  r[2] = (short)bf16_bits(v[2]);
-  r[3] = (short)bf16_bits(v[3]);
+  r[0] = bf16_short(v[0]);
+  r[1] = bf16_short(v[1]);
+  r[2] = bf16_short(v[2]);
+  r[3] = bf16_short(v[3]);
   return r;
+}
+
+// Two accumulator tiles of consecutive 16-feature blocks as one K = 32
+// operand: position 8q + j holds feature 4q + j of the first tile (j < 4)
+// or 4q + j - 4 of the second; the K = 32 weight fragments are packed in
+// this order.
+__device__ __forceinline__ bf16x8 pack_k32(f32x4 lo, f32x4 hi) {
+  bf16x8 r;
+  r[0] = (__bf16)lo[0];
+  r[1] = (__bf16)lo[1];
+  r[2] = (__bf16)lo[2];
+  r[3] = (__bf16)lo[3];
+  r[4] = (__bf16)hi[0];
+  r[5] = (__bf16)hi[1];
+  r[6] = (__bf16)hi[2];
+  r[7] = (__bf16)hi[3];
+  return r;
+}
+
+// tanh x = 1 - 2 / (e^{2x} + 1) on the hardware exp2 and reciprocal
+// (absolute error ~2e-7, far below the bf16 rounding of the activations);
+// saturates to +-1 for large |x|.
+__device__ __forceinline__ float tanh_fast(float x) {
+  const float e = __builtin_amdgcn_exp2f(x * 2.8853900817779268f);  // 2 log2(e)
+  return 1.0f - 2.0f * __builtin_amdgcn_rcpf(e + 1.0f);
 }
 
 __device__ __forceinline__ f32x4 tanh4(f32x4 v) {
   f32x4 r;
-  r[0] = tanhf(v[0]);
-  r[1] = tanhf(v[1]);
-  r[2] = tanhf(v[2]);
-  r[3] = tanhf(v[3]);
+  r[0] = tanh_fast(v[0]);
+  r[1] = tanh_fast(v[1]);
+  r[2] = tanh_fast(v[2]);
+  r[3] = tanh_fast(v[3]);
   return r;
 }
 
@@ -75,36 +113,30 @@ __device__ __forceinline__ f32x4 bias4(const float *b, int row0) {
 // out[0..3] of samples 0..15.  All 64 lanes must call it.
 __device__ __forceinline__ f32x4 mlp_forward(const MlpDev &m, bf16x4 x, f32x4 *h) {
   const int l = threadIdx.x & 63, q = l >> 4;
-  bf16x4 a[4], b[4];
+  f32x4 t[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = bias4(m.bias, 16 * mt + 4 * q);
-    acc = mfma16(m.fwd[(kFwdL1 + mt) * 64 + l], x, acc);
-    const f32x4 t = tanh4(acc);
-    if (h) h[mt] = t;
-    a[mt] = to_bf16x4(t);
+    acc = mfma16(m.fwd1[mt * 64 + l], x, acc);
+    t[mt] = tanh4(acc);
+    if (h) h[mt] = t[mt];
   }
 #pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    f32x4 acc = bias4(m.bias + 64, 16 * mt + 4 * q);
+  for (int layer = 0; layer < 2; ++layer) {
+    const bf16x8 b0 = pack_k32(t[0], t[1]), b1 = pack_k32(t[2], t[3]);
+    const int fr = layer == 0 ? kFwdL2 : kFwdL3;
 #pragma unroll
-    for (int kc = 0; kc < 4; ++kc) acc = mfma16(m.fwd[(kFwdL2 + mt * 4 + kc) * 64 + l], a[kc], acc);
-    const f32x4 t = tanh4(acc);
-    if (h) h[4 + mt] = t;
-    b[mt] = to_bf16x4(t);
-  }
-#pragma unroll
-  for (int mt = 0; mt < 4; ++mt) {
-    f32x4 acc = bias4(m.bias + 128, 16 * mt + 4 * q);
-#pragma unroll
-    for (int kc = 0; kc < 4; ++kc) acc = mfma16(m.fwd[(kFwdL3 + mt * 4 + kc) * 64 + l], b[kc], acc);
-    const f32x4 t = tanh4(acc);
-    if (h) h[8 + mt] = t;
-    a[mt] = to_bf16x4(t);
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 acc = bias4(m.bias + 64 * (layer + 1), 16 * mt + 4 * q);
+      acc = mfma32(m.fwd[(fr + mt * 2) * 64 + l], b0, acc);
+      acc = mfma32(m.fwd[(fr + mt * 2 + 1) * 64 + l], b1, acc);
+      t[mt] = tanh4(acc);
+      if (h) h[4 * (layer + 1) + mt] = t[mt];
+    }
   }
   f32x4 out = bias4(m.bias + 192, 4 * q);
-#pragma unroll
-  for (int kc = 0; kc < 4; ++kc) out = mfma16(m.fwd[(kFwdL4 + kc) * 64 + l], a[kc], out);
+  out = mfma32(m.fwd[kFwdL4 * 64 + l], pack_k32(t[0], t[1]), out);
+  out = mfma32(m.fwd[(kFwdL4 + 1) * 64 + l], pack_k32(t[2], t[3]), out);
   return out;
 }
 
@@ -117,50 +149,58 @@ __device__ __forceinline__ double mlp_log_sigma(double out3) {
 // The 13 inputs of electron i at r (FP64; rounded to bf16 by the caller):
 // nearest-nucleus distance, x_j - r for the three nearest other electrons
 // (nearest first, ties to the lower index, zeros when missing), r.
-// Writes features 4 sub .. 4 sub + 3 into f.
+// Writes features 4 sub .. 4 sub + 3 into f (selected without a runtime
+// index into a private array, which would live in scratch memory).
 __device__ __forceinline__ void mlp_features(const double *pos, int N, const double *nuc, int n_nuc, int i,
                                              const double *r, int sub, float *f) {
-  double v[4];
-  if (sub == 0) {
-    double dmin = __longlong_as_double(0x7FF0000000000000ll);
-    for (int I = 0; I < n_nuc; ++I) {
-      const double d = norm3(r[0] - nuc[3 * I], r[1] - nuc[3 * I + 1], r[2] - nuc[3 * I + 2]);
-      dmin = d < dmin ? d : dmin;
-    }
-    v[0] = dmin;
+  const double kInf = __longlong_as_double(0x7FF0000000000000ll);
+  double dmin = kInf;
+  for (int I = 0; I < n_nuc; ++I) {
+    const double d = norm3(r[0] - nuc[3 * I], r[1] - nuc[3 * I + 1], r[2] - nuc[3 * I + 2]);
+    dmin = d < dmin ? d : dmin;
   }
-  int best[3] = {-1, -1, -1};
-  double bd[3];
-  bd[0] = bd[1] = bd[2] = __longlong_as_double(0x7FF0000000000000ll);
+  int b0 = -1, b1 = -1, b2 = -1;
+  double d0 = kInf, d1 = kInf, d2 = kInf;
   for (int j = 0; j < N; ++j) {
     if (j == i) continue;
     const double *x = pos + 3 * j;
-    const double d = norm3(x[0] - r[0], x[1] - r[1], x[2] - r[2]);
-    if (d < bd[0]) {
-      bd[2] = bd[1]; best[2] = best[1];
-      bd[1] = bd[0]; best[1] = best[0];
-      bd[0] = d; best[0] = j;
-    } else if (d < bd[1]) {
-      bd[2] = bd[1]; best[2] = best[1];
-      bd[1] = d; best[1] = j;
-    } else if (d < bd[2]) {
-      bd[2] = d; best[2] = j;
+    // squared distance (same order as the distance, no square root); ties
+    // of the squared values are ties of the distances
+    const double e0 = x[0] - r[0], e1 = x[1] - r[1], e2 = x[2] - r[2];
+    double d = e0 * e0;
+    d = d + e1 * e1;
+    d = d + e2 * e2;
+    if (d < d0) {
+      d2 = d1; b2 = b1;
+      d1 = d0; b1 = b0;
+      d0 = d; b0 = j;
+    } else if (d < d1) {
+      d2 = d1; b2 = b1;
+      d1 = d; b1 = j;
+    } else if (d < d2) {
+      d2 = d; b2 = j;
     }
   }
-  double all[13];
-  all[0] = sub == 0 ? v[0] : 0.0;
-  for (int k = 0; k < 3; ++k)
-    for (int q = 0; q < 3; ++q) all[1 + 3 * k + q] = best[k] >= 0 ? pos[3 * best[k] + q] - r[q] : 0.0;
-  for (int q = 0; q < 3; ++q) all[10 + q] = r[q];
-  for (int j = 0; j < 4; ++j) {
-    const int k = 4 * sub + j;
-    f[j] = k < 13 ? (float)all[k] : 0.0f;
+  auto vec = [&](int b, int q) { return b >= 0 ? pos[3 * b + q] - r[q] : 0.0; };
+  double v0, v1, v2, v3;
+  if (sub == 0) {
+    v0 = dmin; v1 = vec(b0, 0); v2 = vec(b0, 1); v3 = vec(b0, 2);
+  } else if (sub == 1) {
+    v0 = vec(b1, 0); v1 = vec(b1, 1); v2 = vec(b1, 2); v3 = vec(b2, 0);
+  } else if (sub == 2) {
+    v0 = vec(b2, 1); v1 = vec(b2, 2); v2 = r[0]; v3 = r[1];
+  } else {
+    v0 = r[2]; v1 = 0.0; v2 = 0.0; v3 = 0.0;
   }
+  f[0] = (float)v0;
+  f[1] = (float)v1;
+  f[2] = (float)v2;
+  f[3] = (float)v3;
 }
 
 __device__ __forceinline__ bf16x4 features_bf16(const float *f) {
   bf16x4 r;
-  for (int j = 0; j < 4; ++j) r[j] = (short)bf16_bits(f[j]);
+  for (int j = 0; j < 4; ++j) r[j] = bf16_short(f[j]);
   return r;
 }
 

@@ -28,7 +28,8 @@ constexpr uint32_t kMlpTrainSys = 0x6D6C70u;
 constexpr uint32_t kMlpInitEpoch = 0x58415631u;
 
 hipError_t launch_mlp_init(float *w, uint64_t seed, hipStream_t stream);
-hipError_t launch_mlp_pack(const float *w, bf16x4 *fwd, bf16x4 *bwd, float *bias, hipStream_t stream);
+hipError_t launch_mlp_pack(const float *w, bf16x4 *fwd1, bf16x8 *fwd, bf16x4 *bwd, float *bias,
+                           hipStream_t stream);
 hipError_t launch_mlp_train(const MlpTrainArgs &a, int n_groups, hipStream_t stream);
 // grad[p] = sum_g partial[g][p], obj_sum = sum_g obj[g] (fixed order)
 hipError_t launch_mlp_reduce(const float *partial, const double *obj, int n_groups, float *grad,

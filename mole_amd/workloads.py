@@ -6,6 +6,8 @@ atoms():   Li..Ne Slater-Jastrow atoms, 65,536 walkers each, Sherman-Morrison
 hetero():  256 systems of 2..40 electrons (hydrogen chains and Poisson-disk
            molecules), walkers proportional to electron count, one launch per
            call, autotuned lanes per walker, sweeps replayed from a HIP graph.
+learned(): the learned MH proposal trained on the device from box-sampler
+           configurations, against the box proposal: tau_int, ESS/s, kappa.
 """
 import time
 
@@ -146,3 +148,91 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
             out.append(systems.random_molecule(N, seed=seed * 1000 + k, min_sep=1.2,
                                                basis_per_nucleus=basis_per_nucleus))
     return out
+
+
+def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=2000,
+            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.3, 0.6, 1.0), precision=M.PREC_FP64,
+            lu_period=64, seed=1, log=print):
+    """Learned proposal against the box proposal on one atom: configurations
+    from the box sampler, Adam training on the device, then runs of each
+    proposal with per-walker E_L series.  Efficiency per proposal: ESS/s =
+    W S / (tau_int t) with the statfor estimator (scripts/statfor.rs:30-54)
+    averaged over walkers, and kappa = 1 / (sigma(E)^2 t) with the blocking
+    error, t the device time of the S sampled sweeps."""
+    from mole_amd.stats import tau_int
+
+    sj = systems.atom(Z)
+    N = sj.n_electrons
+
+    def context(proposal, step, flags=0, record=0):
+        return M.Context([M.slater_jastrow(sj, walkers)], seed=seed, step=step, block_size=block,
+                         precision=precision, lu_period=lu_period, proposal=proposal, flags=flags,
+                         record_sweeps=record)
+
+    mlp = M.Mlp(seed=2)
+    # 1. training configurations from the plain sampler; its final ensemble
+    #    is also the common start of every measured run (a slowly mixing
+    #    proposal started from U(-1,1) would otherwise carry its burn-in)
+    with context(M.PROPOSAL_BOX, 0.6) as ctx:
+        ctx.init_walkers()
+        ctx.sweep(equil)
+        n_snap = max(1, -(-n_configs // walkers))
+        t0 = time.perf_counter()
+        mlp.collect(ctx, n_snap, collect_every)
+        t_collect = time.perf_counter() - t0
+        start = ctx.get_configs()
+        # 2. training on the device
+        t0 = time.perf_counter()
+        obj = mlp.train(ctx, train_steps, batch=batch, lr=lr, seed=seed + 10)
+        t_train = time.perf_counter() - t0
+    n_data = mlp.data_size()
+
+    def measure(proposal, step):
+        with context(proposal, step, M.FLAG_TRACE, sweeps + block) as ctx:
+            if proposal == M.PROPOSAL_LEARNED:
+                mlp.attach(ctx)
+            else:
+                autotune.apply(ctx, [sj], autotune.tune([sj], lambda s: walkers, candidates=(4, 8, 16),
+                                                          sweeps=2, precision=precision, lu_period=lu_period)[0])
+            ctx.set_configs(start)
+            ctx.sweep(equil // 2)  # the proposal's own equilibration
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.run(sweeps + block)
+            ctx.synchronize()
+            wall = time.perf_counter() - t0
+            dev_ms, _ = ctx.last_timing()
+            st = ctx.stats()
+            _, el = ctx.trace(sweeps + block, sweeps)
+            lanes = ctx.walker_lanes()
+        t = dev_ms * 1e-3 * sweeps / (sweeps + block)  # device time of the sampled sweeps
+        e, err = energy_and_error(st)
+        tau = tau_int(el)
+        tau_mean = float(np.mean(tau))
+        return dict(proposal={M.PROPOSAL_BOX: "box", M.PROPOSAL_LEARNED: "learned"}[proposal], step=step,
+                    lanes=lanes, energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
+                    tau_int_mean=tau_mean, tau_int_median=float(np.median(tau)), seconds=t, wall=wall,
+                    moves_per_s=walkers * sweeps * N / t, ess_per_s=walkers * sweeps / (tau_mean * t),
+                    kappa=1.0 / (err * err * t))
+
+    runs = [measure(M.PROPOSAL_BOX, s) for s in box_sides]
+    for r in runs:
+        log(dict(workload="learned-run", Z=Z, **r))
+    lrn = measure(M.PROPOSAL_LEARNED, 0.0)
+    log(dict(workload="learned-run", Z=Z, **lrn))
+    mlp.close()
+    best_ess = max(runs, key=lambda r: r["ess_per_s"])
+    best_kappa = max(runs, key=lambda r: r["kappa"])
+    diff = lrn["energy"] - best_kappa["energy"]
+    rec = dict(workload="learned", Z=Z, element=sj.name, walkers=walkers, sweeps=sweeps, block=block,
+               precision="mixed" if precision == M.PREC_MIXED else "fp64",
+               training=dict(configs=n_data, steps=train_steps, batch=batch, lr=lr,
+                             objective_first=float(np.mean(obj[:20])), objective_last=float(np.mean(obj[-20:])),
+                             collect_seconds=t_collect, train_seconds=t_train),
+               box=runs, learned=lrn,
+               ess_ratio_vs_best_box=lrn["ess_per_s"] / best_ess["ess_per_s"],
+               kappa_ratio_vs_best_box=lrn["kappa"] / best_kappa["kappa"],
+               energy_diff_vs_box=diff,
+               energy_diff_sigma=diff / float(np.hypot(lrn["error"], best_kappa["error"])))
+    log(rec)
+    return rec

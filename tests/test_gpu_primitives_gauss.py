@@ -75,3 +75,13 @@ def test_mfma_bf16_lane_placement():
     A = np.arange(256, dtype=np.float32).reshape(16, 16) % 7 - 3
     B = (np.arange(256, dtype=np.float32).reshape(16, 16).T % 5) - 2
     assert np.array_equal(M.debug_mfma_bf16(A, B), A @ B)
+
+
+def test_mfma_bf16_k32_lane_placement():
+    rng = np.random.default_rng(8)
+    A = rng.integers(-8, 9, (16, 32)).astype(np.float32)
+    B = rng.integers(-8, 9, (32, 16)).astype(np.float32)
+    assert np.array_equal(M.debug_mfma_bf16_k32(A, B), A @ B)
+    A = (np.arange(512, dtype=np.float32).reshape(16, 32) % 7) - 3
+    B = (np.arange(512, dtype=np.float32).reshape(32, 16) % 5) - 2
+    assert np.array_equal(M.debug_mfma_bf16_k32(A, B), A @ B)

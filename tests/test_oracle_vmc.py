@@ -150,3 +150,23 @@ def test_vmc_sr_lowers_helium_energy():
     assert en[-1] < en[0] - 5 * er[0]
     assert en[-1] < -2.86  # below the product-STO optimum -(27/16)^2
     assert np.all((ac > 0.3) & (ac < 1.0))
+
+
+def test_vectorised_tau_matches_the_oracle_estimator():
+    from mole_amd.stats import tau_int
+    rng = np.random.default_rng(21)
+    series = []
+    for rho in (0.0, 0.5, 0.9, 0.97):
+        for n in (50, 300, 1000):
+            e = rng.normal(size=n)
+            x = np.empty(n)
+            x[0] = e[0]
+            for t in range(1, n):
+                x[t] = rho * x[t - 1] + e[t]
+            series.append(x)
+    for x in series:
+        want = O.lib().orc_tau_corr(O.dptr(np.ascontiguousarray(x)), x.size)
+        assert tau_int(x)[0] == pytest.approx(want, rel=1e-9, abs=1e-9)
+    # batched rows equal single rows
+    X = np.stack([s for s in series if s.size == 300])
+    assert np.allclose(tau_int(X), [tau_int(r)[0] for r in X], rtol=1e-12)
