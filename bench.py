@@ -47,7 +47,14 @@ def flops_per_launch(walkers, sweeps, sampled):
 VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
 
 
-def gpu_run(iters, warmup, seed=0, variant="auto"):
+def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
+    """The SR optimisation on the device.  Default: mole_optimize, the whole
+    loop (sweeps, reductions, SR solve, parameter update) enqueued without a
+    host round trip.  host_loop: one C-ABI round trip per iteration (read the
+    sums, solve on the host, push the parameters) -- the same trajectory bit
+    for bit.  Afterwards three more iterations through the host path time the
+    sweep kernel alone with HIP events on the context's stream (the roofline's
+    kernel time)."""
     import mole_amd as M
 
     def one_iteration(ctx, p):
@@ -61,38 +68,61 @@ def gpu_run(iters, warmup, seed=0, variant="auto"):
         ms, launches = ctx.last_timing()
         return p, st, ms, launches
 
+    def stats_of(st):
+        mean = st["sum_e"] / st["n_samples"]
+        nb = st["n_blocks"]
+        err = np.sqrt(max(st["sum_block_mean2"] / nb - mean**2, 0.0) / (nb - 1))
+        return mean, err, st["n_accepted"] / st["n_moves"]
+
+    def optimize(ctx, n):
+        return ctx.optimize(n, SWEEPS, BLOCK, M.OPT_SR, SR_STEP, M.SR_TEXTBOOK, SR_EPS)
+
     # warmup on its own context: the timed run starts from the initial parameters
     if warmup:
         with M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed + 1, step=BOX, block_size=BLOCK) as w:
             w.set_launch_config(VARIANTS[variant])
             w.init_walkers()
-            p = np.array(PARAMS0)
-            for _ in range(warmup):
-                p, *_ = one_iteration(w, p)
+            if host_loop:
+                p = np.array(PARAMS0)
+                for _ in range(warmup):
+                    p, *_ = one_iteration(w, p)
+            else:
+                optimize(w, warmup)
             w.synchronize()
 
     ctx = M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed, step=BOX, block_size=BLOCK)
     ctx.set_launch_config(VARIANTS[variant])
     ctx.init_walkers()
     ctx.synchronize()
-    p = np.array(PARAMS0)
-    kernel_ms = []
     energies = []
     launches_total = 0
     t0 = time.perf_counter()
-    for _ in range(iters):
-        p, st, ms, launches = one_iteration(ctx, p)
-        kernel_ms.append(ms)
-        launches_total += launches
-        mean = st["sum_e"] / st["n_samples"]
-        nb = st["n_blocks"]
-        err = np.sqrt(max(st["sum_block_mean2"] / nb - mean**2, 0.0) / (nb - 1))
-        energies.append((mean, err, st["n_accepted"] / st["n_moves"]))
+    if host_loop:
+        p = np.array(PARAMS0)
+        for _ in range(iters):
+            p, st, ms, launches = one_iteration(ctx, p)
+            launches_total += launches
+            energies.append(stats_of(st))
+    else:
+        p, log = optimize(ctx, iters)
     ctx.synchronize()
     t1 = time.perf_counter()
+    if not host_loop:
+        _, launches_total = ctx.last_timing()
+        nb = WALKERS * (SWEEPS // BLOCK - 1)
+        for row in log:
+            mean = row[0] / row[6]
+            err = np.sqrt(max(row[3] / nb - mean**2, 0.0) / (nb - 1))
+            energies.append((mean, err, row[4] / (WALKERS * SWEEPS * N_ELEC)))
+    # the sweep kernel alone, timed live on the context's stream
+    kernel_ms = []
+    q = p.copy()
+    for _ in range(3):
+        q, _, ms, n_l = one_iteration(ctx, q)
+        kernel_ms.append(ms / n_l)
     ctx.close()
     return dict(seconds=t1 - t0, kernel_ms=kernel_ms, launches=launches_total, energies=energies,
-                params=p.tolist())
+                params=[float(v) for v in p])
 
 
 def cpu_baseline(threads, budget_s=20.0):
@@ -168,9 +198,17 @@ def hetero_main(args):
     workloads.hetero(sweeps=args.sweeps or 100, total_walkers=args.walkers or (1 << 20), log=log)
 
 
+def learned_main(args):
+    from mole_amd import workloads
+
+    for Z in (3, 4, 10):
+        workloads.learned(Z=Z, walkers=args.walkers or 65536, sweeps=args.sweeps or 400,
+                          log=lambda r: print(json.dumps(r), flush=True))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero"], default="h2-sr")
+    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero", "learned"], default="h2-sr")
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
@@ -179,6 +217,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", choices=sorted(VARIANTS), default="auto")
+    ap.add_argument("--host-loop", action="store_true",
+                    help="one host round trip per SR iteration instead of the device-side loop")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: walkers are independent Markov chains and this "
@@ -190,15 +230,17 @@ def main():
     if args.workload == "hetero":
         hetero_main(args)
         return
+    if args.workload == "learned":
+        learned_main(args)
+        return
 
-    r = gpu_run(args.steps, args.warmup, variant=args.variant)
+    r = gpu_run(args.steps, args.warmup, variant=args.variant, host_loop=args.host_loop)
     moves = args.steps * WALKERS * SWEEPS * N_ELEC
     value = moves / r["seconds"]
     ms_step = 1e3 * r["seconds"] / args.steps
-    # dominant kernel: one sweep launch per iteration
+    # dominant kernel: one sweep launch of SWEEPS sweeps per iteration
     avg_kernel_s = 1e-3 * float(np.mean(r["kernel_ms"]))
-    launches_per_iter = r["launches"] / args.steps
-    fl = flops_per_launch(WALKERS, SWEEPS // int(launches_per_iter), (SWEEPS - BLOCK) // int(launches_per_iter))
+    fl = flops_per_launch(WALKERS, SWEEPS, SWEEPS - BLOCK)
     achieved = fl / avg_kernel_s / 1e12
     roofline = dict(bound="fp64-valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
@@ -228,7 +270,8 @@ def main():
             "workload": "H2 LCAO sigma_g^2 x Pade Jastrow + stochastic reconfiguration",
             "walkers": WALKERS, "sweeps_per_iteration": SWEEPS, "block": BLOCK,
             "box_side": BOX, "sr_step": SR_STEP, "sr_shift": SR_EPS, "nuclei_z": [-0.7, 0.7],
-            "parallelism": "single GPU, one lane per walker",
+            "parallelism": "single GPU, wave-specialised workgroups of 64 walkers",
+            "optimizer_loop": "host" if args.host_loop else "device (mole_optimize)",
         },
         "roofline": roofline,
         "cpu_baseline": cpu,

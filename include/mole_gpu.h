@@ -268,6 +268,21 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
                                           double step, uint32_t sr_mode, double sr_eps,
                                           double *dp, double *g);
 
+/* VmcRunner::run_optimization (src/vmc/src/vmc.rs:43-106) with persistent
+ * walkers, run on the device: per iteration the statistics restart,
+ * Runner::run(steps, block_size) sweeps the walkers, the accumulators are
+ * reduced and the optimizer step (as mole_compute_parameter_update, same
+ * rounding) updates the parameters in device memory -- no host round trip
+ * between iterations.  log (optional, [iters][8]): sum_e, sum_e2, sum of
+ * block means, sum of squared block means, accepted moves, status,
+ * n_samples, 0.  params_out (optional): the final parameters, which the
+ * context keeps.  One two-electron system, no trace/sample flags.  A
+ * singular SR matrix leaves the parameters unchanged at that iteration and
+ * returns MOLE_ERR_LINALG after the loop. */
+mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size,
+                          uint32_t optimizer, double step, uint32_t sr_mode, double sr_eps,
+                          double *log, double *params_out);
+
 /* ---- launch configuration ----------------------------------------------
  * No counterpart in the reference (rayon's fan-out, src/vmc/src/vmc.rs:63-76,
  * is the closest).  Selects the sweep kernel: AUTO picks the

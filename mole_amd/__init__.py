@@ -179,6 +179,8 @@ _SIGS = [
     ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
     ("mole_debug_mfma_bf16", C.c_int32, [_pf, _pf, _pf]),
     ("mole_debug_mfma_bf16_k32", C.c_int32, [_pf, _pf, _pf]),
+    ("mole_optimize", C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32,
+                                  C.c_double, _pd, _pd]),
     ("mole_mlp_create", C.c_int32, [C.c_uint64, C.POINTER(_P)]),
     ("mole_mlp_destroy", None, [_P]),
     ("mole_mlp_set_weights", C.c_int32, [_P, _pf, C.c_size_t]),
@@ -406,6 +408,17 @@ class Context:
         check(lib().mole_eval(self.h, sys, n, _dp(a), _dp(out["value"]), _dp(out["grad"]),
                               _dp(out["lap"]), _dp(out["e_local"]), _dp(out["pgrad"])))
         return out
+
+    def optimize(self, iters, steps, block_size, optimizer=None, step=0.05, sr_mode=None, sr_eps=1e-3):
+        """mole_optimize: iters optimizer iterations with the loop on the device.
+        Returns (params, log[iters][8])."""
+        optimizer = OPT_SR if optimizer is None else optimizer
+        sr_mode = SR_TEXTBOOK if sr_mode is None else sr_mode
+        log = np.zeros((iters, 8))
+        p = np.zeros(self.systems[0].params.size)
+        check(lib().mole_optimize(self.h, iters, steps, block_size, optimizer, step, sr_mode, sr_eps, _dp(log),
+                                  _dp(p)))
+        return p, log
 
     def set_launch_config(self, variant=VARIANT_AUTO, block_threads=0):
         check(lib().mole_set_launch_config(self.h, variant, block_threads))

@@ -22,6 +22,7 @@
 #include "../../include/mole_gpu.h"
 #include "internal.hpp"
 #include "mlp_internal.hpp"
+#include "sr_math.hpp"
 #include "sj_internal.hpp"
 
 using namespace mole;
@@ -110,6 +111,10 @@ struct mole_ctx {
   size_t lds_bytes = 0, lds_limit = 65536;
   bool sysdev_dirty = true;
   mole_mlp *mlp = nullptr;  // network of the learned proposal (not owned)
+  // device-side optimisation: parameters read by the kernels from d_params
+  double *d_params = nullptr, *d_optlog = nullptr;
+  size_t optlog_cap = 0;
+  bool dev_params = false;
 };
 
 // The learned proposal's network, its Adam state and its training data.
@@ -248,6 +253,8 @@ void free_ctx(mole_ctx *ctx) {
       if (p) (void)hipFree(p);
   }
   if (ctx->d_red) (void)hipFree(ctx->d_red);
+  if (ctx->d_params) (void)hipFree(ctx->d_params);
+  if (ctx->d_optlog) (void)hipFree(ctx->d_optlog);
   if (ctx->d_sysdev) (void)hipFree(ctx->d_sysdev);
   if (ctx->d_state) (void)hipFree(ctx->d_state);
   if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
@@ -413,17 +420,9 @@ mole_status sj_prepare(mole_ctx *ctx, bool init) {
   return MOLE_OK;
 }
 
-// Runs n sweeps; sweeps s >= sample_start of this call are sampled.
-mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
-  if (n == 0) return ok();
-  if (ctx->run.proposal == MOLE_PROPOSAL_LEARNED && !ctx->mlp)
-    return fail(MOLE_ERR_INVALID_ARGUMENT, "learned proposal: attach a network (mole_mlp_attach) first");
-  HIP_TRY(hipSetDevice(ctx->device));
-  if (ctx->family == kFamSJ && ctx->sysdev_dirty) {
-    mole_status st = sj_upload_sysdev(ctx);
-    if (st != MOLE_OK) return st;
-  }
-  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+// Enqueues n sweeps (sampled from sweep sample_start of the call on) with
+// the host bookkeeping; counts the launches.
+mole_status enqueue_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start, uint32_t *n_launches) {
   const double lo = -0.5 * ctx->run.step, hi = 0.5 * ctx->run.step;
   const double scale = hi - lo;
   const double offset = lo - scale;
@@ -443,6 +442,7 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
       if (ctx->family == kFamTwo) {
         Sweep2Args a{};
         a.sp = st.sp;
+        a.dparams = ctx->dev_params ? ctx->d_params : nullptr;
         a.W = st.W;
         a.pos = st.d_pos;
         a.acc = st.d_acc;
@@ -472,6 +472,24 @@ mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
     account(ctx, chunk, n_samp);
     done += chunk;
   }
+  *n_launches += launches;
+  return MOLE_OK;
+}
+
+// Runs n sweeps; sweeps s >= sample_start of this call are sampled.
+mole_status do_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start) {
+  if (n == 0) return ok();
+  if (ctx->run.proposal == MOLE_PROPOSAL_LEARNED && !ctx->mlp)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "learned proposal: attach a network (mole_mlp_attach) first");
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (ctx->family == kFamSJ && ctx->sysdev_dirty) {
+    mole_status st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  uint32_t launches = 0;
+  mole_status st = enqueue_sweeps(ctx, n, sample_start, &launches);
+  if (st != MOLE_OK) return st;
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   ctx->last_launches = launches;
   ctx->timed = true;
@@ -983,50 +1001,9 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
   if (sums->n == 0) return fail(MOLE_ERR_DATA_ACCESS, "no samples");
   if (optimizer > MOLE_OPT_SR || sr_mode > MOLE_SR_REFERENCE)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown optimizer or SR mode");
-  const double n = (double)sums->n;
-  const double ebar = sums->sum_e / n;
-  double g[MOLE_MAX_PARAMS], obar[MOLE_MAX_PARAMS];
-  for (int i = 0; i < np; ++i) {
-    obar[i] = sums->sum_o[i] / n;
-    g[i] = 2.0 * (sums->sum_oe[i] - ebar * sums->sum_o[i]) / n;
-  }
-  if (g_out)
-    for (int i = 0; i < np; ++i) g_out[i] = g[i];
-  if (optimizer == MOLE_OPT_SD) {
-    for (int i = 0; i < np; ++i) dp[i] = -(step * g[i]);
-    return ok();
-  }
-  // augmented matrix [S | -g/2], Gaussian elimination with partial pivoting
-  double A[MOLE_MAX_PARAMS][MOLE_MAX_PARAMS + 1];
-  double osum = 0.0;
-  for (int i = 0; i < np; ++i) osum += obar[i];
-  for (int i = 0; i < np; ++i) {
-    for (int j = 0; j < np; ++j) {
-      double s = sums->sum_oo[i * np + j] / n;
-      s -= (sr_mode == MOLE_SR_REFERENCE) ? osum * osum : obar[i] * obar[j];
-      A[i][j] = s;
-    }
-    A[i][i] *= 1.0 + ((sr_mode == MOLE_SR_REFERENCE) ? 1e-2 : sr_eps);
-    A[i][np] = -0.5 * g[i];
-  }
-  for (int k = 0; k < np; ++k) {
-    int piv = k;
-    for (int i = k + 1; i < np; ++i)
-      if (std::fabs(A[i][k]) > std::fabs(A[piv][k])) piv = i;
-    if (!(std::fabs(A[piv][k]) > 0.0)) return fail(MOLE_ERR_LINALG, "SR matrix is singular");
-    if (piv != k)
-      for (int j = 0; j <= np; ++j) std::swap(A[k][j], A[piv][j]);
-    for (int i = k + 1; i < np; ++i) {
-      const double f = A[i][k] / A[k][k];
-      for (int j = k; j <= np; ++j) A[i][j] -= f * A[k][j];
-    }
-  }
-  for (int i = np - 1; i >= 0; --i) {
-    double s = A[i][np];
-    for (int j = i + 1; j < np; ++j) s -= A[i][j] * dp[j];
-    dp[i] = s / A[i][i];
-  }
-  for (int i = 0; i < np; ++i) dp[i] = step * dp[i];
+  const int rc = sr_update(np, (double)sums->n, sums->sum_e, sums->sum_o, sums->sum_oe, sums->sum_oo, optimizer, step,
+                           sr_mode, sr_eps, dp, g_out);
+  if (rc == MOLE_ERR_LINALG) return fail(MOLE_ERR_LINALG, "SR matrix is singular");
   return ok();
 }
 
@@ -1484,5 +1461,74 @@ mole_status mole_mlp_eval(mole_mlp *m, const double *cfg, uint32_t N, const doub
   if (e == hipSuccess) e = hipStreamSynchronize(m->stream);
   cleanup();
   if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_mlp_eval: %s", hipGetErrorString(e));
+  return ok();
+}
+
+// ---- the optimisation loop on the device ------------------------------------
+
+mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size, uint32_t optimizer,
+                          double step, uint32_t sr_mode, double sr_eps, double *log, double *params_out) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamTwo || ctx->sys.size() != 1)
+    return fail(MOLE_ERR_UNSUPPORTED, "device-side optimisation covers one two-electron system");
+  if (block_size != ctx->run.block_size)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "block_size %u differs from the context's %u", block_size,
+                ctx->run.block_size);
+  if (steps < 2 * block_size)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "steps (%u) < 2 * block_size (%u)", steps, block_size);
+  if (optimizer > MOLE_OPT_SR || sr_mode > MOLE_SR_REFERENCE)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown optimizer or SR mode");
+  if (ctx->run.flags & (MOLE_FLAG_TRACE | MOLE_FLAG_SAMPLES))
+    return fail(MOLE_ERR_UNSUPPORTED, "device-side optimisation records no traces or samples");
+  if (iters == 0) return ok();
+  SysState &s = ctx->sys[0];
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!ctx->d_params) HIP_TRY(hipMalloc(&ctx->d_params, sizeof(double) * MOLE_MAX_PARAMS));
+  if ((size_t)iters > ctx->optlog_cap) {
+    if (ctx->d_optlog) (void)hipFree(ctx->d_optlog);
+    ctx->d_optlog = nullptr;
+    HIP_TRY(hipMalloc(&ctx->d_optlog, sizeof(double) * 8 * iters));
+    ctx->optlog_cap = iters;
+  }
+  HIP_TRY(hipMemcpyAsync(ctx->d_params, s.params.data(), sizeof(double) * s.params.size(), hipMemcpyHostToDevice,
+                         ctx->stream));
+  ctx->dev_params = true;
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  const uint32_t blocks = steps / block_size;
+  uint32_t launches = 0;
+  for (uint32_t it = 0; it < iters; ++it) {
+    // reset statistics (as mole_reset_stats), Runner::run, reduce, step
+    HIP_TRY(launch_fill(s.d_acc, (size_t)s.n_cols * s.W, 0.0, ctx->stream));
+    s.n_samples = s.n_blocks = s.n_moves = 0;
+    ctx->block_fill = 0;
+    ctx->trace_sweeps = ctx->trace_samples = 0;
+    st = enqueue_sweeps(ctx, blocks * block_size, block_size, &launches);
+    if (st != MOLE_OK) {
+      ctx->dev_params = false;
+      return st;
+    }
+    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, ctx->stream));
+    HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, optimizer, step, sr_mode, sr_eps,
+                            ctx->d_optlog + 8 * (size_t)it, ctx->stream));
+    launches += 3;
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->dev_params = false;
+  ctx->last_launches = launches;
+  ctx->timed = true;
+  std::vector<double> p(s.params.size()), lg(8 * (size_t)iters);
+  HIP_TRY(hipMemcpyAsync(p.data(), ctx->d_params, sizeof(double) * p.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(lg.data(), ctx->d_optlog, sizeof(double) * lg.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  s.params = p;
+  s.sp.zeta = p[0];
+  s.sp.b = p[1];
+  s.sp.c = p[2];
+  s.cols_valid = false;
+  if (params_out) std::memcpy(params_out, p.data(), sizeof(double) * p.size());
+  if (log) std::memcpy(log, lg.data(), sizeof(double) * lg.size());
+  for (uint32_t it = 0; it < iters; ++it)
+    if (lg[8 * (size_t)it + 5] != 0.0) return fail(MOLE_ERR_LINALG, "SR matrix singular at iteration %u", it);
   return ok();
 }

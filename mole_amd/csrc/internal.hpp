@@ -37,6 +37,7 @@ struct Sys2 {
 
 struct Sweep2Args {
   Sys2 sp;
+  const double *dparams;  // if set, (zeta, b, c) are read from device memory
   int W;             // walkers of this system
   double *pos;       // [6][W]
   double *acc;       // [kAccColumns2][W]
@@ -68,6 +69,13 @@ struct Eval2Args {
 
 // variant: 0 auto, 1 one lane per walker, 2 wave-specialised (when eligible)
 bool sweep2_wavespec_eligible(const Sys2 &sp);
+
+// One optimizer step on the device from the reduced accumulator columns
+// (red[kAccColumns2]) of n_samples samples: params += dp (same rounding as
+// mole_compute_parameter_update); log[0..7] = sum_e, sum_e2, sum of block
+// means, sum of squared block means, accepted moves, status, n_samples, 0.
+hipError_t launch_vmc_step(const double *red, double n_samples, double *params, uint32_t optimizer, double step,
+                           uint32_t sr_mode, double sr_eps, double *log, hipStream_t stream);
 hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads,
                          hipStream_t stream);
 hipError_t launch_init2(double *pos, int W, uint64_t seed, uint32_t sys,

@@ -18,6 +18,7 @@
 #include "../../include/mole_gpu.h"
 #include "device_common.hpp"
 #include "internal.hpp"
+#include "sr_math.hpp"
 
 namespace mole {
 
@@ -192,7 +193,12 @@ template <int KIND>
 __global__ void __launch_bounds__(256) sweep2_lane_kernel(Sweep2Args a) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
   if (w >= a.W) return;
-  const Sys2 &sp = a.sp;
+  Sys2 sp = a.sp;
+  if (a.dparams) {  // parameters updated on the device between launches
+    sp.zeta = a.dparams[0];
+    sp.b = a.dparams[1];
+    sp.c = a.dparams[2];
+  }
   const int W = a.W;
   double x[6];
 #pragma unroll
@@ -343,7 +349,12 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
   const int w = blockIdx.x * 64 + lane;
   const bool live = w < a.W;
   const int W = a.W;
-  const Sys2 &sp = a.sp;
+  Sys2 sp = a.sp;
+  if (a.dparams) {  // parameters updated on the device between launches
+    sp.zeta = a.dparams[0];
+    sp.b = a.dparams[1];
+    sp.c = a.dparams[2];
+  }
   const uint32_t nchunks = (a.n_sweeps + T - 1) / T;
   const double inv_block = 1.0 / (double)a.block_size;
 
@@ -612,6 +623,42 @@ hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out, i
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t stream) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(fill_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n, v);
+  return hipGetLastError();
+}
+
+namespace {
+__global__ void vmc_step_kernel(const double *red, double n_samples, double *params, uint32_t optimizer,
+                                double step, uint32_t sr_mode, double sr_eps, double *log) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  constexpr int np = kNParam2;
+  double so[np], soe[np], soo[np * np], dp[np];
+  int t = 0;
+  for (int k = 0; k < np; ++k) {
+    so[k] = red[kAccSumO + k];
+    soe[k] = red[kAccSumOE + k];
+    for (int l = k; l < np; ++l, ++t) {
+      soo[k * np + l] = red[kAccSumOO + t];
+      soo[l * np + k] = red[kAccSumOO + t];
+    }
+  }
+  const int rc = sr_update(np, n_samples, red[kAccSumE], so, soe, soo, optimizer, step, sr_mode, sr_eps, dp, nullptr);
+  log[0] = red[kAccSumE];
+  log[1] = red[kAccSumE2];
+  log[2] = red[kAccSumBm];
+  log[3] = red[kAccSumBm2];
+  log[4] = red[kAccAccepted];
+  log[5] = (double)rc;
+  log[6] = n_samples;
+  log[7] = 0.0;
+  if (rc == 0)
+    for (int k = 0; k < np; ++k) params[k] = params[k] + dp[k];
+}
+}  // namespace
+
+hipError_t launch_vmc_step(const double *red, double n_samples, double *params, uint32_t optimizer, double step,
+                           uint32_t sr_mode, double sr_eps, double *log, hipStream_t stream) {
+  hipLaunchKernelGGL(vmc_step_kernel, dim3(1), dim3(64), 0, stream, red, n_samples, params, optimizer, step, sr_mode,
+                     sr_eps, log);
   return hipGetLastError();
 }
 

@@ -299,3 +299,34 @@ def test_uniform_psi_always_accepts():
         ctx.run(40)
         st = ctx.stats()
     assert st["n_accepted"] == st["n_moves"] == 256 * 40 * 2
+
+
+def test_device_optimisation_loop_equals_host_loop():
+    # mole_optimize keeps the SR loop on the device; the trajectory must be
+    # the host loop's bit for bit (same sums, same solve, same rounding)
+    W, sweeps, block, iters = 1024, 60, 10, 5
+    p0 = np.array([1.0, 0.5, 0.3])
+    host_p, host_e = p0.copy(), []
+    with M.Context([M.h2(W, *p0)], seed=7, step=0.5, block_size=block) as ctx:
+        ctx.init_walkers()
+        for _ in range(iters):
+            ctx.reset_stats()
+            ctx.run(sweeps, block)
+            st = ctx.stats()
+            host_e.append((st["sum_e"], st["n_samples"], st["n_accepted"]))
+            dp, _ = M.parameter_update(ctx.sr_sums_raw(), M.OPT_SR, 0.05, M.SR_TEXTBOOK, 1e-3)
+            host_p = host_p + dp
+            ctx.set_params(host_p)
+        cfg_host = ctx.get_configs()
+    with M.Context([M.h2(W, *p0)], seed=7, step=0.5, block_size=block) as ctx:
+        ctx.init_walkers()
+        p, log = ctx.optimize(iters, sweeps, block, M.OPT_SR, 0.05, M.SR_TEXTBOOK, 1e-3)
+        assert np.array_equal(ctx.get_params(), p)
+        cfg_dev = ctx.get_configs()
+        # the context continues from the device loop's parameters and walkers
+        ctx.reset_stats()
+        ctx.run(sweeps, block)
+    assert np.array_equal(p, host_p)
+    for k, (se, n, acc) in enumerate(host_e):
+        assert log[k, 0] == se and log[k, 6] == n and log[k, 4] == acc and log[k, 5] == 0.0
+    assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
