@@ -46,6 +46,11 @@ def flops_per_launch(walkers, sweeps, sampled):
 
 VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
 
+# HBM bytes per sweep launch of this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
+# (factors from the plain copy kernel of mole_amd/copyprobe.py), recorded in
+# profiles/sweep2_ws_kernel_h2_sr_16384w.md; the algorithmic state traffic is 6.29e6 B.
+TRAFFIC_BYTES_PER_LAUNCH = 6712.6 * 1024
+
 
 def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
     """The SR optimisation on the device.  Default: mole_optimize, the whole
@@ -243,7 +248,8 @@ def main():
     fl = flops_per_launch(WALKERS, SWEEPS, SWEEPS - BLOCK)
     achieved = fl / avg_kernel_s / 1e12
     roofline = dict(bound="fp64-valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
-                    frac=achieved / PEAK_FP64_TFLOPS, traffic=None,
+                    frac=achieved / PEAK_FP64_TFLOPS,
+                    traffic=None if args.variant == "lane" else TRAFFIC_BYTES_PER_LAUNCH,
                     kernel=("sweep2_lane_kernel<LCAO2>" if args.variant == "lane"
                             else "sweep2_ws_kernel<LCAO2,2,4>"),
                     kernel_ms=1e3 * avg_kernel_s,
