@@ -45,9 +45,13 @@ struct SJLayout {
     jn = take(8u * (N + nn));
     jo = take(8u * (N + nn));
     f = take(8u * nmax);
+    // scratch: Gauss-Jordan (2 n^2 + n), per-electron local-energy results
+    // (8 N), drift pieces (3K + 3n + 3(N + n_nuc))
     const uint32_t lu = 8u * (2u * nmax * nmax + nmax);
-    const uint32_t el = 8u * (5u * K + 4u * nmax + 8u * (N + nn));
-    scr = take(lu > el ? lu : el);
+    const uint32_t el = 8u * (8u * N);
+    const uint32_t dr = 8u * (3u * K + 3u * nmax + 3u * (N + nn));
+    const uint32_t sc = lu > el ? lu : el;
+    scr = take(sc > dr ? sc : dr);
     flag = take(16u);
     mv = take(64u);  // proposed position and drift (Gaussian proposals)
     acc = take(8u * kSJColumns);
@@ -257,203 +261,137 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
 }
 
 // Local energy and d ln psi/dp (five Jastrow parameters) of the walker's
-// configuration; returns on sub-lane 0.  All lanes must call it.
+// configuration, whole electrons on lanes: lane t takes electrons t, t+L, ...
+// and forms each electron's kinetic term (orbital gradients and laplacians
+// as fma chains over the basis, contracted with the inverse column; Jastrow
+// gradient and laplacian over partners, electrons then nuclei), potential and
+// parameter derivatives in registers; sub-lane 0 adds the electrons' results
+// in electron order.  Every sum has a fixed order, so the result does not
+// depend on L.  One barrier per call; returns on sub-lane 0.
+constexpr int kSJLanesNMax = 8;  // orbitals per register chunk
+
 template <typename T>
 __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl) {
   const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
-  const int nmax = S.n_up > S.n_dn ? S.n_up : S.n_dn;
   const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
-  double *chi = wk.scr, *gch = chi + K, *lch = gch + 3 * K, *orb = lch + K, *pt = orb + 4 * nmax;
-  double kin = 0.0, V = 0.0;
-  for (int q = 0; q < kSJNParam; ++q) dl[q] = 0.0;
-  for (int i = 0; i < N; ++i) {
+  double *res = wk.scr;  // [N][8]: kinetic term, potential, five derivatives
+  for (int i = t; i < N; i += L) {
     const int s = i < S.n_up ? 0 : 1;
     const int cc = s == 0 ? i : i - S.n_up;
     const int n = s == 0 ? S.n_up : S.n_dn;
     const T *B = s == 0 ? wk.Bup : wk.Bdn;
     const double *xi = wk.pos + 3 * i;
-    // basis values and derivatives (lanes over basis functions)
-    for (int m = t; m < K; m += L) {
-      const int I = S.center[m];
-      const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
-      const double r = norm3(d0, d1, d2);
-      const double z = S.zeta[m];
-      const double ch = exp_m(-z * r);
-      const double gf = -z * ch / r;
-      chi[m] = ch;
-      gch[3 * m] = gf * d0;
-      gch[3 * m + 1] = gf * d1;
-      gch[3 * m + 2] = gf * d2;
-      lch[m] = ch * (z * z - 2.0 * z / r);
-    }
-    // Jastrow pair and nuclear terms (lanes over partners); i's own slot is
-    // zero, which leaves every sum below unchanged (they start at +0)
-    for (int j = t; j < N + nn; j += L) {
-      double *o = pt + 8 * j;
-      if (j < N) {
-        if (j == i) {
-          for (int q = 0; q < 7; ++q) o[q] = 0.0;
-          continue;
-        }
-        const double *xj = wk.pos + 3 * j;
-        const double dv0 = xi[0] - xj[0], dv1 = xi[1] - xj[1], dv2 = xi[2] - xj[2];
-        const double r = norm3(dv0, dv1, dv2);
-        const bool same = (j < S.n_up ? 0 : 1) == s;
-        const double b = same ? bP : bA;
-        const double den = 1.0 + c * r;
-        const double up = b / (den * den);
-        const double upp = -2.0 * b * c / (den * den * den);
-        o[0] = up * dv0 / r;
-        o[1] = up * dv1 / r;
-        o[2] = up * dv2 / r;
-        o[3] = upp + 2.0 * up / r;
-        o[4] = 1.0 / r;
-        o[5] = r / den;
-        o[6] = -b * r * r / (den * den);
-      } else {
-        const int I = j - N;
+    // orbitals in register chunks of kSJLanesNMax; the basis values are
+    // recomputed per chunk (bit-identical each time)
+    double Dg0 = 0.0, Dg1 = 0.0, Dg2 = 0.0, Dl = 0.0;
+    for (int k0 = 0; k0 < n; k0 += kSJLanesNMax) {
+      double og[kSJLanesNMax][4];
+#pragma unroll
+      for (int k = 0; k < kSJLanesNMax; ++k) og[k][0] = og[k][1] = og[k][2] = og[k][3] = 0.0;
+      for (int m = 0; m < K; ++m) {
+        const int I = S.center[m];
         const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
         const double r = norm3(d0, d1, d2);
-        const double Z = S.charge[I];
-        const double den = 1.0 + d * r;
-        const double aZ = a * Z;
-        const double vp = -aZ / (den * den);
-        const double vpp = 2.0 * aZ * d / (den * den * den);
-        o[0] = vp * d0 / r;
-        o[1] = vp * d1 / r;
-        o[2] = vp * d2 / r;
-        o[3] = vpp + 2.0 * vp / r;
-        o[4] = -Z / r;
-        o[5] = -Z * r / den;
-        o[6] = aZ * r * r / (den * den);
+        const double z = S.zeta[m];
+        const double ch = exp_m(-z * r);
+        const double gf = -z * ch / r;
+        const double g0 = gf * d0, g1 = gf * d1, g2 = gf * d2;
+        const double lc = ch * (z * z - 2.0 * z / r);
+        const double *cm = S.coef_t + (size_t)m * no + k0;
+#pragma unroll
+        for (int k = 0; k < kSJLanesNMax; ++k)
+          if (k0 + k < n) {
+            const double cv = cm[k];
+            og[k][0] = fma(cv, g0, og[k][0]);
+            og[k][1] = fma(cv, g1, og[k][1]);
+            og[k][2] = fma(cv, g2, og[k][2]);
+            og[k][3] = fma(cv, lc, og[k][3]);
+          }
+      }
+#pragma unroll
+      for (int k = 0; k < kSJLanesNMax; ++k)
+        if (k0 + k < n) {
+          const double bkc = (double)B[(k0 + k) * n + cc];
+          Dg0 = fma(og[k][0], bkc, Dg0);
+          Dg1 = fma(og[k][1], bkc, Dg1);
+          Dg2 = fma(og[k][2], bkc, Dg2);
+          Dl = fma(og[k][3], bkc, Dl);
+        }
+    }
+    double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0;
+    double Vi = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
+    for (int j = 0; j < N; ++j) {
+      if (j == i) continue;
+      const double *xj = wk.pos + 3 * j;
+      const double dv0 = xi[0] - xj[0], dv1 = xi[1] - xj[1], dv2 = xi[2] - xj[2];
+      const double r = norm3(dv0, dv1, dv2);
+      const bool same = (j < S.n_up ? 0 : 1) == s;
+      const double b = same ? bP : bA;
+      const double den = 1.0 + c * r;
+      const double up = b / (den * den);
+      const double upp = -2.0 * b * c / (den * den * den);
+      gJ0 += up * dv0 / r;
+      gJ1 += up * dv1 / r;
+      gJ2 += up * dv2 / r;
+      lJ += upp + 2.0 * up / r;
+      if (j > i) {
+        Vi += 1.0 / r;
+        if (same)
+          e1 += r / den;
+        else
+          e0 += r / den;
+        e2 += -b * r * r / (den * den);
       }
     }
-    __syncthreads();
-    // orbital gradients and laplacians (lanes over orbitals), chains in m order
-    for (int k = t; k < n; k += L) {
-      const double *cf = S.coef_t + k;
-      double g0 = 0.0, g1 = 0.0, g2 = 0.0, l = 0.0;
-      int m = 0;
-      for (; m + 4 <= K; m += 4) {
-        double cv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) cv[u] = cf[(size_t)(m + u) * no];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-          g0 = fma(cv[u], gch[3 * (m + u)], g0);
-          g1 = fma(cv[u], gch[3 * (m + u) + 1], g1);
-          g2 = fma(cv[u], gch[3 * (m + u) + 2], g2);
-          l = fma(cv[u], lch[m + u], l);
-        }
-      }
-      for (; m < K; ++m) {
-        const double cm = cf[(size_t)m * no];
-        g0 = fma(cm, gch[3 * m], g0);
-        g1 = fma(cm, gch[3 * m + 1], g1);
-        g2 = fma(cm, gch[3 * m + 2], g2);
-        l = fma(cm, lch[m], l);
-      }
-      orb[4 * k] = g0;
-      orb[4 * k + 1] = g1;
-      orb[4 * k + 2] = g2;
-      orb[4 * k + 3] = l;
+    for (int I = 0; I < nn; ++I) {
+      const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
+      const double r = norm3(d0, d1, d2);
+      const double Z = S.charge[I];
+      const double den = 1.0 + d * r;
+      const double aZ = a * Z;
+      const double vp = -aZ / (den * den);
+      const double vpp = 2.0 * aZ * d / (den * den * den);
+      gJ0 += vp * d0 / r;
+      gJ1 += vp * d1 / r;
+      gJ2 += vp * d2 / r;
+      lJ += vpp + 2.0 * vp / r;
+      Vi += -Z / r;
+      e3 += -Z * r / den;
+      e4 += aZ * r * r / (den * den);
     }
-    __syncthreads();
-    if (t == 0) {
-      // determinant part, four chains in k order
-      double Dg0 = 0.0, Dg1 = 0.0, Dg2 = 0.0, Dl = 0.0;
-      {
-        int k = 0;
-        for (; k + 2 <= n; k += 2) {
-          double bv[2], ov[2][4];
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            bv[u] = (double)B[(k + u) * n + cc];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) ov[u][q] = orb[4 * (k + u) + q];
-          }
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            Dg0 = fma(ov[u][0], bv[u], Dg0);
-            Dg1 = fma(ov[u][1], bv[u], Dg1);
-            Dg2 = fma(ov[u][2], bv[u], Dg2);
-            Dl = fma(ov[u][3], bv[u], Dl);
-          }
-        }
-        for (; k < n; ++k) {
-          const double bkc = (double)B[k * n + cc];
-          Dg0 = fma(orb[4 * k], bkc, Dg0);
-          Dg1 = fma(orb[4 * k + 1], bkc, Dg1);
-          Dg2 = fma(orb[4 * k + 2], bkc, Dg2);
-          Dl = fma(orb[4 * k + 3], bkc, Dl);
-        }
-      }
-      // Jastrow gradient and laplacian: electrons (i's slot zero), then nuclei
-      double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0;
-      {
-        const int M = N + nn;
-        int j = 0;
-        for (; j + 2 <= M; j += 2) {
-          double v[2][4];
-#pragma unroll
-          for (int u = 0; u < 2; ++u)
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[u][q] = pt[8 * (j + u) + q];
-#pragma unroll
-          for (int u = 0; u < 2; ++u) {
-            gJ0 += v[u][0];
-            gJ1 += v[u][1];
-            gJ2 += v[u][2];
-            lJ += v[u][3];
-          }
-        }
-        for (; j < M; ++j) {
-          gJ0 += pt[8 * j];
-          gJ1 += pt[8 * j + 1];
-          gJ2 += pt[8 * j + 2];
-          lJ += pt[8 * j + 3];
-        }
-      }
-      // pairs j > i: same spin up to same_end, opposite spin after; then nuclei
-      double Vi = 0.0;
-      const int same_end = i < S.n_up ? S.n_up : N;
-#pragma unroll 4
-      for (int j = i + 1; j < same_end; ++j) {
-        Vi += pt[8 * j + 4];
-        dl[1] += pt[8 * j + 5];
-        dl[2] += pt[8 * j + 6];
-      }
-#pragma unroll 4
-      for (int j = same_end; j < N; ++j) {
-        Vi += pt[8 * j + 4];
-        dl[0] += pt[8 * j + 5];
-        dl[2] += pt[8 * j + 6];
-      }
-#pragma unroll 4
-      for (int I = 0; I < nn; ++I) {
-        const double *o = pt + 8 * (N + I);
-        Vi += o[4];
-        dl[3] += o[5];
-        dl[4] += o[6];
-      }
-      const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
-      const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
-      kin += Dl + 2.0 * dot + lJ + gg;
-      V += Vi;
-    }
-    __syncthreads();
+    const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
+    const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
+    double *o = res + 8 * i;
+    o[0] = Dl + 2.0 * dot + lJ + gg;
+    o[1] = Vi;
+    o[2] = e0;
+    o[3] = e1;
+    o[4] = e2;
+    o[5] = e3;
+    o[6] = e4;
   }
-  if (t == 0 && S.hamiltonian != MOLE_H_ELECTRONIC) {
-    V = 0.0;
-    for (int i = 0; i < N; ++i)
-      for (int I = 0; I < nn; ++I) {
-        const double *xi = wk.pos + 3 * i;
-        V += -S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
-      }
+  __syncthreads();
+  double kin = 0.0, V = 0.0;
+  for (int q = 0; q < kSJNParam; ++q) dl[q] = 0.0;
+  if (t == 0) {
+    for (int i = 0; i < N; ++i) {
+      const double *o = res + 8 * i;
+      kin += o[0];
+      V += o[1];
+      for (int q = 0; q < kSJNParam; ++q) dl[q] += o[2 + q];
+    }
+    if (S.hamiltonian != MOLE_H_ELECTRONIC) {
+      V = 0.0;
+      for (int i = 0; i < N; ++i)
+        for (int I = 0; I < nn; ++I) {
+          const double *xi = wk.pos + 3 * i;
+          V += -S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
+        }
+    }
   }
+  __syncthreads();
   return -0.5 * kin + V + S.v_nn;
 }
-
 
 // Pieces of grad_i ln psi at r for the drift-diffusion proposal, FP64
 // (metrop.rs:153-155).  Basis gradients into gch[3m + q] (lanes over basis

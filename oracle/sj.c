@@ -282,7 +282,9 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
       for (int q = 0; q < 3; ++q) Dg[q] = fma(g[q], bkc, Dg[q]);
       Dl = fma(l, bkc, Dl);
     }
-    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0;
+    /* electron i's parameter derivatives are summed on their own and added
+     * to the totals in electron order (the device's order) */
+    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0, dli[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
     for (int j = 0; j < N; ++j) {
       if (j == i) continue;
       const double *xj = cfg + 3 * j;
@@ -298,10 +300,10 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
         Vi += 1.0 / r;
         double t = r / den;
         if (spin_of(wf, j) == s)
-          dl[1] += t;
+          dli[1] += t;
         else
-          dl[0] += t;
-        dl[2] += -b * r * r / (den * den);
+          dli[0] += t;
+        dli[2] += -b * r * r / (den * den);
       }
     }
     for (int I = 0; I < wf->n_nuc; ++I) {
@@ -314,13 +316,14 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
       for (int q = 0; q < 3; ++q) gJ[q] += vp * dI[I][q] / r;
       lJ += vpp + 2.0 * vp / r;
       Vi += -Z / r;
-      dl[3] += -Z * r / den;
-      dl[4] += aZ * r * r / (den * den);
+      dli[3] += -Z * r / den;
+      dli[4] += aZ * r * r / (den * den);
     }
     double dot = Dg[0] * gJ[0] + Dg[1] * gJ[1] + Dg[2] * gJ[2];
     double gg = gJ[0] * gJ[0] + gJ[1] * gJ[1] + gJ[2] * gJ[2];
     kin += Dl + 2.0 * dot + lJ + gg;
     V += Vi;
+    for (int q = 0; q < ORC_SJ_NPARAM; ++q) dl[q] += dli[q];
   }
   double vnn = 0.0;
   for (int I = 0; I < wf->n_nuc; ++I)
