@@ -283,6 +283,63 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
                           uint32_t optimizer, double step, uint32_t sr_mode, double sr_eps,
                           double *log, double *params_out);
 
+/* ---- stateful optimizers ------------------------------------------------
+ * MomentumDescent (src/optimize/src/optimizers.rs:32-61), NesterovMomentum
+ * (:63-95) and OnlineLbfgs (:97-179) keep state between calls of
+ * Optimizer::compute_parameter_update (src/optimize/src/traits.rs:18-25).
+ * mole_optimizer is that state as plain data (it is copied to and from the
+ * device by mole_optimize_with); SD and SR are accepted too and keep none.
+ * With g the energy gradient from the sums (as above) and pars the current
+ * parameters:
+ *   MOMENTUM: m -= step g; dp = mu m                       (:49-60)
+ *   NESTEROV: m_prev = m; m = mu m + step g;
+ *             dp = -(mu m_prev + (1 + mu) m)                (:83-94)
+ *   LBFGS:    the pair (pars - pars_prev, g - grad_prev) is pushed twice
+ *             per call (:163,170), the oldest pairs dropped beyond
+ *             `history`; two-loop recursion from -g scaled by 1e-10 on the
+ *             first call, else by sum(s.y / y.y) / min(iter, history);
+ *             dp = -step p; pars_prev and grad_prev start at 1e-5.
+ * Dot products run in index order. */
+#define MOLE_OPT_MOMENTUM 2
+#define MOLE_OPT_NESTEROV 3
+#define MOLE_OPT_LBFGS 4
+#define MOLE_LBFGS_MAX_HISTORY 16
+typedef struct {
+  uint32_t kind;      /* MOLE_OPT_* */
+  uint32_t n_params;
+  uint32_t sr_mode;   /* SR only */
+  uint32_t history;   /* LBFGS: pairs kept, 1..MOLE_LBFGS_MAX_HISTORY */
+  double step;
+  double sr_eps;      /* SR only */
+  double momentum;    /* MOMENTUM / NESTEROV: momentum_parameter */
+  uint64_t iter;      /* calls so far */
+  uint32_t n_pairs;   /* LBFGS pairs held, oldest at index `head` */
+  uint32_t head;
+  double m[MOLE_MAX_PARAMS];
+  double m_prev[MOLE_MAX_PARAMS];
+  double grad_prev[MOLE_MAX_PARAMS];
+  double pars_prev[MOLE_MAX_PARAMS];
+  double s[MOLE_LBFGS_MAX_HISTORY][MOLE_MAX_PARAMS];
+  double y[MOLE_LBFGS_MAX_HISTORY][MOLE_MAX_PARAMS];
+} mole_optimizer;
+/* The constructors MomentumDescent::new(step, mu, nparm) (:40-46),
+ * NesterovMomentum::new (:71-78), OnlineLbfgs::new(step, history, nparm)
+ * (:108-119), SteepestDescent::new(step) (:14-16) and
+ * StochasticReconfiguration::new(step) (:187-189; sr_mode/sr_eps as for
+ * mole_compute_parameter_update).  Unused arguments are ignored. */
+mole_status mole_optimizer_init(mole_optimizer *opt, uint32_t kind, uint32_t n_params, double step,
+                                double momentum, uint32_t history, uint32_t sr_mode, double sr_eps);
+/* Optimizer::compute_parameter_update from device-reduced sums; pars
+ * (n_params, read by LBFGS only) may be NULL for the other kinds; g may be
+ * NULL.  A singular SR matrix returns MOLE_ERR_LINALG and leaves opt as it
+ * was. */
+mole_status mole_optimizer_update(mole_optimizer *opt, const mole_sr_sums *sums, const double *pars,
+                                  double *dp, double *g);
+/* mole_optimize with any optimizer: opt is copied to the device, advanced
+ * once per iteration there, and copied back (its state after the loop). */
+mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size,
+                               mole_optimizer *opt, double *log, double *params_out);
+
 /* ---- launch configuration ----------------------------------------------
  * No counterpart in the reference (rayon's fan-out, src/vmc/src/vmc.rs:63-76,
  * is the closest).  Selects the sweep kernel: AUTO picks the

@@ -48,6 +48,10 @@ MAX_PARAMS = 16
 
 OPT_SD = 0
 OPT_SR = 1
+OPT_MOMENTUM = 2
+OPT_NESTEROV = 3
+OPT_LBFGS = 4
+LBFGS_MAX_HISTORY = 16
 SR_TEXTBOOK = 0
 SR_REFERENCE = 1
 
@@ -135,6 +139,28 @@ class SrSums(C.Structure):
     ]
 
 
+class OptimizerState(C.Structure):
+    """mole_optimizer: the state of a stateful optimizer (plain data)."""
+    _fields_ = [
+        ("kind", C.c_uint32),
+        ("n_params", C.c_uint32),
+        ("sr_mode", C.c_uint32),
+        ("history", C.c_uint32),
+        ("step", C.c_double),
+        ("sr_eps", C.c_double),
+        ("momentum", C.c_double),
+        ("iter", C.c_uint64),
+        ("n_pairs", C.c_uint32),
+        ("head", C.c_uint32),
+        ("m", C.c_double * MAX_PARAMS),
+        ("m_prev", C.c_double * MAX_PARAMS),
+        ("grad_prev", C.c_double * MAX_PARAMS),
+        ("pars_prev", C.c_double * MAX_PARAMS),
+        ("s", (C.c_double * MAX_PARAMS) * LBFGS_MAX_HISTORY),
+        ("y", (C.c_double * MAX_PARAMS) * LBFGS_MAX_HISTORY),
+    ]
+
+
 _lib = None
 
 # (name, restype, argtypes)
@@ -174,6 +200,11 @@ _SIGS = [
     ("mole_synchronize", C.c_int32, [_P]),
     ("mole_last_timing", C.c_int32, [_P, _pd, C.POINTER(C.c_uint32)]),
     ("mole_compute_parameter_update", C.c_int32, [C.POINTER(SrSums), C.c_uint32, C.c_double, C.c_uint32, C.c_double, _pd, _pd]),
+    ("mole_optimizer_init", C.c_int32, [C.POINTER(OptimizerState), C.c_uint32, C.c_uint32, C.c_double, C.c_double,
+                                        C.c_uint32, C.c_uint32, C.c_double]),
+    ("mole_optimizer_update", C.c_int32, [C.POINTER(OptimizerState), C.POINTER(SrSums), _pd, _pd, _pd]),
+    ("mole_optimize_with", C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(OptimizerState),
+                                       _pd, _pd]),
     ("mole_debug_philox", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t, _pu64]),
     ("mole_debug_math", C.c_int32, [C.c_int32, C.c_size_t, _pd, _pd, _pd]),
     ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
@@ -420,6 +451,14 @@ class Context:
                                   _dp(p)))
         return p, log
 
+    def optimize_with(self, opt, iters, steps, block_size):
+        """mole_optimize_with: the device loop with any Optimizer (its state
+        advances in place).  Returns (params, log[iters][8])."""
+        log = np.zeros((iters, 8))
+        p = np.zeros(self.systems[0].params.size)
+        check(lib().mole_optimize_with(self.h, iters, steps, block_size, C.byref(opt.state), _dp(log), _dp(p)))
+        return p, log
+
     def set_launch_config(self, variant=VARIANT_AUTO, block_threads=0):
         check(lib().mole_set_launch_config(self.h, variant, block_threads))
 
@@ -590,6 +629,25 @@ def parameter_update(sums, optimizer=OPT_SR, step=0.05, sr_mode=SR_TEXTBOOK, sr_
     check(lib().mole_compute_parameter_update(C.byref(sums), optimizer, step, sr_mode, sr_eps,
                                               _dp(dp), _dp(g)))
     return dp, g
+
+
+class Optimizer:
+    """A stateful optimizer (mole_optimizer_init / mole_optimizer_update):
+    SteepestDescent, StochasticReconfiguration, MomentumDescent,
+    NesterovMomentum or OnlineLbfgs of src/optimize/src/optimizers.rs."""
+
+    def __init__(self, kind, n_params, step, momentum=0.0, history=0, sr_mode=SR_TEXTBOOK, sr_eps=1e-3):
+        self.state = OptimizerState()
+        check(lib().mole_optimizer_init(C.byref(self.state), kind, n_params, step, momentum, history, sr_mode,
+                                        sr_eps))
+
+    def compute_parameter_update(self, pars, sums):
+        """Returns (dp, g) and advances the state."""
+        n = self.state.n_params
+        dp, g = np.zeros(n), np.zeros(n)
+        pa = np.ascontiguousarray(pars, dtype=np.float64)
+        check(lib().mole_optimizer_update(C.byref(self.state), C.byref(sums), _dp(pa), _dp(dp), _dp(g)))
+        return dp, g
 
 
 def debug_copy(n_doubles, repeats=1):

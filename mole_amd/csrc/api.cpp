@@ -113,6 +113,7 @@ struct mole_ctx {
   mole_mlp *mlp = nullptr;  // network of the learned proposal (not owned)
   // device-side optimisation: parameters read by the kernels from d_params
   double *d_params = nullptr, *d_optlog = nullptr;
+  mole_optimizer *d_opt = nullptr;
   size_t optlog_cap = 0;
   bool dev_params = false;
 };
@@ -255,6 +256,7 @@ void free_ctx(mole_ctx *ctx) {
   if (ctx->d_red) (void)hipFree(ctx->d_red);
   if (ctx->d_params) (void)hipFree(ctx->d_params);
   if (ctx->d_optlog) (void)hipFree(ctx->d_optlog);
+  if (ctx->d_opt) (void)hipFree(ctx->d_opt);
   if (ctx->d_sysdev) (void)hipFree(ctx->d_sysdev);
   if (ctx->d_state) (void)hipFree(ctx->d_state);
   if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
@@ -1007,6 +1009,52 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
   return ok();
 }
 
+mole_status mole_optimizer_init(mole_optimizer *opt, uint32_t kind, uint32_t n_params, double step, double momentum,
+                                uint32_t history, uint32_t sr_mode, double sr_eps) {
+  if (!opt) return fail(MOLE_ERR_INVALID_ARGUMENT, "null optimizer");
+  if (kind > MOLE_OPT_LBFGS) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown optimizer %u", kind);
+  if (n_params < 1 || n_params > MOLE_MAX_PARAMS) return fail(MOLE_ERR_SHAPE, "n_params out of range");
+  if (kind == MOLE_OPT_SR && sr_mode > MOLE_SR_REFERENCE) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown SR mode");
+  if (kind == MOLE_OPT_LBFGS && (history < 1 || history > MOLE_LBFGS_MAX_HISTORY))
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "LBFGS history must be 1..%d", MOLE_LBFGS_MAX_HISTORY);
+  std::memset(opt, 0, sizeof(*opt));
+  opt->kind = kind;
+  opt->n_params = n_params;
+  opt->sr_mode = kind == MOLE_OPT_SR ? sr_mode : 0;
+  opt->history = kind == MOLE_OPT_LBFGS ? history : 0;
+  opt->step = step;
+  opt->sr_eps = sr_eps;
+  opt->momentum = momentum;
+  if (kind == MOLE_OPT_LBFGS)
+    for (uint32_t i = 0; i < n_params; ++i) opt->grad_prev[i] = opt->pars_prev[i] = 1e-5;  // optimizers.rs:109,115-116
+  return ok();
+}
+
+static mole_status check_optimizer(const mole_optimizer *opt) {
+  if (!opt) return fail(MOLE_ERR_INVALID_ARGUMENT, "null optimizer");
+  if (opt->kind > MOLE_OPT_LBFGS || opt->n_params < 1 || opt->n_params > MOLE_MAX_PARAMS ||
+      (opt->kind == MOLE_OPT_SR && opt->sr_mode > MOLE_SR_REFERENCE) ||
+      (opt->kind == MOLE_OPT_LBFGS && (opt->history < 1 || opt->history > MOLE_LBFGS_MAX_HISTORY ||
+                                       opt->n_pairs > opt->history || opt->head >= opt->history)))
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "optimizer state is not one mole_optimizer_init made");
+  return ok();
+}
+
+mole_status mole_optimizer_update(mole_optimizer *opt, const mole_sr_sums *sums, const double *pars, double *dp,
+                                  double *g) {
+  mole_status st = check_optimizer(opt);
+  if (st != MOLE_OK) return st;
+  if (!sums || !dp) return fail(MOLE_ERR_INVALID_ARGUMENT, "null argument");
+  if (sums->n_params != opt->n_params) return fail(MOLE_ERR_SHAPE, "sums hold %u parameters, the optimizer %u",
+                                                   sums->n_params, opt->n_params);
+  if (sums->n == 0) return fail(MOLE_ERR_DATA_ACCESS, "no samples");
+  if (opt->kind == MOLE_OPT_LBFGS && !pars) return fail(MOLE_ERR_INVALID_ARGUMENT, "LBFGS needs the parameters");
+  const int rc = optimizer_update(opt, (double)sums->n, sums->sum_e, sums->sum_o, sums->sum_oe, sums->sum_oo, pars,
+                                  dp, g);
+  if (rc == MOLE_ERR_LINALG) return fail(MOLE_ERR_LINALG, "SR matrix is singular");
+  return ok();
+}
+
 mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t block_threads) {
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
@@ -1466,8 +1514,8 @@ mole_status mole_mlp_eval(mole_mlp *m, const double *cfg, uint32_t N, const doub
 
 // ---- the optimisation loop on the device ------------------------------------
 
-mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size, uint32_t optimizer,
-                          double step, uint32_t sr_mode, double sr_eps, double *log, double *params_out) {
+mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size,
+                               mole_optimizer *opt, double *log, double *params_out) {
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
   if (ctx->family != kFamTwo || ctx->sys.size() != 1)
@@ -1477,8 +1525,11 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
                 ctx->run.block_size);
   if (steps < 2 * block_size)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "steps (%u) < 2 * block_size (%u)", steps, block_size);
-  if (optimizer > MOLE_OPT_SR || sr_mode > MOLE_SR_REFERENCE)
-    return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown optimizer or SR mode");
+  st = check_optimizer(opt);
+  if (st != MOLE_OK) return st;
+  if (opt->n_params != ctx->sys[0].params.size())
+    return fail(MOLE_ERR_SHAPE, "optimizer has %u parameters, the system %zu", opt->n_params,
+                ctx->sys[0].params.size());
   if (ctx->run.flags & (MOLE_FLAG_TRACE | MOLE_FLAG_SAMPLES))
     return fail(MOLE_ERR_UNSUPPORTED, "device-side optimisation records no traces or samples");
   if (iters == 0) return ok();
@@ -1491,8 +1542,10 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
     HIP_TRY(hipMalloc(&ctx->d_optlog, sizeof(double) * 8 * iters));
     ctx->optlog_cap = iters;
   }
+  if (!ctx->d_opt) HIP_TRY(hipMalloc(&ctx->d_opt, sizeof(mole_optimizer)));
   HIP_TRY(hipMemcpyAsync(ctx->d_params, s.params.data(), sizeof(double) * s.params.size(), hipMemcpyHostToDevice,
                          ctx->stream));
+  HIP_TRY(hipMemcpyAsync(ctx->d_opt, opt, sizeof(mole_optimizer), hipMemcpyHostToDevice, ctx->stream));
   ctx->dev_params = true;
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   const uint32_t blocks = steps / block_size;
@@ -1509,8 +1562,8 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
       return st;
     }
     HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, ctx->stream));
-    HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, optimizer, step, sr_mode, sr_eps,
-                            ctx->d_optlog + 8 * (size_t)it, ctx->stream));
+    HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, ctx->d_opt, ctx->d_optlog + 8 * (size_t)it,
+                            ctx->stream));
     launches += 3;
   }
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1520,6 +1573,7 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
   std::vector<double> p(s.params.size()), lg(8 * (size_t)iters);
   HIP_TRY(hipMemcpyAsync(p.data(), ctx->d_params, sizeof(double) * p.size(), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipMemcpyAsync(lg.data(), ctx->d_optlog, sizeof(double) * lg.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipMemcpyAsync(opt, ctx->d_opt, sizeof(mole_optimizer), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   s.params = p;
   s.sp.zeta = p[0];
@@ -1531,4 +1585,16 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
   for (uint32_t it = 0; it < iters; ++it)
     if (lg[8 * (size_t)it + 5] != 0.0) return fail(MOLE_ERR_LINALG, "SR matrix singular at iteration %u", it);
   return ok();
+}
+
+mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size, uint32_t optimizer,
+                          double step, uint32_t sr_mode, double sr_eps, double *log, double *params_out) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (optimizer > MOLE_OPT_SR || sr_mode > MOLE_SR_REFERENCE)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown optimizer or SR mode");
+  mole_optimizer opt;
+  st = mole_optimizer_init(&opt, optimizer, (uint32_t)ctx->sys[0].params.size(), step, 0.0, 0, sr_mode, sr_eps);
+  if (st != MOLE_OK) return st;
+  return mole_optimize_with(ctx, iters, steps, block_size, &opt, log, params_out);
 }

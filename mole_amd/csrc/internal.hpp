@@ -74,8 +74,8 @@ bool sweep2_wavespec_eligible(const Sys2 &sp);
 // (red[kAccColumns2]) of n_samples samples: params += dp (same rounding as
 // mole_compute_parameter_update); log[0..7] = sum_e, sum_e2, sum of block
 // means, sum of squared block means, accepted moves, status, n_samples, 0.
-hipError_t launch_vmc_step(const double *red, double n_samples, double *params, uint32_t optimizer, double step,
-                           uint32_t sr_mode, double sr_eps, double *log, hipStream_t stream);
+hipError_t launch_vmc_step(const double *red, double n_samples, double *params, mole_optimizer *opt, double *log,
+                           hipStream_t stream);
 hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads,
                          hipStream_t stream);
 hipError_t launch_init2(double *pos, int W, uint64_t seed, uint32_t sys,

@@ -627,21 +627,22 @@ hipError_t launch_fill(double *p, size_t n, double v, hipStream_t stream) {
 }
 
 namespace {
-__global__ void vmc_step_kernel(const double *red, double n_samples, double *params, uint32_t optimizer,
-                                double step, uint32_t sr_mode, double sr_eps, double *log) {
+__global__ void vmc_step_kernel(const double *red, double n_samples, double *params, mole_optimizer *opt,
+                                double *log) {
   if (threadIdx.x != 0 || blockIdx.x != 0) return;
   constexpr int np = kNParam2;
-  double so[np], soe[np], soo[np * np], dp[np];
+  double so[np], soe[np], soo[np * np], dp[np], pars[np];
   int t = 0;
   for (int k = 0; k < np; ++k) {
     so[k] = red[kAccSumO + k];
     soe[k] = red[kAccSumOE + k];
+    pars[k] = params[k];
     for (int l = k; l < np; ++l, ++t) {
       soo[k * np + l] = red[kAccSumOO + t];
       soo[l * np + k] = red[kAccSumOO + t];
     }
   }
-  const int rc = sr_update(np, n_samples, red[kAccSumE], so, soe, soo, optimizer, step, sr_mode, sr_eps, dp, nullptr);
+  const int rc = optimizer_update(opt, n_samples, red[kAccSumE], so, soe, soo, pars, dp, nullptr);
   log[0] = red[kAccSumE];
   log[1] = red[kAccSumE2];
   log[2] = red[kAccSumBm];
@@ -651,14 +652,13 @@ __global__ void vmc_step_kernel(const double *red, double n_samples, double *par
   log[6] = n_samples;
   log[7] = 0.0;
   if (rc == 0)
-    for (int k = 0; k < np; ++k) params[k] = params[k] + dp[k];
+    for (int k = 0; k < np; ++k) params[k] = pars[k] + dp[k];
 }
 }  // namespace
 
-hipError_t launch_vmc_step(const double *red, double n_samples, double *params, uint32_t optimizer, double step,
-                           uint32_t sr_mode, double sr_eps, double *log, hipStream_t stream) {
-  hipLaunchKernelGGL(vmc_step_kernel, dim3(1), dim3(64), 0, stream, red, n_samples, params, optimizer, step, sr_mode,
-                     sr_eps, log);
+hipError_t launch_vmc_step(const double *red, double n_samples, double *params, mole_optimizer *opt, double *log,
+                           hipStream_t stream) {
+  hipLaunchKernelGGL(vmc_step_kernel, dim3(1), dim3(64), 0, stream, red, n_samples, params, opt, log);
   return hipGetLastError();
 }
 

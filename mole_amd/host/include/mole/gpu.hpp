@@ -377,13 +377,14 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
   bool pending_ = false;
 };
 
-// SteepestDescent and StochasticReconfiguration
-// (src/optimize/src/optimizers.rs:9-30, 181-253) on the "SR sums" the
-// device reduced (mole_compute_parameter_update).
+// The optimizers of src/optimize/src/optimizers.rs on the "SR sums" the
+// device reduced (mole_optimizer_update): SteepestDescent (:9-30),
+// MomentumDescent (:32-61), NesterovMomentum (:63-95), OnlineLbfgs
+// (:97-179) and StochasticReconfiguration (:181-253).  The state is a
+// mole_optimizer (plain data), so copies are independent optimizers as the
+// reference's Clone makes them; state() hands it to mole_optimize_with.
 class SumsOptimizer : public Optimizer {
  public:
-  SumsOptimizer(uint32_t kind, double step, uint32_t sr_mode, double sr_eps)
-      : kind_(kind), step_(step), sr_mode_(sr_mode), eps_(sr_eps) {}
   Result<Array1> compute_parameter_update(const Array1 &pars, const AverageMap &averages,
                                           const DataMap &raw_data) override {
     (void)averages;
@@ -395,24 +396,64 @@ class SumsOptimizer : public Optimizer {
     const Result<mole_sr_sums> r = GpuSampler<GpuWaveFunction>::unpack_sums(m.value());
     if (r.is_err()) return r.error();
     if (r.value().n_params != pars.size()) return Error{ErrorKind::ShapeError, "parameter count mismatch"};
+    if (!ready_) {  // SD and SR learn the parameter count from the first call
+      const Status st = status(mole_optimizer_init(&opt_, kind_, (uint32_t)pars.size(), step_, mu_, history_,
+                                                   sr_mode_, eps_));
+      if (st.is_err()) return st.error();
+      ready_ = true;
+    }
     Array1 dp(pars.size()), g(pars.size());
-    const Status st = status(mole_compute_parameter_update(&r.value(), kind_, step_, sr_mode_, eps_, dp.data(), g.data()));
+    const Status st = status(mole_optimizer_update(&opt_, &r.value(), pars.data(), dp.data(), g.data()));
     if (st.is_err()) return st.error();
     last_gradient_ = g;
     return dp;
   }
   const Array1 &last_gradient() const { return last_gradient_; }
+  // the state (valid once ready(): after the first update, or from the
+  // constructor for the optimizers that take nparm)
+  mole_optimizer &state() { return opt_; }
+  bool ready() const { return ready_; }
+
+ protected:
+  SumsOptimizer(uint32_t kind, double step, double mu, uint32_t history, uint32_t sr_mode, double sr_eps,
+                size_t nparm)
+      : kind_(kind), step_(step), mu_(mu), history_(history), sr_mode_(sr_mode), eps_(sr_eps) {
+    if (nparm) {
+      const Status st = status(mole_optimizer_init(&opt_, kind, (uint32_t)nparm, step, mu, history, sr_mode, sr_eps));
+      if (st.is_err()) throw Panic(st.error().message);
+      ready_ = true;
+    }
+  }
 
  private:
   uint32_t kind_;
-  double step_;
-  uint32_t sr_mode_;
+  double step_, mu_;
+  uint32_t history_, sr_mode_;
   double eps_;
+  bool ready_ = false;
+  mole_optimizer opt_{};
   Array1 last_gradient_;
 };
 
 struct SteepestDescent : SumsOptimizer {
-  explicit SteepestDescent(double step) : SumsOptimizer(MOLE_OPT_SD, step, MOLE_SR_TEXTBOOK, 0.0) {}
+  explicit SteepestDescent(double step) : SumsOptimizer(MOLE_OPT_SD, step, 0.0, 0, MOLE_SR_TEXTBOOK, 0.0, 0) {}
+};
+
+struct MomentumDescent : SumsOptimizer {
+  MomentumDescent(double step, double momentum_parameter, size_t nparm)
+      : SumsOptimizer(MOLE_OPT_MOMENTUM, step, momentum_parameter, 0, 0, 0.0, nparm) {}
+};
+
+struct NesterovMomentum : SumsOptimizer {
+  NesterovMomentum(double step, double momentum_parameter, size_t nparm)
+      : SumsOptimizer(MOLE_OPT_NESTEROV, step, momentum_parameter, 0, 0, 0.0, nparm) {}
+};
+
+struct OnlineLbfgs : SumsOptimizer {
+  // history: pairs kept, 1..MOLE_LBFGS_MAX_HISTORY (each call pushes its
+  // pair twice, as optimizers.rs:163,170 do)
+  OnlineLbfgs(double step, size_t history, size_t nparm)
+      : SumsOptimizer(MOLE_OPT_LBFGS, step, 0.0, (uint32_t)history, 0, 0.0, nparm) {}
 };
 
 struct StochasticReconfiguration : SumsOptimizer {
@@ -420,7 +461,7 @@ struct StochasticReconfiguration : SumsOptimizer {
   // (scalar subtraction, diag *= 1.01); MOLE_SR_TEXTBOOK subtracts Obar Obar^T
   // and scales the diagonal by 1 + eps
   StochasticReconfiguration(double step, uint32_t sr_mode = MOLE_SR_TEXTBOOK, double eps = 1e-3)
-      : SumsOptimizer(MOLE_OPT_SR, step, sr_mode, eps) {}
+      : SumsOptimizer(MOLE_OPT_SR, step, 0.0, 0, sr_mode, eps, 0) {}
 };
 
 }  // namespace mole

@@ -134,6 +134,42 @@ static void test_runner_schedule() {
   CHECK(std::fabs(out.value().energy_errors[0] - 5.0) < 1e-12);
 }
 
+// The stateful optimizers of optimizers.rs:32-179 over fabricated "SR sums"
+// (host arithmetic only): n = 4, sum_e = 8, sum_o = (2, 4), sum_oe = (6, 4)
+// give Ebar = 2 and g = 2 (sum_oe - Ebar sum_o) / n = (1, -2).
+static void test_stateful_optimizers() {
+  mole_sr_sums r{};
+  r.n_params = 2;
+  r.n = 4;
+  r.sum_e = 8.0;
+  r.sum_o[0] = 2.0;
+  r.sum_o[1] = 4.0;
+  r.sum_oe[0] = 6.0;
+  r.sum_oe[1] = 4.0;
+  DataMap raw;
+  raw["SR sums"] = {OperatorValue::matrix(GpuSampler<GpuWaveFunction>::pack_sums(r))};
+  const AverageMap avg;
+  const Array1 pars{0.0, 0.0};
+  // MomentumDescent: m -= 0.5 g; dp = 0.9 m
+  MomentumDescent md(0.5, 0.9, 2);
+  Array1 dp = md.compute_parameter_update(pars, avg, raw).value();
+  CHECK(dp[0] == 0.9 * -0.5 && dp[1] == 0.9 * 1.0);
+  MomentumDescent copy = md;  // Clone: an independent optimizer
+  dp = md.compute_parameter_update(pars, avg, raw).value();
+  CHECK(dp[0] == 0.9 * -1.0 && dp[1] == 0.9 * 2.0);
+  CHECK(copy.compute_parameter_update(pars, avg, raw).value()[0] == dp[0]);
+  CHECK(md.last_gradient()[0] == 1.0 && md.last_gradient()[1] == -2.0);
+  // NesterovMomentum: m = 0.9 * 0 + 0.5 g; dp = -(0.9 * 0 + 1.9 m)
+  NesterovMomentum nm(0.5, 0.9, 2);
+  dp = nm.compute_parameter_update(pars, avg, raw).value();
+  CHECK(dp[0] == -(1.9 * 0.5) && dp[1] == -(1.9 * -1.0));
+  // the history must hold at least one pair; the parameter count must match
+  CHECK(panics([] { OnlineLbfgs(0.1, 0, 2); }));
+  OnlineLbfgs lb(0.1, 4, 3);
+  CHECK(lb.compute_parameter_update(pars, avg, raw).is_err());
+  CHECK(lb.state().history == 4 && lb.state().pars_prev[0] == 1e-5);
+}
+
 static int run_gpu(int iters, int walkers, int sweeps, int block, double box, double step, double eps,
                    uint64_t seed) {
   GpuSampler<GpuWaveFunction> sampler(GpuWaveFunction::hydrogen_molecule(1.0, 0.5, 0.3), (uint32_t)walkers, box,
@@ -179,6 +215,7 @@ int main(int argc, char **argv) {
   test_operator_value();
   test_status_mapping();
   test_runner_schedule();
+  test_stateful_optimizers();
   if (g_fail) {
     std::fprintf(stderr, "%d check(s) failed\n", g_fail);
     return 1;

@@ -234,6 +234,32 @@ int orc_vmc_optimize(orc_wf2 *wf, uint64_t seed, double box_side,
                      double *energies, double *errors, double *acceptance,
                      double *grads, double *params_out);
 
+/* Stateful optimizers of src/optimize/src/optimizers.rs, restated on a
+ * given energy gradient g (as compute_energy_gradient returns it):
+ * MomentumDescent (:32-61), NesterovMomentum (:63-95), OnlineLbfgs
+ * (:97-179, with its double update_curvature_pairs at :163,:170).  The
+ * LBFGS pair lists are VecDeques: oldest first, pop_front when full. */
+#define ORC_MAX_HISTORY 64
+typedef struct {
+  int kind; /* ORC_OPT_MOMENTUM / ORC_OPT_NESTEROV / ORC_OPT_LBFGS */
+  int np;
+  double step, mu;
+  int history;
+  size_t iter;
+  double momentum[16], momentum_prev[16];
+  double grad_prev[16], pars_prev[16];
+  int len;
+  double s[ORC_MAX_HISTORY][16], y[ORC_MAX_HISTORY][16];
+} orc_stateful_opt;
+#define ORC_OPT_MOMENTUM 2
+#define ORC_OPT_NESTEROV 3
+#define ORC_OPT_LBFGS 4
+/* ::new of each (optimizers.rs:40-46, :71-78, :108-119).  Returns -1 for
+ * np outside 1..16 or history outside 1..ORC_MAX_HISTORY. */
+int orc_stateful_opt_init(orc_stateful_opt *o, int kind, int np, double step, double mu, int history);
+/* compute_parameter_update after the gradient: writes dp[np]. */
+void orc_stateful_opt_update(orc_stateful_opt *o, const double *pars, const double *g, double *dp);
+
 /* ------------------------------------------------------------------ */
 /* Statistics of scripts/statfor.rs                                    */
 /* ------------------------------------------------------------------ */

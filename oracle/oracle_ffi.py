@@ -21,6 +21,10 @@ SR_TEXTBOOK = 0
 SR_REFERENCE = 1
 OPT_SD = 0
 OPT_SR = 1
+OPT_MOMENTUM = 2
+OPT_NESTEROV = 3
+OPT_LBFGS = 4
+MAX_HISTORY = 64
 MAX_NUC = 8
 
 
@@ -81,6 +85,24 @@ class OptParams(C.Structure):
     ]
 
 
+class StatefulOpt(C.Structure):
+    _fields_ = [
+        ("kind", C.c_int),
+        ("np", C.c_int),
+        ("step", C.c_double),
+        ("mu", C.c_double),
+        ("history", C.c_int),
+        ("iter", C.c_size_t),
+        ("momentum", C.c_double * 16),
+        ("momentum_prev", C.c_double * 16),
+        ("grad_prev", C.c_double * 16),
+        ("pars_prev", C.c_double * 16),
+        ("len", C.c_int),
+        ("s", (C.c_double * 16) * MAX_HISTORY),
+        ("y", (C.c_double * 16) * MAX_HISTORY),
+    ]
+
+
 _lib = None
 
 
@@ -135,6 +157,9 @@ def lib():
     L.orc_vmc_optimize.restype = i
     L.orc_vmc_optimize.argtypes = [pw, u64, d, u32, u32, u32, u32, C.POINTER(OptParams), i,
                                    pd, pd, pd, pd, pd]
+    L.orc_stateful_opt_init.restype = i
+    L.orc_stateful_opt_init.argtypes = [C.POINTER(StatefulOpt), i, i, d, d, i]
+    L.orc_stateful_opt_update.argtypes = [C.POINTER(StatefulOpt), pd, pd, pd]
     L.orc_tau_corr.restype = d
     L.orc_tau_corr.argtypes = [pd, C.c_size_t]
     _lib = L
@@ -234,3 +259,19 @@ def run_box(wf, *, seed, walkers, steps, block_size, box_side, epoch=0, sys=0,
                sum_bm=out.sum_bm, sum_bm2=out.sum_bm2, n_samples=out.n_samples,
                n_blocks=out.n_blocks, n_accepted=out.n_accepted, n_moves=out.n_moves)
     return res
+
+
+class OracleOptimizer:
+    """orc_stateful_opt_*: MomentumDescent / NesterovMomentum / OnlineLbfgs on a given gradient."""
+
+    def __init__(self, kind, np_, step, mu=0.0, history=1):
+        self.st = StatefulOpt()
+        assert lib().orc_stateful_opt_init(C.byref(self.st), kind, np_, step, mu, history) == 0
+        self.np = np_
+
+    def update(self, pars, g):
+        pars = np.ascontiguousarray(pars, dtype=np.float64)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        dp = np.zeros(self.np)
+        lib().orc_stateful_opt_update(C.byref(self.st), dptr(pars), dptr(g), dptr(dp))
+        return dp

@@ -381,3 +381,101 @@ double orc_tau_corr(const double *x, size_t n) {
   }
   return tau > 1.0 ? tau : 1.0;
 }
+
+/* ------------------------------------------------------------------ */
+/* MomentumDescent, NesterovMomentum, OnlineLbfgs (optimizers.rs:32-179) */
+/* ------------------------------------------------------------------ */
+
+int orc_stateful_opt_init(orc_stateful_opt *o, int kind, int np, double step, double mu, int history) {
+  if (np < 1 || np > 16) return -1;
+  if (kind == ORC_OPT_LBFGS && (history < 1 || history > ORC_MAX_HISTORY)) return -1;
+  memset(o, 0, sizeof(*o));
+  o->kind = kind;
+  o->np = np;
+  o->step = step;
+  o->mu = mu;
+  o->history = history;
+  /* OnlineLbfgs::new: grad_prev and pars_prev are filled with EPS = 1e-5 */
+  for (int i = 0; i < np; ++i) {
+    o->grad_prev[i] = 1e-5;
+    o->pars_prev[i] = 1e-5;
+  }
+  return 0;
+}
+
+static double vdot(const double *a, const double *b, int n) {
+  double t = 0.0;
+  for (int i = 0; i < n; ++i) t += a[i] * b[i];
+  return t;
+}
+
+/* update_curvature_pairs (optimizers.rs:146-158) */
+static void lbfgs_pairs(orc_stateful_opt *o, const double *pars, const double *g) {
+  const int n = o->np;
+  if (o->len >= o->history) { /* pop_front */
+    for (int k = 1; k < o->len; ++k) {
+      memcpy(o->s[k - 1], o->s[k], sizeof(o->s[0]));
+      memcpy(o->y[k - 1], o->y[k], sizeof(o->y[0]));
+    }
+    o->len -= 1;
+  }
+  for (int i = 0; i < n; ++i) {
+    o->s[o->len][i] = pars[i] - o->pars_prev[i];
+    o->y[o->len][i] = g[i] - o->grad_prev[i];
+  }
+  o->len += 1;
+}
+
+/* initial_direction (optimizers.rs:121-144) */
+static void lbfgs_direction(const orc_stateful_opt *o, const double *g, double *p) {
+  const int n = o->np;
+  double alphas[ORC_MAX_HISTORY];
+  int na = 0;
+  for (int i = 0; i < n; ++i) p[i] = -g[i];
+  for (int k = o->len - 1; k >= 0; --k) { /* s.iter().rev().zip(y.iter().rev()) */
+    const double alpha = vdot(o->s[k], p, n) / vdot(o->s[k], o->y[k], n);
+    for (int i = 0; i < n; ++i) p[i] -= alpha * o->y[k][i];
+    alphas[na++] = alpha;
+  }
+  double scale;
+  if (o->iter == 0) {
+    scale = 1e-10;
+  } else {
+    double tot = 0.0;
+    for (int k = 0; k < o->len; ++k) tot = tot + vdot(o->s[k], o->y[k], n) / vdot(o->y[k], o->y[k], n);
+    const size_t d = o->iter < (size_t)o->history ? o->iter : (size_t)o->history;
+    scale = tot / (double)d;
+  }
+  for (int i = 0; i < n; ++i) p[i] *= scale;
+  /* izip!(alphas.iter().rev(), s.iter(), y.iter()) */
+  for (int k = 0; k < o->len; ++k) {
+    const double alpha = alphas[na - 1 - k];
+    const double c = alpha - vdot(o->y[k], p, n) / vdot(o->y[k], o->s[k], n);
+    for (int i = 0; i < n; ++i) p[i] = p[i] + c * o->s[k][i];
+  }
+}
+
+void orc_stateful_opt_update(orc_stateful_opt *o, const double *pars, const double *g, double *dp) {
+  const int n = o->np;
+  if (o->kind == ORC_OPT_MOMENTUM) {
+    /* self.momentum -= &(self.step_size * energy_grad);
+       Ok(self.momentum_parameter * &self.momentum) */
+    for (int i = 0; i < n; ++i) {
+      o->momentum[i] -= o->step * g[i];
+      dp[i] = o->mu * o->momentum[i];
+    }
+  } else if (o->kind == ORC_OPT_NESTEROV) {
+    memcpy(o->momentum_prev, o->momentum, sizeof(o->momentum));
+    for (int i = 0; i < n; ++i) o->momentum[i] = o->mu * o->momentum[i] + o->step * g[i];
+    for (int i = 0; i < n; ++i) dp[i] = -(o->mu * o->momentum_prev[i] + (1.0 + o->mu) * o->momentum[i]);
+  } else {
+    double p[16];
+    lbfgs_pairs(o, pars, g);
+    lbfgs_direction(o, g, p);
+    for (int i = 0; i < n; ++i) dp[i] = -o->step * p[i];
+    lbfgs_pairs(o, pars, g);
+    memcpy(o->grad_prev, g, sizeof(double) * n);
+    memcpy(o->pars_prev, pars, sizeof(double) * n);
+  }
+  o->iter += 1;
+}

@@ -330,3 +330,33 @@ def test_device_optimisation_loop_equals_host_loop():
     for k, (se, n, acc) in enumerate(host_e):
         assert log[k, 0] == se and log[k, 6] == n and log[k, 4] == acc and log[k, 5] == 0.0
     assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
+
+
+@pytest.mark.parametrize("kind,kw", [(M.OPT_MOMENTUM, dict(momentum=0.5)), (M.OPT_NESTEROV, dict(momentum=0.5)),
+                                     (M.OPT_LBFGS, dict(history=3))])
+def test_device_loop_stateful_optimizers_equal_host_loop(kind, kw):
+    # MomentumDescent / NesterovMomentum / OnlineLbfgs (optimizers.rs:32-179)
+    # advanced on the device by mole_optimize_with: parameters, walkers and
+    # the optimizer state after the loop equal the host loop's bit for bit
+    W, sweeps, block, iters, step = 512, 40, 10, 4, 0.01
+    p0 = np.array([1.0, 0.5, 0.3])
+    host_opt = M.Optimizer(kind, 3, step, **kw)
+    host_p = p0.copy()
+    with M.Context([M.h2(W, *p0)], seed=5, step=0.5, block_size=block) as ctx:
+        ctx.init_walkers()
+        for _ in range(iters):
+            ctx.reset_stats()
+            ctx.run(sweeps, block)
+            dp, _ = host_opt.compute_parameter_update(host_p, ctx.sr_sums_raw())
+            host_p = host_p + dp
+            ctx.set_params(host_p)
+        cfg_host = ctx.get_configs()
+    dev_opt = M.Optimizer(kind, 3, step, **kw)
+    with M.Context([M.h2(W, *p0)], seed=5, step=0.5, block_size=block) as ctx:
+        ctx.init_walkers()
+        p, log = ctx.optimize_with(dev_opt, iters, sweeps, block)
+        cfg_dev = ctx.get_configs()
+    assert np.all(np.isfinite(p)) and np.all(log[:, 5] == 0.0)
+    assert np.array_equal(p.view(np.uint64), host_p.view(np.uint64))
+    assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
+    assert bytes(dev_opt.state) == bytes(host_opt.state)
