@@ -477,6 +477,167 @@ static void refresh(const orc_sj *wf, const double *cfg, int prec, double *Bu, d
   }
 }
 
+/* One sweep of walker w over all electrons (the body of Runner::run's
+ * move_state loop, samplers.rs:106-117): draws from *pos on, global move
+ * counter *m; returns the accept mask. */
+static uint64_t sj_sweep_once(sj_task *t, uint32_t w, double *cfg, double *Bu, double *Bd, float *Buf, float *Bdf,
+                              uint64_t *pos_io, uint64_t *m_io) {
+  const orc_sj *wf = t->wf;
+  const orc_sj_run_params *rp = t->rp;
+  const int N = wf->n_up + wf->n_dn;
+  const int prec = rp->precision;
+  const double half = 0.5 * rp->box_side;
+  double chi[MAXB];
+  uint64_t pos = *pos_io, m = *m_io;
+  uint64_t mask = 0;
+  for (int i = 0; i < N; ++i) {
+    double xn[3], u;
+    double vold[3] = {0, 0, 0}; /* drift at x (diffusion) */
+    const int diffuse = !t->faithful && rp->proposal == ORC_SJ_DIFFUSE;
+    const int learned = !t->faithful && rp->proposal == ORC_SJ_LEARNED;
+    double lq_ratio = 0.0;
+    if (learned) {
+      uint64_t dr[6];
+      for (int k = 0; k < 6; ++k) dr[k] = orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++);
+      double z[4];
+      orc_box_muller(dr[0], dr[1], &z[0], &z[1]);
+      orc_box_muller(dr[2], dr[3], &z[2], &z[3]);
+      u = orc_u64_standard(dr[4]);
+      double f1[ORC_MLP_NIN], f2[ORC_MLP_NIN], o1[ORC_MLP_NOUT], o2[ORC_MLP_NOUT];
+      orc_mlp_features(wf, cfg, i, cfg + 3 * i, f1);
+      orc_mlp_forward(rp->mlp_w, f1, rp->mlp_emulate_bf16, o1, NULL);
+      double ls = orc_mlp_log_sigma(o1[3]), sg = exp(ls), z2 = 0.0;
+      for (int q = 0; q < 3; ++q) {
+        xn[q] = (cfg[3 * i + q] + o1[q]) + sg * z[q];
+        z2 += z[q] * z[q];
+      }
+      double cfg2[3 * MAXE];
+      memcpy(cfg2, cfg, sizeof(double) * 3 * N);
+      memcpy(cfg2 + 3 * i, xn, sizeof xn);
+      orc_mlp_features(wf, cfg2, i, xn, f2);
+      orc_mlp_forward(rp->mlp_w, f2, rp->mlp_emulate_bf16, o2, NULL);
+      double ls2 = orc_mlp_log_sigma(o2[3]), sg2 = exp(ls2), e2 = 0.0;
+      for (int q = 0; q < 3; ++q) {
+        double e = (cfg[3 * i + q] - xn[q]) - o2[q];
+        e2 += e * e;
+      }
+      lq_ratio = (-3.0 * ls2 - e2 / (2.0 * sg2 * sg2)) - (-3.0 * ls - 0.5 * z2);
+      if (rp->mlp_drop_q_ratio) lq_ratio = 0.0;
+    } else if (!diffuse) {
+      double dd[3];
+      for (int k = 0; k < 3; ++k)
+        dd[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++), -half, half);
+      u = orc_u64_standard(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++));
+      for (int k = 0; k < 3; ++k) xn[k] = cfg[3 * i + k] + dd[k];
+    } else {
+      uint64_t dr[6];
+      for (int k = 0; k < 6; ++k) dr[k] = orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++);
+      double z[4];
+      orc_box_muller(dr[0], dr[1], &z[0], &z[1]);
+      orc_box_muller(dr[2], dr[3], &z[2], &z[3]);
+      u = orc_u64_standard(dr[4]);
+      int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
+      double Bcol[MAXN];
+      for (int k = 0; k < n; ++k)
+        Bcol[k] = prec == ORC_PREC_FP64 ? (sp == 0 ? Bu : Bd)[k * n + c] : (double)(sp == 0 ? Buf : Bdf)[k * n + c];
+      double gd[3], gj[3];
+      det_grad(wf, cfg + 3 * i, n, Bcol, gd);
+      jas_grad(wf, cfg, i, cfg + 3 * i, gj);
+      const double st = sqrt(rp->tau);
+      for (int q = 0; q < 3; ++q) {
+        vold[q] = gd[q] + gj[q];
+        xn[q] = (cfg[3 * i + q] + vold[q] * rp->tau) + st * z[q];
+      }
+    }
+    int acc;
+    t->n_moves++;
+    if (t->faithful) {
+      double prop[3 * MAXE];
+      memcpy(prop, cfg, sizeof(double) * 3 * N);
+      memcpy(prop + 3 * i, xn, sizeof xn);
+      double pn = orc_sj_value(wf, prop);
+      double po = orc_sj_value(wf, cfg);
+      acc = rust_min1((pn * pn) / (po * po)) > u;
+    } else {
+      int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
+      basis_at(wf, xn, chi);
+      if (prec == ORC_PREC_FP64) {
+        double *B = sp == 0 ? Bu : Bd;
+        double phin[MAXN];
+        for (int k = 0; k < n; ++k) phin[k] = orbital(wf, k, chi);
+        double rd = 0.0;
+        for (int k = 0; k < n; ++k) rd = fma(phin[k], B[k * n + c], rd);
+        double Jn = jas_i_d(wf, cfg, i, xn);
+        double Jo = jas_i_d(wf, cfg, i, cfg + 3 * i);
+        double dJ = Jn - Jo;
+        double R2 = (rd * rd) * orc_exp(2.0 * dJ);
+        acc = diffuse   ? diffuse_accept(wf, cfg, i, xn, n, B + c, n, rd, R2, vold, rp->tau, u)
+              : learned ? rust_min1(R2 * exp(lq_ratio)) > u
+                        : rust_min1(R2) > u;
+        if (acc) {
+          double inv = 1.0 / rd;
+          double f[MAXN];
+          for (int l = 0; l < n; ++l) {
+            if (l == c) continue;
+            double v = 0.0;
+            for (int k = 0; k < n; ++k) v = fma(phin[k], B[k * n + l], v);
+            f[l] = v * inv;
+          }
+          for (int k = 0; k < n; ++k)
+            for (int l = 0; l < n; ++l)
+              if (l != c) B[k * n + l] = fma(-B[k * n + c], f[l], B[k * n + l]);
+          for (int k = 0; k < n; ++k) B[k * n + c] = B[k * n + c] * inv;
+        }
+      } else {
+        float *B = sp == 0 ? Buf : Bdf;
+        float phin[MAXN];
+        for (int k = 0; k < n; ++k) phin[k] = (float)orbital(wf, k, chi);
+        float rd = 0.0f;
+        for (int k = 0; k < n; ++k) rd = fmaf(phin[k], B[k * n + c], rd);
+        float Jn = jas_i_f(wf, cfg, i, xn);
+        float Jo = jas_i_f(wf, cfg, i, cfg + 3 * i);
+        float dJ = Jn - Jo;
+        double rdd = (double)rd;
+        double R2 = (rdd * rdd) * orc_exp(2.0 * (double)dJ);
+        if (diffuse) {
+          double Bcol[MAXN];
+          for (int k = 0; k < n; ++k) Bcol[k] = (double)B[k * n + c];
+          acc = diffuse_accept(wf, cfg, i, xn, n, Bcol, 1, rdd, R2, vold, rp->tau, u);
+        } else if (learned) {
+          acc = rust_min1(R2 * exp(lq_ratio)) > u;
+        } else {
+          acc = rust_min1(R2) > u;
+        }
+        if (acc) {
+          float inv = 1.0f / rd;
+          float f[MAXN];
+          for (int l = 0; l < n; ++l) {
+            if (l == c) continue;
+            float v = 0.0f;
+            for (int k = 0; k < n; ++k) v = fmaf(phin[k], B[k * n + l], v);
+            f[l] = v * inv;
+          }
+          for (int k = 0; k < n; ++k)
+            for (int l = 0; l < n; ++l)
+              if (l != c) B[k * n + l] = fmaf(-B[k * n + c], f[l], B[k * n + l]);
+          for (int k = 0; k < n; ++k) B[k * n + c] = B[k * n + c] * inv;
+        }
+      }
+    }
+    if (acc) {
+      memcpy(cfg + 3 * i, xn, sizeof xn);
+      mask |= (uint64_t)1 << i;
+      t->n_accepted++;
+    }
+    ++m;
+    if (!t->faithful && rp->lu_period && m % rp->lu_period == 0)
+      refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
+  }
+  *pos_io = pos;
+  *m_io = m;
+  return mask;
+}
+
 static void sj_walker(sj_task *t, uint32_t w) {
   const orc_sj *wf = t->wf;
   const orc_sj_run_params *rp = t->rp;
@@ -499,156 +660,11 @@ static void sj_walker(sj_task *t, uint32_t w) {
   float Buf[MAXN * MAXN], Bdf[MAXN * MAXN];
   if (!t->faithful) refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
   uint64_t m = rp->move0;
-  const double half = 0.5 * rp->box_side;
   uint64_t sidx = 0;
-  double chi[MAXB];
   for (uint32_t block = 0; block < blocks; ++block) {
     double bsum = 0.0;
     for (uint32_t s = 0; s < rp->block_size; ++s) {
-      uint64_t mask = 0;
-      for (int i = 0; i < N; ++i) {
-        double xn[3], u;
-        double vold[3] = {0, 0, 0}; /* drift at x (diffusion) */
-        const int diffuse = !t->faithful && rp->proposal == ORC_SJ_DIFFUSE;
-        const int learned = !t->faithful && rp->proposal == ORC_SJ_LEARNED;
-        double lq_ratio = 0.0;
-        if (learned) {
-          uint64_t dr[6];
-          for (int k = 0; k < 6; ++k) dr[k] = orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++);
-          double z[4];
-          orc_box_muller(dr[0], dr[1], &z[0], &z[1]);
-          orc_box_muller(dr[2], dr[3], &z[2], &z[3]);
-          u = orc_u64_standard(dr[4]);
-          double f1[ORC_MLP_NIN], f2[ORC_MLP_NIN], o1[ORC_MLP_NOUT], o2[ORC_MLP_NOUT];
-          orc_mlp_features(wf, cfg, i, cfg + 3 * i, f1);
-          orc_mlp_forward(rp->mlp_w, f1, rp->mlp_emulate_bf16, o1, NULL);
-          double ls = orc_mlp_log_sigma(o1[3]), sg = exp(ls), z2 = 0.0;
-          for (int q = 0; q < 3; ++q) {
-            xn[q] = (cfg[3 * i + q] + o1[q]) + sg * z[q];
-            z2 += z[q] * z[q];
-          }
-          double cfg2[3 * MAXE];
-          memcpy(cfg2, cfg, sizeof(double) * 3 * N);
-          memcpy(cfg2 + 3 * i, xn, sizeof xn);
-          orc_mlp_features(wf, cfg2, i, xn, f2);
-          orc_mlp_forward(rp->mlp_w, f2, rp->mlp_emulate_bf16, o2, NULL);
-          double ls2 = orc_mlp_log_sigma(o2[3]), sg2 = exp(ls2), e2 = 0.0;
-          for (int q = 0; q < 3; ++q) {
-            double e = (cfg[3 * i + q] - xn[q]) - o2[q];
-            e2 += e * e;
-          }
-          lq_ratio = (-3.0 * ls2 - e2 / (2.0 * sg2 * sg2)) - (-3.0 * ls - 0.5 * z2);
-          if (rp->mlp_drop_q_ratio) lq_ratio = 0.0;
-        } else if (!diffuse) {
-          double dd[3];
-          for (int k = 0; k < 3; ++k)
-            dd[k] = orc_u64_range(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++), -half, half);
-          u = orc_u64_standard(orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++));
-          for (int k = 0; k < 3; ++k) xn[k] = cfg[3 * i + k] + dd[k];
-        } else {
-          uint64_t dr[6];
-          for (int k = 0; k < 6; ++k) dr[k] = orc_stream_u64(rp->seed, rp->sys, w, rp->epoch, pos++);
-          double z[4];
-          orc_box_muller(dr[0], dr[1], &z[0], &z[1]);
-          orc_box_muller(dr[2], dr[3], &z[2], &z[3]);
-          u = orc_u64_standard(dr[4]);
-          int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
-          double Bcol[MAXN];
-          for (int k = 0; k < n; ++k)
-            Bcol[k] = prec == ORC_PREC_FP64 ? (sp == 0 ? Bu : Bd)[k * n + c] : (double)(sp == 0 ? Buf : Bdf)[k * n + c];
-          double gd[3], gj[3];
-          det_grad(wf, cfg + 3 * i, n, Bcol, gd);
-          jas_grad(wf, cfg, i, cfg + 3 * i, gj);
-          const double st = sqrt(rp->tau);
-          for (int q = 0; q < 3; ++q) {
-            vold[q] = gd[q] + gj[q];
-            xn[q] = (cfg[3 * i + q] + vold[q] * rp->tau) + st * z[q];
-          }
-        }
-        int acc;
-        t->n_moves++;
-        if (t->faithful) {
-          double prop[3 * MAXE];
-          memcpy(prop, cfg, sizeof(double) * 3 * N);
-          memcpy(prop + 3 * i, xn, sizeof xn);
-          double pn = orc_sj_value(wf, prop);
-          double po = orc_sj_value(wf, cfg);
-          acc = rust_min1((pn * pn) / (po * po)) > u;
-        } else {
-          int sp = spin_of(wf, i), c = col_of(wf, i), n = nspin(wf, sp);
-          basis_at(wf, xn, chi);
-          if (prec == ORC_PREC_FP64) {
-            double *B = sp == 0 ? Bu : Bd;
-            double phin[MAXN];
-            for (int k = 0; k < n; ++k) phin[k] = orbital(wf, k, chi);
-            double rd = 0.0;
-            for (int k = 0; k < n; ++k) rd = fma(phin[k], B[k * n + c], rd);
-            double Jn = jas_i_d(wf, cfg, i, xn);
-            double Jo = jas_i_d(wf, cfg, i, cfg + 3 * i);
-            double dJ = Jn - Jo;
-            double R2 = (rd * rd) * orc_exp(2.0 * dJ);
-            acc = diffuse   ? diffuse_accept(wf, cfg, i, xn, n, B + c, n, rd, R2, vold, rp->tau, u)
-                  : learned ? rust_min1(R2 * exp(lq_ratio)) > u
-                            : rust_min1(R2) > u;
-            if (acc) {
-              double inv = 1.0 / rd;
-              double f[MAXN];
-              for (int l = 0; l < n; ++l) {
-                if (l == c) continue;
-                double v = 0.0;
-                for (int k = 0; k < n; ++k) v = fma(phin[k], B[k * n + l], v);
-                f[l] = v * inv;
-              }
-              for (int k = 0; k < n; ++k)
-                for (int l = 0; l < n; ++l)
-                  if (l != c) B[k * n + l] = fma(-B[k * n + c], f[l], B[k * n + l]);
-              for (int k = 0; k < n; ++k) B[k * n + c] = B[k * n + c] * inv;
-            }
-          } else {
-            float *B = sp == 0 ? Buf : Bdf;
-            float phin[MAXN];
-            for (int k = 0; k < n; ++k) phin[k] = (float)orbital(wf, k, chi);
-            float rd = 0.0f;
-            for (int k = 0; k < n; ++k) rd = fmaf(phin[k], B[k * n + c], rd);
-            float Jn = jas_i_f(wf, cfg, i, xn);
-            float Jo = jas_i_f(wf, cfg, i, cfg + 3 * i);
-            float dJ = Jn - Jo;
-            double rdd = (double)rd;
-            double R2 = (rdd * rdd) * orc_exp(2.0 * (double)dJ);
-            if (diffuse) {
-              double Bcol[MAXN];
-              for (int k = 0; k < n; ++k) Bcol[k] = (double)B[k * n + c];
-              acc = diffuse_accept(wf, cfg, i, xn, n, Bcol, 1, rdd, R2, vold, rp->tau, u);
-            } else if (learned) {
-              acc = rust_min1(R2 * exp(lq_ratio)) > u;
-            } else {
-              acc = rust_min1(R2) > u;
-            }
-            if (acc) {
-              float inv = 1.0f / rd;
-              float f[MAXN];
-              for (int l = 0; l < n; ++l) {
-                if (l == c) continue;
-                float v = 0.0f;
-                for (int k = 0; k < n; ++k) v = fmaf(phin[k], B[k * n + l], v);
-                f[l] = v * inv;
-              }
-              for (int k = 0; k < n; ++k)
-                for (int l = 0; l < n; ++l)
-                  if (l != c) B[k * n + l] = fmaf(-B[k * n + c], f[l], B[k * n + l]);
-              for (int k = 0; k < n; ++k) B[k * n + c] = B[k * n + c] * inv;
-            }
-          }
-        }
-        if (acc) {
-          memcpy(cfg + 3 * i, xn, sizeof xn);
-          mask |= (uint64_t)1 << i;
-          t->n_accepted++;
-        }
-        ++m;
-        if (!t->faithful && rp->lu_period && m % rp->lu_period == 0)
-          refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
-      }
+      const uint64_t mask = sj_sweep_once(t, w, cfg, Bu, Bd, Buf, Bdf, &pos, &m);
       if (out->accept_mask)
         out->accept_mask[(size_t)w * blocks * rp->block_size + (size_t)block * rp->block_size + s] = mask;
       if (block > 0) {
@@ -740,4 +756,262 @@ int orc_sj_run_parity(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0
 int orc_sj_run_faithful(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
                         int n_threads, orc_sj_run_out *out) {
   return sj_run(wf, rp, w0, w1, n_threads, out, 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Diffusion Monte Carlo                                               */
+/* ------------------------------------------------------------------ */
+
+static uint64_t uniform_below(uint64_t d, uint64_t n) {
+  return (uint64_t)(((unsigned __int128)d * n) >> 64);
+}
+
+/* Rust's `f64 as u32`: truncation, saturating, NaN -> 0 */
+static uint32_t rust_as_u32(double x) {
+  if (!(x > 0.0)) return 0;
+  if (x >= 4294967295.0) return 0xFFFFFFFFu;
+  return (uint32_t)x;
+}
+
+/* Rust's `f64 as usize` (64-bit) */
+static uint64_t rust_as_usize(double x) {
+  if (!(x > 0.0)) return 0;
+  if (x >= 18446744073709551615.0) return UINT64_MAX;
+  return (uint64_t)x;
+}
+
+int orc_branch_sr(const double *w, uint32_t W, uint64_t seed, uint32_t sys, uint64_t step, uint32_t *src,
+                  double *w_out) {
+  double tot = 0.0, mx = 0.0;
+  for (uint32_t i = 0; i < W; ++i) {
+    tot += w[i];
+    mx = fmax(mx, w[i]); /* f64::max ignores NaN */
+  }
+  const double global_weight = tot / (double)W;
+  const double norm = (double)W / mx;
+  uint64_t *cum = (uint64_t *)malloc(sizeof(uint64_t) * (W ? W : 1));
+  uint64_t run = 0;
+  for (uint32_t i = 0; i < W; ++i) {
+    run += rust_as_u32(w[i] * norm);
+    cum[i] = run;
+  }
+  if (run == 0) {
+    free(cum);
+    return -1;
+  }
+  for (uint32_t j = 0; j < W; ++j) {
+    const uint64_t u = uniform_below(orc_stream_u64(seed, sys, (uint32_t)step, ORC_DMC_BRANCH_EPOCH, j), run);
+    /* first k with cum[k] > u */
+    uint32_t lo = 0, hi = W - 1;
+    while (lo < hi) {
+      const uint32_t mid = lo + (hi - lo) / 2;
+      if (cum[mid] > u) hi = mid;
+      else lo = mid + 1;
+    }
+    src[j] = lo;
+    w_out[j] = global_weight;
+  }
+  free(cum);
+  return 0;
+}
+
+void orc_branch_simple(const double *w, uint32_t W, uint64_t seed, uint32_t sys, uint64_t step, uint32_t *src,
+                       double *w_out) {
+  /* new_walkers = walkers minus to_kill (in order) ++ to_birth */
+  uint32_t *list = (uint32_t *)malloc(sizeof(uint32_t) * 3 * (W ? W : 1));
+  uint32_t *births = (uint32_t *)malloc(sizeof(uint32_t) * 2 * (W ? W : 1));
+  uint32_t M = 0, nb = 0;
+  for (uint32_t i = 0; i < W; ++i) {
+    const double u = orc_u64_standard(orc_stream_u64(seed, sys, (uint32_t)step, ORC_DMC_BRANCH_EPOCH, i));
+    uint64_t n = rust_as_usize(w[i] + u);
+    if (n > 3) n = 3;
+    if (n == 0) continue;
+    list[M++] = i;
+    for (uint64_t c = 0; c + 1 < n; ++c) births[nb++] = i;
+  }
+  for (uint32_t b = 0; b < nb; ++b) list[M++] = births[b];
+  if (W > M) {
+    const uint32_t excess = W - M;
+    for (uint32_t k = 0; k < excess; ++k)
+      list[M + k] = (uint32_t)uniform_below(
+          orc_stream_u64(seed, sys, (uint32_t)step, ORC_DMC_BRANCH_EPOCH, (uint64_t)W + k), W);
+    M = W;
+  } else {
+    const uint32_t excess = M - W;
+    for (uint32_t k = 0; k < excess; ++k) {
+      const uint32_t r = (uint32_t)uniform_below(
+          orc_stream_u64(seed, sys, (uint32_t)step, ORC_DMC_BRANCH_EPOCH, (uint64_t)W + k), M);
+      memmove(list + r, list + r + 1, sizeof(uint32_t) * (M - r - 1)); /* Vec::remove */
+      --M;
+    }
+  }
+  for (uint32_t j = 0; j < W; ++j) {
+    src[j] = list[j];
+    w_out[j] = w[list[j]];
+  }
+  free(list);
+  free(births);
+}
+
+static int dmc_nb(const orc_sj *wf) { return wf->n_up * wf->n_up + wf->n_dn * wf->n_dn; }
+
+static void dmc_task(sj_task *t, const orc_sj *wf, const orc_dmc_params *p, orc_sj_run_params *rp) {
+  memset(t, 0, sizeof(*t));
+  memset(rp, 0, sizeof(*rp));
+  rp->seed = p->seed;
+  rp->sys = p->sys;
+  rp->epoch = p->epoch;
+  rp->precision = p->precision;
+  rp->lu_period = p->lu_period;
+  rp->proposal = ORC_SJ_DIFFUSE;
+  rp->tau = p->tau;
+  t->wf = wf;
+  t->rp = rp;
+}
+
+/* the walker's inverses in the working precision (as the device stores
+ * them) and its local energy from them */
+static double dmc_energy(const orc_sj *wf, int prec, const double *cfg, const double *B) {
+  double dl[ORC_SJ_NPARAM];
+  const int nu = wf->n_up;
+  (void)prec;
+  return orc_sj_local_energy(wf, cfg, B, B + nu * nu, dl);
+}
+
+int orc_dmc_init(const orc_sj *wf, const orc_dmc_params *p, double reference_energy, orc_dmc_state *st) {
+  const int N = wf->n_up + wf->n_dn, nb = dmc_nb(wf);
+  if (N > MAXE || wf->n_up > MAXN || wf->n_dn > MAXN || p->n_walkers == 0) return -1;
+  double cfg[3 * MAXE];
+  for (int k = 0; 2 * k < 3 * N; ++k) {
+    double z0, z1;
+    orc_box_muller(orc_stream_u64(p->seed, p->sys, 0xFFFFFFFFu, ORC_DMC_INIT_EPOCH, 2 * (uint64_t)k),
+                   orc_stream_u64(p->seed, p->sys, 0xFFFFFFFFu, ORC_DMC_INIT_EPOCH, 2 * (uint64_t)k + 1), &z0, &z1);
+    cfg[2 * k] = z0;
+    if (2 * k + 1 < 3 * N) cfg[2 * k + 1] = z1;
+  }
+  double Bu[MAXN * MAXN], Bd[MAXN * MAXN];
+  float Buf[MAXN * MAXN], Bdf[MAXN * MAXN];
+  refresh(wf, cfg, p->precision, Bu, Bd, Buf, Bdf);
+  double B[2 * MAXN * MAXN];
+  for (int k = 0; k < wf->n_up * wf->n_up; ++k)
+    B[k] = p->precision == ORC_PREC_FP64 ? Bu[k] : (double)Buf[k];
+  for (int k = 0; k < wf->n_dn * wf->n_dn; ++k)
+    B[wf->n_up * wf->n_up + k] = p->precision == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
+  const double e0 = dmc_energy(wf, p->precision, cfg, B);
+  for (uint32_t w = 0; w < p->n_walkers; ++w) {
+    memcpy(st->cfg + (size_t)w * 3 * N, cfg, sizeof(double) * 3 * N);
+    memcpy(st->B + (size_t)w * nb, B, sizeof(double) * nb);
+    st->weight[w] = 1.0;
+    st->e_local[w] = e0;
+  }
+  st->reference_energy = reference_energy;
+  st->steps = 0;
+  st->draw = 0;
+  st->moves = 0;
+  return 0;
+}
+
+int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st, double time_step,
+                    uint32_t num_iterations, uint32_t block_size, uint32_t num_eq_blocks, double *energies,
+                    double *errors, double *step_energy) {
+  const int N = wf->n_up + wf->n_dn, nb = dmc_nb(wf), nu = wf->n_up, nd = wf->n_dn;
+  const uint32_t W = p->n_walkers;
+  if (block_size == 0) return -1;
+  sj_task t;
+  orc_sj_run_params rp;
+  dmc_task(&t, wf, p, &rp);
+  double *cfg2 = (double *)malloc(sizeof(double) * W * 3 * N);
+  double *B2 = (double *)malloc(sizeof(double) * W * (nb ? nb : 1));
+  double *e2 = (double *)malloc(sizeof(double) * W);
+  double *w2 = (double *)malloc(sizeof(double) * W);
+  uint32_t *src = (uint32_t *)malloc(sizeof(uint32_t) * W);
+  double *eblock = (double *)malloc(sizeof(double) * block_size);
+  const uint32_t blocks = num_iterations / block_size;
+  int n_out = 0;
+  double *vars = (double *)malloc(sizeof(double) * (blocks ? blocks : 1));
+  uint64_t it = 0;
+  int rc = 0;
+  for (uint32_t block_nr = 0; block_nr < blocks && rc == 0; ++block_nr) {
+    for (uint32_t j = 0; j < block_size; ++j, ++it) {
+      double total_weight = 0.0, ensemble = 0.0;
+      for (uint32_t w = 0; w < W; ++w) {
+        double *c = st->cfg + (size_t)w * 3 * N;
+        double *B = st->B + (size_t)w * nb;
+        double Bu[MAXN * MAXN], Bd[MAXN * MAXN];
+        float Buf[MAXN * MAXN], Bdf[MAXN * MAXN];
+        for (int k = 0; k < nu * nu; ++k) {
+          Bu[k] = B[k];
+          Buf[k] = (float)B[k];
+        }
+        for (int k = 0; k < nd * nd; ++k) {
+          Bd[k] = B[nu * nu + k];
+          Bdf[k] = (float)B[nu * nu + k];
+        }
+        const double local_e = st->e_local[w];
+        uint64_t pos = st->draw, m = st->moves;
+        sj_sweep_once(&t, w, c, Bu, Bd, Buf, Bdf, &pos, &m);
+        for (int k = 0; k < nu * nu; ++k) B[k] = p->precision == ORC_PREC_FP64 ? Bu[k] : (double)Buf[k];
+        for (int k = 0; k < nd * nd; ++k) B[nu * nu + k] = p->precision == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
+        ensemble += st->weight[w] * local_e;
+        total_weight += st->weight[w];
+        const double local_e_new = dmc_energy(wf, p->precision, c, B);
+        st->weight[w] *= orc_exp(-time_step * ((local_e + local_e_new) / 2.0 - st->reference_energy));
+        st->e_local[w] = local_e_new;
+      }
+      st->draw += 6 * (uint64_t)N;
+      st->moves += (uint64_t)N;
+      ensemble /= total_weight;
+      eblock[j] = ensemble;
+      if (step_energy) step_energy[it] = ensemble;
+      if (p->brancher == ORC_BRANCH_SR) {
+        if (orc_branch_sr(st->weight, W, p->seed, p->sys, st->steps, src, w2) != 0) {
+          rc = -1;
+          break;
+        }
+      } else {
+        orc_branch_simple(st->weight, W, p->seed, p->sys, st->steps, src, w2);
+      }
+      for (uint32_t k = 0; k < W; ++k) {
+        memcpy(cfg2 + (size_t)k * 3 * N, st->cfg + (size_t)src[k] * 3 * N, sizeof(double) * 3 * N);
+        memcpy(B2 + (size_t)k * nb, st->B + (size_t)src[k] * nb, sizeof(double) * nb);
+        e2[k] = st->e_local[src[k]];
+      }
+      memcpy(st->cfg, cfg2, sizeof(double) * W * 3 * N);
+      memcpy(st->B, B2, sizeof(double) * W * nb);
+      memcpy(st->e_local, e2, sizeof(double) * W);
+      memcpy(st->weight, w2, sizeof(double) * W);
+      st->steps += 1;
+    }
+    if (rc) break;
+    /* update_energies (dmc.rs:155-202) */
+    if (block_nr == num_eq_blocks) {
+      double energy = 0.0;
+      for (uint32_t j = 0; j < block_size; ++j) energy += eblock[j];
+      energy /= (double)block_size;
+      st->reference_energy = (st->reference_energy + energy) / 2.0;
+      energies[n_out] = energy;
+      vars[n_out] = 0.0;
+      ++n_out;
+    }
+    if (block_nr > num_eq_blocks) {
+      double energy = 0.0;
+      for (uint32_t j = 0; j < block_size; ++j) energy += eblock[j];
+      energy /= (double)block_size;
+      const double prev = energies[n_out - 1];
+      const double k = (double)(block_nr - num_eq_blocks);
+      energies[n_out] = prev + (energy - prev) / k;
+      st->reference_energy = (st->reference_energy + energies[n_out]) / 2.0;
+      vars[n_out] = vars[n_out - 1] + ((energy - prev) * (energy - energies[n_out]) - vars[n_out - 1]) / k;
+      ++n_out;
+    }
+  }
+  for (int i = 0; i < n_out; ++i) errors[i] = sqrt(vars[i] / (double)(i + 1));
+  free(cfg2);
+  free(B2);
+  free(e2);
+  free(w2);
+  free(src);
+  free(eblock);
+  free(vars);
+  return rc ? -1 : n_out;
 }

@@ -121,6 +121,73 @@ int orc_sj_run_parity(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0
 int orc_sj_run_faithful(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, uint32_t w1,
                         int n_threads, orc_sj_run_out *out);
 
+/* ------------------------------------------------------------------ */
+/* Diffusion Monte Carlo (src/dmc/src/dmc.rs, branching.rs)             */
+/* ------------------------------------------------------------------ */
+
+/* Philox streams of the DMC steps (key (seed lo32, seed hi32 ^ sys) as for
+ * the walkers): the starting configuration is stream (walker 0xFFFFFFFF,
+ * epoch ORC_DMC_INIT_EPOCH), draws 0.. in Box-Muller pairs; the branching
+ * of step s (counted from mole_dmc_init) is stream (walker s, epoch
+ * ORC_DMC_BRANCH_EPOCH).  A uniform integer in [0, n) from a draw d is
+ * floor(d n / 2^64). */
+#define ORC_DMC_INIT_EPOCH 0x444D4300u
+#define ORC_DMC_BRANCH_EPOCH 0x444D4301u
+enum { ORC_BRANCH_SR = 0, ORC_BRANCH_SIMPLE = 1 };
+
+/* SRBrancher::branch (branching.rs:15-44): integer weights
+ * (w W / max w) as u32, WeightedChoice (first index whose cumulative
+ * integer weight exceeds floor(d total / 2^64)) for each of the W new
+ * walkers, every new weight = mean(w).  src[W] receives the chosen
+ * indices, w_out[W] the weights.  Returns -1 for a total integer weight of
+ * 0 (the reference panics there). */
+int orc_branch_sr(const double *w, uint32_t W, uint64_t seed, uint32_t sys, uint64_t step, uint32_t *src,
+                  double *w_out);
+/* SimpleBranching::branch (branching.rs:57-91): copies n_i =
+ * min(floor(w_i + U_i), 3) (U_i = Standard of draw i), survivors in order,
+ * then the births in order; W - M clones of uniformly chosen original
+ * walkers (draw W + k), or M - W removals at uniform positions of the
+ * shrinking list (draw W + k).  Weights travel with the walkers. */
+void orc_branch_simple(const double *w, uint32_t W, uint64_t seed, uint32_t sys, uint64_t step, uint32_t *src,
+                       double *w_out);
+
+typedef struct {
+  uint64_t seed;
+  uint32_t sys, epoch;  /* move stream (walker w, epoch) */
+  double tau;           /* MetropolisDiffuse time step (moves) */
+  int precision;        /* ORC_PREC_* */
+  uint32_t lu_period;
+  uint32_t n_walkers;
+  int brancher;         /* ORC_BRANCH_* */
+} orc_dmc_params;
+
+typedef struct {
+  double *cfg;       /* [W][3N] */
+  double *weight;    /* [W] */
+  double *e_local;   /* [W] local energy at the current configuration */
+  double *B;         /* [W][n_up^2 + n_dn^2] inverses (FP64 or FP32 values) */
+  double reference_energy;
+  uint64_t steps;    /* DMC steps since init (branching stream) */
+  uint64_t draw;     /* next draw of the move streams */
+  uint64_t moves;    /* moves per walker since init (LU refresh phase) */
+} orc_dmc_state;
+
+/* DmcRunner::new (dmc.rs:45-73): every walker starts from the same N(0,1)
+ * configuration, weight 1; inverses and local energies of that
+ * configuration.  Buffers of st are caller-owned. */
+int orc_dmc_init(const orc_sj *wf, const orc_dmc_params *p, double reference_energy, orc_dmc_state *st);
+/* DmcRunner::diffuse (dmc.rs:75-153) and update_energies (:155-202):
+ * per step, for every walker the cached local energy E_old, one sweep of
+ * diffusion moves, E_new, ensemble += w E_old, total += w,
+ * w *= exp(-time_step ((E_old + E_new)/2 - E_ref)); the step energy
+ * ensemble/total; then the brancher.  energies/errors: one entry per block
+ * from block num_eq_blocks on (errors = sqrt(var_i / (i + 1)));
+ * step_energy (optional): [num_iterations].  Returns the number of blocks
+ * written, or -1. */
+int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st, double time_step,
+                    uint32_t num_iterations, uint32_t block_size, uint32_t num_eq_blocks, double *energies,
+                    double *errors, double *step_energy);
+
 #ifdef __cplusplus
 }
 #endif

@@ -158,3 +158,93 @@ def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, p
                n_samples=out.n_samples, n_blocks=out.n_blocks, n_accepted=out.n_accepted,
                n_moves=out.n_moves)
     return res
+
+
+# ---- diffusion Monte Carlo (orc_dmc_*, orc_branch_*) -----------------------
+
+BRANCH_SR, BRANCH_SIMPLE = 0, 1
+DMC_INIT_EPOCH = 0x444D4300
+DMC_BRANCH_EPOCH = 0x444D4301
+
+
+class DmcParams(C.Structure):
+    _fields_ = [
+        ("seed", C.c_uint64), ("sys", C.c_uint32), ("epoch", C.c_uint32), ("tau", C.c_double),
+        ("precision", C.c_int), ("lu_period", C.c_uint32), ("n_walkers", C.c_uint32), ("brancher", C.c_int),
+    ]
+
+
+class DmcState(C.Structure):
+    _fields_ = [
+        ("cfg", _pd), ("weight", _pd), ("e_local", _pd), ("B", _pd), ("reference_energy", C.c_double),
+        ("steps", C.c_uint64), ("draw", C.c_uint64), ("moves", C.c_uint64),
+    ]
+
+
+def _bind_dmc(L):
+    if getattr(L, "_dmc_bound", False):
+        return
+    pu = C.POINTER(C.c_uint32)
+    L.orc_branch_sr.restype = C.c_int
+    L.orc_branch_sr.argtypes = [_pd, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, pu, _pd]
+    L.orc_branch_simple.restype = None
+    L.orc_branch_simple.argtypes = [_pd, C.c_uint32, C.c_uint64, C.c_uint32, C.c_uint64, pu, _pd]
+    L.orc_dmc_init.restype = C.c_int
+    L.orc_dmc_init.argtypes = [C.POINTER(Sj), C.POINTER(DmcParams), C.c_double, C.POINTER(DmcState)]
+    L.orc_dmc_diffuse.restype = C.c_int
+    L.orc_dmc_diffuse.argtypes = [C.POINTER(Sj), C.POINTER(DmcParams), C.POINTER(DmcState), C.c_double,
+                                  C.c_uint32, C.c_uint32, C.c_uint32, _pd, _pd, _pd]
+    L._dmc_bound = True
+
+
+def branch(w, seed, sys, step, brancher):
+    """orc_branch_sr / orc_branch_simple: (src indices, new weights)."""
+    L = lib()
+    _bind_dmc(L)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    W = w.size
+    src = np.zeros(W, dtype=np.uint32)
+    wo = np.zeros(W)
+    if brancher == BRANCH_SR:
+        rc = L.orc_branch_sr(OF.dptr(w), W, seed, sys, step, src.ctypes.data_as(C.POINTER(C.c_uint32)),
+                             OF.dptr(wo))
+        if rc != 0:
+            raise ValueError("total integer weight 0")
+    else:
+        L.orc_branch_simple(OF.dptr(w), W, seed, sys, step, src.ctypes.data_as(C.POINTER(C.c_uint32)),
+                            OF.dptr(wo))
+    return src, wo
+
+
+class Dmc:
+    """The oracle's DmcRunner on one Slater-Jastrow system."""
+
+    def __init__(self, wv, *, walkers, seed, tau, reference_energy, brancher=BRANCH_SR, precision=PREC_FP64,
+                 lu_period=0, sys=0, epoch=0):
+        L = lib()
+        _bind_dmc(L)
+        self.wv = wv
+        nb = wv.n_up * wv.n_up + wv.n_dn * wv.n_dn
+        self.cfg = np.zeros((walkers, 3 * wv.N))
+        self.weight = np.zeros(walkers)
+        self.e_local = np.zeros(walkers)
+        self.B = np.zeros((walkers, max(1, nb)))
+        self.p = DmcParams(seed=seed, sys=sys, epoch=epoch, tau=tau, precision=precision, lu_period=lu_period,
+                           n_walkers=walkers, brancher=brancher)
+        self.st = DmcState(cfg=OF.dptr(self.cfg), weight=OF.dptr(self.weight), e_local=OF.dptr(self.e_local),
+                           B=OF.dptr(self.B))
+        assert L.orc_dmc_init(wv.ref(), C.byref(self.p), reference_energy, C.byref(self.st)) == 0
+
+    def diffuse(self, time_step, num_iterations, block_size, num_eq_blocks):
+        L = lib()
+        nb = max(1, num_iterations // block_size)
+        e, err, se = np.zeros(nb), np.zeros(nb), np.zeros(max(1, num_iterations))
+        n = L.orc_dmc_diffuse(self.wv.ref(), C.byref(self.p), C.byref(self.st), time_step, num_iterations,
+                              block_size, num_eq_blocks, OF.dptr(e), OF.dptr(err), OF.dptr(se))
+        if n < 0:
+            raise ValueError("orc_dmc_diffuse failed")
+        return e[:n], err[:n], se[:num_iterations]
+
+    @property
+    def reference_energy(self):
+        return self.st.reference_energy
