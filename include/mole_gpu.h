@@ -340,6 +340,45 @@ mole_status mole_optimizer_update(mole_optimizer *opt, const mole_sr_sums *sums,
 mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size,
                                mole_optimizer *opt, double *log, double *params_out);
 
+/* ---- diffusion Monte Carlo -----------------------------------------------
+ * DmcRunner (src/dmc/src/dmc.rs:16-203) for every Slater-Jastrow system of a
+ * context whose proposal is MOLE_PROPOSAL_DIFFUSE (the MetropolisDiffuse
+ * of dmc.rs:28; run.step is its time step).  Each system has its own
+ * walkers, weights and reference energy; branching never mixes systems.
+ * Streams: the moves continue the walkers' streams from draw 0 of the
+ * context epoch at mole_dmc_init; the starting configuration and the
+ * branching draws come from streams of their own. */
+#define MOLE_BRANCH_SR 0     /* SRBrancher, src/dmc/src/branching.rs:7-44 */
+#define MOLE_BRANCH_SIMPLE 1 /* SimpleBranching, branching.rs:46-91 */
+/* DmcRunner::new (dmc.rs:45-73): every walker of system k starts from one
+ * configuration of N(0,1) coordinates with weight 1; reference_energy[k]
+ * is the trial energy.  The walkers' previous configurations are
+ * replaced. */
+mole_status mole_dmc_init(mole_ctx *ctx, const double *reference_energy, uint32_t brancher);
+/* DmcRunner::diffuse (dmc.rs:75-153): num_iterations / block_size blocks
+ * of block_size steps; a step moves every electron of every walker once,
+ * updates w *= exp(-time_step ((E_old + E_new)/2 - E_ref)), records the
+ * ensemble energy sum(w E_old)/sum(w) and branches; after each block from
+ * num_eq_blocks on, update_energies (dmc.rs:155-202) mixes E_ref.
+ * Unlike the reference's u32 WeightedChoice (which panics once the total
+ * integer weight passes 2^32, W > ~65k walkers), SRBrancher totals are
+ * 64-bit.  A total integer weight of 0 (the reference panics) returns
+ * MOLE_ERR_INVALID_ARGUMENT. */
+mole_status mole_dmc_diffuse(mole_ctx *ctx, double time_step, uint32_t num_iterations, uint32_t block_size,
+                             uint32_t num_eq_blocks);
+/* The (energies, errors) diffuse returns (dmc.rs:147-152) for system sys:
+ * one entry per block from num_eq_blocks on, errors sqrt(var_i / (i+1));
+ * and the current reference energy.  NULL outputs are skipped. */
+mole_status mole_dmc_read(mole_ctx *ctx, uint32_t sys, double *energies, double *errors, size_t cap,
+                          size_t *n_blocks, double *reference_energy);
+/* The ensemble energy of every step of the last diffuse call (the values
+ * dmc.rs:117-121 pushes into each block). */
+mole_status mole_dmc_read_steps(mole_ctx *ctx, uint32_t sys, double *step_energy, size_t cap, size_t *n);
+/* Walker weights and local energies (DmcRunner::walkers, dmc.rs:27);
+ * configurations through mole_get_configs. */
+mole_status mole_dmc_read_walkers(mole_ctx *ctx, uint32_t sys, double *weights, double *e_local,
+                                  size_t n_walkers);
+
 /* ---- launch configuration ----------------------------------------------
  * No counterpart in the reference (rayon's fan-out, src/vmc/src/vmc.rs:63-76,
  * is the closest).  Selects the sweep kernel: AUTO picks the

@@ -49,6 +49,8 @@ MAX_PARAMS = 16
 OPT_SD = 0
 OPT_SR = 1
 OPT_MOMENTUM = 2
+BRANCH_SR = 0
+BRANCH_SIMPLE = 1
 OPT_NESTEROV = 3
 OPT_LBFGS = 4
 LBFGS_MAX_HISTORY = 16
@@ -203,6 +205,11 @@ _SIGS = [
     ("mole_optimizer_init", C.c_int32, [C.POINTER(OptimizerState), C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                         C.c_uint32, C.c_uint32, C.c_double]),
     ("mole_optimizer_update", C.c_int32, [C.POINTER(OptimizerState), C.POINTER(SrSums), _pd, _pd, _pd]),
+    ("mole_dmc_init", C.c_int32, [C.c_void_p, _pd, C.c_uint32]),
+    ("mole_dmc_diffuse", C.c_int32, [C.c_void_p, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32]),
+    ("mole_dmc_read", C.c_int32, [C.c_void_p, C.c_uint32, _pd, _pd, C.c_size_t, C.POINTER(C.c_size_t), _pd]),
+    ("mole_dmc_read_steps", C.c_int32, [C.c_void_p, C.c_uint32, _pd, C.c_size_t, C.POINTER(C.c_size_t)]),
+    ("mole_dmc_read_walkers", C.c_int32, [C.c_void_p, C.c_uint32, _pd, _pd, C.c_size_t]),
     ("mole_optimize_with", C.c_int32, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32, C.POINTER(OptimizerState),
                                        _pd, _pd]),
     ("mole_debug_philox", C.c_int32, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint64, C.c_size_t, _pu64]),
@@ -458,6 +465,45 @@ class Context:
         p = np.zeros(self.systems[0].params.size)
         check(lib().mole_optimize_with(self.h, iters, steps, block_size, C.byref(opt.state), _dp(log), _dp(p)))
         return p, log
+
+    # ---- diffusion Monte Carlo (DmcRunner, src/dmc/src/dmc.rs) ----
+
+    def dmc_init(self, reference_energy, brancher=BRANCH_SR):
+        """mole_dmc_init: one trial energy per system (a scalar for all)."""
+        e = np.ascontiguousarray(np.broadcast_to(np.asarray(reference_energy, dtype=np.float64),
+                                                 (len(self.systems),)))
+        check(lib().mole_dmc_init(self.h, _dp(e), brancher))
+
+    def dmc_diffuse(self, time_step, num_iterations, block_size, num_eq_blocks):
+        """mole_dmc_diffuse; returns (energies, errors) of system 0 as
+        DmcRunner::diffuse does (see dmc_read for the others)."""
+        check(lib().mole_dmc_diffuse(self.h, time_step, num_iterations, block_size, num_eq_blocks))
+        e, err, _ = self.dmc_read(0)
+        return e, err
+
+    def dmc_read(self, sys=0):
+        """(energies, errors, reference_energy) of the last diffuse call."""
+        n = C.c_size_t()
+        eref = C.c_double()
+        check(lib().mole_dmc_read(self.h, sys, None, None, 0, C.byref(n), C.byref(eref)))
+        e, err = np.zeros(n.value), np.zeros(n.value)
+        if n.value:
+            check(lib().mole_dmc_read(self.h, sys, _dp(e), _dp(err), n.value, C.byref(n), C.byref(eref)))
+        return e, err, eref.value
+
+    def dmc_read_steps(self, sys=0):
+        n = C.c_size_t()
+        check(lib().mole_dmc_read_steps(self.h, sys, None, 0, C.byref(n)))
+        out = np.zeros(n.value)
+        if n.value:
+            check(lib().mole_dmc_read_steps(self.h, sys, _dp(out), n.value, C.byref(n)))
+        return out
+
+    def dmc_read_walkers(self, sys=0):
+        W = self.systems[sys].n_walkers
+        w, e = np.zeros(W), np.zeros(W)
+        check(lib().mole_dmc_read_walkers(self.h, sys, _dp(w), _dp(e), W))
+        return w, e
 
     def set_launch_config(self, variant=VARIANT_AUTO, block_threads=0):
         check(lib().mole_set_launch_config(self.h, variant, block_threads))

@@ -24,6 +24,7 @@
 #include "mlp_internal.hpp"
 #include "sr_math.hpp"
 #include "sj_internal.hpp"
+#include "dmc_internal.hpp"
 
 using namespace mole;
 
@@ -80,7 +81,28 @@ struct SysState {
   void *d_B = nullptr;
   double v_nn = 0.0;
   uint64_t draw_base = 0, move_base = 0;
+  // diffusion Monte Carlo (allocated by mole_dmc_init)
+  struct Dmc {
+    double *dmc = nullptr;  // [4][W]
+    DmcState *st = nullptr;
+    double *part = nullptr;
+    uint64_t *cnt_a = nullptr, *cnt_b = nullptr, *bsum_a = nullptr, *bsum_b = nullptr;
+    uint32_t *list = nullptr, *removed = nullptr, *src = nullptr;
+    double *wnew = nullptr, *pos_s = nullptr, *el_s = nullptr;
+    void *B_s = nullptr;
+    double *step_energy = nullptr, *block_buf = nullptr, *energies = nullptr, *vars = nullptr;
+    size_t cap_steps = 0, cap_block = 0, cap_blocks = 0;
+    size_t steps_out = 0;  // steps recorded by the last mole_dmc_diffuse
+  } dmc;
 };
+
+void free_dmc(SysState::Dmc &d) {
+  void *ptrs[] = {d.dmc, d.st, d.part, d.cnt_a, d.cnt_b, d.bsum_a, d.bsum_b, d.list, d.removed, d.src,
+                  d.wnew, d.pos_s, d.el_s, d.B_s, d.step_energy, d.block_buf, d.energies, d.vars};
+  for (void *q : ptrs)
+    if (q) (void)hipFree(q);
+  d = SysState::Dmc{};
+}
 
 }  // namespace
 
@@ -116,6 +138,9 @@ struct mole_ctx {
   mole_optimizer *d_opt = nullptr;
   size_t optlog_cap = 0;
   bool dev_params = false;
+  // diffusion Monte Carlo
+  bool dmc_ready = false;
+  uint32_t dmc_brancher = 0;
 };
 
 // The learned proposal's network, its Adam state and its training data.
@@ -252,6 +277,7 @@ void free_ctx(mole_ctx *ctx) {
                     st.d_samp_g, st.d_tab, st.d_center, st.d_B};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
+    free_dmc(st.dmc);
   }
   if (ctx->d_red) (void)hipFree(ctx->d_red);
   if (ctx->d_params) (void)hipFree(ctx->d_params);
@@ -340,6 +366,8 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.move_base = st.move_base;
     d.sys_index = (uint32_t)s;
     d.lds_per_walker = (uint32_t)st.lds_walker;
+    d.dmc = st.dmc.dmc;
+    d.dmc_eref = st.dmc.st ? &st.dmc.st->eref : nullptr;
   }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   HIP_TRY(hipMemcpy(ctx->d_sysdev, h.data(), sizeof(SJSysDev) * h.size(), hipMemcpyHostToDevice));
@@ -1597,4 +1625,241 @@ mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_
   st = mole_optimizer_init(&opt, optimizer, (uint32_t)ctx->sys[0].params.size(), step, 0.0, 0, sr_mode, sr_eps);
   if (st != MOLE_OK) return st;
   return mole_optimize_with(ctx, iters, steps, block_size, &opt, log, params_out);
+}
+
+// ---- diffusion Monte Carlo --------------------------------------------------
+
+namespace {
+
+template <typename P>
+hipError_t dmalloc(P **p, size_t n) {
+  return hipMalloc((void **)p, sizeof(P) * (n ? n : 1));
+}
+
+mole_status dmc_alloc(mole_ctx *ctx, SysState &s) {
+  SysState::Dmc &d = s.dmc;
+  if (d.dmc) return MOLE_OK;
+  const size_t W = (size_t)s.W;
+  const size_t nB = (size_t)s.n_up * s.n_up + (size_t)s.n_down * s.n_down;
+  HIP_TRY(dmalloc(&d.dmc, 4 * W));
+  HIP_TRY(dmalloc(&d.st, 1));
+  HIP_TRY(dmalloc(&d.part, 4 * dmc_parts(s.W)));
+  HIP_TRY(dmalloc(&d.cnt_a, 3 * W));
+  HIP_TRY(dmalloc(&d.cnt_b, W));
+  HIP_TRY(dmalloc(&d.bsum_a, dmc_scan_blocks(3 * W)));
+  HIP_TRY(dmalloc(&d.bsum_b, dmc_scan_blocks(W)));
+  HIP_TRY(dmalloc(&d.list, 3 * W));
+  HIP_TRY(dmalloc(&d.removed, 2 * W));
+  HIP_TRY(dmalloc(&d.src, W));
+  HIP_TRY(dmalloc(&d.wnew, W));
+  HIP_TRY(dmalloc(&d.pos_s, 3 * (size_t)s.N * W));
+  HIP_TRY(dmalloc(&d.el_s, W));
+  HIP_TRY(hipMalloc(&d.B_s, (nB ? nB : 1) * W * (size_t)sj_tsize(ctx)));
+  return MOLE_OK;
+}
+
+DmcArgs dmc_args(mole_ctx *ctx, uint32_t sys, uint32_t block_size, uint32_t num_eq_blocks) {
+  SysState &s = ctx->sys[sys];
+  SysState::Dmc &d = s.dmc;
+  DmcArgs a{};
+  a.st = d.st;
+  a.dmc = d.dmc;
+  a.W = s.W;
+  a.brancher = (int)ctx->dmc_brancher;
+  a.seed = ctx->run.seed;
+  a.sys = sys;
+  a.block_size = block_size;
+  a.num_eq_blocks = num_eq_blocks;
+  a.cap_steps = (uint32_t)d.cap_steps;
+  a.step_energy = d.step_energy;
+  a.block_buf = d.block_buf;
+  a.energies = d.energies;
+  a.vars = d.vars;
+  a.part = d.part;
+  a.cnt_a = d.cnt_a;
+  a.cnt_b = d.cnt_b;
+  a.bsum_a = d.bsum_a;
+  a.bsum_b = d.bsum_b;
+  a.list = d.list;
+  a.removed = d.removed;
+  a.src = d.src;
+  a.wnew = d.wnew;
+  a.pos = s.d_pos;
+  a.pos_rows = 3 * (int)s.N;
+  a.B = s.d_B;
+  a.b_rows = (int)(s.n_up * s.n_up + s.n_down * s.n_down);
+  a.b_elem = sj_tsize(ctx);
+  a.pos_s = d.pos_s;
+  a.B_s = d.B_s;
+  a.el_s = d.el_s;
+  return a;
+}
+
+// one DMC step of every system: the sweep launch, then each system's
+// reductions, branching, gather and bookkeeping
+mole_status dmc_enqueue_step(mole_ctx *ctx, double time_step, uint32_t block_size, uint32_t num_eq_blocks) {
+  SJArgs a = sj_args(ctx);
+  a.n_sweeps = 1;
+  a.sample_start = 0;
+  a.dmc = kSJDmcStep;
+  a.dmc_tau = time_step;
+  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
+  HIP_TRY(launch_sj_advance(ctx->d_state, 1, 0, ctx->run.block_size, ctx->stream));
+  for (uint32_t s = 0; s < ctx->sys.size(); ++s)
+    HIP_TRY(launch_dmc_after_step(dmc_args(ctx, s, block_size, num_eq_blocks), ctx->stream));
+  return MOLE_OK;
+}
+
+}  // namespace
+
+mole_status mole_dmc_init(mole_ctx *ctx, const double *reference_energy, uint32_t brancher) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (!reference_energy) return fail(MOLE_ERR_INVALID_ARGUMENT, "null reference energies");
+  if (ctx->family != kFamSJ || ctx->run.proposal != MOLE_PROPOSAL_DIFFUSE)
+    return fail(MOLE_ERR_UNSUPPORTED, "DMC runs Slater-Jastrow systems with MOLE_PROPOSAL_DIFFUSE");
+  if (brancher > MOLE_BRANCH_SIMPLE) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown brancher %u", brancher);
+  HIP_TRY(hipSetDevice(ctx->device));
+  for (SysState &s : ctx->sys) {
+    st = dmc_alloc(ctx, s);
+    if (st != MOLE_OK) return st;
+  }
+  // the move streams start over (as mole_set_stream(epoch, 0))
+  for (SysState &s : ctx->sys) {
+    s.draw_base = 0;
+    s.move_base = 0;
+    s.cols_valid = false;
+  }
+  ctx->sweeps_done = 0;
+  ctx->sysdev_dirty = true;
+  st = sj_upload_sysdev(ctx);
+  if (st != MOLE_OK) return st;
+  st = sj_push_state(ctx);
+  if (st != MOLE_OK) return st;
+  for (uint32_t k = 0; k < ctx->sys.size(); ++k) {
+    SysState &s = ctx->sys[k];
+    HIP_TRY(launch_dmc_init_configs(s.d_pos, (int)s.N, s.W, ctx->run.seed, k, ctx->stream));
+    DmcState v{};
+    v.eref = reference_energy[k];
+    HIP_TRY(launch_dmc_state_set(s.dmc.st, v, ctx->stream));
+  }
+  // inverses of the configuration, local energies, unit weights
+  SJArgs a = sj_args(ctx);
+  a.n_sweeps = 0;
+  a.refresh = 1;
+  a.dmc = kSJDmcInit;
+  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->dmc_brancher = brancher;
+  ctx->dmc_ready = true;
+  return ok();
+}
+
+mole_status mole_dmc_diffuse(mole_ctx *ctx, double time_step, uint32_t num_iterations, uint32_t block_size,
+                             uint32_t num_eq_blocks) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (!ctx->dmc_ready) return fail(MOLE_ERR_INVALID_ARGUMENT, "call mole_dmc_init first");
+  if (block_size == 0) return fail(MOLE_ERR_INVALID_ARGUMENT, "block_size must be >= 1");
+  if (!std::isfinite(time_step)) return fail(MOLE_ERR_INVALID_ARGUMENT, "time_step must be finite");
+  HIP_TRY(hipSetDevice(ctx->device));
+  const uint32_t blocks = num_iterations / block_size;
+  const size_t steps = (size_t)blocks * block_size;
+  for (SysState &s : ctx->sys) {
+    SysState::Dmc &d = s.dmc;
+    if (steps > d.cap_steps) {
+      if (d.step_energy) (void)hipFree(d.step_energy);
+      d.step_energy = nullptr;
+      HIP_TRY(dmalloc(&d.step_energy, steps));
+      d.cap_steps = steps;
+    }
+    if (block_size > d.cap_block) {
+      if (d.block_buf) (void)hipFree(d.block_buf);
+      d.block_buf = nullptr;
+      HIP_TRY(dmalloc(&d.block_buf, block_size));
+      d.cap_block = block_size;
+    }
+    if (blocks > d.cap_blocks || !d.energies) {
+      if (d.energies) (void)hipFree(d.energies);
+      if (d.vars) (void)hipFree(d.vars);
+      d.energies = d.vars = nullptr;
+      HIP_TRY(dmalloc(&d.energies, blocks));
+      HIP_TRY(dmalloc(&d.vars, blocks));
+      d.cap_blocks = blocks;
+    }
+    HIP_TRY(launch_dmc_call_start(d.st, ctx->stream));
+    d.steps_out = steps;
+  }
+  if (ctx->sysdev_dirty) {
+    st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  for (size_t it = 0; it < steps; ++it) {
+    st = dmc_enqueue_step(ctx, time_step, block_size, num_eq_blocks);
+    if (st != MOLE_OK) return st;
+    account(ctx, 1, 0);
+  }
+  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  ctx->last_launches = (uint32_t)steps;
+  ctx->timed = true;
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (uint32_t k = 0; k < ctx->sys.size(); ++k) {
+    DmcState v;
+    HIP_TRY(hipMemcpy(&v, ctx->sys[k].dmc.st, sizeof v, hipMemcpyDeviceToHost));
+    if (v.error)
+      return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: every walker's integer weight is 0 (SRBrancher)", k);
+  }
+  return ok();
+}
+
+mole_status mole_dmc_read(mole_ctx *ctx, uint32_t sys, double *energies, double *errors, size_t cap,
+                          size_t *n_blocks, double *reference_energy) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!ctx->dmc_ready) return fail(MOLE_ERR_INVALID_ARGUMENT, "call mole_dmc_init first");
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  const SysState::Dmc &d = ctx->sys[sys].dmc;
+  DmcState v;
+  HIP_TRY(hipMemcpy(&v, d.st, sizeof v, hipMemcpyDeviceToHost));
+  const size_t n = d.energies ? v.n_out : 0;
+  if (n_blocks) *n_blocks = n;
+  if (reference_energy) *reference_energy = v.eref;
+  if ((energies || errors) && cap < n) return fail(MOLE_ERR_SHAPE, "capacity %zu < %zu blocks", cap, n);
+  if (n == 0) return ok();
+  std::vector<double> e(n), var(n);
+  HIP_TRY(hipMemcpy(e.data(), d.energies, sizeof(double) * n, hipMemcpyDeviceToHost));
+  HIP_TRY(hipMemcpy(var.data(), d.vars, sizeof(double) * n, hipMemcpyDeviceToHost));
+  if (energies) std::memcpy(energies, e.data(), sizeof(double) * n);
+  if (errors)
+    for (size_t i = 0; i < n; ++i) errors[i] = std::sqrt(var[i] / (double)(i + 1));  // dmc.rs:147-150
+  return ok();
+}
+
+mole_status mole_dmc_read_steps(mole_ctx *ctx, uint32_t sys, double *step_energy, size_t cap, size_t *n) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!ctx->dmc_ready) return fail(MOLE_ERR_INVALID_ARGUMENT, "call mole_dmc_init first");
+  const SysState::Dmc &d = ctx->sys[sys].dmc;
+  if (n) *n = d.steps_out;
+  if (!step_energy || d.steps_out == 0) return ok();
+  if (cap < d.steps_out) return fail(MOLE_ERR_SHAPE, "capacity %zu < %zu steps", cap, d.steps_out);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(step_energy, d.step_energy, sizeof(double) * d.steps_out, hipMemcpyDeviceToHost));
+  return ok();
+}
+
+mole_status mole_dmc_read_walkers(mole_ctx *ctx, uint32_t sys, double *weights, double *e_local, size_t n_walkers) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!ctx->dmc_ready) return fail(MOLE_ERR_INVALID_ARGUMENT, "call mole_dmc_init first");
+  const SysState &s = ctx->sys[sys];
+  if (n_walkers != (size_t)s.W) return fail(MOLE_ERR_SHAPE, "system %u has %d walkers", sys, s.W);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  if (weights) HIP_TRY(hipMemcpy(weights, s.dmc.dmc, sizeof(double) * s.W, hipMemcpyDeviceToHost));
+  if (e_local) HIP_TRY(hipMemcpy(e_local, s.dmc.dmc + s.W, sizeof(double) * s.W, hipMemcpyDeviceToHost));
+  return ok();
 }

@@ -1,0 +1,467 @@
+// Diffusion Monte Carlo around the Slater-Jastrow DMC steps: the ensemble
+// reductions, SRBrancher and SimpleBranching (src/dmc/src/branching.rs:7-91),
+// the gather of walker state after branching, and update_energies
+// (src/dmc/src/dmc.rs:155-202).  Every kernel reads its counts from the
+// device state, so one step's launches can be captured and replayed.
+//
+// Sums run in a fixed order (per-thread strided sums, then an LDS tree), so
+// results do not depend on scheduling.  A uniform integer in [0, n) from a
+// 64-bit draw d is the high word of d n (no rejection loop: bias below
+// n / 2^64).
+#include "device_common.hpp"
+#include "dmc_internal.hpp"
+
+namespace mole {
+namespace {
+
+constexpr int kRedThreads = 256;
+constexpr int kRedChunk = kRedThreads * 16;  // walkers per reduction block
+constexpr int kScanThreads = 256;
+constexpr int kScanPerThread = 4;
+constexpr int kScanBlock = kScanThreads * kScanPerThread;
+
+__device__ __forceinline__ uint64_t uniform_below(uint64_t d, uint64_t n) { return __umul64hi(d, n); }
+
+// Rust's `f64 as u32` / `f64 as usize`: truncation, saturating, NaN -> 0
+__device__ __forceinline__ uint64_t as_u32(double x) {
+  if (!(x > 0.0)) return 0;
+  if (x >= 4294967295.0) return 0xFFFFFFFFull;
+  return (uint64_t)(uint32_t)x;
+}
+__device__ __forceinline__ uint64_t as_usize(double x) {
+  if (!(x > 0.0)) return 0;
+  if (x >= 18446744073709551615.0) return ~0ull;
+  return (uint64_t)x;
+}
+
+__global__ void init_configs_kernel(double *pos, int N, int W, uint64_t seed, uint32_t sys) {
+  const int w = blockIdx.x * blockDim.x + threadIdx.x;
+  if (w >= W) return;
+  Stream st;
+  st.init(seed, sys, 0xFFFFFFFFu, kDmcInitEpoch);
+  for (int k = 0; 2 * k < 3 * N; ++k) {
+    double z0, z1;
+    box_muller(st.draw(2 * (uint64_t)k), st.draw(2 * (uint64_t)k + 1), &z0, &z1);
+    pos[(size_t)(2 * k) * W + w] = z0;
+    if (2 * k + 1 < 3 * N) pos[(size_t)(2 * k + 1) * W + w] = z1;
+  }
+}
+
+// ---- reductions: sum w E_old, sum w_old, sum w_new, max w_new ----
+
+__global__ void __launch_bounds__(kRedThreads) reduce1_kernel(const double *dmc, int W, double *part, int n_parts) {
+  __shared__ double sh[4][kRedThreads];
+  const int t = threadIdx.x;
+  const size_t b0 = (size_t)blockIdx.x * kRedChunk;
+  double s_we = 0.0, s_w = 0.0, s_wn = 0.0, mx = 0.0;
+  for (int k = 0; k < kRedChunk / kRedThreads; ++k) {
+    const size_t i = b0 + (size_t)k * kRedThreads + t;
+    if (i < (size_t)W) {
+      s_we += dmc[2 * (size_t)W + i];
+      s_w += dmc[3 * (size_t)W + i];
+      s_wn += dmc[i];
+      mx = fmax(mx, dmc[i]);
+    }
+  }
+  sh[0][t] = s_we;
+  sh[1][t] = s_w;
+  sh[2][t] = s_wn;
+  sh[3][t] = mx;
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      sh[0][t] += sh[0][t + off];
+      sh[1][t] += sh[1][t + off];
+      sh[2][t] += sh[2][t + off];
+      sh[3][t] = fmax(sh[3][t], sh[3][t + off]);
+    }
+    __syncthreads();
+  }
+  if (t < 4) part[(size_t)t * n_parts + blockIdx.x] = sh[t][0];
+}
+
+// the step's ensemble energy (dmc.rs:117-121) and SRBrancher's mean weight
+// and normalisation (branching.rs:21-24)
+__global__ void __launch_bounds__(kRedThreads) reduce2_kernel(DmcArgs a, int n_parts) {
+  __shared__ double sh[4][kRedThreads];
+  const int t = threadIdx.x;
+  double v[4] = {0.0, 0.0, 0.0, 0.0};
+  for (int i = t; i < n_parts; i += kRedThreads) {
+    v[0] += a.part[i];
+    v[1] += a.part[(size_t)n_parts + i];
+    v[2] += a.part[2 * (size_t)n_parts + i];
+    v[3] = fmax(v[3], a.part[3 * (size_t)n_parts + i]);
+  }
+  for (int q = 0; q < 4; ++q) sh[q][t] = v[q];
+  __syncthreads();
+  for (int off = kRedThreads / 2; off > 0; off >>= 1) {
+    if (t < off) {
+      sh[0][t] += sh[0][t + off];
+      sh[1][t] += sh[1][t + off];
+      sh[2][t] += sh[2][t + off];
+      sh[3][t] = fmax(sh[3][t], sh[3][t + off]);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    DmcState *st = a.st;
+    st->sum_we = sh[0][0];
+    st->sum_w = sh[1][0];
+    st->sum_wnew = sh[2][0];
+    st->max_w = sh[3][0];
+    const double ens = sh[0][0] / sh[1][0];
+    if (st->it < a.cap_steps) a.step_energy[st->it] = ens;
+    a.block_buf[st->j] = ens;
+    st->gw = sh[2][0] / (double)a.W;
+    st->norm = (double)a.W / sh[3][0];
+  }
+}
+
+// ---- inclusive scan of u64 counts (n from the host bound or a device count) ----
+
+__global__ void __launch_bounds__(kScanThreads) scan_local_kernel(uint64_t *x, size_t n_max, const uint32_t *n_dev,
+                                                                  uint64_t *bsum) {
+  __shared__ uint64_t sh[kScanThreads];
+  const size_t n = n_dev ? (size_t)*n_dev : n_max;
+  const int t = threadIdx.x;
+  const size_t e0 = (size_t)blockIdx.x * kScanBlock + (size_t)t * kScanPerThread;
+  uint64_t v[kScanPerThread];
+  uint64_t run = 0;
+  for (int k = 0; k < kScanPerThread; ++k) {
+    v[k] = (e0 + k < n) ? x[e0 + k] : 0;
+    run += v[k];
+    v[k] = run;
+  }
+  sh[t] = run;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint64_t add = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  const uint64_t before = (t > 0) ? sh[t - 1] : 0;
+  for (int k = 0; k < kScanPerThread; ++k)
+    if (e0 + k < n) x[e0 + k] = v[k] + before;
+  if (t == kScanThreads - 1) bsum[blockIdx.x] = sh[t];
+}
+
+// exclusive scan of the block sums, in place (one workgroup)
+__global__ void __launch_bounds__(kScanThreads) scan_blocks_kernel(uint64_t *bsum, int nb) {
+  __shared__ uint64_t sh[kScanThreads];
+  const int t = threadIdx.x;
+  const int per = (nb + kScanThreads - 1) / kScanThreads;
+  const int b0 = t * per;
+  uint64_t s = 0;
+  for (int k = 0; k < per; ++k)
+    if (b0 + k < nb) s += bsum[b0 + k];
+  sh[t] = s;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint64_t add = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  uint64_t run = (t > 0) ? sh[t - 1] : 0;
+  for (int k = 0; k < per; ++k)
+    if (b0 + k < nb) {
+      const uint64_t v = bsum[b0 + k];
+      bsum[b0 + k] = run;
+      run += v;
+    }
+}
+
+__global__ void scan_add_kernel(uint64_t *x, size_t n_max, const uint32_t *n_dev, const uint64_t *bsum) {
+  const size_t n = n_dev ? (size_t)*n_dev : n_max;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += bsum[i / kScanBlock];
+}
+
+hipError_t enqueue_scan(uint64_t *x, size_t n_max, const uint32_t *n_dev, uint64_t *bsum, hipStream_t stream) {
+  const int nb = (int)dmc_scan_blocks(n_max);
+  hipLaunchKernelGGL(scan_local_kernel, dim3(nb), dim3(kScanThreads), 0, stream, x, n_max, n_dev, bsum);
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(kScanThreads), 0, stream, bsum, nb);
+  hipLaunchKernelGGL(scan_add_kernel, dim3((unsigned)((n_max + 255) / 256)), dim3(256), 0, stream, x, n_max, n_dev,
+                     (const uint64_t *)bsum);
+  return hipGetLastError();
+}
+
+// ---- SRBrancher (branching.rs:15-44) ----
+
+__global__ void sr_counts_kernel(DmcArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.W) a.cnt_a[i] = as_u32(a.dmc[i] * a.st->norm);
+}
+
+// WeightedChoice: the first index whose cumulative weight exceeds u
+__global__ void sr_select_kernel(DmcArgs a) {
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= a.W) return;
+  DmcState *st = a.st;
+  const uint64_t total = a.cnt_a[a.W - 1];
+  if (j == 0) st->total = total;
+  if (total == 0) {  // WeightedChoice::new panics: flag it, keep the walkers
+    if (j == 0) st->error = 1;
+    a.src[j] = (uint32_t)j;
+    a.wnew[j] = a.dmc[j];
+    return;
+  }
+  Stream rs;
+  rs.init(a.seed, a.sys, (uint32_t)st->steps, kDmcBranchEpoch);
+  const uint64_t u = uniform_below(rs.draw((uint64_t)j), total);
+  uint32_t lo = 0, hi = (uint32_t)a.W - 1;
+  while (lo < hi) {
+    const uint32_t mid = lo + (hi - lo) / 2;
+    if (a.cnt_a[mid] > u) hi = mid;
+    else lo = mid + 1;
+  }
+  a.src[j] = lo;
+  a.wnew[j] = st->gw;
+}
+
+// ---- SimpleBranching (branching.rs:57-91) ----
+
+__global__ void simple_counts_kernel(DmcArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.W) return;
+  Stream rs;
+  rs.init(a.seed, a.sys, (uint32_t)a.st->steps, kDmcBranchEpoch);
+  uint64_t n = as_usize(a.dmc[i] + u64_standard(rs.draw((uint64_t)i)));
+  if (n > 3) n = 3;
+  a.cnt_a[i] = n >= 1 ? 1 : 0;
+  a.cnt_b[i] = n >= 1 ? n - 1 : 0;
+}
+
+// survivors in order, then the births in order
+__global__ void simple_list_kernel(DmcArgs a) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.W) return;
+  const uint64_t S = a.cnt_a[a.W - 1], B = a.cnt_b[a.W - 1];
+  if (i == 0) {
+    a.st->S = (uint32_t)S;
+    a.st->M = (uint32_t)(S + B);
+  }
+  const uint64_t ka = a.cnt_a[i], kb = a.cnt_b[i];
+  const uint64_t keep = ka - (i ? a.cnt_a[i - 1] : 0);
+  const uint64_t births = kb - (i ? a.cnt_b[i - 1] : 0);
+  if (keep) a.list[ka - 1] = (uint32_t)i;
+  for (uint64_t r = 0; r < births; ++r) a.list[S + kb - births + r] = (uint32_t)i;
+}
+
+// fewer than W: clones of uniformly chosen original walkers
+__global__ void simple_clone_kernel(DmcArgs a) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t M = a.st->M;
+  if (M >= (uint32_t)a.W || k >= a.W - (int)M) return;
+  Stream rs;
+  rs.init(a.seed, a.sys, (uint32_t)a.st->steps, kDmcBranchEpoch);
+  a.list[M + k] = (uint32_t)uniform_below(rs.draw((uint64_t)a.W + k), (uint64_t)a.W);
+}
+
+// more than W: M - W removals, each at a uniform position of the shrinking
+// list.  The k-th removal's original position is the least p with
+// p = r_k + #{earlier removals <= p} (iterated from p = r_k; p never passes
+// M - 1, so the loop ends).
+constexpr int kRemoveThreads = 1024;
+__global__ void __launch_bounds__(kRemoveThreads) simple_remove_kernel(DmcArgs a) {
+  __shared__ uint32_t sh[kRemoveThreads];
+  const uint32_t M = a.st->M, W = (uint32_t)a.W;
+  if (M <= W) return;
+  const uint32_t E = M - W;
+  const int t = threadIdx.x;
+  Stream rs;
+  rs.init(a.seed, a.sys, (uint32_t)a.st->steps, kDmcBranchEpoch);
+  for (uint32_t k = 0; k < E; ++k) {
+    const uint64_t r = uniform_below(rs.draw((uint64_t)W + k), (uint64_t)(M - k));
+    uint64_t p = r;
+    for (;;) {
+      uint32_t c = 0;
+      for (uint32_t q = t; q < k; q += kRemoveThreads) c += (a.removed[q] <= p) ? 1u : 0u;
+      sh[t] = c;
+      __syncthreads();
+      for (int off = kRemoveThreads / 2; off > 0; off >>= 1) {
+        if (t < off) sh[t] += sh[t + off];
+        __syncthreads();
+      }
+      const uint64_t np = r + sh[0];
+      __syncthreads();
+      if (np == p) break;
+      p = np;
+    }
+    if (t == 0) a.removed[k] = (uint32_t)p;
+    __syncthreads();
+  }
+}
+
+// flags of the list entries that survive the removals (cnt_a, M entries)
+__global__ void simple_flags_kernel(DmcArgs a) {
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < a.st->M) a.cnt_a[i] = 1;
+}
+__global__ void simple_unflag_kernel(DmcArgs a) {
+  const uint32_t M = a.st->M;
+  const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (M > (uint32_t)a.W && k < M - (uint32_t)a.W) a.cnt_a[a.removed[k]] = 0;
+}
+
+// the final W walkers; SimpleBranching keeps each walker's weight
+__global__ void simple_final_kernel(DmcArgs a) {
+  const uint32_t M = a.st->M, W = (uint32_t)a.W;
+  const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (M > W) {
+    if (i < M && (a.cnt_a[i] - (i ? a.cnt_a[i - 1] : 0)) == 1) {
+      const uint64_t j = a.cnt_a[i] - 1;
+      const uint32_t s = a.list[i];
+      a.src[j] = s;
+      a.wnew[j] = a.dmc[s];
+    }
+  } else if (i < W) {
+    const uint32_t s = a.list[i];
+    a.src[i] = s;
+    a.wnew[i] = a.dmc[s];
+  }
+}
+
+// ---- walker state after branching ----
+
+template <typename E>
+__global__ void gather_rows_kernel(const E *in, E *out, int rows, int W, const uint32_t *src) {
+  const size_t n = (size_t)rows * W;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    const size_t r = e / (size_t)W, j = e % (size_t)W;
+    out[e] = in[r * W + src[j]];
+  }
+}
+
+template <typename E>
+__global__ void copy_kernel(const E *in, E *out, size_t n) {
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
+    out[e] = in[e];
+}
+
+// update_energies (dmc.rs:155-202) at the end of a block; counters advance
+__global__ void step_end_kernel(DmcArgs a) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  DmcState *st = a.st;
+  st->steps += 1;
+  st->it += 1;
+  if (++st->j < a.block_size) return;
+  st->j = 0;
+  const uint32_t b = st->block_nr++;
+  if (b < a.num_eq_blocks) return;
+  double energy = 0.0;
+  for (uint32_t k = 0; k < a.block_size; ++k) energy += a.block_buf[k];
+  energy /= (double)a.block_size;
+  const uint32_t o = st->n_out;
+  if (b == a.num_eq_blocks) {
+    st->eref = (st->eref + energy) / 2.0;
+    a.energies[o] = energy;
+    a.vars[o] = 0.0;
+  } else {
+    const double prev = a.energies[o - 1];
+    const double k = (double)(b - a.num_eq_blocks);
+    a.energies[o] = prev + (energy - prev) / k;
+    st->eref = (st->eref + a.energies[o]) / 2.0;
+    a.vars[o] = a.vars[o - 1] + ((energy - prev) * (energy - a.energies[o]) - a.vars[o - 1]) / k;
+  }
+  st->n_out = o + 1;
+}
+
+__global__ void state_set_kernel(DmcState *st, DmcState v) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) *st = v;
+}
+
+// a new DmcRunner::diffuse call: fresh block bookkeeping, E_ref and the
+// branching stream carry on
+__global__ void call_start_kernel(DmcState *st) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  st->it = 0;
+  st->j = 0;
+  st->block_nr = 0;
+  st->n_out = 0;
+  st->error = 0;
+}
+
+unsigned grid_for(size_t n, unsigned cap = 65535u) {
+  const size_t g = (n + 255) / 256;
+  return (unsigned)(g < 1 ? 1 : (g > cap ? cap : g));
+}
+
+}  // namespace
+
+size_t dmc_parts(int W) { return ((size_t)W + kRedChunk - 1) / kRedChunk; }
+size_t dmc_scan_blocks(size_t n) { return (n + kScanBlock - 1) / kScanBlock; }
+
+hipError_t launch_dmc_init_configs(double *pos, int N, int W, uint64_t seed, uint32_t sys, hipStream_t stream) {
+  hipLaunchKernelGGL(init_configs_kernel, dim3((W + 255) / 256), dim3(256), 0, stream, pos, N, W, seed, sys);
+  return hipGetLastError();
+}
+
+hipError_t launch_dmc_state_set(DmcState *st, const DmcState &v, hipStream_t stream) {
+  hipLaunchKernelGGL(state_set_kernel, dim3(1), dim3(64), 0, stream, st, v);
+  return hipGetLastError();
+}
+
+hipError_t launch_dmc_call_start(DmcState *st, hipStream_t stream) {
+  hipLaunchKernelGGL(call_start_kernel, dim3(1), dim3(64), 0, stream, st);
+  return hipGetLastError();
+}
+
+hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream) {
+  const int W = a.W;
+  const int n_parts = (int)dmc_parts(W);
+  const dim3 gw((W + 255) / 256), b256(256);
+  hipLaunchKernelGGL(reduce1_kernel, dim3(n_parts), dim3(kRedThreads), 0, stream, (const double *)a.dmc, W, a.part,
+                     n_parts);
+  hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kRedThreads), 0, stream, a, n_parts);
+  hipError_t e;
+  if (a.brancher == kDmcBranchSR) {
+    hipLaunchKernelGGL(sr_counts_kernel, gw, b256, 0, stream, a);
+    if ((e = enqueue_scan(a.cnt_a, (size_t)W, nullptr, a.bsum_a, stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(sr_select_kernel, gw, b256, 0, stream, a);
+  } else {
+    hipLaunchKernelGGL(simple_counts_kernel, gw, b256, 0, stream, a);
+    if ((e = enqueue_scan(a.cnt_a, (size_t)W, nullptr, a.bsum_a, stream)) != hipSuccess) return e;
+    if ((e = enqueue_scan(a.cnt_b, (size_t)W, nullptr, a.bsum_b, stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(simple_list_kernel, gw, b256, 0, stream, a);
+    hipLaunchKernelGGL(simple_clone_kernel, gw, b256, 0, stream, a);
+    hipLaunchKernelGGL(simple_remove_kernel, dim3(1), dim3(kRemoveThreads), 0, stream, a);
+    const dim3 g3((unsigned)((3 * (size_t)W + 255) / 256));
+    hipLaunchKernelGGL(simple_flags_kernel, g3, b256, 0, stream, a);
+    hipLaunchKernelGGL(simple_unflag_kernel, dim3((unsigned)((2 * (size_t)W + 255) / 256)), b256, 0, stream, a);
+    if ((e = enqueue_scan(a.cnt_a, 3 * (size_t)W, &a.st->M, a.bsum_a, stream)) != hipSuccess) return e;
+    hipLaunchKernelGGL(simple_final_kernel, g3, b256, 0, stream, a);
+  }
+  // gather into scratch, then back
+  const size_t np = (size_t)a.pos_rows * W, nbr = (size_t)a.b_rows * W;
+  hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(grid_for(np)), b256, 0, stream, (const double *)a.pos, a.pos_s,
+                     a.pos_rows, W, (const uint32_t *)a.src);
+  if (nbr) {
+    if (a.b_elem == 4)
+      hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(grid_for(nbr)), b256, 0, stream, (const float *)a.B,
+                         (float *)a.B_s, a.b_rows, W, (const uint32_t *)a.src);
+    else
+      hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(grid_for(nbr)), b256, 0, stream, (const double *)a.B,
+                         (double *)a.B_s, a.b_rows, W, (const uint32_t *)a.src);
+  }
+  hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(grid_for((size_t)W)), b256, 0, stream,
+                     (const double *)(a.dmc + W), a.el_s, 1, W, (const uint32_t *)a.src);
+  hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for(np)), b256, 0, stream, (const double *)a.pos_s, a.pos, np);
+  if (nbr) {
+    if (a.b_elem == 4)
+      hipLaunchKernelGGL(copy_kernel<float>, dim3(grid_for(nbr)), b256, 0, stream, (const float *)a.B_s, (float *)a.B,
+                         nbr);
+    else
+      hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for(nbr)), b256, 0, stream, (const double *)a.B_s,
+                         (double *)a.B, nbr);
+  }
+  hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for((size_t)W)), b256, 0, stream, (const double *)a.el_s,
+                     a.dmc + W, (size_t)W);
+  hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for((size_t)W)), b256, 0, stream, (const double *)a.wnew, a.dmc,
+                     (size_t)W);
+  hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, stream, a);
+  return hipGetLastError();
+}
+
+}  // namespace mole

@@ -54,7 +54,16 @@ struct SJSysDev {
   uint64_t move_base;   // moves made before the stream origin (LU refresh phase)
   uint32_t sys_index;   // mixed into the Philox key
   uint32_t lds_per_walker;  // bytes
+  // diffusion Monte Carlo (null outside DMC): [4][W] = weight, local
+  // energy at the current configuration, and the last step's w E_old and
+  // w_old; the reference energy is a device scalar
+  double *dmc;
+  const double *dmc_eref;
 };
+
+// What a launch does besides the sweeps: nothing (VMC), DMC's start (local
+// energies and unit weights of the configurations), or DMC steps.
+enum SJDmcMode : int { kSJDmcOff = 0, kSJDmcInit = 1, kSJDmcStep = 2 };
 
 // A workgroup's work: walkers [w0, w0 + nw) of one system.
 struct SJTile {
@@ -88,6 +97,8 @@ struct SJArgs {
   int init;                // 1: draw U(-1,1) configurations first (draws 0..3N-1)
   int refresh;             // 1: rebuild the inverses from the positions at entry
   MlpDev net;              // learned proposal: the packed network
+  int dmc;                 // SJDmcMode
+  double dmc_tau;          // DMC: the weights' time step
 };
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);

@@ -560,6 +560,15 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
     const double c1 = refresh_spin<T>(S, wk, 1, t, L);
     if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
   }
+  if (a.dmc == kSJDmcInit) {
+    // DmcRunner::new: weight 1; the local energy of the starting configuration
+    double dlg[kSJNParam];
+    const double E = local_energy<T>(S, wk, t, L, dlg);
+    if (t == 0 && live) {
+      S.dmc[w] = 1.0;
+      S.dmc[(size_t)W + w] = E;
+    }
+  }
   const uint32_t lu_period = a.lu_period;
 
   const SJLaunchState ls = *a.state;
@@ -782,7 +791,20 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
     p += DPM * (uint64_t)N;
     if (t == 0 && live && S.trace_mask && ls.trace_sweeps + sw < a.cap_sweeps)
       S.trace_mask[(size_t)w * a.cap_sweeps + ls.trace_sweeps + sw] = mask;
-    if (sw >= a.sample_start) {
+    if (sw >= a.sample_start && a.dmc == kSJDmcStep) {
+      // DmcRunner::diffuse (src/dmc/src/dmc.rs:88-128): the ensemble terms
+      // with the weight before the step, then
+      // w *= exp(-tau ((E_old + E_new)/2 - E_ref))
+      double dlg[kSJNParam];
+      const double E = local_energy<T>(S, wk, t, L, dlg);
+      if (t == 0 && live) {
+        const double eo = S.dmc[(size_t)W + w], wt = S.dmc[w];
+        S.dmc[2 * (size_t)W + w] = wt * eo;
+        S.dmc[3 * (size_t)W + w] = wt;
+        S.dmc[w] = wt * exp_m(-a.dmc_tau * ((eo + E) / 2.0 - *S.dmc_eref));
+        S.dmc[(size_t)W + w] = E;
+      }
+    } else if (sw >= a.sample_start) {
       double dlg[kSJNParam];
       const double E = local_energy<T>(S, wk, t, L, dlg);
       if (t == 0) {
