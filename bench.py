@@ -211,9 +211,34 @@ def learned_main(args):
                           log=lambda r: print(json.dumps(r), flush=True))
 
 
+def cpu_dmc_sample(walkers=4096, steps=200):
+    """The oracle's DmcRunner (one thread, as the reference's diffuse loop
+    runs) on a bounded sample of a DMC workload."""
+    from oracle import sj_ffi as SJ
+
+    def sample(sj, eref, tau, brancher):
+        wv = SJ.SjWave(sj)
+        d = SJ.Dmc(wv, walkers=walkers, seed=5, tau=tau, reference_energy=eref, brancher=brancher)
+        t0 = time.perf_counter()
+        d.diffuse(tau, steps, 10, 0)
+        dt = time.perf_counter() - t0
+        return dict(value=walkers * steps * sj.n_electrons / dt, unit="walker-moves/s", cores=1, kind="port",
+                    sample=f"{walkers} walkers x {steps} DMC steps on the oracle (parity algorithm: SM moves, "
+                           f"cached local energies), 1 thread, {dt:.2f} s")
+    return sample
+
+
+def dmc_main(args):
+    from mole_amd import workloads
+
+    cpu = None if args.no_cpu_baseline else cpu_dmc_sample()
+    workloads.dmc(walkers=args.walkers or 65536, iters=args.sweeps or 1000, cpu_sample=cpu,
+                  log=lambda r: print(json.dumps(r), flush=True))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero", "learned"], default="h2-sr")
+    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero", "learned", "dmc"], default="h2-sr")
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
@@ -237,6 +262,9 @@ def main():
         return
     if args.workload == "learned":
         learned_main(args)
+        return
+    if args.workload == "dmc":
+        dmc_main(args)
         return
 
     r = gpu_run(args.steps, args.warmup, variant=args.variant, host_loop=args.host_loop)

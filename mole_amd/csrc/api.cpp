@@ -1794,13 +1794,42 @@ mole_status mole_dmc_diffuse(mole_ctx *ctx, double time_step, uint32_t num_itera
     st = sj_upload_sysdev(ctx);
     if (st != MOLE_OK) return st;
   }
-  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
-  for (size_t it = 0; it < steps; ++it) {
+  // one step (sweep launch, then each system's reductions, branching and
+  // bookkeeping) captured once and replayed: every count the kernels need is
+  // in device memory
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  if (steps > 1) {
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
     st = dmc_enqueue_step(ctx, time_step, block_size, num_eq_blocks);
-    if (st != MOLE_OK) return st;
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    if (st != MOLE_OK || e != hipSuccess) {
+      if (graph) (void)hipGraphDestroy(graph);
+      return st != MOLE_OK ? st : fail(MOLE_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(graph);
+      return fail(MOLE_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+  }
+  HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  hipError_t le = hipSuccess;
+  for (size_t it = 0; it < steps && le == hipSuccess; ++it) {
+    if (exec) {
+      le = hipGraphLaunch(exec, ctx->stream);
+    } else {
+      st = dmc_enqueue_step(ctx, time_step, block_size, num_eq_blocks);
+      if (st != MOLE_OK) return st;
+    }
     account(ctx, 1, 0);
   }
-  HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
+  if (le == hipSuccess) le = hipEventRecord(ctx->ev1, ctx->stream);
+  if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
+  if (exec) (void)hipGraphExecDestroy(exec);
+  if (graph) (void)hipGraphDestroy(graph);
+  if (le != hipSuccess) return fail(MOLE_ERR_DEVICE, "DMC steps: %s", hipGetErrorString(le));
   ctx->last_launches = (uint32_t)steps;
   ctx->timed = true;
   HIP_TRY(hipStreamSynchronize(ctx->stream));

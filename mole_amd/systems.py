@@ -67,6 +67,13 @@ def helium_product(zeta=2.0, b=0.0, c=0.3):
                     name="He product")
 
 
+def h2_lcao(R=1.4, zeta=1.0, b=0.5, c=0.5):
+    """H2 on the z axis (nuclei at +-R/2): both electrons in the sigma_g
+    orbital exp(-zeta |r - A|) + exp(-zeta |r - B|), e-e Pade factor."""
+    return SJSystem([(0.0, 0.0, -0.5 * R), (0.0, 0.0, 0.5 * R)], [1, 1], 1, 1, [zeta, zeta], [0, 1], [[1.0, 1.0]],
+                    (b, 0.0, c, 0.0, 1.0), name="H2 LCAO")
+
+
 def poisson_disk(n, min_sep, rng, box=None, max_tries=10000):
     """n points with pairwise distance >= min_sep by dart throwing in a cube
     whose side grows with n (seeded)."""

@@ -236,3 +236,63 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
                energy_diff_sigma=diff / float(np.hypot(lrn["error"], best_kappa["error"])))
     log(rec)
     return rec
+
+
+DMC_EXACT = {"He product": -2.903724, "H2 LCAO": -1.174476}
+
+
+def dmc_systems():
+    """Nodeless trial functions whose local energy stays bounded at the
+    nuclei: He with exp(-2 r) orbitals (the electron-nucleus cusp) and a
+    Pade e-e factor (the electron-electron cusp), and H2 in the sigma_g LCAO
+    orbital; fixed-node DMC is exact for both ground states.  (The random
+    orbitals of the atoms workload break the cusps and have nodes: their
+    local energies diverge, weights overflow, and SRBrancher's total
+    integer weight vanishes -- the reference panics there too.)"""
+    return [systems.helium_product(2.0, 0.5, 0.3), systems.h2_lcao()]
+
+
+def dmc(sys_list=None, walkers=65536, iters=2000, block=100, eq=2, tau=0.01, brancher=M.BRANCH_SR,
+        precision=M.PREC_MIXED, lu_period=64, seed=5, cpu_sample=None, log=print):
+    """DmcRunner::diffuse on the device, one system per context: a VMC
+    estimate as the trial energy, then `iters` DMC steps (each a captured
+    graph of the sweep launch, the reductions, branching and bookkeeping).
+    Rate = walkers x steps x electrons / device time of the diffuse call."""
+    out = []
+    for sj in (sys_list or dmc_systems()):
+        dev = M.slater_jastrow(sj, walkers)
+        table, _ = autotune.tune([sj], lambda s: walkers, candidates=(1, 2, 4, 8), sweeps=4, precision=precision,
+                                 lu_period=lu_period)
+        with M.Context([dev], seed=seed, step=tau, block_size=50, precision=precision, lu_period=lu_period,
+                       proposal=M.PROPOSAL_DIFFUSE) as ctx:
+            autotune.apply(ctx, [sj], table)
+            lanes = ctx.walker_lanes()
+            ctx.init_walkers()
+            ctx.sweep(100)
+            ctx.reset_stats()
+            ctx.run(200, 50)
+            e_vmc, err_vmc = energy_and_error(ctx.stats())
+            ctx.dmc_init(e_vmc, brancher)
+            ctx.dmc_diffuse(tau, 20, 10, 0)  # warm-up (not timed)
+            ctx.reset_stats()
+            t0 = time.perf_counter()
+            e, err = ctx.dmc_diffuse(tau, iters, block, eq)
+            wall = time.perf_counter() - t0
+            dev_ms, _ = ctx.last_timing()
+            w, _ = ctx.dmc_read_walkers()
+            st = ctx.stats()
+        moves = walkers * iters * sj.n_electrons
+        rec = dict(workload="dmc", system=sj.name, N=sj.n_electrons, walkers=walkers, steps=iters, block=block,
+                   eq_blocks=eq, tau=tau, brancher="SR" if brancher == M.BRANCH_SR else "simple",
+                   precision="mixed" if precision == M.PREC_MIXED else "fp64",
+                   vmc_energy=e_vmc, vmc_error=err_vmc,
+                   dmc_energy=float(e[-1]) if len(e) else None, dmc_error=float(err[-1]) if len(err) else None,
+                   exact=DMC_EXACT.get(sj.name),
+                   lanes=lanes, mean_weight=float(np.mean(w)), acceptance=st["n_accepted"] / max(1, st["n_moves"]),
+                   device_seconds=dev_ms * 1e-3, wall_seconds=wall, moves_per_s=moves / (dev_ms * 1e-3),
+                   ms_per_step=dev_ms / iters)
+        if cpu_sample:
+            rec["cpu_baseline"] = cpu_sample(sj, e_vmc, tau, brancher)
+        log(rec)
+        out.append(rec)
+    return out

@@ -129,3 +129,31 @@ def test_dmc_requires_init_and_diffusion():
                    proposal=M.PROPOSAL_DIFFUSE) as ctx:
         with pytest.raises(M.MoleError):
             ctx.dmc_diffuse(0.02, 10, 5, 1)
+
+
+def test_dmc_graph_replays_equal_eager_steps():
+    # a diffuse call of 20 steps replays one captured step 20 times; 20 calls
+    # of one step each run eagerly.  With num_eq_blocks above the block count
+    # E_ref never changes, so both schedules are the same trajectory, bit
+    # for bit, replay after replay
+    sj = S.atom(3)
+    runs = []
+    for mode in ("graph", "eager"):
+        with M.Context([M.slater_jastrow(sj, 64)], seed=2, step=0.02, block_size=10, lu_period=8,
+                       proposal=M.PROPOSAL_DIFFUSE) as ctx:
+            ctx.dmc_init(-7.0, M.BRANCH_SIMPLE)
+            if mode == "graph":
+                ctx.dmc_diffuse(0.02, 20, 1, 100)
+                steps = ctx.dmc_read_steps()
+            else:
+                steps = []
+                for _ in range(20):
+                    ctx.dmc_diffuse(0.02, 1, 1, 100)
+                    steps.append(ctx.dmc_read_steps()[0])
+                steps = np.array(steps)
+            w, el = ctx.dmc_read_walkers()
+            runs.append((steps, w, el, ctx.get_configs()))
+    (gs, gw, ge, gc), (es, ew, ee, ec) = runs
+    assert gs.size == es.size == 20
+    assert np.array_equal(gs, es) and np.array_equal(gw, ew) and np.array_equal(ge, ee)
+    assert np.array_equal(gc, ec)
