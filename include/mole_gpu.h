@@ -227,6 +227,26 @@ mole_status mole_read_sr_sums(mole_ctx *ctx, uint32_t sys, mole_sr_sums *out);
 mole_status mole_read_trace(mole_ctx *ctx, uint32_t sys, uint64_t *accept_mask,
                             double *e_local, size_t walkers, size_t sweeps,
                             size_t samples);
+/* Per-block records of the energy, streamed from the device: the
+ * ensemble form of what Runner::run hands Log::log once per block
+ * (src/montecarlo/src/montecarlo.rs:31-42, src/montecarlo/src/traits.rs:
+ * 43-47) and of the block means VmcRunner::process_monte_carlo_results
+ * forms (src/vmc/src/vmc.rs:150-170).  Block b covers samples
+ * [b*block_size, (b+1)*block_size) since the last reset; a walker's block
+ * mean is its samples summed in order times 1/block_size (the accumulators'
+ * arithmetic), summed over walkers in a fixed order. */
+typedef struct {
+  uint64_t block;          /* index since the last statistics reset */
+  uint64_t n_walkers;
+  double mean;             /* sum_block_mean / n_walkers */
+  double sum_block_mean;
+  double sum_block_mean2;
+} mole_block_record;
+/* Records of the complete blocks [first, first + k), k = min(cap, blocks
+ * complete - first); *n = k.  out may be NULL to count.  Needs
+ * MOLE_FLAG_TRACE (records up to record_sweeps samples). */
+mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_block_record *out, size_t cap,
+                             size_t *n);
 /* MonteCarloSampler::data (src/montecarlo/src/traits.rs:27) in compat
  * form: "Energy", "Wavefunction value" and "Parameter gradient" samples as
  * [walker][sample] (and [walker][sample][param]), the order in which

@@ -141,6 +141,16 @@ class SrSums(C.Structure):
     ]
 
 
+class BlockRecord(C.Structure):
+    _fields_ = [
+        ("block", C.c_uint64),
+        ("n_walkers", C.c_uint64),
+        ("mean", C.c_double),
+        ("sum_block_mean", C.c_double),
+        ("sum_block_mean2", C.c_double),
+    ]
+
+
 class OptimizerState(C.Structure):
     """mole_optimizer: the state of a stateful optimizer (plain data)."""
     _fields_ = [
@@ -205,6 +215,8 @@ _SIGS = [
     ("mole_optimizer_init", C.c_int32, [C.POINTER(OptimizerState), C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                         C.c_uint32, C.c_uint32, C.c_double]),
     ("mole_optimizer_update", C.c_int32, [C.POINTER(OptimizerState), C.POINTER(SrSums), _pd, _pd, _pd]),
+    ("mole_read_blocks", C.c_int32, [C.c_void_p, C.c_uint32, C.c_size_t, C.POINTER(BlockRecord), C.c_size_t,
+                                     C.POINTER(C.c_size_t)]),
     ("mole_dmc_init", C.c_int32, [C.c_void_p, _pd, C.c_uint32]),
     ("mole_dmc_diffuse", C.c_int32, [C.c_void_p, C.c_double, C.c_uint32, C.c_uint32, C.c_uint32]),
     ("mole_dmc_read", C.c_int32, [C.c_void_p, C.c_uint32, _pd, _pd, C.c_size_t, C.POINTER(C.c_size_t), _pd]),
@@ -465,6 +477,19 @@ class Context:
         p = np.zeros(self.systems[0].params.size)
         check(lib().mole_optimize_with(self.h, iters, steps, block_size, C.byref(opt.state), _dp(log), _dp(p)))
         return p, log
+
+    def read_blocks(self, first=0, sys=0):
+        """mole_read_blocks: per-block energy records from block `first` on,
+        as a structured array (block, n_walkers, mean, sum_block_mean,
+        sum_block_mean2)."""
+        n = C.c_size_t()
+        check(lib().mole_read_blocks(self.h, sys, first, None, 0, C.byref(n)))
+        recs = (BlockRecord * max(1, n.value))()
+        if n.value:
+            check(lib().mole_read_blocks(self.h, sys, first, recs, n.value, C.byref(n)))
+        dt = np.dtype([("block", np.uint64), ("n_walkers", np.uint64), ("mean", np.float64),
+                       ("sum_block_mean", np.float64), ("sum_block_mean2", np.float64)])
+        return np.frombuffer(bytes(recs), dtype=dt, count=n.value).copy()
 
     # ---- diffusion Monte Carlo (DmcRunner, src/dmc/src/dmc.rs) ----
 

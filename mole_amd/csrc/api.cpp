@@ -952,6 +952,38 @@ mole_status mole_read_trace(mole_ctx *ctx, uint32_t sys, uint64_t *accept_mask, 
   return ok();
 }
 
+mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_block_record *out, size_t cap,
+                             size_t *n) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!n) return fail(MOLE_ERR_INVALID_ARGUMENT, "null count");
+  if (!(ctx->run.flags & MOLE_FLAG_TRACE)) return fail(MOLE_ERR_DATA_ACCESS, "context has no trace");
+  SysState &s = ctx->sys[sys];
+  const size_t B = ctx->run.block_size, rec = ctx->run.record_sweeps;
+  const size_t samples = std::min((size_t)ctx->trace_samples, rec);
+  const size_t complete = samples / B;
+  const size_t k = first < complete ? std::min(cap, complete - first) : 0;
+  *n = out ? k : (first < complete ? complete - first : 0);
+  if (!out || k == 0) return ok();
+  HIP_TRY(hipSetDevice(ctx->device));
+  double *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * k));
+  hipError_t e = launch_block_records(s.d_trace_e, s.W, rec, B, first, k, d, ctx->stream);
+  std::vector<double> h(2 * k);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * 2 * k, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_read_blocks: %s", hipGetErrorString(e));
+  for (size_t i = 0; i < k; ++i) {
+    out[i].block = first + i;
+    out[i].n_walkers = (uint64_t)s.W;
+    out[i].sum_block_mean = h[2 * i];
+    out[i].sum_block_mean2 = h[2 * i + 1];
+    out[i].mean = h[2 * i] / (double)s.W;
+  }
+  return ok();
+}
+
 mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, double *psi, double *pgrad,
                               size_t walkers, size_t samples) {
   mole_status st = check_ctx(ctx, sys);

@@ -74,6 +74,10 @@ bool sweep2_wavespec_eligible(const Sys2 &sp);
 // (red[kAccColumns2]) of n_samples samples: params += dp (same rounding as
 // mole_compute_parameter_update); log[0..7] = sum_e, sum_e2, sum of block
 // means, sum of squared block means, accepted moves, status, n_samples, 0.
+// per block b in [first, first + k): (sum over walkers of block means, sum
+// of their squares) from the E_L trace [W][cap], blocks of B samples
+hipError_t launch_block_records(const double *trace_e, int W, size_t cap, size_t B, size_t first, size_t k,
+                                double *out, hipStream_t stream);
 hipError_t launch_vmc_step(const double *red, double n_samples, double *params, mole_optimizer *opt, double *log,
                            hipStream_t stream);
 hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads,

@@ -565,6 +565,38 @@ __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *acc, 
   if (threadIdx.x == 0) out[col] = sh[0];
 }
 
+// One workgroup per block: per walker the block's samples summed in order
+// times 1/B, then summed over walkers (strided per thread, LDS tree).
+__global__ void __launch_bounds__(256) block_records_kernel(const double *trace_e, int W, size_t cap, size_t B,
+                                                            size_t first, double *out) {
+  __shared__ double s1[256], s2[256];
+  const size_t b = first + blockIdx.x;
+  const double inv = 1.0 / (double)B;
+  double a1 = 0.0, a2 = 0.0;
+  for (int w = threadIdx.x; w < W; w += 256) {
+    const double *e = trace_e + (size_t)w * cap + b * B;
+    double s = 0.0;
+    for (size_t k = 0; k < B; ++k) s += e[k];
+    const double bm = s * inv;
+    a1 += bm;
+    a2 = fma(bm, bm, a2);
+  }
+  s1[threadIdx.x] = a1;
+  s2[threadIdx.x] = a2;
+  __syncthreads();
+  for (int off = 128; off > 0; off >>= 1) {
+    if (threadIdx.x < off) {
+      s1[threadIdx.x] += s1[threadIdx.x + off];
+      s2[threadIdx.x] += s2[threadIdx.x + off];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    out[2 * blockIdx.x] = s1[0];
+    out[2 * blockIdx.x + 1] = s2[0];
+  }
+}
+
 __global__ void fill_kernel(double *p, size_t n, double v) {
   const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) p[i] = v;
@@ -655,6 +687,12 @@ __global__ void vmc_step_kernel(const double *red, double n_samples, double *par
     for (int k = 0; k < np; ++k) params[k] = pars[k] + dp[k];
 }
 }  // namespace
+
+hipError_t launch_block_records(const double *trace_e, int W, size_t cap, size_t B, size_t first, size_t k,
+                                double *out, hipStream_t stream) {
+  hipLaunchKernelGGL(block_records_kernel, dim3((unsigned)k), dim3(256), 0, stream, trace_e, W, cap, B, first, out);
+  return hipGetLastError();
+}
 
 hipError_t launch_vmc_step(const double *red, double n_samples, double *params, mole_optimizer *opt, double *log,
                            hipStream_t stream) {

@@ -10,6 +10,8 @@
 // The call each one replaces is named at the class.
 #pragma once
 
+#include <functional>
+
 #include <cstring>
 #include <memory>
 #include <string>
@@ -29,6 +31,15 @@ struct FusedSampler {
   // steps sweeps in blocks of block_size, the first block discarded
   // (montecarlo.rs:24-46); statistics restart at the call
   virtual Status run_fused(size_t steps, size_t block_size) = 0;
+  // the same, calling after_block() once every sampled block has finished
+  // with data() holding the blocks so far (the cadence at which Runner::run
+  // prints Log::log's output, montecarlo.rs:31-42); false if the sampler
+  // does not stream blocks
+  virtual bool streams_blocks() const { return false; }
+  virtual Status run_fused_streaming(size_t steps, size_t block_size, const std::function<void()> &after_block) {
+    (void)after_block;
+    return run_fused(steps, block_size);
+  }
   // energy mean, blocking error and the SR sums of the last run
   virtual Result<mole_stats> stats() const = 0;
   virtual Result<mole_sr_sums> sr_sums() const = 0;
@@ -196,8 +207,13 @@ class GpuWaveFunction : public Function, public Differentiate, public WaveFuncti
 template <typename WF>
 class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
  public:
-  GpuSampler(WF wf, uint32_t walkers, double box_side, uint64_t seed, uint32_t block_size = 1)
-      : wf_(std::move(wf)), walkers_(walkers), side_(box_side), seed_(seed), block_(block_size) {
+  // record_sweeps > 0 keeps the local-energy trace of that many samples
+  // (MOLE_FLAG_TRACE), so data() also holds "Energy block means" (one
+  // scalar per finished block, mole_read_blocks) and runs stream blocks
+  GpuSampler(WF wf, uint32_t walkers, double box_side, uint64_t seed, uint32_t block_size = 1,
+             uint32_t record_sweeps = 0)
+      : wf_(std::move(wf)), walkers_(walkers), side_(box_side), seed_(seed), block_(block_size),
+        record_(record_sweeps) {
     open();
   }
 
@@ -271,6 +287,46 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
     if (st.is_err()) return st;
     return refresh_data();
   }
+  bool streams_blocks() const override { return record_ > 0; }
+  Status run_fused_streaming(size_t steps, size_t block_size, const std::function<void()> &after_block) override {
+    if (!record_) return run_fused(steps, block_size);
+    if (steps < 2 * block_size) throw Panic("assertion failed: steps >= 2 * block_size");
+    if (steps - block_size > record_)
+      return Error{ErrorKind::ShapeError, "more samples than the sampler records"};
+    if (block_size != block_) {
+      block_ = (uint32_t)block_size;
+      open_keep_state(false);
+    }
+    Status st = flush();
+    if (st.is_err()) return st;
+    st = push_params();
+    if (st.is_err()) return st;
+    st = status(mole_reset_stats(ctx_->get()));
+    if (st.is_err()) return st;
+    // the first block unsampled, then one launch per block
+    st = status(mole_sweep(ctx_->get(), (uint32_t)block_size, 0));
+    if (st.is_err()) return st;
+    for (size_t b = 1; b < steps / block_size; ++b) {
+      st = status(mole_sweep(ctx_->get(), (uint32_t)block_size, MOLE_SWEEP_SAMPLE));
+      if (st.is_err()) return st;
+      st = refresh_data();
+      if (st.is_err()) return st;
+      if (after_block) after_block();
+    }
+    return Ok();
+  }
+  // the per-block records since the last statistics reset
+  Result<std::vector<mole_block_record>> block_records() const {
+    size_t n = 0;
+    Status st = status(mole_read_blocks(ctx_->get(), 0, 0, nullptr, 0, &n));
+    if (st.is_err()) return st.error();
+    std::vector<mole_block_record> r(n);
+    if (n) {
+      st = status(mole_read_blocks(ctx_->get(), 0, 0, r.data(), n, &n));
+      if (st.is_err()) return st.error();
+    }
+    return r;
+  }
   Result<mole_stats> stats() const override {
     mole_stats s{};
     const Status st = status(mole_read_stats(ctx_->get(), 0, &s));
@@ -287,7 +343,7 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
  private:
   void open() {
     desc_ = wf_.desc(walkers_);
-    ctx_ = std::make_shared<gpu::Context>(std::vector<mole_system_desc>{desc_}, gpu::run_desc(seed_, side_, block_));
+    ctx_ = std::make_shared<gpu::Context>(std::vector<mole_system_desc>{desc_}, run());
     if (ctx_->ok()) (void)mole_init_walkers(ctx_->get());
   }
   // a new block size or seed needs a new context; the walkers carry over
@@ -299,11 +355,14 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
     uint64_t pos = 0;
     (void)mole_get_stream(ctx_->get(), &epoch, &pos);
     desc_ = wf_.desc(walkers_);
-    ctx_ = std::make_shared<gpu::Context>(std::vector<mole_system_desc>{desc_}, gpu::run_desc(seed_, side_, block_));
+    ctx_ = std::make_shared<gpu::Context>(std::vector<mole_system_desc>{desc_}, run());
     if (ctx_->ok()) {
       (void)mole_set_configs(ctx_->get(), 0, cfg.data(), walkers_);
       if (!restart_stream) (void)mole_set_stream(ctx_->get(), epoch, pos);
     }
+  }
+  mole_run_desc run() const {
+    return gpu::run_desc(seed_, side_, block_, record_ ? MOLE_FLAG_TRACE : 0u, record_);
   }
   Status flush() {
     if (!pending_) return Ok();
@@ -335,6 +394,13 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
       }
     }
     data_["SR sums"] = {OperatorValue::matrix(pack_sums(r))};
+    if (record_) {
+      const Result<std::vector<mole_block_record>> recs = block_records();
+      if (recs.is_err()) return recs.error();
+      std::vector<OperatorValue> bm;
+      for (const mole_block_record &b : recs.value()) bm.push_back(OperatorValue::scalar(b.mean));
+      data_["Energy block means"] = bm;
+    }
     return Ok();
   }
 
@@ -371,6 +437,7 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
   double side_;
   uint64_t seed_;
   uint32_t block_;
+  uint32_t record_ = 0;
   mole_system_desc desc_{};
   std::shared_ptr<gpu::Context> ctx_;
   DataMap data_;

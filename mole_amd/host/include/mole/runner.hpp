@@ -31,6 +31,14 @@ class Runner {
   Result<MonteCarloResult<typename S::WaveFunc>> run(size_t steps, size_t block_size) {
     if (!(steps >= 2 * block_size)) throw Panic("assertion failed: steps >= 2 * block_size");
     if (auto *f = dynamic_cast<FusedSampler *>(&sampler_)) {
+      if (f->streams_blocks()) {  // Log::log after every sampled block, as the reference prints
+        const Status st = f->run_fused_streaming(steps, block_size, [this] {
+          const std::string out = logger_.log(sampler_.data());
+          if (!out.empty()) std::printf("%s\n", out.c_str());
+        });
+        if (st.is_err()) return st.error();
+        return sampler_.consume_result();
+      }
       const Status st = f->run_fused(steps, block_size);
       if (st.is_err()) return st.error();
       const std::string out = logger_.log(sampler_.data());

@@ -187,6 +187,26 @@ static int run_gpu(int iters, int walkers, int sweeps, int block, double box, do
     return 3;
   }
   const OptimizationResult &r = out.value();
+  // a streaming Runner::run: Log::log once per sampled block with the block
+  // means so far (mole_read_blocks)
+  struct CountLog : Log {
+    size_t calls = 0, last_blocks = 0;
+    std::string log(const DataMap &d) override {
+      ++calls;
+      const auto it = d.find("Energy block means");
+      last_blocks = it == d.end() ? 0 : it->second.size();
+      return std::string();
+    }
+  } counter;
+  GpuSampler<GpuWaveFunction> streaming(GpuWaveFunction::hydrogen_molecule(1.0, 0.5, 0.3), 256u, box, seed,
+                                        (uint32_t)block, (uint32_t)(5 * block));
+  Runner<GpuSampler<GpuWaveFunction>> runner(std::move(streaming), counter);
+  auto mc = runner.run((size_t)(6 * block), (size_t)block);
+  if (mc.is_err()) {
+    std::fprintf(stderr, "streaming run: %s\n", mc.error().message.c_str());
+    return 4;
+  }
+  const size_t n_bm = mc.value().data.at("Energy block means").size();
   // single-configuration evaluation through the traits
   GpuWaveFunction he = GpuWaveFunction::helium(1.6875, 0.5, 0.3);
   Array2 x(2, 3);
@@ -199,7 +219,8 @@ static int run_gpu(int iters, int walkers, int sweeps, int block, double box, do
   for (size_t k = 0; k < r.energy_errors.size(); ++k) std::printf("%s%.17g", k ? ", " : "", r.energy_errors[k]);
   std::printf("], \"he_value\": %.17g, \"he_laplacian\": %.17g, \"he_local_energy\": %.17g, \"he_grad\": [", v, lap, el);
   for (size_t k = 0; k < 6; ++k) std::printf("%s%.17g", k ? ", " : "", g.data[k]);
-  std::printf("]}\n");
+  std::printf("], \"stream_calls\": %zu, \"stream_blocks\": %zu, \"result_blocks\": %zu}\n", counter.calls,
+              counter.last_blocks, n_bm);
   return 0;
 }
 

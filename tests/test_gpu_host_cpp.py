@@ -45,3 +45,6 @@ def test_cpp_vmc_runner_matches_python_schedule():
     assert out["he_laplacian"] == e["lap"][0]
     assert out["he_local_energy"] == e["e_local"][0]
     assert np.array_equal(np.array(out["he_grad"]), e["grad"].reshape(-1))
+    # the streaming Runner::run of 6 blocks: Log::log after each of the 5
+    # sampled blocks, the last call seeing all 5 block means
+    assert out["stream_calls"] == 5 and out["stream_blocks"] == 5 and out["result_blocks"] == 5
