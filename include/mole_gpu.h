@@ -117,7 +117,13 @@ typedef struct {
   const uint32_t *basis_center;  /* [n_basis] nucleus index */
   uint32_t n_orbitals;           /* >= max(n_up, n_down) */
   const double *orbital_coeff;   /* [n_orbitals][n_basis] */
+  const uint32_t *basis_kind;    /* [n_basis] MOLE_BASIS_*; NULL: all STO */
 } mole_system_desc;
+
+/* Slater-Jastrow basis functions about their centre R: */
+#define MOLE_BASIS_STO 0 /* exp(-zeta |r - R|) */
+#define MOLE_BASIS_GTO 1 /* exp(-zeta |r - R|^2) (the Gaussian trial functions of
+                            examples/custom_operator.rs:64-100, examples/dmc.rs:33-70) */
 
 /* How the walkers move and what they record. */
 typedef struct {
@@ -359,6 +365,54 @@ mole_status mole_optimizer_update(mole_optimizer *opt, const mole_sr_sums *sums,
  * once per iteration there, and copied back (its state after the loop). */
 mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size,
                                mole_optimizer *opt, double *log, double *params_out);
+
+/* ---- local operators on the device --------------------------------------
+ * LocalOperator<T>::act_on(wf, cfg) / psi (src/operator/src/traits.rs:
+ * 263-265, as Sampler::sample divides, src/montecarlo/src/samplers.rs:81-104)
+ * for operators built from these terms, evaluated per walker on the
+ * device from the same derivatives as the local energy (Slater-Jastrow
+ * systems):
+ *   KINETIC               -1/2 lap(psi)/psi       KineticEnergy, operator.rs:96-121
+ *   IONIC_POTENTIAL       -sum_{I,i} Z_I/|r_i - R_I| + ion-ion repulsion
+ *                                                  IonicPotential, operator.rs:12-64
+ *   ELECTRONIC_POTENTIAL  sum_{i<j} 1/r_ij        ElectronicPotential, operator.rs:66-94
+ *   HARMONIC              1/2 w^2 sum_i |r_i - c|^2, w = param[0], c = param[1..3]
+ *                                                  (the potential of examples/custom_operator.rs:48-61)
+ *   RADIAL_POWER          sum_i |r_i - c|^k, k = param[0] in -4..4, k != 0, c = param[1..3]
+ *   DIPOLE                sum_i (r_i - c).u, c = param[0..2], u = param[3..5]
+ * An operator is sum_k coef_k term_k (at most MOLE_OPERATOR_MAX_TERMS terms),
+ * e.g. HarmonicHamiltonian = KINETIC + HARMONIC, ElectronicHamiltonian =
+ * KINETIC + IONIC_POTENTIAL + ELECTRONIC_POTENTIAL. */
+#define MOLE_TERM_KINETIC 0
+#define MOLE_TERM_IONIC_POTENTIAL 1
+#define MOLE_TERM_ELECTRONIC_POTENTIAL 2
+#define MOLE_TERM_HARMONIC 3
+#define MOLE_TERM_RADIAL_POWER 4
+#define MOLE_TERM_DIPOLE 5
+#define MOLE_OPERATOR_MAX_TERMS 4
+#define MOLE_MAX_OBSERVABLES 8
+typedef struct {
+  uint32_t kind; /* MOLE_TERM_* */
+  double coef;
+  double param[6];
+} mole_term;
+typedef struct {
+  uint32_t n_terms;
+  mole_term term[MOLE_OPERATOR_MAX_TERMS];
+} mole_operator;
+/* The operator whose local values are the system's "Energy": the energy
+ * statistics, the SR sums, traces, block records and DMC weights use it
+ * instead of the descriptor's hamiltonian (the operators! "Energy" entry of
+ * examples/custom_operator.rs:124-126).  NULL restores the built-in one. */
+mole_status mole_set_hamiltonian(mole_ctx *ctx, uint32_t sys, const mole_operator *h);
+/* Further observables, sampled with the energy after every sampled sweep
+ * (VMC): per walker sum O and sum O^2 since the last reset.  n = 0 clears
+ * them. */
+mole_status mole_set_observables(mole_ctx *ctx, uint32_t sys, const mole_operator *obs, uint32_t n);
+/* sum over samples of each observable and of its square, walkers summed in
+ * a fixed order; *n_samples the sample count. */
+mole_status mole_read_observables(mole_ctx *ctx, uint32_t sys, double *sum, double *sum2, uint32_t n,
+                                  uint64_t *n_samples);
 
 /* ---- diffusion Monte Carlo -----------------------------------------------
  * DmcRunner (src/dmc/src/dmc.rs:16-203) for every Slater-Jastrow system of a

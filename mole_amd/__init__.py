@@ -46,6 +46,16 @@ VARIANT_LANE = 1
 VARIANT_WAVESPEC = 2
 MAX_PARAMS = 16
 
+BASIS_STO = 0
+BASIS_GTO = 1
+TERM_KINETIC = 0
+TERM_IONIC_POTENTIAL = 1
+TERM_ELECTRONIC_POTENTIAL = 2
+TERM_HARMONIC = 3
+TERM_RADIAL_POWER = 4
+TERM_DIPOLE = 5
+OPERATOR_MAX_TERMS = 4
+MAX_OBSERVABLES = 8
 OPT_SD = 0
 OPT_SR = 1
 OPT_MOMENTUM = 2
@@ -99,6 +109,7 @@ class SystemDesc(C.Structure):
         ("basis_center", C.POINTER(C.c_uint32)),
         ("n_orbitals", C.c_uint32),
         ("orbital_coeff", C.POINTER(C.c_double)),
+        ("basis_kind", C.POINTER(C.c_uint32)),
     ]
 
 
@@ -139,6 +150,36 @@ class SrSums(C.Structure):
         ("sum_oe", C.c_double * MAX_PARAMS),
         ("sum_oo", C.c_double * (MAX_PARAMS * MAX_PARAMS)),
     ]
+
+
+class Term(C.Structure):
+    _fields_ = [("kind", C.c_uint32), ("coef", C.c_double), ("param", C.c_double * 6)]
+
+
+class Operator(C.Structure):
+    """mole_operator: sum of coef * term (see TERM_*)."""
+    _fields_ = [("n_terms", C.c_uint32), ("term", Term * OPERATOR_MAX_TERMS)]
+
+    @classmethod
+    def of(cls, terms):
+        """[(kind, coef, *params)] -> Operator."""
+        op = cls(n_terms=len(terms))
+        for k, t in enumerate(terms):
+            op.term[k].kind = int(t[0])
+            op.term[k].coef = float(t[1])
+            for q, v in enumerate(t[2:]):
+                op.term[k].param[q] = float(v)
+        return op
+
+
+def harmonic_hamiltonian(omega):
+    """HarmonicHamiltonian of examples/custom_operator.rs:30-62: T + 1/2 w^2 |x|^2."""
+    return [(TERM_KINETIC, 1.0), (TERM_HARMONIC, 1.0, omega, 0.0, 0.0, 0.0)]
+
+
+def electronic_hamiltonian():
+    """ElectronicHamiltonian (operator.rs:151-183) as terms."""
+    return [(TERM_KINETIC, 1.0), (TERM_IONIC_POTENTIAL, 1.0), (TERM_ELECTRONIC_POTENTIAL, 1.0)]
 
 
 class BlockRecord(C.Structure):
@@ -215,6 +256,9 @@ _SIGS = [
     ("mole_optimizer_init", C.c_int32, [C.POINTER(OptimizerState), C.c_uint32, C.c_uint32, C.c_double, C.c_double,
                                         C.c_uint32, C.c_uint32, C.c_double]),
     ("mole_optimizer_update", C.c_int32, [C.POINTER(OptimizerState), C.POINTER(SrSums), _pd, _pd, _pd]),
+    ("mole_set_hamiltonian", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Operator)]),
+    ("mole_set_observables", C.c_int32, [C.c_void_p, C.c_uint32, C.POINTER(Operator), C.c_uint32]),
+    ("mole_read_observables", C.c_int32, [C.c_void_p, C.c_uint32, _pd, _pd, C.c_uint32, C.POINTER(C.c_uint64)]),
     ("mole_read_blocks", C.c_int32, [C.c_void_p, C.c_uint32, C.c_size_t, C.POINTER(BlockRecord), C.c_size_t,
                                      C.POINTER(C.c_size_t)]),
     ("mole_dmc_init", C.c_int32, [C.c_void_p, _pd, C.c_uint32]),
@@ -304,8 +348,12 @@ class System:
         self.n_electrons = n_up + n_down
         self.n_walkers = n_walkers
 
-    def with_basis(self, zeta, center, coef):
-        """Attach the s-type STO basis and orbital coefficients (Slater-Jastrow)."""
+    def with_basis(self, zeta, center, coef, kind=None):
+        """Attach the s-type basis (STO, or GTO where kind[m] = BASIS_GTO) and the
+        orbital coefficients (Slater-Jastrow)."""
+        if kind is not None:
+            self.kind = np.ascontiguousarray(kind, dtype=np.uint32)
+            self.desc.basis_kind = self.kind.ctypes.data_as(C.POINTER(C.c_uint32))
         self.zeta = np.ascontiguousarray(zeta, dtype=np.float64)
         self.center = np.ascontiguousarray(center, dtype=np.uint32)
         self.coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1, self.zeta.size)
@@ -322,7 +370,7 @@ def slater_jastrow(sj, n_walkers, hamiltonian=H_ELECTRONIC, params=None):
     p = sj.jastrow if params is None else params
     s = System(WF_SLATER_JASTROW, sj.nuclei, sj.charges, p, n_walkers, n_up=sj.n_up,
                n_down=sj.n_dn, hamiltonian=hamiltonian)
-    return s.with_basis(sj.zeta, sj.center, sj.coef)
+    return s.with_basis(sj.zeta, sj.center, sj.coef, getattr(sj, "kind", None))
 
 
 def helium(n_walkers, zeta=1.6875, b=0.5, c=0.3, hamiltonian=H_ELECTRONIC):
@@ -477,6 +525,33 @@ class Context:
         p = np.zeros(self.systems[0].params.size)
         check(lib().mole_optimize_with(self.h, iters, steps, block_size, C.byref(opt.state), _dp(log), _dp(p)))
         return p, log
+
+    # ---- local operators on the device ----
+
+    def set_hamiltonian(self, terms, sys=0):
+        """mole_set_hamiltonian: the "Energy" operator as [(kind, coef, *params)]; None restores
+        the built-in Hamiltonian."""
+        if terms is None:
+            check(lib().mole_set_hamiltonian(self.h, sys, None))
+        else:
+            op = Operator.of(terms)
+            check(lib().mole_set_hamiltonian(self.h, sys, C.byref(op)))
+
+    def set_observables(self, operators, sys=0):
+        """mole_set_observables: a list of operators (each a list of terms)."""
+        n = len(operators)
+        arr = (Operator * max(1, n))(*[Operator.of(t) for t in operators])
+        check(lib().mole_set_observables(self.h, sys, arr if n else None, n))
+        self._n_obs = getattr(self, "_n_obs", {})
+        self._n_obs[sys] = n
+
+    def read_observables(self, sys=0):
+        """(sum, sum of squares, n_samples) of the observables."""
+        n = getattr(self, "_n_obs", {}).get(sys, 0)
+        s1, s2 = np.zeros(max(1, n)), np.zeros(max(1, n))
+        ns = C.c_uint64()
+        check(lib().mole_read_observables(self.h, sys, _dp(s1), _dp(s2), n, C.byref(ns)))
+        return s1[:n], s2[:n], ns.value
 
     def read_blocks(self, first=0, sys=0):
         """mole_read_blocks: per-block energy records from block `first` on,

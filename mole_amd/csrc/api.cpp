@@ -81,6 +81,13 @@ struct SysState {
   void *d_B = nullptr;
   double v_nn = 0.0;
   uint64_t draw_base = 0, move_base = 0;
+  bool any_gto = false;
+  // local operators: ops[0] the custom Hamiltonian, ops[1..] the observables
+  std::vector<mole_operator> ops = std::vector<mole_operator>(1);
+  bool custom_h = false;
+  int n_obs = 0;
+  mole_operator *d_ops = nullptr;
+  double *d_obs_acc = nullptr;  // [2 MOLE_MAX_OBSERVABLES][W]
   // diffusion Monte Carlo (allocated by mole_dmc_init)
   struct Dmc {
     double *dmc = nullptr;  // [4][W]
@@ -235,6 +242,14 @@ mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run
       for (uint32_t m = 0; m < d.n_basis; ++m)
         if (d.basis_center[m] >= d.n_nuclei)
           return fail(MOLE_ERR_SHAPE, "system %u: basis function %u centred on missing nucleus", s, m);
+      if (d.basis_kind)
+        for (uint32_t m = 0; m < d.n_basis; ++m) {
+          if (d.basis_kind[m] > MOLE_BASIS_GTO)
+            return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: basis function %u has unknown kind %u", s, m,
+                        d.basis_kind[m]);
+          if (d.basis_kind[m] == MOLE_BASIS_GTO && run->proposal == MOLE_PROPOSAL_LEARNED)
+            return fail(MOLE_ERR_UNSUPPORTED, "system %u: the learned proposal takes STO bases only", s);
+        }
       if (run->flags & MOLE_FLAG_SAMPLES)
         return fail(MOLE_ERR_UNSUPPORTED, "per-sample compat data is for the two-electron kinds");
       if (run->o_convention != MOLE_O_LOGDERIV)
@@ -278,6 +293,8 @@ void free_ctx(mole_ctx *ctx) {
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     free_dmc(st.dmc);
+    if (st.d_ops) (void)hipFree(st.d_ops);
+    if (st.d_obs_acc) (void)hipFree(st.d_obs_acc);
   }
   if (ctx->d_red) (void)hipFree(ctx->d_red);
   if (ctx->d_params) (void)hipFree(ctx->d_params);
@@ -368,6 +385,11 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.lds_per_walker = (uint32_t)st.lds_walker;
     d.dmc = st.dmc.dmc;
     d.dmc_eref = st.dmc.st ? &st.dmc.st->eref : nullptr;
+    d.kind = st.d_center + st.n_basis;
+    d.ops = st.d_ops;
+    d.custom_h = st.custom_h ? 1 : 0;
+    d.n_obs = st.n_obs;
+    d.obs_acc = st.d_obs_acc;
   }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   HIP_TRY(hipMemcpy(ctx->d_sysdev, h.data(), sizeof(SJSysDev) * h.size(), hipMemcpyHostToDevice));
@@ -666,7 +688,10 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       tab.insert(tab.end(), d.basis_exponent, d.basis_exponent + d.n_basis);
       for (uint32_t m = 0; m < d.n_basis; ++m)
         for (uint32_t k = 0; k < d.n_orbitals; ++k) tab.push_back(d.orbital_coeff[(size_t)k * d.n_basis + m]);
+      // centres, then the basis kinds (MOLE_BASIS_*)
       std::vector<int> cen(d.basis_center, d.basis_center + d.n_basis);
+      for (uint32_t m = 0; m < d.n_basis; ++m) cen.push_back(d.basis_kind ? (int)d.basis_kind[m] : MOLE_BASIS_STO);
+      S.any_gto = std::any_of(cen.begin() + d.n_basis, cen.end(), [](int k) { return k == MOLE_BASIS_GTO; });
       CTX_TRY(hipMalloc(&S.d_tab, sizeof(double) * tab.size()));
       CTX_TRY(hipMemcpy(S.d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
       CTX_TRY(hipMalloc(&S.d_center, sizeof(int) * cen.size()));
@@ -867,6 +892,7 @@ mole_status mole_reset_stats(mole_ctx *ctx) {
   HIP_TRY(hipSetDevice(ctx->device));
   for (SysState &s : ctx->sys) {
     HIP_TRY(launch_fill(s.d_acc, (size_t)s.n_cols * s.W, 0.0, ctx->stream));
+    if (s.d_obs_acc) HIP_TRY(launch_fill(s.d_obs_acc, 2 * (size_t)MOLE_MAX_OBSERVABLES * s.W, 0.0, ctx->stream));
     s.n_samples = s.n_blocks = s.n_moves = 0;
     s.cols_valid = false;
   }
@@ -1295,6 +1321,7 @@ mole_status mlp_pack(mole_mlp *m) {
 mole_status mlp_check_system(const SysState &s) {
   if (s.N > 16 || s.n_up > 8 || s.n_down > 8)
     return fail(MOLE_ERR_UNSUPPORTED, "learned-proposal training supports N <= 16 and spin blocks <= 8");
+  if (s.any_gto) return fail(MOLE_ERR_UNSUPPORTED, "learned-proposal training takes STO bases only");
   return MOLE_OK;
 }
 
@@ -1922,5 +1949,97 @@ mole_status mole_dmc_read_walkers(mole_ctx *ctx, uint32_t sys, double *weights, 
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   if (weights) HIP_TRY(hipMemcpy(weights, s.dmc.dmc, sizeof(double) * s.W, hipMemcpyDeviceToHost));
   if (e_local) HIP_TRY(hipMemcpy(e_local, s.dmc.dmc + s.W, sizeof(double) * s.W, hipMemcpyDeviceToHost));
+  return ok();
+}
+
+// ---- local operators -----------------------------------------------------------
+
+namespace {
+
+mole_status check_operator(const mole_operator &op) {
+  if (op.n_terms < 1 || op.n_terms > MOLE_OPERATOR_MAX_TERMS)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "an operator has 1..%d terms", MOLE_OPERATOR_MAX_TERMS);
+  for (uint32_t k = 0; k < op.n_terms; ++k) {
+    const mole_term &t = op.term[k];
+    if (t.kind > MOLE_TERM_DIPOLE) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown term kind %u", t.kind);
+    if (t.kind == MOLE_TERM_RADIAL_POWER) {
+      const double p = t.param[0];
+      if (!(p == std::floor(p)) || p == 0.0 || p < -4.0 || p > 4.0)
+        return fail(MOLE_ERR_INVALID_ARGUMENT, "RADIAL_POWER needs an integer power in -4..4, not 0");
+    }
+  }
+  return MOLE_OK;
+}
+
+mole_status push_ops(mole_ctx *ctx, SysState &s) {
+  HIP_TRY(hipSetDevice(ctx->device));
+  if (!s.d_ops) HIP_TRY(hipMalloc(&s.d_ops, sizeof(mole_operator) * (1 + MOLE_MAX_OBSERVABLES)));
+  if (!s.d_obs_acc) {
+    HIP_TRY(hipMalloc(&s.d_obs_acc, sizeof(double) * 2 * MOLE_MAX_OBSERVABLES * (size_t)s.W));
+    HIP_TRY(launch_fill(s.d_obs_acc, 2 * (size_t)MOLE_MAX_OBSERVABLES * s.W, 0.0, ctx->stream));
+  }
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  HIP_TRY(hipMemcpy(s.d_ops, s.ops.data(), sizeof(mole_operator) * s.ops.size(), hipMemcpyHostToDevice));
+  ctx->sysdev_dirty = true;
+  return MOLE_OK;
+}
+
+}  // namespace
+
+mole_status mole_set_hamiltonian(mole_ctx *ctx, uint32_t sys, const mole_operator *h) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "local operators run on Slater-Jastrow systems");
+  SysState &s = ctx->sys[sys];
+  if (h) {
+    st = check_operator(*h);
+    if (st != MOLE_OK) return st;
+    s.ops[0] = *h;
+  }
+  s.custom_h = h != nullptr;
+  return push_ops(ctx, s);
+}
+
+mole_status mole_set_observables(mole_ctx *ctx, uint32_t sys, const mole_operator *obs, uint32_t n) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "local operators run on Slater-Jastrow systems");
+  if (n > MOLE_MAX_OBSERVABLES) return fail(MOLE_ERR_INVALID_ARGUMENT, "at most %d observables", MOLE_MAX_OBSERVABLES);
+  if (n && !obs) return fail(MOLE_ERR_INVALID_ARGUMENT, "null observables");
+  for (uint32_t k = 0; k < n; ++k) {
+    st = check_operator(obs[k]);
+    if (st != MOLE_OK) return st;
+  }
+  SysState &s = ctx->sys[sys];
+  s.ops.resize(1 + n);
+  for (uint32_t k = 0; k < n; ++k) s.ops[1 + k] = obs[k];
+  s.n_obs = (int)n;
+  st = push_ops(ctx, s);
+  if (st != MOLE_OK) return st;
+  HIP_TRY(launch_fill(s.d_obs_acc, 2 * (size_t)MOLE_MAX_OBSERVABLES * s.W, 0.0, ctx->stream));
+  return ok();
+}
+
+mole_status mole_read_observables(mole_ctx *ctx, uint32_t sys, double *sum, double *sum2, uint32_t n,
+                                  uint64_t *n_samples) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  SysState &s = ctx->sys[sys];
+  if (n != (uint32_t)s.n_obs) return fail(MOLE_ERR_SHAPE, "system %u has %d observables", sys, s.n_obs);
+  if (n_samples) *n_samples = s.n_samples;
+  if (n == 0) return ok();
+  HIP_TRY(hipSetDevice(ctx->device));
+  double *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * n));
+  hipError_t e = launch_reduce_columns(s.d_obs_acc, s.W, (int)(2 * n), d, -1, ctx->stream);
+  std::vector<double> h(2 * n);
+  if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_read_observables: %s", hipGetErrorString(e));
+  for (uint32_t k = 0; k < n; ++k) {
+    if (sum) sum[k] = h[2 * k];
+    if (sum2) sum2[k] = h[2 * k + 1];
+  }
   return ok();
 }

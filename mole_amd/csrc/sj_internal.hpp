@@ -5,6 +5,7 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "../../include/mole_gpu.h"
 #include "mlp.hpp"
 
 namespace mole {
@@ -42,6 +43,7 @@ struct SJSysDev {
   const double *charge;  // [n_nuc]
   const double *zeta;    // [n_basis]
   const int *center;     // [n_basis]
+  const int *kind;       // [n_basis] MOLE_BASIS_*
   const double *coef_t;  // [n_basis][n_orb]: lanes over orbitals read consecutive words
   double p[kSJNParam];
   double v_nn;
@@ -59,6 +61,11 @@ struct SJSysDev {
   // w_old; the reference energy is a device scalar
   double *dmc;
   const double *dmc_eref;
+  // local operators: ops[0] the custom Hamiltonian (when custom_h), ops[1..n_obs]
+  // the observables, whose per-walker sums are obs_acc [2 n_obs][W] (sum, sum of squares)
+  const mole_operator *ops;
+  int custom_h, n_obs;
+  double *obs_acc;
 };
 
 // What a launch does besides the sweeps: nothing (VMC), DMC's start (local

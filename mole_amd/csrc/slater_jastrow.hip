@@ -125,7 +125,10 @@ __device__ __forceinline__ void basis_to_lds(const SJSysDev &S, const double *r,
   for (int I = t; I < S.n_nuc; I += L)
     rn[I] = norm3(r[0] - S.nuc[3 * I], r[1] - S.nuc[3 * I + 1], r[2] - S.nuc[3 * I + 2]);
   __syncthreads();
-  for (int m = t; m < S.n_basis; m += L) chi[m] = exp_m(-S.zeta[m] * rn[S.center[m]]);
+  for (int m = t; m < S.n_basis; m += L) {
+    const double r = rn[S.center[m]];
+    chi[m] = S.kind[m] == MOLE_BASIS_GTO ? exp_m(-S.zeta[m] * (r * r)) : exp_m(-S.zeta[m] * r);
+  }
   __syncthreads();
 }
 
@@ -270,11 +273,64 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
 // depend on L.  One barrier per call; returns on sub-lane 0.
 constexpr int kSJLanesNMax = 8;  // orbitals per register chunk
 
+// The pieces of the local energy the operator terms need (sub-lane 0).
+struct SJParts {
+  double kin;  // sum_i lap_i psi / psi
+  double vee;  // sum_{i<j} 1/r_ij
+  double ven;  // -sum_{I,i} Z_I / |r_i - R_I|
+};
+
+// sum_k coef_k term_k of one operator (terms as the C ABI defines them).
+__device__ double operator_value(const mole_operator &op, const SJParts &P, const SJSysDev &S, const double *pos) {
+  const int N = S.N;
+  double v = 0.0;
+  for (uint32_t k = 0; k < op.n_terms; ++k) {
+    const mole_term &tm = op.term[k];
+    double x = 0.0;
+    switch (tm.kind) {
+      case MOLE_TERM_KINETIC: x = -0.5 * P.kin; break;
+      case MOLE_TERM_IONIC_POTENTIAL: x = P.ven + S.v_nn; break;
+      case MOLE_TERM_ELECTRONIC_POTENTIAL: x = P.vee; break;
+      case MOLE_TERM_HARMONIC: {
+        double s2 = 0.0;
+        for (int i = 0; i < N; ++i)
+          for (int q = 0; q < 3; ++q) {
+            const double dq = pos[3 * i + q] - tm.param[1 + q];
+            s2 += dq * dq;
+          }
+        x = 0.5 * (tm.param[0] * tm.param[0]) * s2;
+        break;
+      }
+      case MOLE_TERM_RADIAL_POWER: {
+        const int pw = (int)tm.param[0];
+        for (int i = 0; i < N; ++i) {
+          const double r = norm3(pos[3 * i] - tm.param[1], pos[3 * i + 1] - tm.param[2], pos[3 * i + 2] - tm.param[3]);
+          const double b = pw > 0 ? r : 1.0 / r;
+          double y = b;
+          for (int e = 1; e < (pw > 0 ? pw : -pw); ++e) y *= b;
+          x += y;
+        }
+        break;
+      }
+      case MOLE_TERM_DIPOLE:
+        for (int i = 0; i < N; ++i)
+          x += ((pos[3 * i] - tm.param[0]) * tm.param[3] + (pos[3 * i + 1] - tm.param[1]) * tm.param[4]) +
+               (pos[3 * i + 2] - tm.param[2]) * tm.param[5];
+        break;
+      default: break;
+    }
+    v += tm.coef * x;
+  }
+  return v;
+}
+
 template <typename T>
-__device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl) {
+__device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl,
+                               SJParts *parts = nullptr) {
   const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
   const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
-  double *res = wk.scr;  // [N][8]: kinetic term, potential, five derivatives
+  double *res = wk.scr;  // [N][8]: kinetic term, potential, five derivatives, e-e potential
+  SJParts own;
   for (int i = t; i < N; i += L) {
     const int s = i < S.n_up ? 0 : 1;
     const int cc = s == 0 ? i : i - S.n_up;
@@ -293,10 +349,18 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
         const double r = norm3(d0, d1, d2);
         const double z = S.zeta[m];
-        const double ch = exp_m(-z * r);
-        const double gf = -z * ch / r;
+        double ch, gf, lc;
+        if (S.kind[m] == MOLE_BASIS_GTO) {
+          const double r2 = r * r;
+          ch = exp_m(-z * r2);
+          gf = -2.0 * z * ch;
+          lc = ch * (4.0 * z * z * r2 - 6.0 * z);
+        } else {
+          ch = exp_m(-z * r);
+          gf = -z * ch / r;
+          lc = ch * (z * z - 2.0 * z / r);
+        }
         const double g0 = gf * d0, g1 = gf * d1, g2 = gf * d2;
-        const double lc = ch * (z * z - 2.0 * z / r);
         const double *cm = S.coef_t + (size_t)m * no + k0;
 #pragma unroll
         for (int k = 0; k < kSJLanesNMax; ++k)
@@ -319,7 +383,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         }
     }
     double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0;
-    double Vi = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
+    double Vi = 0.0, Vee = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
     for (int j = 0; j < N; ++j) {
       if (j == i) continue;
       const double *xj = wk.pos + 3 * j;
@@ -335,7 +399,9 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
       gJ2 += up * dv2 / r;
       lJ += upp + 2.0 * up / r;
       if (j > i) {
-        Vi += 1.0 / r;
+        const double inv = 1.0 / r;
+        Vi += inv;
+        Vee += inv;
         if (same)
           e1 += r / den;
         else
@@ -369,6 +435,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     o[4] = e2;
     o[5] = e3;
     o[6] = e4;
+    o[7] = Vee;
   }
   __syncthreads();
   double kin = 0.0, V = 0.0;
@@ -388,8 +455,23 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
           V += -S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
         }
     }
+    if (S.custom_h || S.n_obs) {
+      if (!parts) parts = &own;
+      parts->kin = kin;
+      double vee = 0.0, ven = 0.0;
+      for (int i = 0; i < N; ++i) vee += res[8 * i + 7];
+      // IonicPotential::value's order: ions outer, electrons inner
+      for (int I = 0; I < nn; ++I)
+        for (int i = 0; i < N; ++i) {
+          const double *xi = wk.pos + 3 * i;
+          ven -= S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
+        }
+      parts->vee = vee;
+      parts->ven = ven;
+    }
   }
   __syncthreads();
+  if (t == 0 && S.custom_h) return operator_value(S.ops[0], *parts, S, wk.pos);
   return -0.5 * kin + V + S.v_nn;
 }
 
@@ -404,7 +486,7 @@ __device__ __forceinline__ void drift_terms(const SJSysDev &S, const double *pos
   for (int m = t; m < S.n_basis; m += L) {
     const int I = S.center[m];
     const double d0 = r[0] - S.nuc[3 * I], d1 = r[1] - S.nuc[3 * I + 1], d2 = r[2] - S.nuc[3 * I + 2];
-    const double gf = -S.zeta[m] * chi[m] / rn[I];
+    const double gf = S.kind[m] == MOLE_BASIS_GTO ? -2.0 * S.zeta[m] * chi[m] : -S.zeta[m] * chi[m] / rn[I];
     gch[3 * m] = gf * d0;
     gch[3 * m + 1] = gf * d1;
     gch[3 * m + 2] = gf * d2;
@@ -806,7 +888,17 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
       }
     } else if (sw >= a.sample_start) {
       double dlg[kSJNParam];
-      const double E = local_energy<T>(S, wk, t, L, dlg);
+      SJParts parts;
+      const double E = local_energy<T>(S, wk, t, L, dlg, &parts);
+      if (t == 0 && live && S.n_obs) {
+        // further observables: per-walker sums in HBM (opt-in, sampled sweeps only)
+        for (int k = 0; k < S.n_obs; ++k) {
+          const double v = operator_value(S.ops[1 + k], parts, S, wk.pos);
+          double *o = S.obs_acc + (size_t)(2 * k) * W + w;
+          o[0] += v;
+          o[(size_t)W] = fma(v, v, o[(size_t)W]);
+        }
+      }
       if (t == 0) {
         acc[kSJSumE] += E;
         acc[kSJSumE2] = fma(E, E, acc[kSJSumE2]);

@@ -19,7 +19,7 @@ class SJSystem:
     s-type STO orbitals, times the Pade e-e and e-n Jastrow factor."""
 
     def __init__(self, nuclei, charges, n_up, n_dn, zeta, center, coef, jastrow=DEFAULT_JASTROW,
-                 name=""):
+                 name="", kind=None):
         self.nuclei = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 3)
         self.charges = np.ascontiguousarray(charges, dtype=np.int32)
         self.n_up = int(n_up)
@@ -29,6 +29,8 @@ class SJSystem:
         self.coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1, self.zeta.size)
         self.jastrow = np.ascontiguousarray(jastrow, dtype=np.float64)
         self.name = name
+        # basis kinds: 0 STO exp(-zeta r), 1 GTO exp(-zeta r^2)
+        self.kind = None if kind is None else np.ascontiguousarray(kind, dtype=np.int32)
         assert self.coef.shape[0] >= max(self.n_up, self.n_dn)
         assert self.center.size == self.zeta.size
         assert np.all((self.center >= 0) & (self.center < self.nuclei.shape[0]))
@@ -59,6 +61,16 @@ def hydrogenic(Z=1):
     """One electron in exp(-Z r): an exact eigenfunction, E_L = -Z^2/2."""
     return SJSystem([(0.0, 0.0, 0.0)], [Z], 1, 0, [float(Z)], [0], [[1.0]], (0, 0, 0, 0, 0),
                     name=f"hydrogenic Z={Z}")
+
+
+def harmonic_oscillator(omega=1.0, a=None):
+    """One electron in exp(-(r/a)^2) about a charge-0 centre: the Gaussian
+    trial function and oscillator of examples/custom_operator.rs:64-100,
+    a = sqrt(2/omega) by default (exact: E_L = 3 omega/2 under
+    T + 1/2 omega^2 r^2)."""
+    a = math.sqrt(2.0 / omega) if a is None else a
+    return SJSystem([(0.0, 0.0, 0.0)], [0], 1, 0, [1.0 / (a * a)], [0], [[1.0]], (0, 0, 0, 0, 0),
+                    name=f"oscillator w={omega}", kind=[1])
 
 
 def helium_product(zeta=2.0, b=0.0, c=0.3):

@@ -24,12 +24,17 @@ static int electron_of(const orc_sj *wf, int s, int c) { return s == 0 ? c : wf-
 
 static double rust_min1(double a) { return (a < 1.0) ? a : 1.0; }
 
+static int is_gto(const orc_sj *wf, int m) { return wf->kind && wf->kind[m] == ORC_BASIS_GTO; }
+
 /* basis values chi[m] at r (distances to all nuclei first) */
 static void basis_at(const orc_sj *wf, const double *r, double *chi) {
   double rn[MAXNUC];
   for (int I = 0; I < wf->n_nuc; ++I)
     rn[I] = orc_norm3(r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]);
-  for (int m = 0; m < wf->n_basis; ++m) chi[m] = orc_exp(-wf->zeta[m] * rn[wf->center[m]]);
+  for (int m = 0; m < wf->n_basis; ++m) {
+    const double rr = rn[wf->center[m]];
+    chi[m] = is_gto(wf, m) ? orc_exp(-wf->zeta[m] * (rr * rr)) : orc_exp(-wf->zeta[m] * rr);
+  }
 }
 
 static double orbital(const orc_sj *wf, int k, const double *chi) {
@@ -246,11 +251,14 @@ double orc_sj_value(const orc_sj *wf, const double *cfg) {
 
 /* ---- local energy ---------------------------------------------------- */
 
-double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bup,
-                           const double *Bdn, double *dlog) {
+/* the built-in local energy; kin_out / vee_out (optional) receive
+ * sum_i lap_i psi/psi and sum_{i<j} 1/r_ij (per electron, then in electron
+ * order) */
+static double sj_local_parts(const orc_sj *wf, const double *cfg, const double *Bup, const double *Bdn,
+                             double *dlog, double *kin_out, double *vee_out) {
   int N = wf->n_up + wf->n_dn;
   const double bA = wf->p[0], bP = wf->p[1], c = wf->p[2], a = wf->p[3], d = wf->p[4];
-  double kin = 0.0, V = 0.0;
+  double kin = 0.0, V = 0.0, vee = 0.0;
   double dl[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
   for (int i = 0; i < N; ++i) {
     int s = spin_of(wf, i), cc = col_of(wf, i), n = nspin(wf, s);
@@ -265,10 +273,18 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
     for (int m = 0; m < wf->n_basis; ++m) {
       int I = wf->center[m];
       double z = wf->zeta[m];
-      chi[m] = orc_exp(-z * rI[I]);
-      double gf = -z * chi[m] / rI[I];
+      double gf;
+      if (is_gto(wf, m)) {
+        const double r2 = rI[I] * rI[I];
+        chi[m] = orc_exp(-z * r2);
+        gf = -2.0 * z * chi[m];
+        lch[m] = chi[m] * (4.0 * z * z * r2 - 6.0 * z);
+      } else {
+        chi[m] = orc_exp(-z * rI[I]);
+        gf = -z * chi[m] / rI[I];
+        lch[m] = chi[m] * (z * z - 2.0 * z / rI[I]);
+      }
       for (int q = 0; q < 3; ++q) gch[m][q] = gf * dI[I][q];
-      lch[m] = chi[m] * (z * z - 2.0 * z / rI[I]);
     }
     double Dg[3] = {0, 0, 0}, Dl = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -284,7 +300,7 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
     }
     /* electron i's parameter derivatives are summed on their own and added
      * to the totals in electron order (the device's order) */
-    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0, dli[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
+    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0, Vee = 0.0, dli[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
     for (int j = 0; j < N; ++j) {
       if (j == i) continue;
       const double *xj = cfg + 3 * j;
@@ -297,7 +313,9 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
       for (int q = 0; q < 3; ++q) gJ[q] += up * dv[q] / r;
       lJ += upp + 2.0 * up / r;
       if (j > i) {
-        Vi += 1.0 / r;
+        const double inv = 1.0 / r;
+        Vi += inv;
+        Vee += inv;
         double t = r / den;
         if (spin_of(wf, j) == s)
           dli[1] += t;
@@ -323,6 +341,7 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
     double gg = gJ[0] * gJ[0] + gJ[1] * gJ[1] + gJ[2] * gJ[2];
     kin += Dl + 2.0 * dot + lJ + gg;
     V += Vi;
+    vee += Vee;
     for (int q = 0; q < ORC_SJ_NPARAM; ++q) dl[q] += dli[q];
   }
   double vnn = 0.0;
@@ -343,7 +362,78 @@ double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bu
       }
   }
   if (dlog) memcpy(dlog, dl, sizeof dl);
+  if (kin_out) *kin_out = kin;
+  if (vee_out) *vee_out = vee;
   return -0.5 * kin + V + vnn;
+}
+
+double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bup, const double *Bdn,
+                           double *dlog) {
+  return sj_local_parts(wf, cfg, Bup, Bdn, dlog, NULL, NULL);
+}
+
+static double nuclear_repulsion(const orc_sj *wf) {
+  double vnn = 0.0;
+  for (int I = 0; I < wf->n_nuc; ++I)
+    for (int J = I + 1; J < wf->n_nuc; ++J) {
+      double r = orc_norm3(wf->nuc[3 * J] - wf->nuc[3 * I], wf->nuc[3 * J + 1] - wf->nuc[3 * I + 1],
+                           wf->nuc[3 * J + 2] - wf->nuc[3 * I + 2]);
+      vnn += (double)(wf->charge[I] * wf->charge[J]) / r;
+    }
+  return vnn;
+}
+
+double orc_sj_operator(const orc_sj *wf, const double *cfg, const double *Bup, const double *Bdn,
+                       const orc_operator *op) {
+  const int N = wf->n_up + wf->n_dn;
+  double kin, vee, ven = 0.0;
+  sj_local_parts(wf, cfg, Bup, Bdn, NULL, &kin, &vee);
+  for (int I = 0; I < wf->n_nuc; ++I)
+    for (int i = 0; i < N; ++i) {
+      const double *xi = cfg + 3 * i;
+      ven -= (double)wf->charge[I] /
+             orc_norm3(xi[0] - wf->nuc[3 * I], xi[1] - wf->nuc[3 * I + 1], xi[2] - wf->nuc[3 * I + 2]);
+    }
+  double v = 0.0;
+  for (int k = 0; k < op->n_terms; ++k) {
+    const orc_term *t = &op->term[k];
+    double x = 0.0;
+    switch (t->kind) {
+      case ORC_TERM_KINETIC: x = -0.5 * kin; break;
+      case ORC_TERM_IONIC_POTENTIAL: x = ven + nuclear_repulsion(wf); break;
+      case ORC_TERM_ELECTRONIC_POTENTIAL: x = vee; break;
+      case ORC_TERM_HARMONIC: {
+        double s2 = 0.0;
+        for (int i = 0; i < N; ++i)
+          for (int q = 0; q < 3; ++q) {
+            const double dq = cfg[3 * i + q] - t->param[1 + q];
+            s2 += dq * dq;
+          }
+        x = 0.5 * (t->param[0] * t->param[0]) * s2;
+        break;
+      }
+      case ORC_TERM_RADIAL_POWER: {
+        const int pw = (int)t->param[0];
+        for (int i = 0; i < N; ++i) {
+          const double r =
+              orc_norm3(cfg[3 * i] - t->param[1], cfg[3 * i + 1] - t->param[2], cfg[3 * i + 2] - t->param[3]);
+          const double b = pw > 0 ? r : 1.0 / r;
+          double y = b;
+          for (int e = 1; e < (pw > 0 ? pw : -pw); ++e) y *= b;
+          x += y;
+        }
+        break;
+      }
+      case ORC_TERM_DIPOLE:
+        for (int i = 0; i < N; ++i)
+          x += ((cfg[3 * i] - t->param[0]) * t->param[3] + (cfg[3 * i + 1] - t->param[1]) * t->param[4]) +
+               (cfg[3 * i + 2] - t->param[2]) * t->param[5];
+        break;
+      default: break;
+    }
+    v += t->coef * x;
+  }
+  return v;
 }
 
 double orc_sj_local_energy_full(const orc_sj *wf, const double *cfg, double *dlog) {
@@ -364,8 +454,14 @@ static void det_grad(const orc_sj *wf, const double *r, int n, const double *Bco
     double d[3] = {r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]};
     double rr = orc_norm3(d[0], d[1], d[2]);
     double z = wf->zeta[m];
-    double ch = orc_exp(-z * rr);
-    double gf = -z * ch / rr;
+    double gf;
+    if (is_gto(wf, m)) {
+      const double ch = orc_exp(-z * (rr * rr));
+      gf = -2.0 * z * ch;
+    } else {
+      const double ch = orc_exp(-z * rr);
+      gf = -z * ch / rr;
+    }
     for (int q = 0; q < 3; ++q) gch[m][q] = gf * d[q];
   }
   g[0] = g[1] = g[2] = 0.0;
@@ -465,6 +561,7 @@ typedef struct {
   int faithful;
   double sum_e, sum_e2, sum_bm, sum_bm2;
   uint64_t n_samples, n_blocks, n_accepted, n_moves;
+  double obs_sum[8], obs_sum2[8];
 } sj_task;
 
 static void refresh(const orc_sj *wf, const double *cfg, int prec, double *Bu, double *Bd,
@@ -680,6 +777,17 @@ static void sj_walker(sj_task *t, uint32_t w) {
           for (int k = 0; k < wf->n_dn * wf->n_dn; ++k) bd[k] = (double)Bdf[k];
           el = orc_sj_local_energy(wf, cfg, bu, bd, dl);
         }
+        if (!t->faithful && (rp->hamiltonian || rp->n_obs)) {
+          double bu[MAXN * MAXN], bd[MAXN * MAXN];
+          for (int k = 0; k < wf->n_up * wf->n_up; ++k) bu[k] = prec == ORC_PREC_FP64 ? Bu[k] : (double)Buf[k];
+          for (int k = 0; k < wf->n_dn * wf->n_dn; ++k) bd[k] = prec == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
+          if (rp->hamiltonian) el = orc_sj_operator(wf, cfg, bu, bd, rp->hamiltonian);
+          for (int k = 0; k < rp->n_obs; ++k) {
+            const double v = orc_sj_operator(wf, cfg, bu, bd, &rp->obs[k]);
+            t->obs_sum[k] += v;
+            t->obs_sum2[k] += v * v;
+          }
+        }
         size_t si = (size_t)w * nsw + sidx;
         if (out->e_local) out->e_local[si] = el;
         if (out->dlog) memcpy(out->dlog + si * ORC_SJ_NPARAM, dl, sizeof dl);
@@ -714,6 +822,7 @@ static int sj_run(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, ui
     return -1;
   if (out->accept_mask && N > 64) return -1;
   if (!rp->init_from_stream && !out->cfg) return -1;
+  if (rp->n_obs < 0 || rp->n_obs > 8) return -1;
   if (n_threads < 1) n_threads = 1;
   uint32_t nw = w1 - w0;
   if ((uint32_t)n_threads > nw) n_threads = nw ? (int)nw : 1;
@@ -742,6 +851,10 @@ static int sj_run(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, ui
     out->n_blocks += tasks[i].n_blocks;
     out->n_accepted += tasks[i].n_accepted;
     out->n_moves += tasks[i].n_moves;
+    for (int k = 0; k < rp->n_obs; ++k) {
+      if (out->obs_sum) out->obs_sum[k] += tasks[i].obs_sum[k];
+      if (out->obs_sum2) out->obs_sum2[k] += tasks[i].obs_sum2[k];
+    }
   }
   free(tasks);
   free(th);
@@ -871,10 +984,10 @@ static void dmc_task(sj_task *t, const orc_sj *wf, const orc_dmc_params *p, orc_
 
 /* the walker's inverses in the working precision (as the device stores
  * them) and its local energy from them */
-static double dmc_energy(const orc_sj *wf, int prec, const double *cfg, const double *B) {
+static double dmc_energy(const orc_sj *wf, const orc_dmc_params *p, const double *cfg, const double *B) {
   double dl[ORC_SJ_NPARAM];
   const int nu = wf->n_up;
-  (void)prec;
+  if (p->hamiltonian) return orc_sj_operator(wf, cfg, B, B + nu * nu, p->hamiltonian);
   return orc_sj_local_energy(wf, cfg, B, B + nu * nu, dl);
 }
 
@@ -897,7 +1010,7 @@ int orc_dmc_init(const orc_sj *wf, const orc_dmc_params *p, double reference_ene
     B[k] = p->precision == ORC_PREC_FP64 ? Bu[k] : (double)Buf[k];
   for (int k = 0; k < wf->n_dn * wf->n_dn; ++k)
     B[wf->n_up * wf->n_up + k] = p->precision == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
-  const double e0 = dmc_energy(wf, p->precision, cfg, B);
+  const double e0 = dmc_energy(wf, p, cfg, B);
   for (uint32_t w = 0; w < p->n_walkers; ++w) {
     memcpy(st->cfg + (size_t)w * 3 * N, cfg, sizeof(double) * 3 * N);
     memcpy(st->B + (size_t)w * nb, B, sizeof(double) * nb);
@@ -954,7 +1067,7 @@ int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st
         for (int k = 0; k < nd * nd; ++k) B[nu * nu + k] = p->precision == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
         ensemble += st->weight[w] * local_e;
         total_weight += st->weight[w];
-        const double local_e_new = dmc_energy(wf, p->precision, c, B);
+        const double local_e_new = dmc_energy(wf, p, c, B);
         st->weight[w] *= orc_exp(-time_step * ((local_e + local_e_new) / 2.0 - st->reference_energy));
         st->e_local[w] = local_e_new;
       }

@@ -41,7 +41,41 @@ typedef struct {
   const int *center;    /* [n_basis] */
   const double *coef;   /* [n_orb][n_basis] */
   double p[ORC_SJ_NPARAM];
+  const int *kind;      /* [n_basis] ORC_BASIS_*; NULL: all STO */
 } orc_sj;
+
+/* basis functions: exp(-zeta r) (STO) or exp(-zeta r^2) (GTO), r = |x - R| */
+enum { ORC_BASIS_STO = 0, ORC_BASIS_GTO = 1 };
+
+/* Local operators (O psi)/psi built from terms (LocalOperator::act_on,
+ * src/operator/src/traits.rs:263-265, divided by psi as Sampler::sample
+ * does): KINETIC -1/2 lap psi/psi (operator.rs:96-121); IONIC_POTENTIAL
+ * -sum_{I,i} Z_I/r_iI + ion-ion repulsion (operator.rs:12-64, ions outer,
+ * electrons inner); ELECTRONIC_POTENTIAL sum_{i<j} 1/r_ij (operator.rs:
+ * 66-94); HARMONIC 1/2 w^2 sum_i |r_i - c|^2 (w = param[0], c = param[1..3];
+ * examples/custom_operator.rs:48-61); RADIAL_POWER sum_i |r_i - c|^k
+ * (k = param[0] integer, c = param[1..3]); DIPOLE sum_i (r_i - c).u
+ * (c = param[0..2], u = param[3..5]).  value = sum_k coef_k term_k. */
+enum {
+  ORC_TERM_KINETIC = 0,
+  ORC_TERM_IONIC_POTENTIAL = 1,
+  ORC_TERM_ELECTRONIC_POTENTIAL = 2,
+  ORC_TERM_HARMONIC = 3,
+  ORC_TERM_RADIAL_POWER = 4,
+  ORC_TERM_DIPOLE = 5
+};
+typedef struct {
+  int kind;
+  double coef;
+  double param[6];
+} orc_term;
+typedef struct {
+  int n_terms;
+  orc_term term[4];
+} orc_operator;
+/* the operator's local value at cfg from the inverses (FP64) */
+double orc_sj_operator(const orc_sj *wf, const double *cfg, const double *Bup, const double *Bdn,
+                       const orc_operator *op);
 
 enum { ORC_PREC_FP64 = 0, ORC_PREC_MIXED = 1 };
 
@@ -87,6 +121,9 @@ typedef struct {
   const float *mlp_w;   /* learned proposal: network parameters (mlp.h) */
   int mlp_emulate_bf16; /* learned proposal: round operands to bf16 */
   int mlp_drop_q_ratio; /* tests only: a planted mistake (omit q(x|x')/q(x'|x)) */
+  const orc_operator *hamiltonian; /* parity runner: the local energy's operator; NULL: built in */
+  const orc_operator *obs;         /* parity runner: observables summed over samples */
+  int n_obs;
 } orc_sj_run_params;
 
 /* Proposals of the parity runner.  BOX: 4 draws per move (3 Range, 1
@@ -109,6 +146,7 @@ typedef struct {
   double *dlog;          /* [W][samples][5] */
   double sum_e, sum_e2, sum_bm, sum_bm2;
   uint64_t n_samples, n_blocks, n_accepted, n_moves;
+  double *obs_sum, *obs_sum2; /* [n_obs]: sums over samples (walkers in order) */
 } orc_sj_run_out;
 
 /* Parity runner: walkers [w0, w1), each its own Runner::run
@@ -159,6 +197,7 @@ typedef struct {
   uint32_t lu_period;
   uint32_t n_walkers;
   int brancher;         /* ORC_BRANCH_* */
+  const orc_operator *hamiltonian; /* the local energy's operator; NULL: built in */
 } orc_dmc_params;
 
 typedef struct {

@@ -21,7 +21,30 @@ class Sj(C.Structure):
         ("n_orb", C.c_int), ("hamiltonian", C.c_int),
         ("nuc", _pd), ("charge", _pi), ("zeta", _pd), ("center", _pi), ("coef", _pd),
         ("p", C.c_double * NPARAM),
+        ("kind", _pi),
     ]
+
+
+TERM_KINETIC, TERM_IONIC_POTENTIAL, TERM_ELECTRONIC_POTENTIAL, TERM_HARMONIC, TERM_RADIAL_POWER, TERM_DIPOLE = range(6)
+
+
+class OrcTerm(C.Structure):
+    _fields_ = [("kind", C.c_int), ("coef", C.c_double), ("param", C.c_double * 6)]
+
+
+class OrcOperator(C.Structure):
+    _fields_ = [("n_terms", C.c_int), ("term", OrcTerm * 4)]
+
+
+def operator(terms):
+    """[(kind, coef, params...)] -> OrcOperator."""
+    op = OrcOperator(n_terms=len(terms))
+    for k, t in enumerate(terms):
+        op.term[k].kind = int(t[0])
+        op.term[k].coef = float(t[1])
+        for q, v in enumerate(t[2:]):
+            op.term[k].param[q] = float(v)
+    return op
 
 
 class SjRunParams(C.Structure):
@@ -31,6 +54,7 @@ class SjRunParams(C.Structure):
         ("pos0", C.c_uint64), ("move0", C.c_uint64), ("precision", C.c_int),
         ("lu_period", C.c_uint32), ("proposal", C.c_int), ("tau", C.c_double),
         ("mlp_w", C.POINTER(C.c_float)), ("mlp_emulate_bf16", C.c_int), ("mlp_drop_q_ratio", C.c_int),
+        ("hamiltonian", C.POINTER(OrcOperator)), ("obs", C.POINTER(OrcOperator)), ("n_obs", C.c_int),
     ]
 
 
@@ -39,7 +63,7 @@ class SjRunOut(C.Structure):
         ("cfg", _pd), ("accept_mask", C.POINTER(C.c_uint64)), ("e_local", _pd), ("dlog", _pd),
         ("sum_e", C.c_double), ("sum_e2", C.c_double), ("sum_bm", C.c_double), ("sum_bm2", C.c_double),
         ("n_samples", C.c_uint64), ("n_blocks", C.c_uint64), ("n_accepted", C.c_uint64),
-        ("n_moves", C.c_uint64),
+        ("n_moves", C.c_uint64), ("obs_sum", _pd), ("obs_sum2", _pd),
     ]
 
 
@@ -64,6 +88,8 @@ def lib():
         L.orc_sj_local_energy_full.restype = C.c_double
         L.orc_sj_local_energy_full.argtypes = [ps, _pd, _pd]
         L.orc_sj_drift.argtypes = [ps, _pd, C.c_int, _pd]
+        L.orc_sj_operator.restype = C.c_double
+        L.orc_sj_operator.argtypes = [ps, _pd, _pd, _pd, C.POINTER(OrcOperator)]
         for f in (L.orc_sj_run_parity, L.orc_sj_run_faithful):
             f.restype = C.c_int
             f.argtypes = [ps, C.POINTER(SjRunParams), C.c_uint32, C.c_uint32, C.c_int,
@@ -77,12 +103,14 @@ class SjWave:
 
     def __init__(self, system, hamiltonian=0, params=None):
         s = system
-        self._keep = [s.nuclei, s.charges.astype(np.intc), s.zeta, s.center.astype(np.intc), s.coef]
-        nuc, ch, ze, ce, co = self._keep
+        kind = getattr(s, "kind", None)
+        kind = np.zeros(s.zeta.size, np.intc) if kind is None else np.ascontiguousarray(kind, dtype=np.intc)
+        self._keep = [s.nuclei, s.charges.astype(np.intc), s.zeta, s.center.astype(np.intc), s.coef, kind]
+        nuc, ch, ze, ce, co, kd = self._keep
         self.st = Sj(n_up=s.n_up, n_dn=s.n_dn, n_nuc=nuc.shape[0], n_basis=ze.size, n_orb=co.shape[0],
                      hamiltonian=hamiltonian, nuc=nuc.ctypes.data_as(_pd), charge=ch.ctypes.data_as(_pi),
                      zeta=ze.ctypes.data_as(_pd), center=ce.ctypes.data_as(_pi),
-                     coef=co.ctypes.data_as(_pd))
+                     coef=co.ctypes.data_as(_pd), kind=kd.ctypes.data_as(_pi))
         p = s.jastrow if params is None else params
         for k in range(NPARAM):
             self.st.p[k] = float(p[k])
@@ -123,9 +151,21 @@ def local_energy(wv, x):
 BOX, DIFFUSE, LEARNED = 0, 1, 2
 
 
+def operator_value(wv, x, terms):
+    """orc_sj_operator at configuration x (fresh inverses)."""
+    x = np.ascontiguousarray(x, dtype=np.float64).ravel()
+    Bu, _ = inverse(wv, x, 0)
+    Bd, _ = inverse(wv, x, 1)
+    Bu = np.ascontiguousarray(Bu.ravel() if Bu.size else np.zeros(1))
+    Bd = np.ascontiguousarray(Bd.ravel() if Bd.size else np.zeros(1))
+    op = operator(terms)
+    return lib().orc_sj_operator(wv.ref(), OF.dptr(x), OF.dptr(Bu), OF.dptr(Bd), C.byref(op))
+
+
 def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, precision=PREC_FP64,
         lu_period=0, sys=0, epoch=0, cfg=None, pos0=0, move0=0, n_threads=1, record=True,
-        proposal=BOX, tau=0.0, mlp_w=None, emulate_bf16=False, drop_q_ratio=False):
+        proposal=BOX, tau=0.0, mlp_w=None, emulate_bf16=False, drop_q_ratio=False, hamiltonian=None,
+        observables=()):
     L = lib()
     N = wv.N
     rp = SjRunParams(seed=seed, sys=sys, epoch=epoch, box_side=box_side, steps=steps,
@@ -136,9 +176,23 @@ def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, p
     if mlp_w is not None:
         mlp_w = np.ascontiguousarray(mlp_w, dtype=np.float32)
         rp.mlp_w = mlp_w.ctypes.data_as(C.POINTER(C.c_float))
+    keep = []
+    if hamiltonian is not None:
+        h = operator(hamiltonian)
+        keep.append(h)
+        rp.hamiltonian = C.pointer(h)
+    n_obs = len(observables)
+    obs_sum, obs_sum2 = np.zeros(max(1, n_obs)), np.zeros(max(1, n_obs))
+    if n_obs:
+        arr = (OrcOperator * n_obs)(*[operator(o) for o in observables])
+        keep.append(arr)
+        rp.obs = C.cast(arr, C.POINTER(OrcOperator))
+        rp.n_obs = n_obs
     blocks = steps // block_size
     ns = (blocks - 1) * block_size
     out = SjRunOut()
+    out.obs_sum = OF.dptr(obs_sum)
+    out.obs_sum2 = OF.dptr(obs_sum2)
     c = np.zeros((walkers, 3 * N)) if cfg is None else np.ascontiguousarray(cfg, dtype=np.float64).copy()
     out.cfg = OF.dptr(c)
     res = {}
@@ -156,7 +210,7 @@ def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, p
         raise ValueError("orc_sj_run failed (steps < 2*block_size or sizes too large)")
     res.update(cfg=c, sum_e=out.sum_e, sum_e2=out.sum_e2, sum_bm=out.sum_bm, sum_bm2=out.sum_bm2,
                n_samples=out.n_samples, n_blocks=out.n_blocks, n_accepted=out.n_accepted,
-               n_moves=out.n_moves)
+               n_moves=out.n_moves, obs_sum=obs_sum[:n_obs], obs_sum2=obs_sum2[:n_obs])
     return res
 
 
@@ -171,6 +225,7 @@ class DmcParams(C.Structure):
     _fields_ = [
         ("seed", C.c_uint64), ("sys", C.c_uint32), ("epoch", C.c_uint32), ("tau", C.c_double),
         ("precision", C.c_int), ("lu_period", C.c_uint32), ("n_walkers", C.c_uint32), ("brancher", C.c_int),
+        ("hamiltonian", C.POINTER(OrcOperator)),
     ]
 
 
@@ -220,7 +275,7 @@ class Dmc:
     """The oracle's DmcRunner on one Slater-Jastrow system."""
 
     def __init__(self, wv, *, walkers, seed, tau, reference_energy, brancher=BRANCH_SR, precision=PREC_FP64,
-                 lu_period=0, sys=0, epoch=0):
+                 lu_period=0, sys=0, epoch=0, hamiltonian=None):
         L = lib()
         _bind_dmc(L)
         self.wv = wv
@@ -231,6 +286,9 @@ class Dmc:
         self.B = np.zeros((walkers, max(1, nb)))
         self.p = DmcParams(seed=seed, sys=sys, epoch=epoch, tau=tau, precision=precision, lu_period=lu_period,
                            n_walkers=walkers, brancher=brancher)
+        if hamiltonian is not None:
+            self._h = operator(hamiltonian)
+            self.p.hamiltonian = C.pointer(self._h)
         self.st = DmcState(cfg=OF.dptr(self.cfg), weight=OF.dptr(self.weight), e_local=OF.dptr(self.e_local),
                            B=OF.dptr(self.B))
         assert L.orc_dmc_init(wv.ref(), C.byref(self.p), reference_energy, C.byref(self.st)) == 0

@@ -41,7 +41,7 @@ def test_rust_declarations_match_header_functions():
 def test_rust_structs_match_header_fields():
     _, structs = header_functions_and_structs()
     rb = rust_block()
-    for name in ("mole_system_desc", "mole_run_desc", "mole_stats", "mole_sr_sums", "mole_train_desc", "mole_optimizer", "mole_block_record"):
+    for name in ("mole_system_desc", "mole_run_desc", "mole_stats", "mole_sr_sums", "mole_train_desc", "mole_optimizer", "mole_block_record", "mole_term", "mole_operator"):
         m = re.search(r"pub struct " + name + r" \{(.*?)\n\}", rb, flags=re.S)
         assert m, name
         rust_fields = re.findall(r"pub (\w+):", m.group(1))
