@@ -386,6 +386,7 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.dmc = st.dmc.dmc;
     d.dmc_eref = st.dmc.st ? &st.dmc.st->eref : nullptr;
     d.kind = st.d_center + st.n_basis;
+    d.any_gto = st.any_gto ? 1 : 0;
     d.ops = st.d_ops;
     d.custom_h = st.custom_h ? 1 : 0;
     d.n_obs = st.n_obs;
@@ -414,6 +415,8 @@ SJArgs sj_args(const mole_ctx *ctx) {
   a.cap_sweeps = (ctx->run.flags & MOLE_FLAG_TRACE) ? ctx->run.record_sweeps : 0;
   a.cap_samples = a.cap_sweeps;
   if (ctx->mlp) a.net = ctx->mlp->net();
+  for (const SysState &s : ctx->sys)
+    if (s.any_gto || s.custom_h || s.n_obs) a.ext = 1;
   return a;
 }
 
@@ -1762,6 +1765,7 @@ mole_status dmc_enqueue_step(mole_ctx *ctx, double time_step, uint32_t block_siz
   a.sample_start = 0;
   a.dmc = kSJDmcStep;
   a.dmc_tau = time_step;
+  a.ext = 1;
   HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
   HIP_TRY(launch_sj_advance(ctx->d_state, 1, 0, ctx->run.block_size, ctx->stream));
   for (uint32_t s = 0; s < ctx->sys.size(); ++s)
@@ -1807,6 +1811,7 @@ mole_status mole_dmc_init(mole_ctx *ctx, const double *reference_energy, uint32_
   a.n_sweeps = 0;
   a.refresh = 1;
   a.dmc = kSJDmcInit;
+  a.ext = 1;
   HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->dmc_brancher = brancher;

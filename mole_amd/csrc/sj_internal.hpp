@@ -44,6 +44,7 @@ struct SJSysDev {
   const double *zeta;    // [n_basis]
   const int *center;     // [n_basis]
   const int *kind;       // [n_basis] MOLE_BASIS_*
+  int any_gto;           // 0: every basis function is an STO (kind is not read)
   const double *coef_t;  // [n_basis][n_orb]: lanes over orbitals read consecutive words
   double p[kSJNParam];
   double v_nn;
@@ -106,6 +107,7 @@ struct SJArgs {
   MlpDev net;              // learned proposal: the packed network
   int dmc;                 // SJDmcMode
   double dmc_tau;          // DMC: the weights' time step
+  int ext;                 // 1: DMC, local operators or Gaussian bases in use (the extended kernel)
 };
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);

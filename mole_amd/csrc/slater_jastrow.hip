@@ -121,13 +121,13 @@ __device__ __forceinline__ int small_div(int e, float inv_n) { return (int)(((fl
 // Basis values chi[m] of the point r into LDS (lanes over nuclei, then over
 // basis functions).  Ends with a barrier.
 __device__ __forceinline__ void basis_to_lds(const SJSysDev &S, const double *r, double *rn, double *chi,
-                                             int t, int L) {
+                                             int t, int L, bool gto) {
   for (int I = t; I < S.n_nuc; I += L)
     rn[I] = norm3(r[0] - S.nuc[3 * I], r[1] - S.nuc[3 * I + 1], r[2] - S.nuc[3 * I + 2]);
   __syncthreads();
   for (int m = t; m < S.n_basis; m += L) {
     const double r = rn[S.center[m]];
-    chi[m] = S.kind[m] == MOLE_BASIS_GTO ? exp_m(-S.zeta[m] * (r * r)) : exp_m(-S.zeta[m] * r);
+    chi[m] = (gto && S.kind[m] == MOLE_BASIS_GTO) ? exp_m(-S.zeta[m] * (r * r)) : exp_m(-S.zeta[m] * r);
   }
   __syncthreads();
 }
@@ -194,14 +194,14 @@ __device__ __forceinline__ double orbital_value(const SJSysDev &S, int k, const 
 // into B (as T); returns ||A||_1 ||A^-1||_1 on sub-lane 0.  All lanes of the
 // workgroup must call it.
 template <typename T>
-__device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t, int L) {
+__device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t, int L, bool gto) {
   const int n = s == 0 ? S.n_up : S.n_dn;
   if (n == 0) return 1.0;
   const int first = s == 0 ? 0 : S.n_up;
   T *B = s == 0 ? wk.Bup : wk.Bdn;
   double *M = wk.scr, *I = wk.scr + n * n, *colp = wk.scr + 2 * n * n;
   for (int c = 0; c < n; ++c) {
-    basis_to_lds(S, wk.pos + 3 * (first + c), wk.rn, wk.chi, t, L);
+    basis_to_lds(S, wk.pos + 3 * (first + c), wk.rn, wk.chi, t, L, gto);
     for (int k = t; k < n; k += L) M[c * n + k] = orbital_value(S, k, wk.chi);
     __syncthreads();
   }
@@ -324,7 +324,7 @@ __device__ double operator_value(const mole_operator &op, const SJParts &P, cons
   return v;
 }
 
-template <typename T>
+template <typename T, bool EXT>
 __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl,
                                SJParts *parts = nullptr) {
   const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
@@ -350,7 +350,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         const double r = norm3(d0, d1, d2);
         const double z = S.zeta[m];
         double ch, gf, lc;
-        if (S.kind[m] == MOLE_BASIS_GTO) {
+        if (EXT && S.any_gto && S.kind[m] == MOLE_BASIS_GTO) {
           const double r2 = r * r;
           ch = exp_m(-z * r2);
           gf = -2.0 * z * ch;
@@ -401,7 +401,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
       if (j > i) {
         const double inv = 1.0 / r;
         Vi += inv;
-        Vee += inv;
+        if constexpr (EXT) Vee += inv;
         if (same)
           e1 += r / den;
         else
@@ -435,7 +435,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     o[4] = e2;
     o[5] = e3;
     o[6] = e4;
-    o[7] = Vee;
+    if constexpr (EXT) o[7] = Vee;
   }
   __syncthreads();
   double kin = 0.0, V = 0.0;
@@ -455,7 +455,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
           V += -S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
         }
     }
-    if (S.custom_h || S.n_obs) {
+    if (EXT && (S.custom_h || S.n_obs)) {
       if (!parts) parts = &own;
       parts->kin = kin;
       double vee = 0.0, ven = 0.0;
@@ -471,7 +471,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     }
   }
   __syncthreads();
-  if (t == 0 && S.custom_h) return operator_value(S.ops[0], *parts, S, wk.pos);
+  if (EXT && t == 0 && S.custom_h) return operator_value(S.ops[0], *parts, S, wk.pos);
   return -0.5 * kin + V + S.v_nn;
 }
 
@@ -482,11 +482,11 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
 // No barrier.
 __device__ __forceinline__ void drift_terms(const SJSysDev &S, const double *pos, int i, const double *r,
                                             const double *rn, const double *chi, double *gch, double *jg,
-                                            int t, int L) {
+                                            int t, int L, bool gto) {
   for (int m = t; m < S.n_basis; m += L) {
     const int I = S.center[m];
     const double d0 = r[0] - S.nuc[3 * I], d1 = r[1] - S.nuc[3 * I + 1], d2 = r[2] - S.nuc[3 * I + 2];
-    const double gf = S.kind[m] == MOLE_BASIS_GTO ? -2.0 * S.zeta[m] * chi[m] : -S.zeta[m] * chi[m] / rn[I];
+    const double gf = (gto && S.kind[m] == MOLE_BASIS_GTO) ? -2.0 * S.zeta[m] * chi[m] : -S.zeta[m] * chi[m] / rn[I];
     gch[3 * m] = gf * d0;
     gch[3 * m + 1] = gf * d1;
     gch[3 * m + 2] = gf * d2;
@@ -572,7 +572,10 @@ __device__ __forceinline__ void drift_sum(const double *og, const T *B, int n, i
 
 }  // namespace
 
-template <typename T, int PROP>
+// EXT: the DMC modes, local operators and Gaussian basis functions are
+// compiled in (the plain VMC instantiations leave them out, keeping their
+// register budget).
+template <typename T, int PROP, bool EXT>
 __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
   constexpr int DPM = sj_draws_per_move(PROP);
   extern __shared__ __align__(16) unsigned char lds[];
@@ -607,6 +610,7 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
   wk.mv = (double *)(base + lay.mv);
   const int nB = nu * nu + nd * nd;
   const T *Bg = (const T *)S.B;
+  const bool gto = EXT && S.any_gto;
 
   // ---- state in ----
   for (int e = lane; e < 3 * N * nslots; e += 64) {
@@ -638,14 +642,14 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
     __syncthreads();
   }
   if (a.init || a.refresh) {
-    const double c0 = refresh_spin<T>(S, wk, 0, t, L);
-    const double c1 = refresh_spin<T>(S, wk, 1, t, L);
+    const double c0 = refresh_spin<T>(S, wk, 0, t, L, gto);
+    const double c1 = refresh_spin<T>(S, wk, 1, t, L, gto);
     if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
   }
-  if (a.dmc == kSJDmcInit) {
+  if (EXT && a.dmc == kSJDmcInit) {
     // DmcRunner::new: weight 1; the local energy of the starting configuration
     double dlg[kSJNParam];
-    const double E = local_energy<T>(S, wk, t, L, dlg);
+    const double E = local_energy<T, EXT>(S, wk, t, L, dlg);
     if (t == 0 && live) {
       S.dmc[w] = 1.0;
       S.dmc[(size_t)W + w] = E;
@@ -743,8 +747,8 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
         __syncthreads();
       } else {
         // drift at x (metrop.rs:150-163): x' = (x_i + tau v) + sqrt(tau) z
-        basis_to_lds(S, xi, wk.rn, wk.chi, t, L);
-        drift_terms(S, wk.pos, i, xi, wk.rn, wk.chi, gch, jg, t, L);
+        basis_to_lds(S, xi, wk.rn, wk.chi, t, L, gto);
+        drift_terms(S, wk.pos, i, xi, wk.rn, wk.chi, gch, jg, t, L, gto);
         __syncthreads();
         orbital_gradients(S, n, gch, og, t, L);
         __syncthreads();
@@ -779,10 +783,10 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
           wk.jo[j] = en_term<T>(aZ, S.p[4], xi, S.nuc + 3 * I);
         }
       }
-      basis_to_lds(S, xn, wk.rn, wk.chi, t, L);
+      basis_to_lds(S, xn, wk.rn, wk.chi, t, L, gto);
       for (int k = t; k < n; k += L) wk.phin[k] = (T)orbital_value(S, k, wk.chi);
       if constexpr (PROP == kSJPropDiffuse) {
-        drift_terms(S, wk.pos, i, xn, wk.rn, wk.chi, gch, jg, t, L);
+        drift_terms(S, wk.pos, i, xn, wk.rn, wk.chi, gch, jg, t, L, gto);
         __syncthreads();
         orbital_gradients(S, n, gch, og, t, L);
       }
@@ -865,20 +869,20 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
       __syncthreads();
       ++m;
       if (lu_period && m % lu_period == 0) {
-        const double c0 = refresh_spin<T>(S, wk, 0, t, L);
-        const double c1 = refresh_spin<T>(S, wk, 1, t, L);
+        const double c0 = refresh_spin<T>(S, wk, 0, t, L, gto);
+        const double c1 = refresh_spin<T>(S, wk, 1, t, L, gto);
         if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
       }
     }
     p += DPM * (uint64_t)N;
     if (t == 0 && live && S.trace_mask && ls.trace_sweeps + sw < a.cap_sweeps)
       S.trace_mask[(size_t)w * a.cap_sweeps + ls.trace_sweeps + sw] = mask;
-    if (sw >= a.sample_start && a.dmc == kSJDmcStep) {
+    if (EXT && sw >= a.sample_start && a.dmc == kSJDmcStep) {
       // DmcRunner::diffuse (src/dmc/src/dmc.rs:88-128): the ensemble terms
       // with the weight before the step, then
       // w *= exp(-tau ((E_old + E_new)/2 - E_ref))
       double dlg[kSJNParam];
-      const double E = local_energy<T>(S, wk, t, L, dlg);
+      const double E = local_energy<T, EXT>(S, wk, t, L, dlg);
       if (t == 0 && live) {
         const double eo = S.dmc[(size_t)W + w], wt = S.dmc[w];
         S.dmc[2 * (size_t)W + w] = wt * eo;
@@ -889,8 +893,8 @@ __global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
     } else if (sw >= a.sample_start) {
       double dlg[kSJNParam];
       SJParts parts;
-      const double E = local_energy<T>(S, wk, t, L, dlg, &parts);
-      if (t == 0 && live && S.n_obs) {
+      const double E = local_energy<T, EXT>(S, wk, t, L, dlg, &parts);
+      if (EXT && t == 0 && live && S.n_obs) {
         // further observables: per-walker sums in HBM (opt-in, sampled sweeps only)
         for (int k = 0; k < S.n_obs; ++k) {
           const double v = operator_value(S.ops[1 + k], parts, S, wk.pos);
@@ -964,16 +968,22 @@ hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n
   return hipGetLastError();
 }
 
-template <typename T, int PROP>
-static hipError_t launch_sj_t(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
+template <typename T, int PROP, bool EXT>
+static hipError_t launch_sj_e(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
   if (lds_bytes > 65536) {
     // more than the default 64 KiB of dynamic LDS needs an opt-in per kernel
-    const hipError_t e = hipFuncSetAttribute((const void *)sj_sweep_kernel<T, PROP>,
+    const hipError_t e = hipFuncSetAttribute((const void *)sj_sweep_kernel<T, PROP, EXT>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((sj_sweep_kernel<T, PROP>), dim3(n_tiles), dim3(64), lds_bytes, stream, a);
+  hipLaunchKernelGGL((sj_sweep_kernel<T, PROP, EXT>), dim3(n_tiles), dim3(64), lds_bytes, stream, a);
   return hipGetLastError();
+}
+
+template <typename T, int PROP>
+static hipError_t launch_sj_t(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
+  return a.ext ? launch_sj_e<T, PROP, true>(a, n_tiles, lds_bytes, stream)
+               : launch_sj_e<T, PROP, false>(a, n_tiles, lds_bytes, stream);
 }
 
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_t lds_bytes,
