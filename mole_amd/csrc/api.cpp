@@ -415,8 +415,13 @@ SJArgs sj_args(const mole_ctx *ctx) {
   a.cap_sweeps = (ctx->run.flags & MOLE_FLAG_TRACE) ? ctx->run.record_sweeps : 0;
   a.cap_samples = a.cap_sweeps;
   if (ctx->mlp) a.net = ctx->mlp->net();
-  for (const SysState &s : ctx->sys)
+  for (const SysState &s : ctx->sys) {
     if (s.any_gto || s.custom_h || s.n_obs) a.ext = 1;
+    // learned proposal: from 6 electrons on, the register budget of one
+    // wave per SIMD (no spills) beats two waves (measured: Li..B faster
+    // with two, C even, O and Ne faster with one)
+    if (s.N >= 6) a.one_wave = 1;
+  }
   return a;
 }
 

@@ -108,6 +108,7 @@ struct SJArgs {
   int dmc;                 // SJDmcMode
   double dmc_tau;          // DMC: the weights' time step
   int ext;                 // 1: DMC, local operators or Gaussian bases in use (the extended kernel)
+  int one_wave;            // learned proposal: the kernel fitted to one wave per SIMD
 };
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);

@@ -575,8 +575,11 @@ __device__ __forceinline__ void drift_sum(const double *og, const T *B, int n, i
 // EXT: the DMC modes, local operators and Gaussian basis functions are
 // compiled in (the plain VMC instantiations leave them out, keeping their
 // register budget).
-template <typename T, int PROP, bool EXT>
-__global__ void __launch_bounds__(64, 2) sj_sweep_kernel(SJArgs a) {
+// MINW: minimum waves per SIMD the register budget is fitted to (2: up to
+// 256 registers; 1: 512, for the learned proposal's larger systems, where
+// spills cost more than the lost occupancy).
+template <typename T, int PROP, bool EXT, int MINW>
+__global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
   constexpr int DPM = sj_draws_per_move(PROP);
   extern __shared__ __align__(16) unsigned char lds[];
   const SJTile tile = a.tiles[blockIdx.x];
@@ -968,16 +971,23 @@ hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n
   return hipGetLastError();
 }
 
-template <typename T, int PROP, bool EXT>
-static hipError_t launch_sj_e(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
+template <typename T, int PROP, bool EXT, int MINW>
+static hipError_t launch_sj_m(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
   if (lds_bytes > 65536) {
     // more than the default 64 KiB of dynamic LDS needs an opt-in per kernel
-    const hipError_t e = hipFuncSetAttribute((const void *)sj_sweep_kernel<T, PROP, EXT>,
+    const hipError_t e = hipFuncSetAttribute((const void *)sj_sweep_kernel<T, PROP, EXT, MINW>,
                                              hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
     if (e != hipSuccess) return e;
   }
-  hipLaunchKernelGGL((sj_sweep_kernel<T, PROP, EXT>), dim3(n_tiles), dim3(64), lds_bytes, stream, a);
+  hipLaunchKernelGGL((sj_sweep_kernel<T, PROP, EXT, MINW>), dim3(n_tiles), dim3(64), lds_bytes, stream, a);
   return hipGetLastError();
+}
+
+template <typename T, int PROP, bool EXT>
+static hipError_t launch_sj_e(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
+  if constexpr (PROP == kSJPropLearned)
+    if (a.one_wave) return launch_sj_m<T, PROP, EXT, 1>(a, n_tiles, lds_bytes, stream);
+  return launch_sj_m<T, PROP, EXT, 2>(a, n_tiles, lds_bytes, stream);
 }
 
 template <typename T, int PROP>
