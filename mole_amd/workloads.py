@@ -150,8 +150,8 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
     return out
 
 
-def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=2000,
-            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.3, 0.6, 1.0), precision=M.PREC_FP64,
+def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=8000,
+            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.6, 1.0, 1.5, 2.0, 3.0), precision=M.PREC_FP64,
             lu_period=64, seed=1, log=print):
     """Learned proposal against the box proposal on one atom: configurations
     from the box sampler, Adam training on the device, then runs of each
