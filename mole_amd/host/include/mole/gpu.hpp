@@ -287,6 +287,13 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
     if (st.is_err()) return st;
     return refresh_data();
   }
+  // the "Energy" operator (mole_set_hamiltonian): the operators! "Energy"
+  // entry of a run with a user LocalOperator made of device terms
+  Status set_hamiltonian(const mole_operator &h) {
+    h_ = h;
+    has_h_ = true;
+    return status(mole_set_hamiltonian(ctx_->get(), 0, &h_));
+  }
   bool streams_blocks() const override { return record_ > 0; }
   Status run_fused_streaming(size_t steps, size_t block_size, const std::function<void()> &after_block) override {
     if (!record_) return run_fused(steps, block_size);
@@ -359,6 +366,7 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
     if (ctx_->ok()) {
       (void)mole_set_configs(ctx_->get(), 0, cfg.data(), walkers_);
       if (!restart_stream) (void)mole_set_stream(ctx_->get(), epoch, pos);
+      if (has_h_) (void)mole_set_hamiltonian(ctx_->get(), 0, &h_);
     }
   }
   mole_run_desc run() const {
@@ -438,6 +446,8 @@ class GpuSampler : public MonteCarloSampler<WF>, public FusedSampler {
   uint64_t seed_;
   uint32_t block_;
   uint32_t record_ = 0;
+  mole_operator h_{};
+  bool has_h_ = false;
   mole_system_desc desc_{};
   std::shared_ptr<gpu::Context> ctx_;
   DataMap data_;

@@ -12,7 +12,9 @@
 
 #include "mole/errors.hpp"
 #include "mole/gpu.hpp"
+#include "mole/dmc.hpp"
 #include "mole/operator_value.hpp"
+#include "mole/slater_jastrow.hpp"
 #include "mole/runner.hpp"
 #include "mole/traits.hpp"
 
@@ -207,6 +209,29 @@ static int run_gpu(int iters, int walkers, int sweeps, int block, double box, do
     return 4;
   }
   const size_t n_bm = mc.value().data.at("Energy block means").size();
+  // examples/custom_operator.rs on the device: the Gaussian trial under
+  // HarmonicHamiltonian, Runner::run(1000, 1), box side 1
+  GpuSampler<GpuSlaterJastrow> sho(GpuSlaterJastrow::gaussian_oscillator(1.0), 64u, 1.0, 0, 1u);
+  const Status hs = sho.set_hamiltonian(operators::harmonic_hamiltonian(1.0).op);
+  if (hs.is_err()) {
+    std::fprintf(stderr, "set_hamiltonian: %s\n", hs.error().message.c_str());
+    return 5;
+  }
+  Runner<GpuSampler<GpuSlaterJastrow>> sho_runner(std::move(sho), log);
+  auto sho_res = sho_runner.run(1000, 1);
+  if (sho_res.is_err()) return 6;
+  const double sho_energy = sho_res.value().data.at("Energy").front().get_scalar().value();
+  // DmcRunner on the hydrogen 1s function (exact: every energy -1/2)
+  GpuDmcRunner dmc(GpuSlaterJastrow::hydrogenic(1), 128, -0.5, std::nullopt, 0.05, 2, Branching::SR);
+  const auto de = dmc.diffuse(0.05, 40, 10, 1);
+  const double dmc_energy = de.first.empty() ? 0.0 : de.first.back();
+  bool dmc_panics = false;
+  try {
+    GpuDmcRunner bad(GpuSlaterJastrow::helium(1.6875, 0.5, 0.3), 16, -1e3, std::nullopt, 0.02, 2, Branching::SR);
+    (void)bad.diffuse(1e4, 2, 1, 0);  // every weight underflows: WeightedChoice's panic
+  } catch (const Panic &) {
+    dmc_panics = true;
+  }
   // single-configuration evaluation through the traits
   GpuWaveFunction he = GpuWaveFunction::helium(1.6875, 0.5, 0.3);
   Array2 x(2, 3);
@@ -219,8 +244,9 @@ static int run_gpu(int iters, int walkers, int sweeps, int block, double box, do
   for (size_t k = 0; k < r.energy_errors.size(); ++k) std::printf("%s%.17g", k ? ", " : "", r.energy_errors[k]);
   std::printf("], \"he_value\": %.17g, \"he_laplacian\": %.17g, \"he_local_energy\": %.17g, \"he_grad\": [", v, lap, el);
   for (size_t k = 0; k < 6; ++k) std::printf("%s%.17g", k ? ", " : "", g.data[k]);
-  std::printf("], \"stream_calls\": %zu, \"stream_blocks\": %zu, \"result_blocks\": %zu}\n", counter.calls,
-              counter.last_blocks, n_bm);
+  std::printf("], \"stream_calls\": %zu, \"stream_blocks\": %zu, \"result_blocks\": %zu, \"sho_energy\": %.17g, "
+              "\"dmc_energy\": %.17g, \"dmc_blocks\": %zu, \"dmc_panics\": %d}\n",
+              counter.calls, counter.last_blocks, n_bm, sho_energy, dmc_energy, de.first.size(), dmc_panics ? 1 : 0);
   return 0;
 }
 

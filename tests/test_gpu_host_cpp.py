@@ -48,3 +48,8 @@ def test_cpp_vmc_runner_matches_python_schedule():
     # the streaming Runner::run of 6 blocks: Log::log after each of the 5
     # sampled blocks, the last call seeing all 5 block means
     assert out["stream_calls"] == 5 and out["stream_blocks"] == 5 and out["result_blocks"] == 5
+    # examples/custom_operator.rs through GpuSampler<GpuSlaterJastrow> with a
+    # device HarmonicHamiltonian; DmcRunner on hydrogen; the panic path
+    assert abs(out["sho_energy"] - 1.5) < 1e-12
+    assert abs(out["dmc_energy"] + 0.5) < 1e-12 and out["dmc_blocks"] == 3
+    assert out["dmc_panics"] == 1
