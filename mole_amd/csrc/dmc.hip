@@ -189,9 +189,38 @@ hipError_t enqueue_scan(uint64_t *x, size_t n_max, const uint32_t *n_dev, uint64
 
 // ---- SRBrancher (branching.rs:15-44) ----
 
-__global__ void sr_counts_kernel(DmcArgs a) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < a.W) a.cnt_a[i] = as_u32(a.dmc[i] * a.st->norm);
+// block-local inclusive scan of the integer weights (w W / max w) as u32,
+// computed on the fly; block totals to bsum_a
+__global__ void __launch_bounds__(kScanThreads) sr_scan_local_kernel(DmcArgs a) {
+  __shared__ uint64_t sh[kScanThreads];
+  const size_t n = (size_t)a.W;
+  const double norm = a.st->norm;
+  const int t = threadIdx.x;
+  const size_t e0 = (size_t)blockIdx.x * kScanBlock + (size_t)t * kScanPerThread;
+  uint64_t v[kScanPerThread];
+  uint64_t run = 0;
+  for (int k = 0; k < kScanPerThread; ++k) {
+    run += (e0 + k < n) ? as_u32(a.dmc[e0 + k] * norm) : 0;
+    v[k] = run;
+  }
+  sh[t] = run;
+  __syncthreads();
+  for (int off = 1; off < kScanThreads; off <<= 1) {
+    const uint64_t add = (t >= off) ? sh[t - off] : 0;
+    __syncthreads();
+    sh[t] += add;
+    __syncthreads();
+  }
+  const uint64_t before = (t > 0) ? sh[t - 1] : 0;
+  for (int k = 0; k < kScanPerThread; ++k)
+    if (e0 + k < n) a.cnt_a[e0 + k] = v[k] + before;
+  if (t == kScanThreads - 1) a.bsum_a[blockIdx.x] = sh[t];
+}
+
+// the cumulative integer weight through walker k: block-local sum plus the
+// block's offset (bsum_a after the exclusive block scan)
+__device__ __forceinline__ uint64_t sr_cum(const DmcArgs &a, uint32_t k) {
+  return a.cnt_a[k] + a.bsum_a[k / kScanBlock];
 }
 
 // WeightedChoice: the first index whose cumulative weight exceeds u
@@ -199,7 +228,7 @@ __global__ void sr_select_kernel(DmcArgs a) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= a.W) return;
   DmcState *st = a.st;
-  const uint64_t total = a.cnt_a[a.W - 1];
+  const uint64_t total = sr_cum(a, (uint32_t)a.W - 1);
   if (j == 0) st->total = total;
   if (total == 0) {  // WeightedChoice::new panics: flag it, keep the walkers
     if (j == 0) st->error = 1;
@@ -213,7 +242,7 @@ __global__ void sr_select_kernel(DmcArgs a) {
   uint32_t lo = 0, hi = (uint32_t)a.W - 1;
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
-    if (a.cnt_a[mid] > u) hi = mid;
+    if (sr_cum(a, mid) > u) hi = mid;
     else lo = mid + 1;
   }
   a.src[j] = lo;
@@ -340,9 +369,57 @@ __global__ void copy_kernel(const E *in, E *out, size_t n) {
     out[e] = in[e];
 }
 
+// Positions, inverses and cached local energies of the chosen walkers into
+// scratch: one flat index over [3N positions | inverses | energy] x W.
+__global__ void gather_all_kernel(DmcArgs a) {
+  const size_t W = (size_t)a.W;
+  const size_t np = (size_t)a.pos_rows * W, nb = (size_t)a.b_rows * W, n = np + nb + W;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < np) {
+      const size_t r = e / W, j = e % W;
+      a.pos_s[e] = a.pos[r * W + a.src[j]];
+    } else if (e < np + nb) {
+      const size_t f = e - np, r = f / W, j = f % W;
+      if (a.b_elem == 4)
+        ((float *)a.B_s)[f] = ((const float *)a.B)[r * W + a.src[j]];
+      else
+        ((double *)a.B_s)[f] = ((const double *)a.B)[r * W + a.src[j]];
+    } else {
+      const size_t j = e - np - nb;
+      a.el_s[j] = a.dmc[W + a.src[j]];
+    }
+  }
+}
+
+__device__ void step_end(const DmcArgs &a);
+
+// scratch back into the walker state, the new weights into row 0; one
+// thread then runs the step's bookkeeping
+__global__ void copy_all_kernel(DmcArgs a) {
+  const size_t W = (size_t)a.W;
+  const size_t np = (size_t)a.pos_rows * W, nb = (size_t)a.b_rows * W, n = np + nb + 2 * W;
+  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+    if (e < np) {
+      a.pos[e] = a.pos_s[e];
+    } else if (e < np + nb) {
+      const size_t f = e - np;
+      if (a.b_elem == 4)
+        ((float *)a.B)[f] = ((const float *)a.B_s)[f];
+      else
+        ((double *)a.B)[f] = ((const double *)a.B_s)[f];
+    } else if (e < np + nb + W) {
+      const size_t j = e - np - nb;
+      a.dmc[W + j] = a.el_s[j];
+    } else {
+      const size_t j = e - np - nb - W;
+      a.dmc[j] = a.wnew[j];
+    }
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) step_end(a);
+}
+
 // update_energies (dmc.rs:155-202) at the end of a block; counters advance
-__global__ void step_end_kernel(DmcArgs a) {
-  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+__device__ void step_end(const DmcArgs &a) {
   DmcState *st = a.st;
   st->steps += 1;
   st->it += 1;
@@ -417,8 +494,9 @@ hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream) {
   hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kRedThreads), 0, stream, a, n_parts);
   hipError_t e;
   if (a.brancher == kDmcBranchSR) {
-    hipLaunchKernelGGL(sr_counts_kernel, gw, b256, 0, stream, a);
-    if ((e = enqueue_scan(a.cnt_a, (size_t)W, nullptr, a.bsum_a, stream)) != hipSuccess) return e;
+    const int nb = (int)dmc_scan_blocks((size_t)W);
+    hipLaunchKernelGGL(sr_scan_local_kernel, dim3(nb), dim3(kScanThreads), 0, stream, a);
+    hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(kScanThreads), 0, stream, a.bsum_a, nb);
     hipLaunchKernelGGL(sr_select_kernel, gw, b256, 0, stream, a);
   } else {
     hipLaunchKernelGGL(simple_counts_kernel, gw, b256, 0, stream, a);
@@ -433,34 +511,9 @@ hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream) {
     if ((e = enqueue_scan(a.cnt_a, 3 * (size_t)W, &a.st->M, a.bsum_a, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(simple_final_kernel, g3, b256, 0, stream, a);
   }
-  // gather into scratch, then back
-  const size_t np = (size_t)a.pos_rows * W, nbr = (size_t)a.b_rows * W;
-  hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(grid_for(np)), b256, 0, stream, (const double *)a.pos, a.pos_s,
-                     a.pos_rows, W, (const uint32_t *)a.src);
-  if (nbr) {
-    if (a.b_elem == 4)
-      hipLaunchKernelGGL(gather_rows_kernel<float>, dim3(grid_for(nbr)), b256, 0, stream, (const float *)a.B,
-                         (float *)a.B_s, a.b_rows, W, (const uint32_t *)a.src);
-    else
-      hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(grid_for(nbr)), b256, 0, stream, (const double *)a.B,
-                         (double *)a.B_s, a.b_rows, W, (const uint32_t *)a.src);
-  }
-  hipLaunchKernelGGL(gather_rows_kernel<double>, dim3(grid_for((size_t)W)), b256, 0, stream,
-                     (const double *)(a.dmc + W), a.el_s, 1, W, (const uint32_t *)a.src);
-  hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for(np)), b256, 0, stream, (const double *)a.pos_s, a.pos, np);
-  if (nbr) {
-    if (a.b_elem == 4)
-      hipLaunchKernelGGL(copy_kernel<float>, dim3(grid_for(nbr)), b256, 0, stream, (const float *)a.B_s, (float *)a.B,
-                         nbr);
-    else
-      hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for(nbr)), b256, 0, stream, (const double *)a.B_s,
-                         (double *)a.B, nbr);
-  }
-  hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for((size_t)W)), b256, 0, stream, (const double *)a.el_s,
-                     a.dmc + W, (size_t)W);
-  hipLaunchKernelGGL(copy_kernel<double>, dim3(grid_for((size_t)W)), b256, 0, stream, (const double *)a.wnew, a.dmc,
-                     (size_t)W);
-  hipLaunchKernelGGL(step_end_kernel, dim3(1), dim3(64), 0, stream, a);
+  const size_t n_all = (size_t)(a.pos_rows + a.b_rows + 2) * W;
+  hipLaunchKernelGGL(gather_all_kernel, dim3(grid_for(n_all)), b256, 0, stream, a);
+  hipLaunchKernelGGL(copy_all_kernel, dim3(grid_for(n_all)), b256, 0, stream, a);
   return hipGetLastError();
 }
 

@@ -893,13 +893,48 @@ static uint64_t rust_as_usize(double x) {
   return (uint64_t)x;
 }
 
+/* Sums over walkers in the device's fixed order (the reference's
+ * sequential fold, `walkers.iter().fold`, differs from it in the last bits;
+ * its order is ndarray's/std's and not pinned): blocks of 4096 walkers, in
+ * each block thread t of 256 adds elements t, t + 256, ... in order, then a
+ * pairwise tree over the threads (offsets 128, 64, ..., 1); the block sums
+ * are reduced the same way (thread t adds blocks t, t + 256, ...; tree). */
+static double tree256(double *sh) {
+  for (int off = 128; off > 0; off >>= 1)
+    for (int t = 0; t < off; ++t) sh[t] += sh[t + off];
+  return sh[0];
+}
+
+double orc_device_sum(const double *x, size_t n) {
+  const size_t chunk = 4096, nb = (n + chunk - 1) / chunk;
+  double *part = (double *)malloc(sizeof(double) * (nb ? nb : 1));
+  double sh[256];
+  for (size_t b = 0; b < nb; ++b) {
+    for (int t = 0; t < 256; ++t) {
+      double s = 0.0;
+      for (int k = 0; k < 16; ++k) {
+        const size_t i = b * chunk + (size_t)k * 256 + (size_t)t;
+        if (i < n) s += x[i];
+      }
+      sh[t] = s;
+    }
+    part[b] = tree256(sh);
+  }
+  for (int t = 0; t < 256; ++t) {
+    double s = 0.0;
+    for (size_t i = (size_t)t; i < nb; i += 256) s += part[i];
+    sh[t] = s;
+  }
+  const double r = tree256(sh);
+  free(part);
+  return r;
+}
+
 int orc_branch_sr(const double *w, uint32_t W, uint64_t seed, uint32_t sys, uint64_t step, uint32_t *src,
                   double *w_out) {
-  double tot = 0.0, mx = 0.0;
-  for (uint32_t i = 0; i < W; ++i) {
-    tot += w[i];
-    mx = fmax(mx, w[i]); /* f64::max ignores NaN */
-  }
+  double mx = 0.0;
+  for (uint32_t i = 0; i < W; ++i) mx = fmax(mx, w[i]); /* f64::max ignores NaN; max is order-free */
+  const double tot = orc_device_sum(w, W);
   const double global_weight = tot / (double)W;
   const double norm = (double)W / mx;
   uint64_t *cum = (uint64_t *)malloc(sizeof(uint64_t) * (W ? W : 1));
@@ -1042,11 +1077,11 @@ int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st
   const uint32_t blocks = num_iterations / block_size;
   int n_out = 0;
   double *vars = (double *)malloc(sizeof(double) * (blocks ? blocks : 1));
+  double *we = (double *)malloc(sizeof(double) * W), *wo = (double *)malloc(sizeof(double) * W);
   uint64_t it = 0;
   int rc = 0;
   for (uint32_t block_nr = 0; block_nr < blocks && rc == 0; ++block_nr) {
     for (uint32_t j = 0; j < block_size; ++j, ++it) {
-      double total_weight = 0.0, ensemble = 0.0;
       for (uint32_t w = 0; w < W; ++w) {
         double *c = st->cfg + (size_t)w * 3 * N;
         double *B = st->B + (size_t)w * nb;
@@ -1065,15 +1100,15 @@ int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st
         sj_sweep_once(&t, w, c, Bu, Bd, Buf, Bdf, &pos, &m);
         for (int k = 0; k < nu * nu; ++k) B[k] = p->precision == ORC_PREC_FP64 ? Bu[k] : (double)Buf[k];
         for (int k = 0; k < nd * nd; ++k) B[nu * nu + k] = p->precision == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
-        ensemble += st->weight[w] * local_e;
-        total_weight += st->weight[w];
+        we[w] = st->weight[w] * local_e;
+        wo[w] = st->weight[w];
         const double local_e_new = dmc_energy(wf, p, c, B);
         st->weight[w] *= orc_exp(-time_step * ((local_e + local_e_new) / 2.0 - st->reference_energy));
         st->e_local[w] = local_e_new;
       }
       st->draw += 6 * (uint64_t)N;
       st->moves += (uint64_t)N;
-      ensemble /= total_weight;
+      const double ensemble = orc_device_sum(we, W) / orc_device_sum(wo, W);
       eblock[j] = ensemble;
       if (step_energy) step_energy[it] = ensemble;
       if (p->brancher == ORC_BRANCH_SR) {
@@ -1126,5 +1161,7 @@ int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st
   free(src);
   free(eblock);
   free(vars);
+  free(we);
+  free(wo);
   return rc ? -1 : n_out;
 }

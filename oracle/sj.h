@@ -173,6 +173,10 @@ int orc_sj_run_faithful(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t 
 #define ORC_DMC_BRANCH_EPOCH 0x444D4301u
 enum { ORC_BRANCH_SR = 0, ORC_BRANCH_SIMPLE = 1 };
 
+/* Sum over walkers in the device's fixed order (blocks of 4096, 256
+ * strided partial sums, pairwise trees); the reference folds sequentially,
+ * an order that is not pinned and differs only in the last bits. */
+double orc_device_sum(const double *x, size_t n);
 /* SRBrancher::branch (branching.rs:15-44): integer weights
  * (w W / max w) as u32, WeightedChoice (first index whose cumulative
  * integer weight exceeds floor(d total / 2^64)) for each of the W new

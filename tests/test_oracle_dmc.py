@@ -185,3 +185,28 @@ def test_helium_ground_state_and_block_bookkeeping(brancher):
     assert abs(e[-1] - HE_EXACT) < 3 * err[-1] + 0.01, (e[-1], err[-1])
     # the DMC energy is below the trial function's variational energy (-2.878)
     assert e[-1] < -2.885
+
+
+def test_device_order_sum():
+    # orc_device_sum follows the device's fixed order; it is a sum: within a
+    # few ulps of the exact (fsum) value, and for n <= 256 it is one pairwise
+    # tree over the inputs (each thread holds one element)
+    import ctypes as C
+    import math
+    from oracle import oracle_ffi as OF
+    L = OF.lib()
+    L.orc_device_sum.restype = C.c_double
+    L.orc_device_sum.argtypes = [C.POINTER(C.c_double), C.c_size_t]
+    rng = np.random.default_rng(1)
+    for n in (1, 7, 256, 257, 4096, 4097, 70000):
+        x = np.ascontiguousarray(rng.exponential(1.0, n))
+        s = L.orc_device_sum(OF.dptr(x), n)
+        assert abs(s - math.fsum(x)) <= 4 * n * np.finfo(float).eps * math.fsum(x)
+    x = np.ascontiguousarray(rng.normal(size=200))
+    sh = np.zeros(256)
+    sh[:200] = x
+    off = 128
+    while off:
+        sh[:off] = sh[:off] + sh[off:2 * off]
+        off //= 2
+    assert L.orc_device_sum(OF.dptr(x), 200) == sh[0]
