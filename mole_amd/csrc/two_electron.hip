@@ -422,45 +422,54 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
         const uint32_t s = s0 + t;
         if (s >= a.n_sweeps) break;
         uint64_t mask = 0;
+        // electron-local parts of both moves first (trial position, distances to
+        // the nuclei, orbital values: none depends on the other electron's
+        // decision), then each move's coupled part (r12, Jastrow, ratio) in
+        // order; the updates are selects.  Every value is computed by the same
+        // operations from the same operands as move-by-move code, so the chain
+        // is bit-identical while its two halves overlap in the pipeline.
+        double nx[2][3], nr[2][NNUC], ne[2][NNUC], nphi[2], uu[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          const double dx = rng_lds[buf][t][e][0][lane];
-          const double dy = rng_lds[buf][t][e][1][lane];
-          const double dz = rng_lds[buf][t][e][2][lane];
-          const double u = rng_lds[buf][t][e][3][lane];
-          const double nx = x[3 * e] + dx, ny = x[3 * e + 1] + dy, nz = x[3 * e + 2] + dz;
-          const double n12 = (e == 0) ? norm3(nx - x[3], ny - x[4], nz - x[5])
-                                      : norm3(x[0] - nx, x[1] - ny, x[2] - nz);
-          double nr[NNUC], ne[NNUC], nphi = 0.0, psin;
+          nx[e][0] = x[3 * e] + rng_lds[buf][t][e][0][lane];
+          nx[e][1] = x[3 * e + 1] + rng_lds[buf][t][e][1][lane];
+          nx[e][2] = x[3 * e + 2] + rng_lds[buf][t][e][2][lane];
+          uu[e] = rng_lds[buf][t][e][3][lane];
+          nphi[e] = 0.0;
 #pragma unroll
           for (int I = 0; I < NNUC; ++I) {
-            nr[I] = norm3(nx - sp.nuc[I][0], ny - sp.nuc[I][1], nz - sp.nuc[I][2]);
-            ne[I] = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : exp_m(-sp.zeta * nr[I]);
-            nphi = (I == 0) ? ne[I] : nphi + ne[I];
+            nr[e][I] = norm3(nx[e][0] - sp.nuc[I][0], nx[e][1] - sp.nuc[I][1], nx[e][2] - sp.nuc[I][2]);
+            ne[e][I] = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : exp_m(-sp.zeta * nr[e][I]);
+            nphi[e] = (I == 0) ? ne[e][I] : nphi[e] + ne[e][I];
           }
+        }
+#pragma unroll
+        for (int e = 0; e < 2; ++e) {
+          const double n12 = (e == 0) ? norm3(nx[0][0] - x[3], nx[0][1] - x[4], nx[0][2] - x[5])
+                                      : norm3(x[0] - nx[1][0], x[1] - nx[1][1], x[2] - nx[1][2]);
+          double psin;
           if (KIND == MOLE_WF_HELIUM_PADE) {
-            const double s12 = (e == 0) ? (nr[0] + r[1][0]) : (r[0][0] + nr[0]);
+            const double s12 = (e == 0) ? (nr[0][0] + r[1][0]) : (r[0][0] + nr[1][0]);
             psin = exp_m(-sp.zeta * s12 + pade(sp.b, sp.c, n12));
           } else {
             const double jas = exp_m(pade(sp.b, sp.c, n12));
-            psin = ((e == 0) ? (nphi * phi[1]) : (phi[0] * nphi)) * jas;
+            psin = ((e == 0) ? (nphi[0] * phi[1]) : (phi[0] * nphi[1])) * jas;
           }
           const double ratio = min_one((psin * psin) / (psi * psi));
-          if (ratio > u) {
-            x[3 * e] = nx;
-            x[3 * e + 1] = ny;
-            x[3 * e + 2] = nz;
+          const bool acc = ratio > uu[e];
+          x[3 * e] = acc ? nx[e][0] : x[3 * e];
+          x[3 * e + 1] = acc ? nx[e][1] : x[3 * e + 1];
+          x[3 * e + 2] = acc ? nx[e][2] : x[3 * e + 2];
 #pragma unroll
-            for (int I = 0; I < NNUC; ++I) {
-              r[e][I] = nr[I];
-              ee[e][I] = ne[I];
-            }
-            phi[e] = nphi;
-            r12 = n12;
-            psi = psin;
-            accepted += 1.0;
-            mask |= (uint64_t)1 << e;
+          for (int I = 0; I < NNUC; ++I) {
+            r[e][I] = acc ? nr[e][I] : r[e][I];
+            ee[e][I] = acc ? ne[e][I] : ee[e][I];
           }
+          phi[e] = acc ? nphi[e] : phi[e];
+          r12 = acc ? n12 : r12;
+          psi = acc ? psin : psi;
+          accepted += acc ? 1.0 : 0.0;
+          mask |= acc ? ((uint64_t)1 << e) : 0;
         }
         if (live && a.trace_mask && a.trace_sweep0 + s < a.cap_sweeps)
           a.trace_mask[(size_t)w * a.cap_sweeps + a.trace_sweep0 + s] = mask;
