@@ -628,8 +628,12 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
   }
   // accumulators: LDS, touched by sub-lane 0 only
   double *acc = (double *)(base + lay.acc);
-  if (t == 0)
-    for (int k = 0; k < kSJColumns; ++k) acc[k] = live ? S.acc[(size_t)k * W + w] : 0.0;
+  if (t == 0) {
+    // DMC steps touch only the accepted-move count and the condition maximum
+    const bool dmc_step = EXT && a.dmc == kSJDmcStep;
+    for (int k = 0; k < kSJColumns; ++k)
+      if (!dmc_step || k == kSJAccepted || k == kSJCondMax) acc[k] = live ? S.acc[(size_t)k * W + w] : 0.0;
+  }
   __syncthreads();
 
   const uint32_t k0 = (uint32_t)a.seed, k1 = (uint32_t)(a.seed >> 32) ^ S.sys_index;
@@ -943,7 +947,8 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
     if (sl < tile.nw) Bo[(size_t)k * W + tile.w0 + sl] = ((T *)(lds + (size_t)sl * lay.bytes + lay.B))[k];
   }
   if (t == 0 && live)
-    for (int k = 0; k < kSJColumns; ++k) S.acc[(size_t)k * W + w] = acc[k];
+    for (int k = 0; k < kSJColumns; ++k)
+      if (!(EXT && a.dmc == kSJDmcStep) || k == kSJAccepted || k == kSJCondMax) S.acc[(size_t)k * W + w] = acc[k];
 }
 
 __global__ void sj_advance_kernel(SJLaunchState *st, uint32_t n_sweeps, uint32_t n_samples,
