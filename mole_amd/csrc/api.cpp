@@ -1772,9 +1772,14 @@ mole_status dmc_enqueue_step(mole_ctx *ctx, double time_step, uint32_t block_siz
   a.dmc_tau = time_step;
   a.ext = 1;
   HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
-  HIP_TRY(launch_sj_advance(ctx->d_state, 1, 0, ctx->run.block_size, ctx->stream));
-  for (uint32_t s = 0; s < ctx->sys.size(); ++s)
-    HIP_TRY(launch_dmc_after_step(dmc_args(ctx, s, block_size, num_eq_blocks), ctx->stream));
+  for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
+    DmcArgs d = dmc_args(ctx, s, block_size, num_eq_blocks);
+    if (s == 0) {  // the step's sweep, counted once (no samples)
+      d.sweeps_done = &ctx->d_state->sweeps_done;
+      d.trace_sweeps = &ctx->d_state->trace_sweeps;
+    }
+    HIP_TRY(launch_dmc_after_step(d, ctx->stream));
+  }
   return MOLE_OK;
 }
 

@@ -80,10 +80,11 @@ __global__ void __launch_bounds__(kRedThreads) reduce1_kernel(const double *dmc,
   if (t < 4) part[(size_t)t * n_parts + blockIdx.x] = sh[t][0];
 }
 
-// the step's ensemble energy (dmc.rs:117-121) and SRBrancher's mean weight
-// and normalisation (branching.rs:21-24)
-__global__ void __launch_bounds__(kRedThreads) reduce2_kernel(DmcArgs a, int n_parts) {
-  __shared__ double sh[4][kRedThreads];
+// The second stage over the n_parts partials, in one fixed order (per-thread
+// strided sums, then an LDS tree), leaving the four results in sh[q][0]; with
+// `write` the step's ensemble energy (dmc.rs:117-121) and SRBrancher's mean
+// weight and normalisation (branching.rs:21-24) go to the state.
+__device__ void finish_reduction(const DmcArgs &a, int n_parts, double (*sh)[kRedThreads], bool write) {
   const int t = threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
   for (int i = t; i < n_parts; i += kRedThreads) {
@@ -103,7 +104,7 @@ __global__ void __launch_bounds__(kRedThreads) reduce2_kernel(DmcArgs a, int n_p
     }
     __syncthreads();
   }
-  if (t == 0) {
+  if (write && t == 0) {
     DmcState *st = a.st;
     st->sum_we = sh[0][0];
     st->sum_w = sh[1][0];
@@ -115,6 +116,11 @@ __global__ void __launch_bounds__(kRedThreads) reduce2_kernel(DmcArgs a, int n_p
     st->gw = sh[2][0] / (double)a.W;
     st->norm = (double)a.W / sh[3][0];
   }
+}
+
+__global__ void __launch_bounds__(kRedThreads) reduce2_kernel(DmcArgs a, int n_parts) {
+  __shared__ double sh[4][kRedThreads];
+  finish_reduction(a, n_parts, sh, true);
 }
 
 // ---- inclusive scan of u64 counts (n from the host bound or a device count) ----
@@ -189,12 +195,17 @@ hipError_t enqueue_scan(uint64_t *x, size_t n_max, const uint32_t *n_dev, uint64
 
 // ---- SRBrancher (branching.rs:15-44) ----
 
-// block-local inclusive scan of the integer weights (w W / max w) as u32,
-// computed on the fly; block totals to bsum_a
-__global__ void __launch_bounds__(kScanThreads) sr_scan_local_kernel(DmcArgs a) {
+// Block-local inclusive scan of the integer weights (w W / max w) as u32,
+// computed on the fly; block totals to bsum_a.  Every workgroup first
+// finishes the ensemble reduction itself (the same partials in the same
+// order, so the same W / max w everywhere); workgroup 0 stores the results.
+__global__ void __launch_bounds__(kScanThreads) sr_scan_local_kernel(DmcArgs a, int n_parts) {
+  static_assert(kScanThreads == kRedThreads, "one workgroup shape for both stages");
+  __shared__ double red[4][kRedThreads];
   __shared__ uint64_t sh[kScanThreads];
+  finish_reduction(a, n_parts, red, blockIdx.x == 0);
   const size_t n = (size_t)a.W;
-  const double norm = a.st->norm;
+  const double norm = (double)a.W / red[3][0];
   const int t = threadIdx.x;
   const size_t e0 = (size_t)blockIdx.x * kScanBlock + (size_t)t * kScanPerThread;
   uint64_t v[kScanPerThread];
@@ -217,18 +228,37 @@ __global__ void __launch_bounds__(kScanThreads) sr_scan_local_kernel(DmcArgs a) 
   if (t == kScanThreads - 1) a.bsum_a[blockIdx.x] = sh[t];
 }
 
-// the cumulative integer weight through walker k: block-local sum plus the
-// block's offset (bsum_a after the exclusive block scan)
-__device__ __forceinline__ uint64_t sr_cum(const DmcArgs &a, uint32_t k) {
-  return a.cnt_a[k] + a.bsum_a[k / kScanBlock];
-}
-
-// WeightedChoice: the first index whose cumulative weight exceeds u
-__global__ void sr_select_kernel(DmcArgs a) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+// WeightedChoice: the first index whose cumulative weight exceeds u.  The
+// cumulative weight through walker k is its block-local sum plus the
+// exclusive offset of its scan block.  With LOCAL_OFFSETS (at most
+// kScanThreads scan blocks) every workgroup scans the block totals in LDS
+// itself; otherwise bsum_a already holds the offsets.
+template <bool LOCAL_OFFSETS>
+__global__ void __launch_bounds__(kScanThreads) sr_select_kernel(DmcArgs a) {
+  __shared__ uint64_t off[kScanThreads];
+  const int t = threadIdx.x;
+  if constexpr (LOCAL_OFFSETS) {
+    const int nb = (a.W + kScanBlock - 1) / kScanBlock;
+    const uint64_t v = t < nb ? a.bsum_a[t] : 0;
+    off[t] = v;
+    __syncthreads();
+    for (int o = 1; o < kScanThreads; o <<= 1) {
+      const uint64_t add = (t >= o) ? off[t - o] : 0;
+      __syncthreads();
+      off[t] += add;
+      __syncthreads();
+    }
+    off[t] -= v;  // each thread its own entry: inclusive to exclusive
+    __syncthreads();
+  }
+  auto cum = [&](uint32_t k) -> uint64_t {
+    if constexpr (LOCAL_OFFSETS) return a.cnt_a[k] + off[k / kScanBlock];
+    else return a.cnt_a[k] + a.bsum_a[k / kScanBlock];
+  };
+  const int j = blockIdx.x * blockDim.x + t;
   if (j >= a.W) return;
   DmcState *st = a.st;
-  const uint64_t total = sr_cum(a, (uint32_t)a.W - 1);
+  const uint64_t total = cum((uint32_t)a.W - 1);
   if (j == 0) st->total = total;
   if (total == 0) {  // WeightedChoice::new panics: flag it, keep the walkers
     if (j == 0) st->error = 1;
@@ -242,7 +272,7 @@ __global__ void sr_select_kernel(DmcArgs a) {
   uint32_t lo = 0, hi = (uint32_t)a.W - 1;
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
-    if (sr_cum(a, mid) > u) hi = mid;
+    if (cum(mid) > u) hi = mid;
     else lo = mid + 1;
   }
   a.src[j] = lo;
@@ -354,21 +384,6 @@ __global__ void simple_final_kernel(DmcArgs a) {
 
 // ---- walker state after branching ----
 
-template <typename E>
-__global__ void gather_rows_kernel(const E *in, E *out, int rows, int W, const uint32_t *src) {
-  const size_t n = (size_t)rows * W;
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
-    const size_t r = e / (size_t)W, j = e % (size_t)W;
-    out[e] = in[r * W + src[j]];
-  }
-}
-
-template <typename E>
-__global__ void copy_kernel(const E *in, E *out, size_t n) {
-  for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x)
-    out[e] = in[e];
-}
-
 // Positions, inverses and cached local energies of the chosen walkers into
 // scratch: one flat index over [3N positions | inverses | energy] x W.
 __global__ void gather_all_kernel(DmcArgs a) {
@@ -418,8 +433,13 @@ __global__ void copy_all_kernel(DmcArgs a) {
   if (blockIdx.x == 0 && threadIdx.x == 0) step_end(a);
 }
 
-// update_energies (dmc.rs:155-202) at the end of a block; counters advance
+// update_energies (dmc.rs:155-202) at the end of a block; counters advance,
+// the sweep counters too for the one system that carries them
 __device__ void step_end(const DmcArgs &a) {
+  if (a.sweeps_done) {
+    *a.sweeps_done += 1;
+    *a.trace_sweeps += 1;
+  }
   DmcState *st = a.st;
   st->steps += 1;
   st->it += 1;
@@ -491,14 +511,18 @@ hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream) {
   const dim3 gw((W + 255) / 256), b256(256);
   hipLaunchKernelGGL(reduce1_kernel, dim3(n_parts), dim3(kRedThreads), 0, stream, (const double *)a.dmc, W, a.part,
                      n_parts);
-  hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kRedThreads), 0, stream, a, n_parts);
   hipError_t e;
   if (a.brancher == kDmcBranchSR) {
     const int nb = (int)dmc_scan_blocks((size_t)W);
-    hipLaunchKernelGGL(sr_scan_local_kernel, dim3(nb), dim3(kScanThreads), 0, stream, a);
-    hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(kScanThreads), 0, stream, a.bsum_a, nb);
-    hipLaunchKernelGGL(sr_select_kernel, gw, b256, 0, stream, a);
+    hipLaunchKernelGGL(sr_scan_local_kernel, dim3(nb), dim3(kScanThreads), 0, stream, a, n_parts);
+    if (nb <= kScanThreads) {
+      hipLaunchKernelGGL(sr_select_kernel<true>, gw, dim3(kScanThreads), 0, stream, a);
+    } else {
+      hipLaunchKernelGGL(scan_blocks_kernel, dim3(1), dim3(kScanThreads), 0, stream, a.bsum_a, nb);
+      hipLaunchKernelGGL(sr_select_kernel<false>, gw, dim3(kScanThreads), 0, stream, a);
+    }
   } else {
+    hipLaunchKernelGGL(reduce2_kernel, dim3(1), dim3(kRedThreads), 0, stream, a, n_parts);
     hipLaunchKernelGGL(simple_counts_kernel, gw, b256, 0, stream, a);
     if ((e = enqueue_scan(a.cnt_a, (size_t)W, nullptr, a.bsum_a, stream)) != hipSuccess) return e;
     if ((e = enqueue_scan(a.cnt_b, (size_t)W, nullptr, a.bsum_b, stream)) != hipSuccess) return e;

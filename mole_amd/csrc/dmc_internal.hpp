@@ -61,7 +61,11 @@ struct DmcArgs {
   int b_rows, b_elem;
   double *pos_s;
   void *B_s;
-  double *el_s, *w_s;
+  double *el_s;
+  // the sweep counters of the Slater-Jastrow launch state, which every DMC
+  // step advances by one (set for one system per step, null for the rest)
+  uint64_t *sweeps_done;
+  uint32_t *trace_sweeps;
 };
 
 // scratch sizes for W walkers

@@ -84,6 +84,20 @@ def test_dmc_mixed_precision_parity_with_oracle(brancher):
     check_parity(d, o)
 
 
+@pytest.mark.parametrize("W,brancher,iters,block", [
+    (9000, M.BRANCH_SR, 12, 4),        # 3 reduction partials, 9 scan blocks (offsets scanned in the selection)
+    (9000, M.BRANCH_SIMPLE, 12, 4),    # list scans over up to 27,000 entries
+    (263000, M.BRANCH_SR, 4, 2),       # 257 scan blocks: the separate block-offset scan
+])
+def test_dmc_parity_multi_block(W, brancher, iters, block):
+    # walker counts that span several reduction and scan workgroups, ragged
+    kw = dict(tau=0.02, time_step=0.02, iters=iters, block=block, eq=1, eref=-2.88, brancher=brancher)
+    print(f"dmc multi-block parity W={W} brancher={brancher}", flush=True)
+    d = run_device([S.helium_product(1.6875, 0.5, 0.3)], W, **kw)[0]
+    o = run_oracle(S.helium_product(1.6875, 0.5, 0.3), W, **kw)
+    check_parity(d, o)
+
+
 def test_dmc_lane_independence():
     kw = dict(tau=0.02, time_step=0.02, iters=20, block=5, eq=1, eref=-7.0, brancher=M.BRANCH_SR, lu=8)
     a = run_device([S.atom(3)], 64, lanes=1, **kw)[0]
