@@ -385,6 +385,10 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.lds_per_walker = (uint32_t)st.lds_walker;
     d.dmc = st.dmc.dmc;
     d.dmc_eref = st.dmc.st ? &st.dmc.st->eref : nullptr;
+    d.dmc_pos_in = st.dmc.pos_s;
+    d.dmc_B_in = st.dmc.B_s;
+    d.dmc_el_in = st.dmc.el_s;
+    d.dmc_w_in = st.dmc.wnew;
     d.kind = st.d_center + st.n_basis;
     d.any_gto = st.any_gto ? 1 : 0;
     d.ops = st.d_ops;
@@ -1889,6 +1893,11 @@ mole_status mole_dmc_diffuse(mole_ctx *ctx, double time_step, uint32_t num_itera
     }
   }
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
+  // the steps start from the gathered copies of the walkers and leave the
+  // last branching's choice there; the walker state is brought up to date
+  // at the end
+  for (uint32_t k = 0; k < ctx->sys.size(); ++k)
+    HIP_TRY(launch_dmc_sync_state(dmc_args(ctx, k, block_size, num_eq_blocks), false, ctx->stream));
   hipError_t le = hipSuccess;
   for (size_t it = 0; it < steps && le == hipSuccess; ++it) {
     if (exec) {
@@ -1899,6 +1908,8 @@ mole_status mole_dmc_diffuse(mole_ctx *ctx, double time_step, uint32_t num_itera
     }
     account(ctx, 1, 0);
   }
+  for (uint32_t k = 0; k < ctx->sys.size() && le == hipSuccess; ++k)
+    le = launch_dmc_sync_state(dmc_args(ctx, k, block_size, num_eq_blocks), true, ctx->stream);
   if (le == hipSuccess) le = hipEventRecord(ctx->ev1, ctx->stream);
   if (le == hipSuccess) le = hipStreamSynchronize(ctx->stream);
   if (exec) (void)hipGraphExecDestroy(exec);

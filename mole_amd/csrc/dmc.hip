@@ -384,8 +384,12 @@ __global__ void simple_final_kernel(DmcArgs a) {
 
 // ---- walker state after branching ----
 
+__device__ void step_end(const DmcArgs &a);
+
 // Positions, inverses and cached local energies of the chosen walkers into
-// scratch: one flat index over [3N positions | inverses | energy] x W.
+// the gathered copies, one flat index over [3N positions | inverses | energy]
+// x W; the next step starts from them.  One thread then runs the step's
+// bookkeeping (no workgroup of this launch reads the DMC state).
 __global__ void gather_all_kernel(DmcArgs a) {
   const size_t W = (size_t)a.W;
   const size_t np = (size_t)a.pos_rows * W, nb = (size_t)a.b_rows * W, n = np + nb + W;
@@ -404,33 +408,40 @@ __global__ void gather_all_kernel(DmcArgs a) {
       a.el_s[j] = a.dmc[W + a.src[j]];
     }
   }
+  if (blockIdx.x == 0 && threadIdx.x == 0) step_end(a);
 }
 
-__device__ void step_end(const DmcArgs &a);
-
-// scratch back into the walker state, the new weights into row 0; one
-// thread then runs the step's bookkeeping
-__global__ void copy_all_kernel(DmcArgs a) {
+// Between calls the walkers live in the walker state (positions, inverses,
+// dmc rows 0-1: weight, cached local energy); within a call, between steps,
+// in the gathered copies (weights in wnew).  to_main: the copies into the
+// walker state (a call's end), else the walker state into the copies (its
+// start).
+__global__ void sync_state_kernel(DmcArgs a, int to_main) {
   const size_t W = (size_t)a.W;
   const size_t np = (size_t)a.pos_rows * W, nb = (size_t)a.b_rows * W, n = np + nb + 2 * W;
   for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
     if (e < np) {
-      a.pos[e] = a.pos_s[e];
+      if (to_main) a.pos[e] = a.pos_s[e];
+      else a.pos_s[e] = a.pos[e];
     } else if (e < np + nb) {
       const size_t f = e - np;
-      if (a.b_elem == 4)
-        ((float *)a.B)[f] = ((const float *)a.B_s)[f];
-      else
-        ((double *)a.B)[f] = ((const double *)a.B_s)[f];
+      if (a.b_elem == 4) {
+        if (to_main) ((float *)a.B)[f] = ((const float *)a.B_s)[f];
+        else ((float *)a.B_s)[f] = ((const float *)a.B)[f];
+      } else {
+        if (to_main) ((double *)a.B)[f] = ((const double *)a.B_s)[f];
+        else ((double *)a.B_s)[f] = ((const double *)a.B)[f];
+      }
     } else if (e < np + nb + W) {
       const size_t j = e - np - nb;
-      a.dmc[W + j] = a.el_s[j];
+      if (to_main) a.dmc[W + j] = a.el_s[j];
+      else a.el_s[j] = a.dmc[W + j];
     } else {
       const size_t j = e - np - nb - W;
-      a.dmc[j] = a.wnew[j];
+      if (to_main) a.dmc[j] = a.wnew[j];
+      else a.wnew[j] = a.dmc[j];
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) step_end(a);
 }
 
 // update_energies (dmc.rs:155-202) at the end of a block; counters advance,
@@ -535,9 +546,14 @@ hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream) {
     if ((e = enqueue_scan(a.cnt_a, 3 * (size_t)W, &a.st->M, a.bsum_a, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(simple_final_kernel, g3, b256, 0, stream, a);
   }
-  const size_t n_all = (size_t)(a.pos_rows + a.b_rows + 2) * W;
-  hipLaunchKernelGGL(gather_all_kernel, dim3(grid_for(n_all)), b256, 0, stream, a);
-  hipLaunchKernelGGL(copy_all_kernel, dim3(grid_for(n_all)), b256, 0, stream, a);
+  const size_t n_gather = (size_t)(a.pos_rows + a.b_rows + 1) * W;
+  hipLaunchKernelGGL(gather_all_kernel, dim3(grid_for(n_gather)), b256, 0, stream, a);
+  return hipGetLastError();
+}
+
+hipError_t launch_dmc_sync_state(const DmcArgs &a, bool to_main, hipStream_t stream) {
+  const size_t n_all = (size_t)(a.pos_rows + a.b_rows + 2) * (size_t)a.W;
+  hipLaunchKernelGGL(sync_state_kernel, dim3(grid_for(n_all)), dim3(256), 0, stream, a, to_main ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -53,8 +53,9 @@ struct DmcArgs {
   uint32_t *removed;  // [2W]
   uint32_t *src;      // [W]
   double *wnew;       // [W]
-  // walker state gathered after branching: positions [pos_rows][W] f64,
-  // inverses [b_rows][W] of b_elem bytes; scratch copies of each
+  // walker state (positions [pos_rows][W] f64, inverses [b_rows][W] of
+  // b_elem bytes) and the copies gathered from it after branching, which the
+  // next step reads (el_s the cached local energies, wnew the weights)
   double *pos;
   int pos_rows;
   void *B;
@@ -77,6 +78,12 @@ hipError_t launch_dmc_init_configs(double *pos, int N, int W, uint64_t seed, uin
 // one system's work after a DMC step's sweep launch: reductions, the step
 // energy, branching, the gather of walker state, the block bookkeeping
 hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream);
+// The walkers between DMC calls are in the walker state (positions,
+// inverses, weights and cached local energies in dmc rows 0-1); within a
+// call, between steps, in the gathered copies that each step starts from.
+// to_main copies the gathered copies into the walker state (a call's end),
+// otherwise the walker state into the copies (its start).
+hipError_t launch_dmc_sync_state(const DmcArgs &a, bool to_main, hipStream_t stream);
 // start of a diffuse call: block counters to zero
 hipError_t launch_dmc_call_start(DmcState *st, hipStream_t stream);
 // *st = v by a kernel (capturable)

@@ -62,6 +62,12 @@ struct SJSysDev {
   // w_old; the reference energy is a device scalar
   double *dmc;
   const double *dmc_eref;
+  // the walkers a DMC step starts from, as the branching gathered them:
+  // positions, inverses, cached local energies and weights (the step writes
+  // the moved walkers to pos, B and dmc)
+  const double *dmc_pos_in;
+  const void *dmc_B_in;
+  const double *dmc_el_in, *dmc_w_in;
   // local operators: ops[0] the custom Hamiltonian (when custom_h), ops[1..n_obs]
   // the observables, whose per-walker sums are obs_acc [2 n_obs][W] (sum, sum of squares)
   const mole_operator *ops;

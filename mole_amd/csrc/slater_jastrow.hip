@@ -612,14 +612,16 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
   wk.flag = (double *)(base + lay.flag);
   wk.mv = (double *)(base + lay.mv);
   const int nB = nu * nu + nd * nd;
-  const T *Bg = (const T *)S.B;
   const bool gto = EXT && S.any_gto;
 
-  // ---- state in ----
+  // ---- state in (a DMC step reads the walkers the branching gathered) ----
+  const bool dmc_in = EXT && a.dmc == kSJDmcStep;
+  const double *pin = dmc_in ? S.dmc_pos_in : S.pos;
+  const T *Bg = (const T *)(dmc_in ? S.dmc_B_in : S.B);
   for (int e = lane; e < 3 * N * nslots; e += 64) {
     const int k = e >> lg_slots, sl = e & (nslots - 1);
     double *pp = (double *)(lds + (size_t)sl * lay.bytes + lay.pos);
-    pp[k] = (sl < tile.nw) ? S.pos[(size_t)k * W + tile.w0 + sl] : 0.0;
+    pp[k] = (sl < tile.nw) ? pin[(size_t)k * W + tile.w0 + sl] : 0.0;
   }
   for (int e = lane; e < nB * nslots; e += 64) {
     const int k = e >> lg_slots, sl = e & (nslots - 1);
@@ -891,7 +893,7 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
       double dlg[kSJNParam];
       const double E = local_energy<T, EXT>(S, wk, t, L, dlg);
       if (t == 0 && live) {
-        const double eo = S.dmc[(size_t)W + w], wt = S.dmc[w];
+        const double eo = S.dmc_el_in[w], wt = S.dmc_w_in[w];
         S.dmc[2 * (size_t)W + w] = wt * eo;
         S.dmc[3 * (size_t)W + w] = wt;
         S.dmc[w] = wt * exp_m(-a.dmc_tau * ((eo + E) / 2.0 - *S.dmc_eref));
