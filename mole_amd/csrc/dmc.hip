@@ -80,10 +80,14 @@ __global__ void __launch_bounds__(kRedThreads) reduce1_kernel(const double *dmc,
   if (t < 4) part[(size_t)t * n_parts + blockIdx.x] = sh[t][0];
 }
 
+__device__ void step_end(const DmcArgs &a);
+
 // The second stage over the n_parts partials, in one fixed order (per-thread
 // strided sums, then an LDS tree), leaving the four results in sh[q][0]; with
 // `write` the step's ensemble energy (dmc.rs:117-121) and SRBrancher's mean
-// weight and normalisation (branching.rs:21-24) go to the state.
+// weight and normalisation (branching.rs:21-24) go to the state, and the same
+// thread then runs the step's bookkeeping (no later launch of the step reads
+// what it changes: the branching reads its stream key from `branch`).
 __device__ void finish_reduction(const DmcArgs &a, int n_parts, double (*sh)[kRedThreads], bool write) {
   const int t = threadIdx.x;
   double v[4] = {0.0, 0.0, 0.0, 0.0};
@@ -115,6 +119,7 @@ __device__ void finish_reduction(const DmcArgs &a, int n_parts, double (*sh)[kRe
     a.block_buf[st->j] = ens;
     st->gw = sh[2][0] / (double)a.W;
     st->norm = (double)a.W / sh[3][0];
+    step_end(a);
   }
 }
 
@@ -228,6 +233,19 @@ __global__ void __launch_bounds__(kScanThreads) sr_scan_local_kernel(DmcArgs a, 
   if (t == kScanThreads - 1) a.bsum_a[blockIdx.x] = sh[t];
 }
 
+// Walker s's positions, inverses and cached local energy into column j of
+// the gathered copies, which the next step starts from
+__device__ __forceinline__ void gather_walker(const DmcArgs &a, uint32_t j, uint32_t s) {
+  const size_t W = (size_t)a.W;
+  for (int r = 0; r < a.pos_rows; ++r) a.pos_s[r * W + j] = a.pos[r * W + s];
+  if (a.b_elem == 4) {
+    for (int r = 0; r < a.b_rows; ++r) ((float *)a.B_s)[r * W + j] = ((const float *)a.B)[r * W + s];
+  } else {
+    for (int r = 0; r < a.b_rows; ++r) ((double *)a.B_s)[r * W + j] = ((const double *)a.B)[r * W + s];
+  }
+  a.el_s[j] = a.dmc[W + s];
+}
+
 // WeightedChoice: the first index whose cumulative weight exceeds u.  The
 // cumulative weight through walker k is its block-local sum plus the
 // exclusive offset of its scan block.  With LOCAL_OFFSETS (at most
@@ -264,10 +282,11 @@ __global__ void __launch_bounds__(kScanThreads) sr_select_kernel(DmcArgs a) {
     if (j == 0) st->error = 1;
     a.src[j] = (uint32_t)j;
     a.wnew[j] = a.dmc[j];
+    gather_walker(a, (uint32_t)j, (uint32_t)j);
     return;
   }
   Stream rs;
-  rs.init(a.seed, a.sys, (uint32_t)st->steps, kDmcBranchEpoch);
+  rs.init(a.seed, a.sys, st->branch, kDmcBranchEpoch);
   const uint64_t u = uniform_below(rs.draw((uint64_t)j), total);
   uint32_t lo = 0, hi = (uint32_t)a.W - 1;
   while (lo < hi) {
@@ -277,6 +296,7 @@ __global__ void __launch_bounds__(kScanThreads) sr_select_kernel(DmcArgs a) {
   }
   a.src[j] = lo;
   a.wnew[j] = st->gw;
+  gather_walker(a, (uint32_t)j, lo);
 }
 
 // ---- SimpleBranching (branching.rs:57-91) ----
@@ -285,7 +305,7 @@ __global__ void simple_counts_kernel(DmcArgs a) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.W) return;
   Stream rs;
-  rs.init(a.seed, a.sys, (uint32_t)a.st->steps, kDmcBranchEpoch);
+  rs.init(a.seed, a.sys, a.st->branch, kDmcBranchEpoch);
   uint64_t n = as_usize(a.dmc[i] + u64_standard(rs.draw((uint64_t)i)));
   if (n > 3) n = 3;
   a.cnt_a[i] = n >= 1 ? 1 : 0;
@@ -314,7 +334,7 @@ __global__ void simple_clone_kernel(DmcArgs a) {
   const uint32_t M = a.st->M;
   if (M >= (uint32_t)a.W || k >= a.W - (int)M) return;
   Stream rs;
-  rs.init(a.seed, a.sys, (uint32_t)a.st->steps, kDmcBranchEpoch);
+  rs.init(a.seed, a.sys, a.st->branch, kDmcBranchEpoch);
   a.list[M + k] = (uint32_t)uniform_below(rs.draw((uint64_t)a.W + k), (uint64_t)a.W);
 }
 
@@ -330,7 +350,7 @@ __global__ void __launch_bounds__(kRemoveThreads) simple_remove_kernel(DmcArgs a
   const uint32_t E = M - W;
   const int t = threadIdx.x;
   Stream rs;
-  rs.init(a.seed, a.sys, (uint32_t)a.st->steps, kDmcBranchEpoch);
+  rs.init(a.seed, a.sys, a.st->branch, kDmcBranchEpoch);
   for (uint32_t k = 0; k < E; ++k) {
     const uint64_t r = uniform_below(rs.draw((uint64_t)W + k), (uint64_t)(M - k));
     uint64_t p = r;
@@ -384,12 +404,9 @@ __global__ void simple_final_kernel(DmcArgs a) {
 
 // ---- walker state after branching ----
 
-__device__ void step_end(const DmcArgs &a);
-
-// Positions, inverses and cached local energies of the chosen walkers into
-// the gathered copies, one flat index over [3N positions | inverses | energy]
-// x W; the next step starts from them.  One thread then runs the step's
-// bookkeeping (no workgroup of this launch reads the DMC state).
+// SimpleBranching: positions, inverses and cached local energies of the
+// chosen walkers into the gathered copies, one flat index over
+// [3N positions | inverses | energy] x W; the next step starts from them.
 __global__ void gather_all_kernel(DmcArgs a) {
   const size_t W = (size_t)a.W;
   const size_t np = (size_t)a.pos_rows * W, nb = (size_t)a.b_rows * W, n = np + nb + W;
@@ -408,7 +425,6 @@ __global__ void gather_all_kernel(DmcArgs a) {
       a.el_s[j] = a.dmc[W + a.src[j]];
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) step_end(a);
 }
 
 // Between calls the walkers live in the walker state (positions, inverses,
@@ -452,6 +468,7 @@ __device__ void step_end(const DmcArgs &a) {
     *a.trace_sweeps += 1;
   }
   DmcState *st = a.st;
+  st->branch = (uint32_t)st->steps;
   st->steps += 1;
   st->it += 1;
   if (++st->j < a.block_size) return;
@@ -545,9 +562,9 @@ hipError_t launch_dmc_after_step(const DmcArgs &a, hipStream_t stream) {
     hipLaunchKernelGGL(simple_unflag_kernel, dim3((unsigned)((2 * (size_t)W + 255) / 256)), b256, 0, stream, a);
     if ((e = enqueue_scan(a.cnt_a, 3 * (size_t)W, &a.st->M, a.bsum_a, stream)) != hipSuccess) return e;
     hipLaunchKernelGGL(simple_final_kernel, g3, b256, 0, stream, a);
+    const size_t n_gather = (size_t)(a.pos_rows + a.b_rows + 1) * W;
+    hipLaunchKernelGGL(gather_all_kernel, dim3(grid_for(n_gather)), b256, 0, stream, a);
   }
-  const size_t n_gather = (size_t)(a.pos_rows + a.b_rows + 1) * W;
-  hipLaunchKernelGGL(gather_all_kernel, dim3(grid_for(n_gather)), b256, 0, stream, a);
   return hipGetLastError();
 }
 

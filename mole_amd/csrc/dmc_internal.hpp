@@ -27,7 +27,8 @@ struct DmcState {
   uint64_t total;                           // SRBrancher: total integer weight
   uint32_t it, j, block_nr, n_out;          // step of the call, step in block, block, blocks out
   uint32_t M, S;                            // SimpleBranching: list length, survivors
-  uint32_t error, pad;                      // 1: total integer weight 0 (the reference panics)
+  uint32_t error;                           // 1: total integer weight 0 (the reference panics)
+  uint32_t branch;                          // step whose branching is in progress (stream key)
 };
 
 struct DmcArgs {
