@@ -145,6 +145,9 @@ struct mole_ctx {
   mole_optimizer *d_opt = nullptr;
   size_t optlog_cap = 0;
   bool dev_params = false;
+  // the next two-electron launch starts every walker's accumulators from zero
+  // (a reset folded into the sweep kernel); cleared once a launch used it
+  bool fresh_acc = false;
   // diffusion Monte Carlo
   bool dmc_ready = false;
   uint32_t dmc_brancher = 0;
@@ -510,6 +513,7 @@ mole_status enqueue_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start, uin
         a.W = st.W;
         a.pos = st.d_pos;
         a.acc = st.d_acc;
+        a.fresh = ctx->fresh_acc ? 1 : 0;
         a.seed = ctx->run.seed;
         a.sys = s;
         a.epoch = ctx->epoch;
@@ -533,6 +537,7 @@ mole_status enqueue_sweeps(mole_ctx *ctx, uint32_t n, uint32_t sample_start, uin
         ++launches;
       }
     }
+    ctx->fresh_acc = false;
     account(ctx, chunk, n_samp);
     done += chunk;
   }
@@ -1650,12 +1655,15 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
   const uint32_t blocks = steps / block_size;
   uint32_t launches = 0;
   for (uint32_t it = 0; it < iters; ++it) {
-    // reset statistics (as mole_reset_stats), Runner::run, reduce, step
-    HIP_TRY(launch_fill(s.d_acc, (size_t)s.n_cols * s.W, 0.0, ctx->stream));
+    // reset statistics (as mole_reset_stats: the first sweep launch starts
+    // the accumulators from zero; steps >= 2 blocks, so there is one),
+    // Runner::run, reduce, step
+    ctx->fresh_acc = true;
     s.n_samples = s.n_blocks = s.n_moves = 0;
     ctx->block_fill = 0;
     ctx->trace_sweeps = ctx->trace_samples = 0;
     st = enqueue_sweeps(ctx, blocks * block_size, block_size, &launches);
+    ctx->fresh_acc = false;
     if (st != MOLE_OK) {
       ctx->dev_params = false;
       return st;
@@ -1663,7 +1671,7 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
     HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, ctx->stream));
     HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, ctx->d_opt, ctx->d_optlog + 8 * (size_t)it,
                             ctx->stream));
-    launches += 3;
+    launches += 2;
   }
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
   ctx->dev_params = false;

@@ -41,6 +41,8 @@ struct Sweep2Args {
   int W;             // walkers of this system
   double *pos;       // [6][W]
   double *acc;       // [kAccColumns2][W]
+  int fresh;         // 1: every walker's accumulators start from zero (as after a
+                     // reset) instead of being read
   // stream
   uint64_t seed;
   uint32_t sys, epoch;

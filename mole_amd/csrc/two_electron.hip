@@ -133,7 +133,7 @@ struct SampleAcc {
   __device__ __forceinline__ void load(const Sweep2Args &a, int w) {
 #pragma unroll
     for (int k = 0; k < kAccColumns2; ++k)
-      if (k != kAccAccepted) acc[k] = a.acc[(size_t)k * a.W + w];
+      if (k != kAccAccepted) acc[k] = a.fresh ? 0.0 : a.acc[(size_t)k * a.W + w];
     fill = a.block_fill;
     sample_idx = a.trace_sample0;
   }
@@ -220,7 +220,7 @@ __global__ void __launch_bounds__(256) sweep2_lane_kernel(Sweep2Args a) {
 
   SampleAcc sa;
   sa.load(a, w);
-  double accepted = a.acc[(size_t)kAccAccepted * W + w];
+  double accepted = a.fresh ? 0.0 : a.acc[(size_t)kAccAccepted * W + w];
   Stream rng;
   rng.init(a.seed, a.sys, (uint32_t)w, a.epoch);
   uint64_t p = a.draw0;
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
     } else {
       psi = (phi[0] * phi[1]) * exp_m(pade(sp.b, sp.c, r12));
     }
-    accepted = live ? a.acc[(size_t)kAccAccepted * W + w] : 0.0;
+    accepted = (live && !a.fresh) ? a.acc[(size_t)kAccAccepted * W + w] : 0.0;
   } else if (role == 3) {
     if (live) sa.load(a, w);
   } else {
@@ -563,7 +563,16 @@ __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *acc, 
   const bool mx = col == max_col;
   const double *p = acc + (size_t)col * W;
   double s = 0.0;
-  for (int i = threadIdx.x; i < W; i += 256) s = mx ? fmax(s, p[i]) : s + p[i];
+  // loads in batches of 16 (all in flight together), added in index order
+  int i = threadIdx.x;
+  for (; i + 15 * 256 < W; i += 16 * 256) {
+    double v[16];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) v[q] = p[i + q * 256];
+#pragma unroll
+    for (int q = 0; q < 16; ++q) s = mx ? fmax(s, v[q]) : s + v[q];
+  }
+  for (; i < W; i += 256) s = mx ? fmax(s, p[i]) : s + p[i];
   sh[threadIdx.x] = s;
   __syncthreads();
   for (int off = 128; off > 0; off >>= 1) {
