@@ -15,12 +15,17 @@
 namespace mole {
 
 // Returns 0, or MOLE_ERR_LINALG for a singular SR matrix.  g (optional)
-// receives the energy gradient.
+// receives the energy gradient.  MAXP sizes the working arrays; below
+// MOLE_MAX_PARAMS it is also the parameter count (a caller with a fixed
+// count, whose small arrays then stay in registers), with the same
+// arithmetic in the same order.
+template <int MAXP = MOLE_MAX_PARAMS>
 __host__ __device__ inline int sr_update(int np, double n, double sum_e, const double *sum_o, const double *sum_oe,
                                          const double *sum_oo, uint32_t optimizer, double step, uint32_t sr_mode,
                                          double sr_eps, double *dp, double *g_out) {
+  if (MAXP < MOLE_MAX_PARAMS) np = MAXP;
   const double ebar = sum_e / n;
-  double g[MOLE_MAX_PARAMS], obar[MOLE_MAX_PARAMS];
+  double g[MAXP], obar[MAXP];
   for (int i = 0; i < np; ++i) {
     obar[i] = sum_o[i] / n;
     g[i] = 2.0 * (sum_oe[i] - ebar * sum_o[i]) / n;
@@ -32,7 +37,7 @@ __host__ __device__ inline int sr_update(int np, double n, double sum_e, const d
     return 0;
   }
   // augmented matrix [S | -g/2], Gaussian elimination with partial pivoting
-  double A[MOLE_MAX_PARAMS][MOLE_MAX_PARAMS + 1];
+  double A[MAXP][MAXP + 1];
   double osum = 0.0;
   for (int i = 0; i < np; ++i) osum += obar[i];
   for (int i = 0; i < np; ++i) {
@@ -129,19 +134,22 @@ __host__ __device__ inline void lbfgs_direction(const mole_optimizer *o, const d
 
 // Optimizer::compute_parameter_update for every kind from the sums.
 // Returns 0 or MOLE_ERR_LINALG (SR, state untouched).
+// MAXP as for sr_update (below MOLE_MAX_PARAMS: o->n_params must equal it).
+template <int MAXP = MOLE_MAX_PARAMS>
 __host__ __device__ inline int optimizer_update(mole_optimizer *o, double n, double sum_e, const double *sum_o,
                                                 const double *sum_oe, const double *sum_oo, const double *pars,
                                                 double *dp, double *g_out) {
-  const int np = (int)o->n_params;
-  double g[MOLE_MAX_PARAMS];
+  const int np = MAXP < MOLE_MAX_PARAMS ? MAXP : (int)o->n_params;
+  double g[MAXP];
   if (o->kind == MOLE_OPT_SD || o->kind == MOLE_OPT_SR) {
-    const int rc = sr_update(np, n, sum_e, sum_o, sum_oe, sum_oo, o->kind, o->step, o->sr_mode, o->sr_eps, dp, g);
+    const int rc =
+        sr_update<MAXP>(np, n, sum_e, sum_o, sum_oe, sum_oo, o->kind, o->step, o->sr_mode, o->sr_eps, dp, g);
     if (g_out)
       for (int i = 0; i < np; ++i) g_out[i] = g[i];
     if (rc == 0) ++o->iter;
     return rc;
   }
-  sr_update(np, n, sum_e, sum_o, sum_oe, sum_oo, MOLE_OPT_SD, 0.0, 0, 0.0, dp, g);
+  sr_update<MAXP>(np, n, sum_e, sum_o, sum_oe, sum_oo, MOLE_OPT_SD, 0.0, 0, 0.0, dp, g);
   if (g_out)
     for (int i = 0; i < np; ++i) g_out[i] = g[i];
   const double mu = o->momentum;
@@ -157,7 +165,7 @@ __host__ __device__ inline int optimizer_update(mole_optimizer *o, double n, dou
       dp[i] = -(mu * o->m_prev[i] + (1.0 + mu) * o->m[i]);
     }
   } else {  // MOLE_OPT_LBFGS
-    double p[MOLE_MAX_PARAMS];
+    double p[MAXP];
     lbfgs_push(o, pars, g);
     lbfgs_direction(o, g, p);
     for (int i = 0; i < np; ++i) dp[i] = -o->step * p[i];

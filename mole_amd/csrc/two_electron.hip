@@ -692,7 +692,8 @@ __global__ void vmc_step_kernel(const double *red, double n_samples, double *par
       soo[l * np + k] = red[kAccSumOO + t];
     }
   }
-  const int rc = optimizer_update(opt, n_samples, red[kAccSumE], so, soe, soo, pars, dp, nullptr);
+  // the parameter count is fixed here (mole_optimize_with checks opt->n_params)
+  const int rc = optimizer_update<np>(opt, n_samples, red[kAccSumE], so, soe, soo, pars, dp, nullptr);
   log[0] = red[kAccSumE];
   log[1] = red[kAccSumE2];
   log[2] = red[kAccSumBm];
