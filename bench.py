@@ -46,10 +46,12 @@ def flops_per_launch(walkers, sweeps, sampled):
 
 VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
 
-# HBM bytes per sweep launch of this workload: rocprofv3 FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0
-# (factors from the plain copy kernel of mole_amd/copyprobe.py), recorded in
-# profiles/sweep2_ws_kernel_h2_sr_16384w.md; the algorithmic state traffic is 6.29e6 B.
-TRAFFIC_BYTES_PER_LAUNCH = 6712.6 * 1024
+# HBM bytes per sweep launch of the timed optimisation loop: rocprofv3
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from the plain copy kernel of
+# mole_amd/copyprobe.py), recorded in profiles/sweep2_ws_kernel_h2_sr_16384w.md.
+# Those launches start the accumulators from zero, so the algorithmic state
+# traffic is positions in and out plus accumulators out: 3.93e6 B.
+TRAFFIC_BYTES_PER_LAUNCH = (672.3 * 2.0 + 3328.0) * 1024
 
 
 def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
