@@ -284,7 +284,7 @@ def main():
                     frac=achieved / PEAK_FP64_TFLOPS,
                     traffic=None if args.variant == "lane" else TRAFFIC_BYTES_PER_LAUNCH,
                     kernel=("sweep2_lane_kernel<LCAO2>" if args.variant == "lane"
-                            else "sweep2_ws_kernel<LCAO2,2,4>"),
+                            else "sweep2_ws_kernel<LCAO2,2,6>"),
                     kernel_ms=1e3 * avg_kernel_s,
                     flop_per_launch=fl)
     cpu = None

@@ -630,7 +630,7 @@ hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads, hi
   if (variant != 1 && sweep2_wavespec_eligible(a.sp)) {
     // wave-specialised path: 64 walkers per 256-thread workgroup
     const dim3 grid((a.W + 63) / 64), blk(256);
-    constexpr int T = 4;
+    constexpr int T = 6;
     if (he)
       hipLaunchKernelGGL((sweep2_ws_kernel<MOLE_WF_HELIUM_PADE, 1, T>), grid, blk, 0, stream, a);
     else if (nn == 1)
