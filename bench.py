@@ -51,7 +51,7 @@ VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
 # mole_amd/copyprobe.py), recorded in profiles/sweep2_ws_kernel_h2_sr_16384w.md.
 # Those launches start the accumulators from zero, so the algorithmic state
 # traffic is positions in and out plus accumulators out: 3.93e6 B.
-TRAFFIC_BYTES_PER_LAUNCH = (672.3 * 2.0 + 3328.0) * 1024
+TRAFFIC_BYTES_PER_LAUNCH = (680.3 * 2.0 + 3328.0) * 1024
 
 
 def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
