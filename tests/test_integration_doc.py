@@ -1,6 +1,7 @@
-"""The Rust binding written out in INTEGRATION.md declares exactly the entry
-points of include/mole_gpu.h, and its repr(C) structs list the header's
-fields in the header's order (the Rust cannot be compiled in this image)."""
+"""The Rust binding written out in the repository's integration guide
+declares exactly the entry points of include/mole_gpu.h, and its repr(C)
+structs list the header's fields in the header's order (the Rust cannot be
+compiled in this image)."""
 import os
 import re
 
