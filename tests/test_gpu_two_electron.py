@@ -332,6 +332,49 @@ def test_device_optimisation_loop_equals_host_loop():
     assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
 
 
+def test_benchmark_shape_parity_and_device_loop():
+    # the H2 + SR benchmark at its full shape: 16,384 walkers x 200 sweeps,
+    # block 20, box side 0.5, parameters (1.0, 0.5, 0.3).  One Runner::run
+    # against the oracle (every accept decision, the final configurations,
+    # the sums), then three SR iterations (step 0.05, shift 1e-3) of the
+    # device loop against the host loop, bit for bit.
+    W, S, B, side, p0 = 16384, 200, 20, 0.5, (1.0, 0.5, 0.3)
+    print(f"benchmark shape W={W} sweeps={S}", flush=True)
+    with M.Context([M.h2(W, *p0)], seed=0, step=side, block_size=B, flags=M.FLAG_TRACE,
+                   record_sweeps=S) as ctx:
+        ctx.init_walkers()
+        ctx.run(S)
+        masks, el = ctx.trace(S, S - B)
+        cfg = ctx.get_configs()
+        st = ctx.stats()
+    ref = O.run_box(O.h2(*p0), seed=0, walkers=W, steps=S, block_size=B, box_side=side)
+    assert np.array_equal(masks, ref["accept_mask"]), "accept/reject decisions differ"
+    assert np.array_equal(bits(cfg), bits(ref["cfg"]))
+    assert st["n_accepted"] == ref["n_accepted"] and st["n_samples"] == ref["n_samples"]
+    scale = np.maximum(1.0, np.abs(ref["e_local"]))
+    assert np.max(np.abs(el - ref["e_local"]) / scale) <= E_TOL
+    assert abs(st["sum_e"] - ref["sum_e"]) <= 1e-11 * ref["n_samples"]
+
+    iters = 3
+    host_p = np.array(p0)
+    with M.Context([M.h2(W, *p0)], seed=7, step=side, block_size=B) as ctx:
+        ctx.init_walkers()
+        for _ in range(iters):
+            ctx.reset_stats()
+            ctx.run(S, B)
+            dp, _ = M.parameter_update(ctx.sr_sums_raw(), M.OPT_SR, 0.05, M.SR_TEXTBOOK, 1e-3)
+            host_p = host_p + dp
+            ctx.set_params(host_p)
+        cfg_host = ctx.get_configs()
+    with M.Context([M.h2(W, *p0)], seed=7, step=side, block_size=B) as ctx:
+        ctx.init_walkers()
+        p, log = ctx.optimize(iters, S, B, M.OPT_SR, 0.05, M.SR_TEXTBOOK, 1e-3)
+        cfg_dev = ctx.get_configs()
+    assert np.all(log[:, 5] == 0.0)
+    assert np.array_equal(p.view(np.uint64), host_p.view(np.uint64))
+    assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
+
+
 @pytest.mark.parametrize("kind,kw", [(M.OPT_MOMENTUM, dict(momentum=0.5)), (M.OPT_NESTEROV, dict(momentum=0.5)),
                                      (M.OPT_LBFGS, dict(history=3))])
 def test_device_loop_stateful_optimizers_equal_host_loop(kind, kw):
