@@ -72,6 +72,24 @@ def test_mixed_precision_parity_with_oracle(Z):
     assert np.max(rel) <= E_TOL
 
 
+@pytest.mark.parametrize("Z,lanes,steps", [(3, 4, 30), (10, 8, 10)])
+def test_mixed_precision_parity_at_workload_width(Z, lanes, steps):
+    # the atoms workload's width (65,536 walkers, LU refresh every 64 moves)
+    # with the lanes per walker its autotuning picks: every accept decision,
+    # the configurations and the FP32 inverses' consequences bit for bit
+    W, B = 65536, 2
+    print(f"workload width Z={Z} W={W} lanes={lanes} sweeps={steps}", flush=True)
+    system = S.atom(Z)
+    d = run_device([system], W, steps, B, precision=M.PREC_MIXED, lu=64, lanes=lanes, seed=1)[0]
+    o = run_oracle(system, W, steps, B, precision=SJ.PREC_MIXED, lu=64, seed=1)
+    assert d["lanes"] == lanes
+    assert np.array_equal(d["masks"], o["accept_mask"]), "accept/reject decisions differ"
+    assert np.array_equal(bits(d["cfg"]), bits(o["cfg"]))
+    rel = np.abs(d["el"] - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+    assert np.max(rel) <= 1e-9
+    assert d["stats"]["n_accepted"] == o["n_accepted"]
+
+
 def test_results_independent_of_lanes_per_walker():
     system = S.atom(7)
     ref = None
