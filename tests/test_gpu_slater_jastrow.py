@@ -90,6 +90,43 @@ def test_mixed_precision_parity_at_workload_width(Z, lanes, steps):
     assert d["stats"]["n_accepted"] == o["n_accepted"]
 
 
+def test_heterogeneous_batch_at_workload_shape():
+    # the heterogeneous workload's one launch: 256 systems of 2..40 electrons,
+    # 2^20 walkers split in proportion to N, the lanes its autotuning picked
+    # (longest tiles first, one LDS footprint for every tile), mixed
+    # precision; the smallest, a middle and the largest system of the launch
+    # against their own oracle runs (systems are independent streams)
+    from mole_amd import autotune
+    from mole_amd import workloads as WL
+    lanes = {2: 4, 3: 4, 4: 4, 5: 8, 6: 8, 7: 8, 8: 8, 9: 16, 10: 8, 11: 8, 12: 16, 13: 16, 14: 16, 15: 16,
+             16: 16, 17: 16, 18: 16, 19: 32, 20: 16}
+    lanes.update({n: 32 for n in range(21, 32)})
+    lanes.update({n: 64 for n in range(32, 41)})
+    sys_list = WL.hetero_systems(256, 3)
+    Ns = np.array([s.n_electrons for s in sys_list])
+    W = np.maximum(1, np.floor(Ns / Ns.sum() * (1 << 20))).astype(int)
+    W[np.argmax(Ns)] += (1 << 20) - W.sum()
+    order = np.argsort(Ns, kind="stable")
+    picks = [int(order[0]), int(order[len(order) // 2]), int(np.argmax(Ns))]
+    steps, B = 4, 2
+    print(f"heterogeneous batch: {len(sys_list)} systems, {int(W.sum())} walkers, checking {picks}", flush=True)
+    dev = [M.slater_jastrow(s, int(w)) for s, w in zip(sys_list, W)]
+    with M.Context(dev, seed=3, step=0.6, block_size=B, flags=M.FLAG_TRACE, record_sweeps=steps,
+                   precision=M.PREC_MIXED, lu_period=64) as ctx:
+        autotune.apply(ctx, sys_list, lanes)
+        ctx.init_walkers()
+        ctx.run(steps)
+        got = {k: (ctx.trace(steps, steps - B, sys=k), ctx.get_configs(sys=k)) for k in picks}
+    for k in picks:
+        (masks, el), cfg = got[k]
+        o = SJ.run(SJ.SjWave(sys_list[k]), seed=3, walkers=int(W[k]), steps=steps, block_size=B, box_side=0.6,
+                   precision=SJ.PREC_MIXED, lu_period=64, sys=k, n_threads=8)
+        assert np.array_equal(masks, o["accept_mask"]), f"system {k}: accept/reject decisions differ"
+        assert np.array_equal(bits(cfg), bits(o["cfg"])), f"system {k}"
+        rel = np.abs(el - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+        assert np.max(rel) <= 1e-9, f"system {k}"
+
+
 def test_results_independent_of_lanes_per_walker():
     system = S.atom(7)
     ref = None
