@@ -98,6 +98,21 @@ def test_dmc_parity_multi_block(W, brancher, iters, block):
     check_parity(d, o)
 
 
+@pytest.mark.parametrize("k,eref", [(0, -2.87), (1, -1.14)])
+def test_dmc_parity_at_workload_shape(k, eref):
+    # the DMC workload's shape: its trial functions (He exp(-2r) orbitals with
+    # a Pade factor, H2 sigma_g LCAO), 65,536 walkers, 2 lanes per walker,
+    # mixed precision with an LU refresh every 64 moves, SRBrancher, tau 0.01;
+    # a few steps bit for bit against the oracle
+    from mole_amd import workloads as WL
+    system = WL.dmc_systems()[k]
+    kw = dict(tau=0.01, time_step=0.01, iters=6, block=2, eq=1, eref=eref, brancher=M.BRANCH_SR, lu=64, seed=5)
+    print(f"dmc workload shape {system.name} W=65536", flush=True)
+    d = run_device([system], 65536, precision=M.PREC_MIXED, lanes=2, **kw)[0]
+    o = run_oracle(system, 65536, precision=SJ.PREC_MIXED, **kw)
+    check_parity(d, o)
+
+
 def test_dmc_lane_independence():
     kw = dict(tau=0.02, time_step=0.02, iters=20, block=5, eq=1, eref=-7.0, brancher=M.BRANCH_SR, lu=8)
     a = run_device([S.atom(3)], 64, lanes=1, **kw)[0]
