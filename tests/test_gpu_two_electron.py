@@ -332,6 +332,27 @@ def test_device_optimisation_loop_equals_host_loop():
     assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
 
 
+def test_helium_energy_run_full_shape():
+    # He (zeta 1.6875, Pade b 0.5, c 0.3), 4,096 walkers x 10,000 sweeps,
+    # block 100 (the first discarded), box side 0.5: one Runner::run whose
+    # final configurations are the oracle's bit for bit (so are all 8.2e7
+    # accept decisions on the way), with the same sums
+    W, S, B, side = 4096, 10000, 100, 0.5
+    print(f"helium energy run W={W} sweeps={S}", flush=True)
+    with M.Context([M.helium(W, 1.6875, 0.5, 0.3)], seed=0, step=side, block_size=B) as ctx:
+        ctx.init_walkers()
+        ctx.run(S)
+        cfg = ctx.get_configs()
+        st = ctx.stats()
+    ref = O.run_box(O.helium(1.6875, 0.5, 0.3), seed=0, walkers=W, steps=S, block_size=B, box_side=side,
+                    record=False, n_threads=8)
+    assert np.array_equal(bits(cfg), bits(ref["cfg"]))
+    assert st["n_accepted"] == ref["n_accepted"] and st["n_moves"] == ref["n_moves"]
+    assert st["n_samples"] == ref["n_samples"] and st["n_blocks"] == ref["n_blocks"]
+    assert abs(st["sum_e"] - ref["sum_e"]) <= 1e-12 * abs(ref["sum_e"])
+    assert abs(st["sum_block_mean2"] - ref["sum_bm2"]) <= 1e-12 * ref["sum_bm2"]
+
+
 def test_benchmark_shape_parity_and_device_loop():
     # the H2 + SR benchmark at its full shape: 16,384 walkers x 200 sweeps,
     # block 20, box side 0.5, parameters (1.0, 0.5, 0.3).  One Runner::run
