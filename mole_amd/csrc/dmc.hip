@@ -250,7 +250,8 @@ __device__ __forceinline__ void gather_walker(const DmcArgs &a, uint32_t j, uint
 // cumulative weight through walker k is its block-local sum plus the
 // exclusive offset of its scan block.  With LOCAL_OFFSETS (at most
 // kScanThreads scan blocks) every workgroup scans the block totals in LDS
-// itself; otherwise bsum_a already holds the offsets.
+// itself; otherwise bsum_a already holds the offsets.  Thread j then gathers
+// its chosen walker into column j of the copies the next step starts from.
 template <bool LOCAL_OFFSETS>
 __global__ void __launch_bounds__(kScanThreads) sr_select_kernel(DmcArgs a) {
   __shared__ uint64_t off[kScanThreads];
