@@ -41,9 +41,9 @@ def time_lanes(system, walkers, lanes, *, sweeps=6, precision=M.PREC_MIXED, lu_p
 def tune(systems, walkers_of, *, candidates=CANDIDATES, sweeps=6, precision=M.PREC_MIXED,
          lu_period=64, lds_budget=None, verbose=False):
     """Best L per electron count.  walkers_of(system) gives the walker count
-    to time with; lds_budget caps the LDS of a workgroup (one launch shares
-    the largest footprint of its tiles, so a budget keeps large systems from
-    lowering the occupancy of small ones)."""
+    to time with; lds_budget caps the LDS of a workgroup (the tiles of one
+    4 KiB footprint class share a launch and its largest footprint, so the
+    cap bounds the occupancy the largest systems run at)."""
     table = {}
     timings = {}
     reps = {}

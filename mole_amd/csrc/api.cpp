@@ -138,6 +138,14 @@ struct mole_ctx {
   SJTile *d_tiles = nullptr;
   int n_tiles = 0;
   size_t lds_bytes = 0, lds_limit = 65536;
+  // tiles of one LDS footprint class (4 KiB steps) are launched together with
+  // that class's dynamic LDS, so small walkers are not held to the largest
+  // footprint's occupancy
+  struct TileGroup {
+    int first, count;
+    size_t lds;
+  };
+  std::vector<TileGroup> tile_groups;
   bool sysdev_dirty = true;
   mole_mlp *mlp = nullptr;  // network of the learned proposal (not owned)
   // device-side optimisation: parameters read by the kernels from d_params
@@ -322,11 +330,12 @@ int sj_prop(const mole_ctx *ctx) {
                                                       : kSJPropBox;
 }
 
-// Tiles of 64/L walkers, most expensive first (longest-processing-time
-// order, so the tail of the launch is short tiles), and the dynamic LDS of
-// the launch: the largest tile footprint.
+// Tiles of 64/L walkers grouped by LDS footprint class (4 KiB steps, the
+// largest class first), most expensive first within a group
+// (longest-processing-time order, so the tail of each launch is short
+// tiles); each group's dynamic LDS is its largest tile footprint.
 mole_status sj_build_tiles(mole_ctx *ctx) {
-  struct T { SJTile t; double cost; };
+  struct T { SJTile t; double cost; size_t fp; };
   std::vector<T> tiles;
   size_t lds = 0;
   for (size_t s = 0; s < ctx->sys.size(); ++s) {
@@ -338,15 +347,26 @@ mole_status sj_build_tiles(mole_ctx *ctx) {
     const int nmax = (int)std::max(st.n_up, st.n_down);
     const double per_walker = (double)st.N * (st.N + st.n_nuc + st.n_basis + (double)nmax * st.n_basis +
                                                (double)nmax * nmax);
+    const size_t fp = (size_t)wpt * st.lds_walker;
     for (int w0 = 0; w0 < st.W; w0 += wpt)
-      tiles.push_back({{(int)s, w0, std::min(wpt, st.W - w0)}, per_walker / st.L});
+      tiles.push_back({{(int)s, w0, std::min(wpt, st.W - w0)}, per_walker / st.L, fp});
   }
   if (lds > ctx->lds_limit)
     return fail(MOLE_ERR_UNSUPPORTED, "walker state needs %zu B of LDS per workgroup (limit %zu): use "
                 "more lanes per walker", lds, ctx->lds_limit);
-  std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+  auto cls = [](size_t fp) { return (fp + 4095) / 4096; };
+  std::stable_sort(tiles.begin(), tiles.end(), [&](const T &a, const T &b) {
+    return cls(a.fp) != cls(b.fp) ? cls(a.fp) > cls(b.fp) : a.cost > b.cost;
+  });
   std::vector<SJTile> h(tiles.size());
-  for (size_t i = 0; i < tiles.size(); ++i) h[i] = tiles[i].t;
+  ctx->tile_groups.clear();
+  for (size_t i = 0; i < tiles.size(); ++i) {
+    h[i] = tiles[i].t;
+    if (i == 0 || cls(tiles[i].fp) != cls(tiles[i - 1].fp)) ctx->tile_groups.push_back({(int)i, 0, 0});
+    mole_ctx::TileGroup &g = ctx->tile_groups.back();
+    ++g.count;
+    g.lds = std::max(g.lds, tiles[i].fp);
+  }
   if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
   ctx->d_tiles = nullptr;
   HIP_TRY(hipMalloc(&ctx->d_tiles, sizeof(SJTile) * h.size()));
@@ -405,6 +425,17 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
   return MOLE_OK;
 }
 
+// One launch per tile group, each with its group's dynamic LDS.
+hipError_t sj_launch_groups(const mole_ctx *ctx, SJArgs a) {
+  const int fp32 = ctx->run.precision == MOLE_PREC_MIXED;
+  for (const mole_ctx::TileGroup &g : ctx->tile_groups) {
+    a.tiles = ctx->d_tiles + g.first;
+    const hipError_t e = launch_sj(a, g.count, fp32, sj_prop(ctx), g.lds, ctx->stream);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
+}
+
 SJArgs sj_args(const mole_ctx *ctx) {
   SJArgs a{};
   a.systems = ctx->d_sysdev;
@@ -449,7 +480,7 @@ mole_status sj_enqueue(mole_ctx *ctx, uint32_t n, uint32_t ss) {
   SJArgs a = sj_args(ctx);
   a.n_sweeps = n;
   a.sample_start = ss;
-  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
+  HIP_TRY(sj_launch_groups(ctx, a));
   HIP_TRY(launch_sj_advance(ctx->d_state, n, n - std::min(ss, n), ctx->run.block_size, ctx->stream));
   return MOLE_OK;
 }
@@ -483,7 +514,7 @@ mole_status sj_prepare(mole_ctx *ctx, bool init) {
   a.init = init ? 1 : 0;
   a.refresh = 1;
   for (SysState &s : ctx->sys) s.cols_valid = false;
-  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
+  HIP_TRY(sj_launch_groups(ctx, a));
   return MOLE_OK;
 }
 
@@ -1783,7 +1814,7 @@ mole_status dmc_enqueue_step(mole_ctx *ctx, double time_step, uint32_t block_siz
   a.dmc = kSJDmcStep;
   a.dmc_tau = time_step;
   a.ext = 1;
-  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
+  HIP_TRY(sj_launch_groups(ctx, a));
   for (uint32_t s = 0; s < ctx->sys.size(); ++s) {
     DmcArgs d = dmc_args(ctx, s, block_size, num_eq_blocks);
     if (s == 0) {  // the step's sweep, counted once (no samples)
@@ -1834,7 +1865,7 @@ mole_status mole_dmc_init(mole_ctx *ctx, const double *reference_energy, uint32_
   a.refresh = 1;
   a.dmc = kSJDmcInit;
   a.ext = 1;
-  HIP_TRY(launch_sj(a, ctx->n_tiles, ctx->run.precision == MOLE_PREC_MIXED, sj_prop(ctx), ctx->lds_bytes, ctx->stream));
+  HIP_TRY(sj_launch_groups(ctx, a));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   ctx->dmc_brancher = brancher;
   ctx->dmc_ready = true;

@@ -74,10 +74,11 @@ def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, s
 
 
 def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, chunk=10, block=10,
-           precision=M.PREC_MIXED, lu_period=64, seed=3, tune=True, table=None, lds_budget=20 << 10,
+           precision=M.PREC_MIXED, lu_period=64, seed=3, tune=True, table=None, lds_budget=40 << 10,
            log=print):
-    """All systems in one context: one launch per `chunk` sweeps (every
-    system's tiles, most expensive first), replayed from a HIP graph."""
+    """All systems in one context: per `chunk` sweeps one launch per LDS
+    footprint class of the tiles (most expensive first within a class),
+    replayed from a HIP graph."""
     sys_list = hetero_systems(n_systems, seed)
     Ns = np.array([s.n_electrons for s in sys_list])
     W = np.maximum(1, np.floor(Ns / Ns.sum() * total_walkers)).astype(int)
