@@ -468,8 +468,9 @@ mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t blo
  * on it; it only trades per-move parallelism against walkers per wave. */
 mole_status mole_set_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t lanes);
 mole_status mole_get_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t *lanes);
-/* Slater-Jastrow launch shape: workgroups per launch and dynamic LDS bytes
- * per workgroup (the largest tile footprint of the context). */
+/* Slater-Jastrow launch shape: workgroups of one sweep step over all its
+ * launches (one launch per LDS footprint class of the tiles) and the
+ * largest dynamic LDS bytes per workgroup among them. */
 mole_status mole_get_launch_info(mole_ctx *ctx, uint32_t *workgroups, size_t *lds_bytes);
 
 /* ---- captured sweeps ---------------------------------------------------
