@@ -113,11 +113,19 @@ __device__ __forceinline__ f32x4 bias4(const float *b, int row0) {
 // out[0..3] of samples 0..15.  All 64 lanes must call it.
 __device__ __forceinline__ f32x4 mlp_forward(const MlpDev &m, bf16x4 x, f32x4 *h) {
   const int l = threadIdx.x & 63, q = l >> 4;
+  // every weight fragment of the pass is loaded before the first MFMA, so
+  // the loads are in flight together instead of one layer at a time
+  bf16x4 w1[4];
+  bf16x8 wf[kFwdFrags];
+#pragma unroll
+  for (int mt = 0; mt < 4; ++mt) w1[mt] = m.fwd1[mt * 64 + l];
+#pragma unroll
+  for (int f = 0; f < kFwdFrags; ++f) wf[f] = m.fwd[f * 64 + l];
   f32x4 t[4];
 #pragma unroll
   for (int mt = 0; mt < 4; ++mt) {
     f32x4 acc = bias4(m.bias, 16 * mt + 4 * q);
-    acc = mfma16(m.fwd1[mt * 64 + l], x, acc);
+    acc = mfma16(w1[mt], x, acc);
     t[mt] = tanh4(acc);
     if (h) h[mt] = t[mt];
   }
@@ -128,15 +136,15 @@ __device__ __forceinline__ f32x4 mlp_forward(const MlpDev &m, bf16x4 x, f32x4 *h
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt) {
       f32x4 acc = bias4(m.bias + 64 * (layer + 1), 16 * mt + 4 * q);
-      acc = mfma32(m.fwd[(fr + mt * 2) * 64 + l], b0, acc);
-      acc = mfma32(m.fwd[(fr + mt * 2 + 1) * 64 + l], b1, acc);
+      acc = mfma32(wf[fr + mt * 2], b0, acc);
+      acc = mfma32(wf[fr + mt * 2 + 1], b1, acc);
       t[mt] = tanh4(acc);
       if (h) h[4 * (layer + 1) + mt] = t[mt];
     }
   }
   f32x4 out = bias4(m.bias + 192, 4 * q);
-  out = mfma32(m.fwd[kFwdL4 * 64 + l], pack_k32(t[0], t[1]), out);
-  out = mfma32(m.fwd[(kFwdL4 + 1) * 64 + l], pack_k32(t[2], t[3]), out);
+  out = mfma32(wf[kFwdL4], pack_k32(t[0], t[1]), out);
+  out = mfma32(wf[kFwdL4 + 1], pack_k32(t[2], t[3]), out);
   return out;
 }
 
