@@ -396,16 +396,35 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
   }
 }
 
+// Per parameter the partials of the G workgroups summed in workgroup order;
+// the loads go out 16 at a time and are added in that order.
 __global__ void mlp_reduce_kernel(const float *partial, const double *obj, int G, float *grad, double *obj_sum) {
+  constexpr int kBatch = 16;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < kMlpParams) {
     float t = 0.0f;
-    for (int gi = 0; gi < G; ++gi) t += partial[(size_t)gi * kMlpParams + p];
+    int gi = 0;
+    for (; gi + kBatch <= G; gi += kBatch) {
+      float v[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) v[q] = partial[(size_t)(gi + q) * kMlpParams + p];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) t += v[q];
+    }
+    for (; gi < G; ++gi) t += partial[(size_t)gi * kMlpParams + p];
     grad[p] = t;
   }
   if (p == 0) {
     double t = 0.0;
-    for (int gi = 0; gi < G; ++gi) t += obj[gi];
+    int gi = 0;
+    for (; gi + kBatch <= G; gi += kBatch) {
+      double v[kBatch];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) v[q] = obj[gi + q];
+#pragma unroll
+      for (int q = 0; q < kBatch; ++q) t += v[q];
+    }
+    for (; gi < G; ++gi) t += obj[gi];
     *obj_sum = t;
   }
 }
