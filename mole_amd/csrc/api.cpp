@@ -1513,7 +1513,7 @@ namespace {
 // One gradient evaluation of `batch` samples starting at global sample
 // `first` into m->d_grad and *obj_dev (device); returns after enqueueing.
 mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
-                                 uint64_t seed, double *obj_dev) {
+                                 uint64_t seed, double *obj_dev, const MlpAdamStep *adam = nullptr) {
   const int n_mb = (int)(batch / 16);
   const int G = std::min(n_mb, 1024);
   if (G > m->partial_groups) {
@@ -1525,7 +1525,6 @@ mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint3
     HIP_TRY(hipMalloc(&m->d_obj, sizeof(double) * (size_t)G));
     m->partial_groups = G;
   }
-  HIP_TRY(launch_fill((double *)m->d_partial, (size_t)kMlpParams * G / 2, 0.0, m->stream));
   MlpTrainArgs a{};
   a.net = m->net();
   a.sys = ctx->d_sysdev + sys;
@@ -1537,7 +1536,7 @@ mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint3
   a.partial = m->d_partial;
   a.obj = m->d_obj;
   HIP_TRY(launch_mlp_train(a, G, m->stream));
-  HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_grad, obj_dev, m->stream));
+  HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_grad, obj_dev, adam, m->stream));
   return MOLE_OK;
 }
 
@@ -1597,11 +1596,12 @@ mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_
   }
   for (uint32_t k = 0; k < d->steps; ++k) {
     const uint64_t first = (uint64_t)m->adam_step * d->batch;
-    st = mlp_gradient_enqueue(m, ctx, sys, d->batch, first, d->seed, m->d_obj_steps + k);
+    // the gradient reduction applies the Adam step (step counts from 1)
+    const MlpAdamStep adam{m->d_w, m->d_m, m->d_v, 1.0f / (float)d->batch, d->lr, d->beta1, d->beta2, d->eps,
+                           (int)m->adam_step + 1};
+    st = mlp_gradient_enqueue(m, ctx, sys, d->batch, first, d->seed, m->d_obj_steps + k, &adam);
     if (st != MOLE_OK) return st;
     ++m->adam_step;
-    HIP_TRY(launch_mlp_adam(m->d_w, m->d_m, m->d_v, m->d_grad, 1.0f / (float)d->batch, d->lr, d->beta1,
-                            d->beta2, d->eps, m->adam_step, m->stream));
     HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd1, m->d_fwd, m->d_bwd, m->d_bias, m->stream));
   }
   if (objective) {

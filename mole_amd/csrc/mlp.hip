@@ -302,6 +302,9 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
   const int N = S.N;
   const int n_mb = (int)(a.batch / 16);
   float *g = a.partial + (size_t)blockIdx.x * kMlpParams;
+  // this workgroup's partial gradient starts from zero (the passes below add
+  // to it; the loop's first barrier orders these stores before them)
+  for (int p = l; p < kMlpParams; p += 64) g[p] = 0.0f;
   double obj_acc = 0.0;
   for (int mb = blockIdx.x; mb < n_mb; mb += gridDim.x) {
     const uint64_t gi = a.first + (uint64_t)mb * 16 + sl;
@@ -396,9 +399,23 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
   }
 }
 
+// Adam on parameter p from its summed gradient (g = -grad / batch)
+__device__ __forceinline__ void adam_update(const MlpAdamStep &a, int p, float grad) {
+  const float gr = -grad * a.inv_batch;  // loss = -mean objective
+  const float mm = a.beta1 * a.m[p] + (1.0f - a.beta1) * gr;
+  const float vv = a.beta2 * a.v[p] + (1.0f - a.beta2) * gr * gr;
+  a.m[p] = mm;
+  a.v[p] = vv;
+  const float mh = mm / (1.0f - powf(a.beta1, (float)a.step));
+  const float vh = vv / (1.0f - powf(a.beta2, (float)a.step));
+  a.w[p] = a.w[p] - a.lr * mh / (sqrtf(vh) + a.eps);
+}
+
 // Per parameter the partials of the G workgroups summed in workgroup order;
-// the loads go out 16 at a time and are added in that order.
-__global__ void mlp_reduce_kernel(const float *partial, const double *obj, int G, float *grad, double *obj_sum) {
+// the loads go out 16 at a time and are added in that order.  With
+// do_adam the thread then takes the Adam step of its parameter.
+__global__ void mlp_reduce_kernel(const float *partial, const double *obj, int G, float *grad, double *obj_sum,
+                                  MlpAdamStep adam, int do_adam) {
   constexpr int kBatch = 16;
   const int p = blockIdx.x * blockDim.x + threadIdx.x;
   if (p < kMlpParams) {
@@ -413,6 +430,7 @@ __global__ void mlp_reduce_kernel(const float *partial, const double *obj, int G
     }
     for (; gi < G; ++gi) t += partial[(size_t)gi * kMlpParams + p];
     grad[p] = t;
+    if (do_adam) adam_update(adam, p, t);
   }
   if (p == 0) {
     double t = 0.0;
@@ -429,19 +447,6 @@ __global__ void mlp_reduce_kernel(const float *partial, const double *obj, int G
   }
 }
 
-__global__ void mlp_adam_kernel(float *w, float *m, float *v, const float *grad, float inv_batch, float lr,
-                                float b1, float b2, float eps, int step) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kMlpParams) return;
-  const float gr = -grad[p] * inv_batch;  // loss = -mean objective
-  const float mm = b1 * m[p] + (1.0f - b1) * gr;
-  const float vv = b2 * v[p] + (1.0f - b2) * gr * gr;
-  m[p] = mm;
-  v[p] = vv;
-  const float mh = mm / (1.0f - powf(b1, (float)step));
-  const float vh = vv / (1.0f - powf(b2, (float)step));
-  w[p] = w[p] - lr * mh / (sqrtf(vh) + eps);
-}
 
 // out[4 e + k] = network output k for electron electron[e] of configuration e
 __global__ void __launch_bounds__(64) mlp_eval_kernel(MlpDev net, const double *cfg, int N, const double *nuc,
@@ -479,16 +484,10 @@ hipError_t launch_mlp_train(const MlpTrainArgs &a, int n_groups, hipStream_t str
 }
 
 hipError_t launch_mlp_reduce(const float *partial, const double *obj, int n_groups, float *grad,
-                             double *obj_sum, hipStream_t stream) {
+                             double *obj_sum, const MlpAdamStep *adam, hipStream_t stream) {
+  const MlpAdamStep none{};
   hipLaunchKernelGGL(mlp_reduce_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, partial, obj,
-                     n_groups, grad, obj_sum);
-  return hipGetLastError();
-}
-
-hipError_t launch_mlp_adam(float *w, float *m, float *v, const float *grad, float inv_batch, float lr,
-                           float beta1, float beta2, float eps, int step, hipStream_t stream) {
-  hipLaunchKernelGGL(mlp_adam_kernel, dim3((kMlpParams + 255) / 256), dim3(256), 0, stream, w, m, v, grad,
-                     inv_batch, lr, beta1, beta2, eps, step);
+                     n_groups, grad, obj_sum, adam ? *adam : none, adam ? 1 : 0);
   return hipGetLastError();
 }
 

@@ -31,12 +31,17 @@ hipError_t launch_mlp_init(float *w, uint64_t seed, hipStream_t stream);
 hipError_t launch_mlp_pack(const float *w, bf16x4 *fwd1, bf16x8 *fwd, bf16x4 *bwd, float *bias,
                            hipStream_t stream);
 hipError_t launch_mlp_train(const MlpTrainArgs &a, int n_groups, hipStream_t stream);
-// grad[p] = sum_g partial[g][p], obj_sum = sum_g obj[g] (fixed order)
+// One Adam step on the loss -mean(A |dx|^2) (g = -grad / batch) of the FP32
+// master weights; step counts from 1 (bias correction)
+struct MlpAdamStep {
+  float *w, *m, *v;
+  float inv_batch, lr, beta1, beta2, eps;
+  int step;
+};
+// grad[p] = sum_g partial[g][p], obj_sum = sum_g obj[g] (fixed order); with
+// adam, the same thread then applies the Adam step to parameter p
 hipError_t launch_mlp_reduce(const float *partial, const double *obj, int n_groups, float *grad,
-                             double *obj_sum, hipStream_t stream);
-// Adam on the loss -mean(A |dx|^2): g = -grad / batch
-hipError_t launch_mlp_adam(float *w, float *m, float *v, const float *grad, float inv_batch, float lr,
-                           float beta1, float beta2, float eps, int step, hipStream_t stream);
+                             double *obj_sum, const MlpAdamStep *adam, hipStream_t stream);
 // features and network outputs of given configurations (debug / tests)
 hipError_t launch_mlp_eval(const MlpDev &net, const double *cfg, int N, const double *nuc, int n_nuc,
                            const int *electron, int n, float *out, hipStream_t stream);
