@@ -124,6 +124,12 @@ typedef struct {
 #define MOLE_BASIS_STO 0 /* exp(-zeta |r - R|) */
 #define MOLE_BASIS_GTO 1 /* exp(-zeta |r - R|^2) (the Gaussian trial functions of
                             examples/custom_operator.rs:64-100, examples/dmc.rs:33-70) */
+/* p-type STOs d_q exp(-zeta |d|), d = r - R, q = x, y, z: the angular
+ * functions of the 2p shell (the 2p hydrogen eigenfunction is
+ * z exp(-r/2), E = -1/8). */
+#define MOLE_BASIS_STO_PX 2
+#define MOLE_BASIS_STO_PY 3
+#define MOLE_BASIS_STO_PZ 4
 
 /* How the walkers move and what they record. */
 typedef struct {
@@ -155,6 +161,17 @@ typedef struct {
    * the last reset (0 otherwise) -- the condition estimate that says
    * whether FP32 inverses are trustworthy. */
   double max_condition;
+  /* Slater-Jastrow: spin blocks found numerically singular at a
+   * re-inversion (a pivot |p| <= n 2^-52 ||A||_1): the division by the pivot
+   * is skipped and the block's log|det| is -inf until the next
+   * re-inversion. */
+  uint64_t n_singular;
+  /* Slater-Jastrow VMC: largest |log|det A_s| accumulated over the
+   * Sherman-Morrison updates since the previous re-inversion - log|det A_s|
+   * from the re-inversion's pivots|, both FP64 (the accumulation multiplies
+   * the ratios of the update's precision): the error the FP32 (MIXED) or
+   * FP64 update chain has built up between re-inversions. */
+  double max_logdet_drift;
 } mole_stats;
 
 /* Sums for the energy gradient and the SR matrix
@@ -261,6 +278,18 @@ mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_blo
 mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local,
                               double *psi, double *pgrad, size_t walkers,
                               size_t samples);
+/* Slater-Jastrow: log|det A_up|, log|det A_dn| and their signs per walker,
+ * as [walker][2] each -- the determinant factor of Function::value
+ * (src/wavefunction_traits/src/lib.rs:7-11; psi = sign_up sign_dn
+ * exp(logdet_up + logdet_dn + J)).  FP64, carried through every accepted
+ * Sherman-Morrison update as log|det| <- log|det| + log|R_D| (a mantissa
+ * product and a binary exponent, so nothing over- or underflows) and
+ * replaced at every re-inversion by the sum of log|pivot| of its
+ * Gauss-Jordan elimination.  A singular block reads -inf with sign 0.
+ * DMC steps do not carry it (the next VMC re-inversion restores it).
+ * sign may be NULL. */
+mole_status mole_read_logdet(mole_ctx *ctx, uint32_t sys, double *logdet, double *sign,
+                             size_t n_walkers);
 
 /* ---- single configurations ---------------------------------------------
  * Function::value, Differentiate::gradient/laplacian

@@ -48,6 +48,7 @@ MAX_PARAMS = 16
 
 BASIS_STO = 0
 BASIS_GTO = 1
+BASIS_STO_PX, BASIS_STO_PY, BASIS_STO_PZ = 2, 3, 4
 TERM_KINETIC = 0
 TERM_IONIC_POTENTIAL = 1
 TERM_ELECTRONIC_POTENTIAL = 2
@@ -138,6 +139,8 @@ class Stats(C.Structure):
         ("sum_block_mean", C.c_double),
         ("sum_block_mean2", C.c_double),
         ("max_condition", C.c_double),
+        ("n_singular", C.c_uint64),
+        ("max_logdet_drift", C.c_double),
     ]
 
 
@@ -242,6 +245,7 @@ _SIGS = [
     ("mole_read_sr_sums", C.c_int32, [_P, C.c_uint32, C.POINTER(SrSums)]),
     ("mole_read_trace", C.c_int32, [_P, C.c_uint32, _pu64, _pd, C.c_size_t, C.c_size_t, C.c_size_t]),
     ("mole_read_samples", C.c_int32, [_P, C.c_uint32, _pd, _pd, _pd, C.c_size_t, C.c_size_t]),
+    ("mole_read_logdet", C.c_int32, [_P, C.c_uint32, _pd, _pd, C.c_size_t]),
     ("mole_eval", C.c_int32, [_P, C.c_uint32, C.c_size_t, _pd, _pd, _pd, _pd, _pd, _pd]),
     ("mole_set_launch_config", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_set_walker_lanes", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
@@ -486,6 +490,14 @@ class Context:
         e = np.zeros((W, samples))
         check(lib().mole_read_trace(self.h, sys, _u64p(m), _dp(e), W, sweeps, samples))
         return m, e
+
+    def logdet(self, sys=0):
+        """mole_read_logdet: (log|det A_s| [W][2], sign [W][2]) per walker."""
+        W = self.systems[sys].n_walkers
+        ld = np.zeros((W, 2))
+        sg = np.zeros((W, 2))
+        check(lib().mole_read_logdet(self.h, sys, _dp(ld), _dp(sg), W))
+        return ld, sg
 
     def samples(self, samples, sys=0):
         W = self.systems[sys].n_walkers

@@ -81,7 +81,8 @@ struct SysState {
   void *d_B = nullptr;
   double v_nn = 0.0;
   uint64_t draw_base = 0, move_base = 0;
-  bool any_gto = false;
+  bool any_gto = false, any_p = false;
+  double *d_ld = nullptr;  // [kSJLdColumns][W] log-determinant state
   // local operators: ops[0] the custom Hamiltonian, ops[1..] the observables
   std::vector<mole_operator> ops = std::vector<mole_operator>(1);
   bool custom_h = false;
@@ -255,7 +256,7 @@ mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run
           return fail(MOLE_ERR_SHAPE, "system %u: basis function %u centred on missing nucleus", s, m);
       if (d.basis_kind)
         for (uint32_t m = 0; m < d.n_basis; ++m) {
-          if (d.basis_kind[m] > MOLE_BASIS_GTO)
+          if (d.basis_kind[m] > MOLE_BASIS_STO_PZ)
             return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: basis function %u has unknown kind %u", s, m,
                         d.basis_kind[m]);
           if (d.basis_kind[m] == MOLE_BASIS_GTO && run->proposal == MOLE_PROPOSAL_LEARNED)
@@ -300,7 +301,7 @@ void free_ctx(mole_ctx *ctx) {
   if (!ctx) return;
   for (SysState &st : ctx->sys) {
     void *ptrs[] = {st.d_pos, st.d_acc, st.d_trace_mask, st.d_trace_e, st.d_samp_psi,
-                    st.d_samp_g, st.d_tab, st.d_center, st.d_B};
+                    st.d_samp_g, st.d_tab, st.d_center, st.d_B, st.d_ld};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     free_dmc(st.dmc);
@@ -400,6 +401,7 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.pos = st.d_pos;
     d.B = st.d_B;
     d.acc = st.d_acc;
+    d.ld = st.d_ld;
     d.trace_mask = st.d_trace_mask;
     d.trace_e = st.d_trace_e;
     d.draw_base = st.draw_base;
@@ -414,6 +416,7 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.dmc_w_in = st.dmc.wnew;
     d.kind = st.d_center + st.n_basis;
     d.any_gto = st.any_gto ? 1 : 0;
+    d.any_p = st.any_p ? 1 : 0;
     d.ops = st.d_ops;
     d.custom_h = st.custom_h ? 1 : 0;
     d.n_obs = st.n_obs;
@@ -601,8 +604,8 @@ mole_status reduce_sys(mole_ctx *ctx, uint32_t sys, const double **cols) {
   if (!s.cols_valid) {
     HIP_TRY(hipSetDevice(ctx->device));
     s.cols.resize(s.n_cols);
-    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red,
-                                  ctx->family == kFamSJ ? (int)kSJCondMax : -1, ctx->stream));
+    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, ctx->family == kFamSJ ? kSJMaxFirst : -1,
+                                  ctx->family == kFamSJ ? kSJMaxCount : 0, ctx->stream));
     HIP_TRY(hipMemcpyAsync(s.cols.data(), ctx->d_red, sizeof(double) * s.n_cols, hipMemcpyDeviceToHost,
                            ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -740,6 +743,7 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       std::vector<int> cen(d.basis_center, d.basis_center + d.n_basis);
       for (uint32_t m = 0; m < d.n_basis; ++m) cen.push_back(d.basis_kind ? (int)d.basis_kind[m] : MOLE_BASIS_STO);
       S.any_gto = std::any_of(cen.begin() + d.n_basis, cen.end(), [](int k) { return k == MOLE_BASIS_GTO; });
+      S.any_p = std::any_of(cen.begin() + d.n_basis, cen.end(), [](int k) { return k >= MOLE_BASIS_STO_PX; });
       CTX_TRY(hipMalloc(&S.d_tab, sizeof(double) * tab.size()));
       CTX_TRY(hipMemcpy(S.d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
       CTX_TRY(hipMalloc(&S.d_center, sizeof(int) * cen.size()));
@@ -747,6 +751,8 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       const size_t nB = (size_t)S.n_up * S.n_up + (size_t)S.n_down * S.n_down;
       CTX_TRY(hipMalloc(&S.d_B, (size_t)sj_tsize(ctx) * nB * W + 8));
       CTX_TRY(launch_fill((double *)S.d_B, (sj_tsize(ctx) * nB * W) / 8, 0.0, ctx->stream));
+      CTX_TRY(hipMalloc(&S.d_ld, sizeof(double) * kSJLdColumns * W));
+      CTX_TRY(launch_fill(S.d_ld, kSJLdColumns * W, 0.0, ctx->stream));
     }
   }
   if (ctx->family == kFamSJ) {
@@ -971,6 +977,8 @@ mole_status mole_read_stats(mole_ctx *ctx, uint32_t sys, mole_stats *out) {
   out->sum_block_mean = cols[sj ? kSJSumBm : kAccSumBm];
   out->sum_block_mean2 = cols[sj ? kSJSumBm2 : kAccSumBm2];
   out->max_condition = sj ? cols[kSJCondMax] : 0.0;
+  out->n_singular = sj ? (uint64_t)cols[kSJSingular] : 0;
+  out->max_logdet_drift = sj ? cols[kSJLdDrift] : 0.0;
   return ok();
 }
 
@@ -1086,6 +1094,28 @@ mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, doub
       std::memcpy(pgrad + w * samples * kNParam2, h.data() + w * cap * kNParam2,
                   samples * kNParam2 * sizeof(double));
   }
+  return ok();
+}
+
+mole_status mole_read_logdet(mole_ctx *ctx, uint32_t sys, double *logdet, double *sign, size_t n_walkers) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "log-determinants are kept for Slater-Jastrow systems");
+  if (!logdet) return fail(MOLE_ERR_INVALID_ARGUMENT, "null logdet");
+  SysState &s = ctx->sys[sys];
+  if (n_walkers != (size_t)s.W) return fail(MOLE_ERR_SHAPE, "expected %d walkers", s.W);
+  const size_t W = (size_t)s.W;
+  std::vector<double> h((size_t)kSJLdColumns * W);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipMemcpyAsync(h.data(), s.d_ld, h.size() * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  for (size_t w = 0; w < W; ++w)
+    for (int sp = 0; sp < 2; ++sp) {
+      const double mant = h[(size_t)(2 * sp) * W + w], ex = h[(size_t)(2 * sp + 1) * W + w];
+      // det = mant 2^ex
+      logdet[2 * w + sp] = mant == 0.0 ? -INFINITY : std::log(std::fabs(mant)) + ex * 0.69314718055994530942;
+      if (sign) sign[2 * w + sp] = mant > 0.0 ? 1.0 : (mant < 0.0 ? -1.0 : 0.0);
+    }
   return ok();
 }
 
@@ -1699,7 +1729,7 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
       ctx->dev_params = false;
       return st;
     }
-    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, ctx->stream));
+    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, 0, ctx->stream));
     HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, ctx->d_opt, ctx->d_optlog + 8 * (size_t)it,
                             ctx->stream));
     launches += 2;
@@ -1956,6 +1986,14 @@ mole_status mole_dmc_diffuse(mole_ctx *ctx, double time_step, uint32_t num_itera
   if (le != hipSuccess) return fail(MOLE_ERR_DEVICE, "DMC steps: %s", hipGetErrorString(le));
   ctx->last_launches = (uint32_t)steps;
   ctx->timed = true;
+  {
+    // the steps do not carry the log-determinant state of the walkers they
+    // branch: take it from the walkers' configurations (inverses kept)
+    SJArgs a = sj_args(ctx);
+    a.n_sweeps = 0;
+    a.refresh = 2;
+    HIP_TRY(sj_launch_groups(ctx, a));
+  }
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   for (uint32_t k = 0; k < ctx->sys.size(); ++k) {
     DmcState v;
@@ -2096,7 +2134,7 @@ mole_status mole_read_observables(mole_ctx *ctx, uint32_t sys, double *sum, doub
   HIP_TRY(hipSetDevice(ctx->device));
   double *d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(double) * 2 * n));
-  hipError_t e = launch_reduce_columns(s.d_obs_acc, s.W, (int)(2 * n), d, -1, ctx->stream);
+  hipError_t e = launch_reduce_columns(s.d_obs_acc, s.W, (int)(2 * n), d, -1, 0, ctx->stream);
   std::vector<double> h(2 * n);
   if (e == hipSuccess) e = hipMemcpyAsync(h.data(), d, sizeof(double) * 2 * n, hipMemcpyDeviceToHost, ctx->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);

@@ -87,8 +87,10 @@ hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads,
 hipError_t launch_init2(double *pos, int W, uint64_t seed, uint32_t sys,
                         uint32_t epoch, hipStream_t stream);
 hipError_t launch_eval2(const Eval2Args &a, hipStream_t stream);
+// out[k] = sum over walkers of column k of acc [K][W] in a fixed order; the
+// columns [max_first, max_first + max_count) by their maximum instead
 hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out,
-                                 int max_col, hipStream_t stream);
+                                 int max_first, int max_count, hipStream_t stream);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t stream);
 
 }  // namespace mole

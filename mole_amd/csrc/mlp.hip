@@ -3,6 +3,7 @@
 // 16 samples, with the exact single-electron determinant ratio and drift of
 // the Slater-Jastrow wavefunction), the fixed-order gradient reduction and
 // Adam.
+#include "basis.hpp"
 #include "mlp_internal.hpp"
 
 namespace mole {
@@ -197,8 +198,9 @@ __device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const do
     for (int k = 0; k < n; ++k) A[row * n + k] = 0.0;
     for (int m = 0; m < K; ++m) {
       const int I = S.center[m];
-      const double ch = exp_m(-S.zeta[m] * norm3(re[0] - S.nuc[3 * I], re[1] - S.nuc[3 * I + 1],
-                                                re[2] - S.nuc[3 * I + 2]));
+      const double d0 = re[0] - S.nuc[3 * I], d1 = re[1] - S.nuc[3 * I + 1], d2 = re[2] - S.nuc[3 * I + 2];
+      const double ch = basis_value<false>(S.any_p ? S.kind[m] : MOLE_BASIS_STO, S.zeta[m], d0, d1, d2,
+                                           norm3(d0, d1, d2));
       for (int k = 0; k < n; ++k) A[row * n + k] = fma(S.coef_t[(size_t)m * no + k], ch, A[row * n + k]);
     }
     y[row] = row == c ? 1.0 : 0.0;
@@ -236,14 +238,13 @@ __device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const do
       const int I = S.center[m];
       const double d0 = xn[0] - S.nuc[3 * I], d1 = xn[1] - S.nuc[3 * I + 1], d2 = xn[2] - S.nuc[3 * I + 2];
       const double rr = norm3(d0, d1, d2);
-      const double z = S.zeta[m];
-      const double ch = exp_m(-z * rr);
+      double ch, b0, b1, b2, bl;
+      basis_vgl<false>(S.any_p ? S.kind[m] : MOLE_BASIS_STO, S.zeta[m], d0, d1, d2, rr, ch, b0, b1, b2, bl);
       const double cm = S.coef_t[(size_t)m * no + k];
-      const double gf = -z * ch / rr;
       phi = fma(cm, ch, phi);
-      g0 = fma(cm, gf * d0, g0);
-      g1 = fma(cm, gf * d1, g1);
-      g2 = fma(cm, gf * d2, g2);
+      g0 = fma(cm, b0, g0);
+      g1 = fma(cm, b1, g1);
+      g2 = fma(cm, b2, g2);
     }
     rd = fma(phi, y[k], rd);
     gd[0] = fma(g0, y[k], gd[0]);

@@ -26,12 +26,21 @@ enum SJAccColumn : int {
   kSJSumBm2 = 3,
   kSJBlockSum = 4,
   kSJAccepted = 5,
-  kSJCondMax = 6,  // largest ||A||_1 ||A^-1||_1 seen at an LU refresh
-  kSJSumO = 7,
-  kSJSumOE = 7 + kSJNParam,
-  kSJSumOO = 7 + 2 * kSJNParam,  // upper triangle, 15 columns
-  kSJColumns = 7 + 2 * kSJNParam + kSJNParam * (kSJNParam + 1) / 2
+  kSJCondMax = 6,    // largest ||A||_1 ||A^-1||_1 seen at an LU refresh
+  kSJLdDrift = 7,    // largest |log|det| accumulated - log|det| refreshed| (FP64)
+  kSJSingular = 8,   // spin blocks found singular at a refresh
+  kSJSumO = 9,
+  kSJSumOE = 9 + kSJNParam,
+  kSJSumOO = 9 + 2 * kSJNParam,  // upper triangle, 15 columns
+  kSJColumns = 9 + 2 * kSJNParam + kSJNParam * (kSJNParam + 1) / 2
 };
+// The columns reduced by their maximum (the rest by their sum).
+constexpr int kSJMaxFirst = kSJCondMax, kSJMaxCount = 2;
+
+// Per-walker log-determinant state, [kSJLdColumns][W]: for each spin block
+// (mantissa, exponent) with det A_s = mantissa 2^exponent, the mantissa in
+// [1/2, 1) in magnitude (0: singular).
+constexpr int kSJLdColumns = 4;
 
 // One system as the kernels see it (device memory).
 struct SJSysDev {
@@ -44,13 +53,15 @@ struct SJSysDev {
   const double *zeta;    // [n_basis]
   const int *center;     // [n_basis]
   const int *kind;       // [n_basis] MOLE_BASIS_*
-  int any_gto;           // 0: every basis function is an STO (kind is not read)
+  int any_gto;           // 0: no Gaussian basis function
+  int any_p;             // 0: no p-type STO (with any_gto == 0, kind is not read)
   const double *coef_t;  // [n_basis][n_orb]: lanes over orbitals read consecutive words
   double p[kSJNParam];
   double v_nn;
   double *pos;          // [3N][W]
   void *B;              // [n_up^2 + n_dn^2][W], element type of the launch
   double *acc;          // [kSJColumns][W]
+  double *ld;           // [kSJLdColumns][W] log-determinant state
   uint64_t *trace_mask; // [W][cap_sweeps] or null
   double *trace_e;      // [W][cap_samples] or null
   uint64_t draw_base;   // draw index of the first move after the stream origin
@@ -109,7 +120,8 @@ struct SJArgs {
   uint32_t lu_period;      // 0: never
   uint32_t cap_sweeps, cap_samples;
   int init;                // 1: draw U(-1,1) configurations first (draws 0..3N-1)
-  int refresh;             // 1: rebuild the inverses from the positions at entry
+  int refresh;             // 1: rebuild the inverses from the positions at entry;
+                           // 2: only the log-determinant state (inverses kept)
   MlpDev net;              // learned proposal: the packed network
   int dmc;                 // SJDmcMode
   double dmc_tau;          // DMC: the weights' time step

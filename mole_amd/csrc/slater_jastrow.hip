@@ -19,6 +19,7 @@
 //   update B[k][l] -= B[k][c] f_l (l != c), B[k][c] /= R_D,
 //          f_l = (sum_k phi_k(r') B[k][l]) / R_D  (Sherman-Morrison)
 #include "../../include/mole_gpu.h"
+#include "basis.hpp"
 #include "device_common.hpp"
 #include "mlp.hpp"
 #include "sj_internal.hpp"
@@ -27,7 +28,7 @@ namespace mole {
 
 // LDS layout of one walker (bytes); identical on host and device.
 struct SJLayout {
-  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, mv, acc, bytes;
+  uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, mv, acc, ld, bytes;
   __host__ __device__ SJLayout(int N, int nu, int nd, int nn, int K, int tsize) {
     const int nmax = nu > nd ? nu : nd;
     uint32_t o = 0;
@@ -55,6 +56,7 @@ struct SJLayout {
     flag = take(16u);
     mv = take(64u);  // proposed position and drift (Gaussian proposals)
     acc = take(8u * kSJColumns);
+    ld = take(8u * kSJLdColumns);  // directly after acc: addressed as acc + kSJColumns
     bytes = o;
   }
 };
@@ -119,15 +121,23 @@ __device__ __forceinline__ float fma_t(float a, float b, float c) { return fmaf(
 __device__ __forceinline__ int small_div(int e, float inv_n) { return (int)(((float)e + 0.5f) * inv_n); }
 
 // Basis values chi[m] of the point r into LDS (lanes over nuclei, then over
-// basis functions).  Ends with a barrier.
+// basis functions).  gto/pk: Gaussians / p-type STOs may occur (otherwise
+// every function is an s-type STO and the kinds are not read).  Ends with a
+// barrier.
 __device__ __forceinline__ void basis_to_lds(const SJSysDev &S, const double *r, double *rn, double *chi,
-                                             int t, int L, bool gto) {
+                                             int t, int L, bool gto, bool pk) {
   for (int I = t; I < S.n_nuc; I += L)
     rn[I] = norm3(r[0] - S.nuc[3 * I], r[1] - S.nuc[3 * I + 1], r[2] - S.nuc[3 * I + 2]);
   __syncthreads();
   for (int m = t; m < S.n_basis; m += L) {
-    const double r = rn[S.center[m]];
-    chi[m] = (gto && S.kind[m] == MOLE_BASIS_GTO) ? exp_m(-S.zeta[m] * (r * r)) : exp_m(-S.zeta[m] * r);
+    const int I = S.center[m];
+    const double rr = rn[I];
+    if (gto || pk) {
+      const double *R = S.nuc + 3 * I;
+      chi[m] = basis_value<true>(S.kind[m], S.zeta[m], r[0] - R[0], r[1] - R[1], r[2] - R[2], rr);
+    } else {
+      chi[m] = exp_m(-S.zeta[m] * rr);
+    }
   }
   __syncthreads();
 }
@@ -190,18 +200,30 @@ __device__ __forceinline__ double orbital_value(const SJSysDev &S, int k, const 
   return phi;
 }
 
+// What a re-inversion of one spin block found (valid on sub-lane 0).
+struct SJRefresh {
+  double cond;      // ||A||_1 ||A^-1||_1
+  double mant, ex;  // det A = mant 2^ex (mant = 0: singular)
+  bool singular;
+};
+
 // Gauss-Jordan inverse of the spin-s Slater matrix from the positions, FP64,
-// into B (as T); returns ||A||_1 ||A^-1||_1 on sub-lane 0.  All lanes of the
-// workgroup must call it.
+// with partial pivoting, into B (as T) when write_B; the determinant from the
+// pivots (the row swaps' signs times their product, carried as mant 2^ex).
+// A pivot |p| <= n 2^-52 ||A||_1 (or NaN) marks the block singular: its
+// division is skipped (the row is zeroed), so the inverse stays finite and
+// the determinant reads 0.  All lanes of the workgroup must call it.
 template <typename T>
-__device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t, int L, bool gto) {
+__device__ SJRefresh refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t, int L, bool gto, bool pk,
+                                  bool write_B) {
+  SJRefresh res{1.0, 1.0, 0.0, false};
   const int n = s == 0 ? S.n_up : S.n_dn;
-  if (n == 0) return 1.0;
+  if (n == 0) return res;
   const int first = s == 0 ? 0 : S.n_up;
   T *B = s == 0 ? wk.Bup : wk.Bdn;
   double *M = wk.scr, *I = wk.scr + n * n, *colp = wk.scr + 2 * n * n;
   for (int c = 0; c < n; ++c) {
-    basis_to_lds(S, wk.pos + 3 * (first + c), wk.rn, wk.chi, t, L, gto);
+    basis_to_lds(S, wk.pos + 3 * (first + c), wk.rn, wk.chi, t, L, gto, pk);
     for (int k = t; k < n; k += L) M[c * n + k] = orbital_value(S, k, wk.chi);
     __syncthreads();
   }
@@ -212,6 +234,7 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
       for (int c = 0; c < n; ++c) cs += fabs(M[c * n + k]);
       anorm = cs > anorm ? cs : anorm;
     }
+  const double thresh = anorm * ((double)n * 0x1p-52);
   const float inv_n = 1.0f / (float)n;
   for (int e = t; e < n * n; e += L) I[e] = (e == small_div(e, inv_n) * (n + 1)) ? 1.0 : 0.0;
   __syncthreads();
@@ -220,10 +243,20 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
       int piv = p;
       for (int r = p + 1; r < n; ++r)
         if (fabs(M[r * n + p]) > fabs(M[piv * n + p])) piv = r;
+      const double pv = M[piv * n + p];
+      const bool bad = !(fabs(pv) > thresh);
+      if (bad) {
+        res.singular = true;
+      } else {
+        if (piv != p) res.mant = -res.mant;
+        logdet_mul(res.mant, res.ex, pv);
+      }
       wk.flag[0] = (double)piv;
+      wk.flag[1] = bad ? 1.0 : 0.0;
     }
     __syncthreads();
     const int piv = (int)wk.flag[0];
+    const bool bad = wk.flag[1] != 0.0;
     if (piv != p)
       for (int j = t; j < n; j += L) {
         double tm = M[p * n + j];
@@ -234,7 +267,7 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
         I[piv * n + j] = tm;
       }
     __syncthreads();
-    const double ip = 1.0 / M[p * n + p];
+    const double ip = bad ? 0.0 : 1.0 / M[p * n + p];
     __syncthreads();
     for (int j = t; j < n; j += L) {
       M[p * n + j] = M[p * n + j] * ip;
@@ -251,7 +284,8 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
     }
     __syncthreads();
   }
-  for (int e = t; e < n * n; e += L) B[e] = (T)I[e];
+  if (write_B)
+    for (int e = t; e < n * n; e += L) B[e] = (T)I[e];
   double bnorm = 0.0;
   if (t == 0)
     for (int c = 0; c < n; ++c) {
@@ -260,7 +294,40 @@ __device__ double refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int t,
       bnorm = cs > bnorm ? cs : bnorm;
     }
   __syncthreads();
-  return anorm * bnorm;
+  res.cond = anorm * bnorm;
+  if (res.singular) {
+    res.mant = 0.0;
+    res.ex = 0.0;
+  }
+  return res;
+}
+
+// Re-inverts both spin blocks and keeps the books on sub-lane 0: with count,
+// the condition maximum and the singular blocks; the log-determinant state
+// when ld_mode is 1 (taken from the re-inversion) or 2 (the same, after
+// recording how far the value the updates carried drifted from it).
+template <typename T>
+__device__ void refresh_walker(const SJSysDev &S, SJWalker<T> &wk, double *acc, int t, int L, bool gto, bool pk,
+                               bool write_B, bool count, int ld_mode) {
+  double *ld = acc + kSJColumns;
+#pragma unroll 1
+  for (int q = 0; q < 2; ++q) {
+    const SJRefresh r = refresh_spin<T>(S, wk, q, t, L, gto, pk, write_B);
+    if (t == 0) {
+      if (count) {
+        acc[kSJCondMax] = fmax(acc[kSJCondMax], r.cond);
+        if (r.singular) acc[kSJSingular] += 1.0;
+      }
+      if (ld_mode == 2 && !r.singular && ld[2 * q] != 0.0) {
+        const double carried = logdet_value(ld[2 * q], ld[2 * q + 1]);
+        acc[kSJLdDrift] = fmax(acc[kSJLdDrift], fabs(carried - logdet_value(r.mant, r.ex)));
+      }
+      if (ld_mode) {
+        ld[2 * q] = r.mant;
+        ld[2 * q + 1] = r.ex;
+      }
+    }
+  }
 }
 
 // Local energy and d ln psi/dp (five Jastrow parameters) of the walker's
@@ -348,19 +415,9 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         const int I = S.center[m];
         const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
         const double r = norm3(d0, d1, d2);
-        const double z = S.zeta[m];
-        double ch, gf, lc;
-        if (EXT && S.any_gto && S.kind[m] == MOLE_BASIS_GTO) {
-          const double r2 = r * r;
-          ch = exp_m(-z * r2);
-          gf = -2.0 * z * ch;
-          lc = ch * (4.0 * z * z * r2 - 6.0 * z);
-        } else {
-          ch = exp_m(-z * r);
-          gf = -z * ch / r;
-          lc = ch * (z * z - 2.0 * z / r);
-        }
-        const double g0 = gf * d0, g1 = gf * d1, g2 = gf * d2;
+        const int kd = ((EXT && S.any_gto) || S.any_p) ? S.kind[m] : MOLE_BASIS_STO;
+        double ch, g0, g1, g2, lc;
+        basis_vgl<EXT>(kd, S.zeta[m], d0, d1, d2, r, ch, g0, g1, g2, lc);
         const double *cm = S.coef_t + (size_t)m * no + k0;
 #pragma unroll
         for (int k = 0; k < kSJLanesNMax; ++k)
@@ -482,14 +539,20 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
 // No barrier.
 __device__ __forceinline__ void drift_terms(const SJSysDev &S, const double *pos, int i, const double *r,
                                             const double *rn, const double *chi, double *gch, double *jg,
-                                            int t, int L, bool gto) {
+                                            int t, int L, bool gto, bool pk) {
   for (int m = t; m < S.n_basis; m += L) {
     const int I = S.center[m];
     const double d0 = r[0] - S.nuc[3 * I], d1 = r[1] - S.nuc[3 * I + 1], d2 = r[2] - S.nuc[3 * I + 2];
-    const double gf = (gto && S.kind[m] == MOLE_BASIS_GTO) ? -2.0 * S.zeta[m] * chi[m] : -S.zeta[m] * chi[m] / rn[I];
-    gch[3 * m] = gf * d0;
-    gch[3 * m + 1] = gf * d1;
-    gch[3 * m + 2] = gf * d2;
+    const int kd = (gto || pk) ? S.kind[m] : MOLE_BASIS_STO;
+    if (kd == MOLE_BASIS_STO) {
+      // chi[m] = exp(-z r) is in LDS
+      const double gf = -S.zeta[m] * chi[m] / rn[I];
+      gch[3 * m] = gf * d0;
+      gch[3 * m + 1] = gf * d1;
+      gch[3 * m + 2] = gf * d2;
+    } else {
+      basis_grad<true>(kd, S.zeta[m], d0, d1, d2, rn[I], gch[3 * m], gch[3 * m + 1], gch[3 * m + 2]);
+    }
   }
   const int N = S.N, si = i < S.n_up ? 0 : 1;
   for (int j = t; j < N + S.n_nuc; j += L) {
@@ -613,6 +676,10 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
   wk.mv = (double *)(base + lay.mv);
   const int nB = nu * nu + nd * nd;
   const bool gto = EXT && S.any_gto;
+  const bool pk = S.any_p != 0;
+  // DMC steps move walkers the branching gathered: the log-determinant state
+  // is not carried there (mole_dmc_diffuse recomputes it at its end)
+  const bool track_ld = !(EXT && a.dmc == kSJDmcStep);
 
   // ---- state in (a DMC step reads the walkers the branching gathered) ----
   const bool dmc_in = EXT && a.dmc == kSJDmcStep;
@@ -634,7 +701,10 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
     // DMC steps touch only the accepted-move count and the condition maximum
     const bool dmc_step = EXT && a.dmc == kSJDmcStep;
     for (int k = 0; k < kSJColumns; ++k)
-      if (!dmc_step || k == kSJAccepted || k == kSJCondMax) acc[k] = live ? S.acc[(size_t)k * W + w] : 0.0;
+      if (!dmc_step || k == kSJAccepted || k == kSJCondMax || k == kSJSingular)
+        acc[k] = live ? S.acc[(size_t)k * W + w] : 0.0;
+    if (track_ld)
+      for (int k = 0; k < kSJLdColumns; ++k) acc[kSJColumns + k] = live ? S.ld[(size_t)k * W + w] : 1.0;
   }
   __syncthreads();
 
@@ -651,9 +721,10 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
     __syncthreads();
   }
   if (a.init || a.refresh) {
-    const double c0 = refresh_spin<T>(S, wk, 0, t, L, gto);
-    const double c1 = refresh_spin<T>(S, wk, 1, t, L, gto);
-    if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
+    // new configurations: the determinants are taken from the re-inversion
+    // (nothing accumulated to compare with); refresh 2 keeps the inverses
+    const bool wb = a.init || a.refresh == 1;
+    refresh_walker<T>(S, wk, acc, t, L, gto, pk, wb, wb, track_ld ? 1 : 0);
   }
   if (EXT && a.dmc == kSJDmcInit) {
     // DmcRunner::new: weight 1; the local energy of the starting configuration
@@ -756,8 +827,8 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
         __syncthreads();
       } else {
         // drift at x (metrop.rs:150-163): x' = (x_i + tau v) + sqrt(tau) z
-        basis_to_lds(S, xi, wk.rn, wk.chi, t, L, gto);
-        drift_terms(S, wk.pos, i, xi, wk.rn, wk.chi, gch, jg, t, L, gto);
+        basis_to_lds(S, xi, wk.rn, wk.chi, t, L, gto, pk);
+        drift_terms(S, wk.pos, i, xi, wk.rn, wk.chi, gch, jg, t, L, gto, pk);
         __syncthreads();
         orbital_gradients(S, n, gch, og, t, L);
         __syncthreads();
@@ -792,10 +863,10 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
           wk.jo[j] = en_term<T>(aZ, S.p[4], xi, S.nuc + 3 * I);
         }
       }
-      basis_to_lds(S, xn, wk.rn, wk.chi, t, L, gto);
+      basis_to_lds(S, xn, wk.rn, wk.chi, t, L, gto, pk);
       for (int k = t; k < n; k += L) wk.phin[k] = (T)orbital_value(S, k, wk.chi);
       if constexpr (PROP == kSJPropDiffuse) {
-        drift_terms(S, wk.pos, i, xn, wk.rn, wk.chi, gch, jg, t, L, gto);
+        drift_terms(S, wk.pos, i, xn, wk.rn, wk.chi, gch, jg, t, L, gto, pk);
         __syncthreads();
         orbital_gradients(S, n, gch, og, t, L);
       }
@@ -831,6 +902,8 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
         }
         wk.flag[0] = accept ? 1.0 : 0.0;
         wk.flag[1] = (double)((T)1 / rd);
+        // det A_s' = R_D det A_s, carried in FP64
+        if (accept && track_ld) logdet_mul(acc[kSJColumns + 2 * s], acc[kSJColumns + 2 * s + 1], rdd);
       }
       __syncthreads();
       const bool accept = wk.flag[0] != 0.0;
@@ -878,9 +951,8 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
       __syncthreads();
       ++m;
       if (lu_period && m % lu_period == 0) {
-        const double c0 = refresh_spin<T>(S, wk, 0, t, L, gto);
-        const double c1 = refresh_spin<T>(S, wk, 1, t, L, gto);
-        if (t == 0) acc[kSJCondMax] = fmax(acc[kSJCondMax], fmax(c0, c1));
+        // the FP64 log|det| the update chain carried is checked against the re-inversion's
+        refresh_walker<T>(S, wk, acc, t, L, gto, pk, true, true, track_ld ? 2 : 0);
       }
     }
     p += DPM * (uint64_t)N;
@@ -948,9 +1020,13 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
     const int k = e >> lg_slots, sl = e & (nslots - 1);
     if (sl < tile.nw) Bo[(size_t)k * W + tile.w0 + sl] = ((T *)(lds + (size_t)sl * lay.bytes + lay.B))[k];
   }
-  if (t == 0 && live)
+  if (t == 0 && live) {
     for (int k = 0; k < kSJColumns; ++k)
-      if (!(EXT && a.dmc == kSJDmcStep) || k == kSJAccepted || k == kSJCondMax) S.acc[(size_t)k * W + w] = acc[k];
+      if (!(EXT && a.dmc == kSJDmcStep) || k == kSJAccepted || k == kSJCondMax || k == kSJSingular)
+        S.acc[(size_t)k * W + w] = acc[k];
+    if (track_ld)
+      for (int k = 0; k < kSJLdColumns; ++k) S.ld[(size_t)k * W + w] = acc[kSJColumns + k];
+  }
 }
 
 __global__ void sj_advance_kernel(SJLaunchState *st, uint32_t n_sweeps, uint32_t n_samples,

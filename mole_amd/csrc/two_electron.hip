@@ -557,10 +557,10 @@ __global__ void eval2_kernel(Eval2Args a) {
 // Column sums of a [K][W] array, one workgroup per column, fixed order.
 // Column max_col, if in range, is reduced with max instead of +.
 __global__ void __launch_bounds__(256) reduce_columns_kernel(const double *acc, int W,
-                                                             double *out, int max_col) {
+                                                             double *out, int max_first, int max_count) {
   __shared__ double sh[256];
   const int col = blockIdx.x;
-  const bool mx = col == max_col;
+  const bool mx = col >= max_first && col < max_first + max_count;
   const double *p = acc + (size_t)col * W;
   double s = 0.0;
   // loads in batches of 16 (all in flight together), added in index order
@@ -664,9 +664,9 @@ hipError_t launch_eval2(const Eval2Args &a, hipStream_t stream) {
   return hipGetLastError();
 }
 
-hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out, int max_col,
+hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out, int max_first, int max_count,
                                  hipStream_t stream) {
-  hipLaunchKernelGGL(reduce_columns_kernel, dim3(K), dim3(256), 0, stream, acc, W, out, max_col);
+  hipLaunchKernelGGL(reduce_columns_kernel, dim3(K), dim3(256), 0, stream, acc, W, out, max_first, max_count);
   return hipGetLastError();
 }
 

@@ -1,8 +1,9 @@
 """Synthetic systems of the benchmark workloads (host-side definitions).
 
-Slater-Jastrow systems are plain data: nuclei, an s-type STO basis (exponent
-and centre per function), orbital coefficients, spin occupations and the
-five Jastrow parameters (b_anti, b_para, c, a, d) of
+Slater-Jastrow systems are plain data: nuclei, a basis (exponent, centre and
+kind per function: s-type STO exp(-zeta r), Gaussian exp(-zeta r^2), or a p-type
+STO x exp(-zeta r), y exp(-zeta r), z exp(-zeta r)), orbital coefficients, spin
+occupations and the five Jastrow parameters (b_anti, b_para, c, a, d) of
     J = sum_{i<j} b_ij r_ij/(1 + c r_ij) - sum_{i,I} a Z_I r_iI/(1 + d r_iI).
 The same arrays feed the device (through the C ABI) and the CPU oracle.
 """
@@ -11,12 +12,15 @@ import math
 import numpy as np
 
 DEFAULT_JASTROW = (0.5, 0.25, 1.0, 0.3, 1.0)
+# basis kinds (the C ABI's MOLE_BASIS_*)
+STO, GTO, STO_PX, STO_PY, STO_PZ = 0, 1, 2, 3, 4
 ELEMENTS = {1: "H", 2: "He", 3: "Li", 4: "Be", 5: "B", 6: "C", 7: "N", 8: "O", 9: "F", 10: "Ne"}
 
 
 class SJSystem:
     """A Slater-Jastrow system: determinants of n_up and n_dn electrons over
-    s-type STO orbitals, times the Pade e-e and e-n Jastrow factor."""
+    orbitals of STO (s and p) or Gaussian basis functions, times the Pade e-e
+    and e-n Jastrow factor."""
 
     def __init__(self, nuclei, charges, n_up, n_dn, zeta, center, coef, jastrow=DEFAULT_JASTROW,
                  name="", kind=None):
@@ -29,7 +33,7 @@ class SJSystem:
         self.coef = np.ascontiguousarray(coef, dtype=np.float64).reshape(-1, self.zeta.size)
         self.jastrow = np.ascontiguousarray(jastrow, dtype=np.float64)
         self.name = name
-        # basis kinds: 0 STO exp(-zeta r), 1 GTO exp(-zeta r^2)
+        # basis kinds (STO, GTO, STO_PX..STO_PZ); None: all s-type STOs
         self.kind = None if kind is None else np.ascontiguousarray(kind, dtype=np.int32)
         assert self.coef.shape[0] >= max(self.n_up, self.n_dn)
         assert self.center.size == self.zeta.size
@@ -44,17 +48,30 @@ class SJSystem:
         return self.zeta.size
 
 
-def atom(Z, n_basis=5, seed=1, jastrow=DEFAULT_JASTROW):
-    """Neutral atom Z at the origin: n_up = ceil(Z/2), n_dn = floor(Z/2);
-    n_basis STO exponents uniform in [0.5, Z] and N(0,1) coefficients for
-    n_up orbitals, from numpy's default_rng(seed)."""
+def atom(Z, seed=1, jastrow=DEFAULT_JASTROW):
+    """Neutral atom Z at the origin: n_up = ceil(Z/2), n_dn = floor(Z/2), in a
+    minimal STO basis of five functions -- two s functions exp(-zeta r) and
+    one p shell (x, y, z) exp(-zeta r), the 1s, 2s and 2p of the first row --
+    with the three exponents uniform in [0.5, Z] and N(0,1) coefficients for
+    n_up orbitals, from numpy's default_rng(seed).  (The p shell gives the
+    spin determinants of B..Ne their angular rank: over s functions of one
+    centre alone they are numerically singular from four orbitals on.)"""
     rng = np.random.default_rng(seed)
     n_up = (Z + 1) // 2
     n_dn = Z // 2
-    zeta = rng.uniform(0.5, float(Z), n_basis)
-    coef = rng.normal(size=(n_up, n_basis))
-    return SJSystem([(0.0, 0.0, 0.0)], [Z], n_up, n_dn, zeta, np.zeros(n_basis, np.int32), coef,
-                    jastrow, name=ELEMENTS.get(Z, f"Z{Z}"))
+    zs = rng.uniform(0.5, float(Z), 3)
+    zeta = [zs[0], zs[1], zs[2], zs[2], zs[2]]
+    kind = [STO, STO, STO_PX, STO_PY, STO_PZ]
+    coef = rng.normal(size=(n_up, 5))
+    return SJSystem([(0.0, 0.0, 0.0)], [Z], n_up, n_dn, zeta, np.zeros(5, np.int32), coef,
+                    jastrow, name=ELEMENTS.get(Z, f"Z{Z}"), kind=kind)
+
+
+def hydrogenic_2p(Z=1, axis=2):
+    """One electron in the 2p orbital (x, y or z) exp(-Z r/2) of a charge-Z
+    nucleus: an exact eigenfunction, E_L = -Z^2/8."""
+    return SJSystem([(0.0, 0.0, 0.0)], [Z], 1, 0, [0.5 * Z], [0], [[1.0]], (0, 0, 0, 0, 0),
+                    name=f"hydrogenic 2p Z={Z}", kind=[STO_PX + axis])
 
 
 def hydrogenic(Z=1):
@@ -104,21 +121,30 @@ def poisson_disk(n, min_sep, rng, box=None, max_tries=10000):
     return pts - pts.mean(axis=0)
 
 
-def molecule(charges, positions, basis_per_nucleus=2, seed=0, jastrow=DEFAULT_JASTROW, name=""):
+def molecule(charges, positions, basis_per_nucleus=2, seed=0, jastrow=DEFAULT_JASTROW, name="", p_shells=True):
     """Neutral molecule: electrons = sum Z, closed shell where the count is
-    even (n_up = ceil(N/2)); basis_per_nucleus STOs per nucleus with
-    exponents uniform in [0.5, max(1, Z)], N(0,1) coefficients."""
+    even (n_up = ceil(N/2)); basis_per_nucleus s-type STOs per nucleus and,
+    with p_shells, one p shell (x, y, z) on every nucleus with Z >= 3, all
+    with exponents uniform in [0.5, max(1, Z)]; N(0,1) coefficients.  (A
+    nucleus then carries at least ceil(Z/2) functions for Z <= 10, so the
+    orbitals span n_up dimensions.)"""
     rng = np.random.default_rng(seed)
     charges = np.asarray(charges, dtype=np.int32)
     N = int(charges.sum())
     n_up, n_dn = (N + 1) // 2, N // 2
-    zeta, center = [], []
+    zeta, center, kind = [], [], []
     for I, Z in enumerate(charges):
         zeta.extend(rng.uniform(0.5, max(1.0, float(Z)), basis_per_nucleus))
         center.extend([I] * basis_per_nucleus)
+        kind.extend([STO] * basis_per_nucleus)
+        if p_shells and Z >= 3:
+            zp = rng.uniform(0.5, float(Z))
+            zeta.extend([zp] * 3)
+            center.extend([I] * 3)
+            kind.extend([STO_PX, STO_PY, STO_PZ])
     nb = len(zeta)
     coef = rng.normal(size=(n_up, nb)) / math.sqrt(nb)
-    return SJSystem(positions, charges, n_up, n_dn, zeta, center, coef, jastrow, name=name)
+    return SJSystem(positions, charges, n_up, n_dn, zeta, center, coef, jastrow, name=name, kind=kind)
 
 
 def hydrogen_chain(n, spacing=1.4, basis_per_nucleus=2, seed=0):

@@ -24,7 +24,37 @@ static int electron_of(const orc_sj *wf, int s, int c) { return s == 0 ? c : wf-
 
 static double rust_min1(double a) { return (a < 1.0) ? a : 1.0; }
 
-static int is_gto(const orc_sj *wf, int m) { return wf->kind && wf->kind[m] == ORC_BASIS_GTO; }
+static int kind_of(const orc_sj *wf, int m) { return wf->kind ? wf->kind[m] : ORC_BASIS_STO; }
+
+/* Value v, gradient g and laplacian l of a basis function of the given kind
+ * at d = x - R, r = |d| (the device's basis_vgl, same operations):
+ *   STO    e = exp(-z r), g = (-z e / r) d, l = e (z^2 - 2 z/r)
+ *   GTO    v = exp(-z r^2), g = (-2 z v) d, l = v (4 z^2 r^2 - 6 z)
+ *   STO_P  v = d_q e, g_k = ((-z e / r) d_q) d_k + (k == q ? e : 0),
+ *          l = v (z^2 - 4 z/r)  (lap(x f(r)) = x (f'' + 4 f'/r)) */
+static void basis_vgl(int kind, double z, const double d[3], double r, double *v, double g[3], double *l) {
+  if (kind == ORC_BASIS_GTO) {
+    const double r2 = r * r;
+    *v = orc_exp(-z * r2);
+    const double gf = -2.0 * z * *v;
+    for (int q = 0; q < 3; ++q) g[q] = gf * d[q];
+    *l = *v * (4.0 * z * z * r2 - 6.0 * z);
+    return;
+  }
+  const double e = orc_exp(-z * r);
+  const double gf = -z * e / r;
+  if (kind >= ORC_BASIS_STO_PX) {
+    const int q = kind - ORC_BASIS_STO_PX;
+    const double gq = gf * d[q];
+    *v = d[q] * e;
+    for (int k = 0; k < 3; ++k) g[k] = gq * d[k] + (k == q ? e : 0.0);
+    *l = *v * (z * z - 4.0 * z / r);
+  } else {
+    *v = e;
+    for (int k = 0; k < 3; ++k) g[k] = gf * d[k];
+    *l = e * (z * z - 2.0 * z / r);
+  }
+}
 
 /* basis values chi[m] at r (distances to all nuclei first) */
 static void basis_at(const orc_sj *wf, const double *r, double *chi) {
@@ -32,8 +62,17 @@ static void basis_at(const orc_sj *wf, const double *r, double *chi) {
   for (int I = 0; I < wf->n_nuc; ++I)
     rn[I] = orc_norm3(r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]);
   for (int m = 0; m < wf->n_basis; ++m) {
-    const double rr = rn[wf->center[m]];
-    chi[m] = is_gto(wf, m) ? orc_exp(-wf->zeta[m] * (rr * rr)) : orc_exp(-wf->zeta[m] * rr);
+    const int I = wf->center[m];
+    const double rr = rn[I];
+    const int kd = kind_of(wf, m);
+    if (kd == ORC_BASIS_GTO) {
+      chi[m] = orc_exp(-wf->zeta[m] * (rr * rr));
+    } else if (kd >= ORC_BASIS_STO_PX) {
+      const int q = kd - ORC_BASIS_STO_PX;
+      chi[m] = (r[q] - wf->nuc[3 * I + q]) * orc_exp(-wf->zeta[m] * rr);
+    } else {
+      chi[m] = orc_exp(-wf->zeta[m] * rr);
+    }
   }
 }
 
@@ -132,17 +171,47 @@ long double orc_sj_det_leibniz(const orc_sj *wf, const double *cfg, int spin) {
   return total;
 }
 
-int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B) {
+double orc_sj_logdet_value(double mant, double ex) {
+  if (mant == 0.0) return -INFINITY;
+  /* ln 2 split as in orc_log (fdlibm) */
+  const double ln2_hi = 6.93147180369123816490e-01, ln2_lo = 1.90821492927058770002e-10;
+  return fma(ex, ln2_hi, fma(ex, ln2_lo, orc_log(fabs(mant))));
+}
+
+void orc_sj_logdet_mul(double *mant, double *ex, double f) {
+  int e;
+  *mant = frexp(*mant * f, &e);
+  *ex = *ex + (double)e;
+}
+
+int orc_sj_inverse_det(const orc_sj *wf, const double *cfg, int spin, double *B, double *mant, double *ex) {
   int n = nspin(wf, spin);
   double M[MAXN * MAXN], I[MAXN * MAXN];
   slater_matrix(wf, cfg, spin, M);
+  /* ||A||_1: column sums in row order, their maximum */
+  double anorm = 0.0;
+  for (int k = 0; k < n; ++k) {
+    double cs = 0.0;
+    for (int c = 0; c < n; ++c) cs += fabs(M[c * n + k]);
+    anorm = cs > anorm ? cs : anorm;
+  }
+  const double thresh = anorm * ((double)n * 0x1p-52);
   for (int r = 0; r < n; ++r)
     for (int j = 0; j < n; ++j) I[r * n + j] = (r == j) ? 1.0 : 0.0;
   int rc = 0;
+  double dm = 1.0, de = 0.0;
   for (int p = 0; p < n; ++p) {
     int piv = p;
     for (int r = p + 1; r < n; ++r)
       if (fabs(M[r * n + p]) > fabs(M[piv * n + p])) piv = r;
+    const double pv0 = M[piv * n + p];
+    const int bad = !(fabs(pv0) > thresh);
+    if (bad) {
+      rc = -1;
+    } else {
+      if (piv != p) dm = -dm;
+      orc_sj_logdet_mul(&dm, &de, pv0);
+    }
     if (piv != p)
       for (int j = 0; j < n; ++j) {
         double t = M[p * n + j];
@@ -152,9 +221,7 @@ int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B) {
         I[p * n + j] = I[piv * n + j];
         I[piv * n + j] = t;
       }
-    double pv = M[p * n + p];
-    if (pv == 0.0) rc = -1;
-    double ip = 1.0 / pv;
+    double ip = bad ? 0.0 : 1.0 / M[p * n + p];
     for (int j = 0; j < n; ++j) {
       M[p * n + j] = M[p * n + j] * ip;
       I[p * n + j] = I[p * n + j] * ip;
@@ -169,7 +236,17 @@ int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B) {
     }
   }
   memcpy(B, I, sizeof(double) * n * n);
+  if (rc != 0) {
+    dm = 0.0;
+    de = 0.0;
+  }
+  if (mant) *mant = dm;
+  if (ex) *ex = de;
   return rc;
+}
+
+int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B) {
+  return orc_sj_inverse_det(wf, cfg, spin, B, NULL, NULL);
 }
 
 /* ---- Jastrow ------------------------------------------------------- */
@@ -272,19 +349,7 @@ static double sj_local_parts(const orc_sj *wf, const double *cfg, const double *
     double chi[MAXB], gch[MAXB][3], lch[MAXB];
     for (int m = 0; m < wf->n_basis; ++m) {
       int I = wf->center[m];
-      double z = wf->zeta[m];
-      double gf;
-      if (is_gto(wf, m)) {
-        const double r2 = rI[I] * rI[I];
-        chi[m] = orc_exp(-z * r2);
-        gf = -2.0 * z * chi[m];
-        lch[m] = chi[m] * (4.0 * z * z * r2 - 6.0 * z);
-      } else {
-        chi[m] = orc_exp(-z * rI[I]);
-        gf = -z * chi[m] / rI[I];
-        lch[m] = chi[m] * (z * z - 2.0 * z / rI[I]);
-      }
-      for (int q = 0; q < 3; ++q) gch[m][q] = gf * dI[I][q];
+      basis_vgl(kind_of(wf, m), wf->zeta[m], dI[I], rI[I], &chi[m], gch[m], &lch[m]);
     }
     double Dg[3] = {0, 0, 0}, Dl = 0.0;
     for (int k = 0; k < n; ++k) {
@@ -453,16 +518,8 @@ static void det_grad(const orc_sj *wf, const double *r, int n, const double *Bco
     int I = wf->center[m];
     double d[3] = {r[0] - wf->nuc[3 * I], r[1] - wf->nuc[3 * I + 1], r[2] - wf->nuc[3 * I + 2]};
     double rr = orc_norm3(d[0], d[1], d[2]);
-    double z = wf->zeta[m];
-    double gf;
-    if (is_gto(wf, m)) {
-      const double ch = orc_exp(-z * (rr * rr));
-      gf = -2.0 * z * ch;
-    } else {
-      const double ch = orc_exp(-z * rr);
-      gf = -z * ch / rr;
-    }
-    for (int q = 0; q < 3; ++q) gch[m][q] = gf * d[q];
+    double v, l;
+    basis_vgl(kind_of(wf, m), wf->zeta[m], d, rr, &v, gch[m], &l);
   }
   g[0] = g[1] = g[2] = 0.0;
   for (int k = 0; k < n; ++k) {
@@ -562,23 +619,48 @@ typedef struct {
   double sum_e, sum_e2, sum_bm, sum_bm2;
   uint64_t n_samples, n_blocks, n_accepted, n_moves;
   double obs_sum[8], obs_sum2[8];
+  double max_ld_drift;
+  uint64_t n_singular;
 } sj_task;
 
-static void refresh(const orc_sj *wf, const double *cfg, int prec, double *Bu, double *Bd,
-                    float *Buf, float *Bdf) {
-  orc_sj_inverse(wf, cfg, 0, Bu);
-  orc_sj_inverse(wf, cfg, 1, Bd);
+/* Re-inverts both spin blocks (the device's refresh).  With ld (the
+ * walker's carried determinants, mant/ex per spin): ld_mode 1 replaces them
+ * by the re-inversion's; 2 first records the drift of the carried log|det|
+ * in *drift (non-singular blocks whose carried mantissa is not 0).  Singular
+ * blocks are counted in *n_singular (may be NULL). */
+static void refresh_ld(const orc_sj *wf, const double *cfg, int prec, double *Bu, double *Bd, float *Buf, float *Bdf,
+                       double *ld, int ld_mode, double *drift, uint64_t *n_singular) {
+  double m[2], e[2];
+  const int r0 = orc_sj_inverse_det(wf, cfg, 0, Bu, &m[0], &e[0]);
+  const int r1 = orc_sj_inverse_det(wf, cfg, 1, Bd, &m[1], &e[1]);
+  if (n_singular) *n_singular += (uint64_t)(r0 != 0) + (uint64_t)(r1 != 0);
+  const int bad[2] = {r0 != 0, r1 != 0};
+  for (int q = 0; q < 2 && ld; ++q) {
+    if (ld_mode == 2 && !bad[q] && ld[2 * q] != 0.0) {
+      const double carried = orc_sj_logdet_value(ld[2 * q], ld[2 * q + 1]);
+      const double dv = fabs(carried - orc_sj_logdet_value(m[q], e[q]));
+      *drift = fmax(*drift, dv);
+    }
+    if (ld_mode) {
+      ld[2 * q] = m[q];
+      ld[2 * q + 1] = e[q];
+    }
+  }
   if (prec == ORC_PREC_MIXED) {
     for (int k = 0; k < wf->n_up * wf->n_up; ++k) Buf[k] = (float)Bu[k];
     for (int k = 0; k < wf->n_dn * wf->n_dn; ++k) Bdf[k] = (float)Bd[k];
   }
 }
 
+static void refresh(const orc_sj *wf, const double *cfg, int prec, double *Bu, double *Bd, float *Buf, float *Bdf) {
+  refresh_ld(wf, cfg, prec, Bu, Bd, Buf, Bdf, NULL, 0, NULL, NULL);
+}
+
 /* One sweep of walker w over all electrons (the body of Runner::run's
  * move_state loop, samplers.rs:106-117): draws from *pos on, global move
  * counter *m; returns the accept mask. */
 static uint64_t sj_sweep_once(sj_task *t, uint32_t w, double *cfg, double *Bu, double *Bd, float *Buf, float *Bdf,
-                              uint64_t *pos_io, uint64_t *m_io) {
+                              double *ld, uint64_t *pos_io, uint64_t *m_io) {
   const orc_sj *wf = t->wf;
   const orc_sj_run_params *rp = t->rp;
   const int N = wf->n_up + wf->n_dn;
@@ -671,6 +753,7 @@ static uint64_t sj_sweep_once(sj_task *t, uint32_t w, double *cfg, double *Bu, d
         acc = diffuse   ? diffuse_accept(wf, cfg, i, xn, n, B + c, n, rd, R2, vold, rp->tau, u)
               : learned ? rust_min1(R2 * exp(lq_ratio)) > u
                         : rust_min1(R2) > u;
+        if (acc) orc_sj_logdet_mul(&ld[2 * sp], &ld[2 * sp + 1], rd);
         if (acc) {
           double inv = 1.0 / rd;
           double f[MAXN];
@@ -705,6 +788,7 @@ static uint64_t sj_sweep_once(sj_task *t, uint32_t w, double *cfg, double *Bu, d
         } else {
           acc = rust_min1(R2) > u;
         }
+        if (acc) orc_sj_logdet_mul(&ld[2 * sp], &ld[2 * sp + 1], rdd);
         if (acc) {
           float inv = 1.0f / rd;
           float f[MAXN];
@@ -728,7 +812,7 @@ static uint64_t sj_sweep_once(sj_task *t, uint32_t w, double *cfg, double *Bu, d
     }
     ++m;
     if (!t->faithful && rp->lu_period && m % rp->lu_period == 0)
-      refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
+      refresh_ld(wf, cfg, prec, Bu, Bd, Buf, Bdf, ld, 2, &t->max_ld_drift, &t->n_singular);
   }
   *pos_io = pos;
   *m_io = m;
@@ -755,13 +839,14 @@ static void sj_walker(sj_task *t, uint32_t w) {
   }
   double Bu[MAXN * MAXN], Bd[MAXN * MAXN];
   float Buf[MAXN * MAXN], Bdf[MAXN * MAXN];
-  if (!t->faithful) refresh(wf, cfg, prec, Bu, Bd, Buf, Bdf);
+  double ld[4] = {1.0, 0.0, 1.0, 0.0};
+  if (!t->faithful) refresh_ld(wf, cfg, prec, Bu, Bd, Buf, Bdf, ld, 1, NULL, &t->n_singular);
   uint64_t m = rp->move0;
   uint64_t sidx = 0;
   for (uint32_t block = 0; block < blocks; ++block) {
     double bsum = 0.0;
     for (uint32_t s = 0; s < rp->block_size; ++s) {
-      const uint64_t mask = sj_sweep_once(t, w, cfg, Bu, Bd, Buf, Bdf, &pos, &m);
+      const uint64_t mask = sj_sweep_once(t, w, cfg, Bu, Bd, Buf, Bdf, ld, &pos, &m);
       if (out->accept_mask)
         out->accept_mask[(size_t)w * blocks * rp->block_size + (size_t)block * rp->block_size + s] = mask;
       if (block > 0) {
@@ -806,6 +891,7 @@ static void sj_walker(sj_task *t, uint32_t w) {
     }
   }
   if (out->cfg) memcpy(out->cfg + (size_t)w * 3 * N, cfg, sizeof(double) * 3 * N);
+  if (out->ld && !t->faithful) memcpy(out->ld + (size_t)w * 4, ld, sizeof ld);
 }
 
 static void *sj_task_main(void *arg) {
@@ -851,6 +937,8 @@ static int sj_run(const orc_sj *wf, const orc_sj_run_params *rp, uint32_t w0, ui
     out->n_blocks += tasks[i].n_blocks;
     out->n_accepted += tasks[i].n_accepted;
     out->n_moves += tasks[i].n_moves;
+    out->max_ld_drift = fmax(out->max_ld_drift, tasks[i].max_ld_drift);
+    out->n_singular += tasks[i].n_singular;
     for (int k = 0; k < rp->n_obs; ++k) {
       if (out->obs_sum) out->obs_sum[k] += tasks[i].obs_sum[k];
       if (out->obs_sum2) out->obs_sum2[k] += tasks[i].obs_sum2[k];
@@ -1097,7 +1185,8 @@ int orc_dmc_diffuse(const orc_sj *wf, const orc_dmc_params *p, orc_dmc_state *st
         }
         const double local_e = st->e_local[w];
         uint64_t pos = st->draw, m = st->moves;
-        sj_sweep_once(&t, w, c, Bu, Bd, Buf, Bdf, &pos, &m);
+        double ld_unused[4] = {1.0, 0.0, 1.0, 0.0}; /* DMC steps carry no determinants */
+        sj_sweep_once(&t, w, c, Bu, Bd, Buf, Bdf, ld_unused, &pos, &m);
         for (int k = 0; k < nu * nu; ++k) B[k] = p->precision == ORC_PREC_FP64 ? Bu[k] : (double)Buf[k];
         for (int k = 0; k < nd * nd; ++k) B[nu * nu + k] = p->precision == ORC_PREC_FP64 ? Bd[k] : (double)Bdf[k];
         we[w] = st->weight[w] * local_e;

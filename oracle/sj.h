@@ -8,7 +8,8 @@
  *
  *   psi = det A_up * det A_dn * exp(J)
  *   A_s[c][k] = phi_k(r of the c-th electron of spin s),
- *   phi_k(r)  = sum_m coef[k][m] exp(-zeta_m |r - R_center(m)|)
+ *   phi_k(r)  = sum_m coef[k][m] chi_m(r - R_center(m))
+ *   chi_m     = exp(-zeta r), exp(-zeta r^2) or d_q exp(-zeta r) (kind)
  *   J = sum_{i<j} b_ij r_ij/(1 + c r_ij) - sum_{i,I} a Z_I r_iI/(1 + d r_iI)
  *   b_ij = b_anti (opposite spins) or b_para (same spin)
  *
@@ -44,8 +45,9 @@ typedef struct {
   const int *kind;      /* [n_basis] ORC_BASIS_*; NULL: all STO */
 } orc_sj;
 
-/* basis functions: exp(-zeta r) (STO) or exp(-zeta r^2) (GTO), r = |x - R| */
-enum { ORC_BASIS_STO = 0, ORC_BASIS_GTO = 1 };
+/* basis functions about R, d = x - R, r = |d|: exp(-zeta r) (STO),
+ * exp(-zeta r^2) (GTO), d_q exp(-zeta r) (p-type STO, q = x, y, z) */
+enum { ORC_BASIS_STO = 0, ORC_BASIS_GTO = 1, ORC_BASIS_STO_PX = 2, ORC_BASIS_STO_PY = 3, ORC_BASIS_STO_PZ = 4 };
 
 /* Local operators (O psi)/psi built from terms (LocalOperator::act_on,
  * src/operator/src/traits.rs:263-265, divided by psi as Sampler::sample
@@ -86,9 +88,20 @@ double orc_sj_logdet(const orc_sj *wf, const double *cfg, int spin, double *sign
 /* Determinant of A_s by the Leibniz expansion in long double (n <= 6). */
 long double orc_sj_det_leibniz(const orc_sj *wf, const double *cfg, int spin);
 /* Inverse B_s[k][c] = (A_s^-1)[k][c] by Gauss-Jordan with partial
- * pivoting, FP64, exactly as the device refreshes it.  Returns 0, -1 if
- * singular. */
+ * pivoting, FP64, exactly as the device refreshes it: a pivot
+ * |p| <= n 2^-52 ||A||_1 (or NaN) marks the block singular, its division is
+ * skipped (the row is zeroed).  Returns 0, -1 if singular. */
 int orc_sj_inverse(const orc_sj *wf, const double *cfg, int spin, double *B);
+/* The same, and the determinant from the pivots (row-swap signs times
+ * their product) as *mant 2^*ex, the mantissa renormalised by frexp after
+ * every factor, as the device carries it; *mant = 0 for a singular block.
+ * log|det| = orc_sj_logdet_value(mant, ex). */
+int orc_sj_inverse_det(const orc_sj *wf, const double *cfg, int spin, double *B, double *mant, double *ex);
+/* log|mant 2^ex| = log|mant| + ex ln2 (ln2 split as in orc_log); -inf for
+ * mant = 0. */
+double orc_sj_logdet_value(double mant, double ex);
+/* mant 2^ex times f, renormalised: the device's per-move update. */
+void orc_sj_logdet_mul(double *mant, double *ex, double f);
 /* Local energy -1/2 lap(psi)/psi + V and d ln psi / dp of the five
  * Jastrow parameters, FP64, from the inverses B_up, B_dn (as doubles). */
 double orc_sj_local_energy(const orc_sj *wf, const double *cfg, const double *Bup,
@@ -147,6 +160,13 @@ typedef struct {
   double sum_e, sum_e2, sum_bm, sum_bm2;
   uint64_t n_samples, n_blocks, n_accepted, n_moves;
   double *obs_sum, *obs_sum2; /* [n_obs]: sums over samples (walkers in order) */
+  /* parity runner: the determinants as the device carries them, per walker
+   * (mant_up, ex_up, mant_dn, ex_dn) at the end ([W][4], optional); the
+   * largest |carried - re-inverted| log|det| at a periodic re-inversion;
+   * singular blocks met at re-inversions */
+  double *ld;
+  double max_ld_drift;
+  uint64_t n_singular;
 } orc_sj_run_out;
 
 /* Parity runner: walkers [w0, w1), each its own Runner::run

@@ -64,6 +64,7 @@ class SjRunOut(C.Structure):
         ("sum_e", C.c_double), ("sum_e2", C.c_double), ("sum_bm", C.c_double), ("sum_bm2", C.c_double),
         ("n_samples", C.c_uint64), ("n_blocks", C.c_uint64), ("n_accepted", C.c_uint64),
         ("n_moves", C.c_uint64), ("obs_sum", _pd), ("obs_sum2", _pd),
+        ("ld", _pd), ("max_ld_drift", C.c_double), ("n_singular", C.c_uint64),
     ]
 
 
@@ -83,6 +84,12 @@ def lib():
         L.orc_sj_det_leibniz.argtypes = [ps, _pd, C.c_int]
         L.orc_sj_inverse.restype = C.c_int
         L.orc_sj_inverse.argtypes = [ps, _pd, C.c_int, _pd]
+        L.orc_sj_inverse_det.restype = C.c_int
+        L.orc_sj_inverse_det.argtypes = [ps, _pd, C.c_int, _pd, _pd, _pd]
+        L.orc_sj_logdet_value.restype = C.c_double
+        L.orc_sj_logdet_value.argtypes = [C.c_double, C.c_double]
+        L.orc_sj_logdet_mul.restype = None
+        L.orc_sj_logdet_mul.argtypes = [_pd, _pd, C.c_double]
         L.orc_sj_local_energy.restype = C.c_double
         L.orc_sj_local_energy.argtypes = [ps, _pd, _pd, _pd, _pd]
         L.orc_sj_local_energy_full.restype = C.c_double
@@ -132,6 +139,30 @@ def inverse(wv, x, spin):
     B = np.zeros(max(1, n * n))
     rc = lib().orc_sj_inverse(wv.ref(), OF.dptr(x), spin, OF.dptr(B))
     return B[:n * n].reshape(n, n), rc
+
+
+def inverse_det(wv, x, spin):
+    """(inverse, mant, ex, rc): the device's re-inversion and the determinant
+    mant 2^ex from its pivots (mant 0 for a singular block)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    n = wv.n_up if spin == 0 else wv.n_dn
+    B = np.zeros(max(1, n * n))
+    m, e = C.c_double(), C.c_double()
+    rc = lib().orc_sj_inverse_det(wv.ref(), OF.dptr(x), spin, OF.dptr(B), C.byref(m), C.byref(e))
+    return B[:n * n].reshape(n, n), m.value, e.value, rc
+
+
+def logdet_value(mant, ex):
+    return lib().orc_sj_logdet_value(float(mant), float(ex))
+
+
+def logdet(wv, x, spin):
+    """(log|det A_s|, sign) by the oracle's LU (an elimination independent of
+    the Gauss-Jordan re-inversion)."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    sg = C.c_double()
+    v = lib().orc_sj_logdet(wv.ref(), OF.dptr(x), int(spin), C.byref(sg))
+    return v, sg.value
 
 
 def drift(wv, x, i):
@@ -193,6 +224,8 @@ def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, p
     out = SjRunOut()
     out.obs_sum = OF.dptr(obs_sum)
     out.obs_sum2 = OF.dptr(obs_sum2)
+    ld = np.zeros((walkers, 4))
+    out.ld = OF.dptr(ld)
     c = np.zeros((walkers, 3 * N)) if cfg is None else np.ascontiguousarray(cfg, dtype=np.float64).copy()
     out.cfg = OF.dptr(c)
     res = {}
@@ -210,7 +243,8 @@ def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, p
         raise ValueError("orc_sj_run failed (steps < 2*block_size or sizes too large)")
     res.update(cfg=c, sum_e=out.sum_e, sum_e2=out.sum_e2, sum_bm=out.sum_bm, sum_bm2=out.sum_bm2,
                n_samples=out.n_samples, n_blocks=out.n_blocks, n_accepted=out.n_accepted,
-               n_moves=out.n_moves, obs_sum=obs_sum[:n_obs], obs_sum2=obs_sum2[:n_obs])
+               n_moves=out.n_moves, obs_sum=obs_sum[:n_obs], obs_sum2=obs_sum2[:n_obs],
+               ld=ld, max_ld_drift=out.max_ld_drift, n_singular=out.n_singular)
     return res
 
 

@@ -1,7 +1,9 @@
 """Parity of the gfx950 Slater-Jastrow path (through the C ABI) with the CPU
 oracle's Sherman-Morrison runner on the same Philox streams, in FP64 and in
 mixed precision; independence of the result from lanes-per-walker;
-heterogeneous systems in one launch; zero-variance anchors."""
+heterogeneous systems in one launch; zero-variance anchors (1s and 2p); the
+FP64 log-determinants carried through the updates and their drift; the
+singular-pivot guard on planted singular configurations."""
 import numpy as np
 import pytest
 
@@ -90,12 +92,25 @@ def test_mixed_precision_parity_at_workload_width(Z, lanes, steps):
     assert d["stats"]["n_accepted"] == o["n_accepted"]
 
 
+def tile_footprint(system, lanes):
+    """LDS bytes of one full tile of the system at the given lanes per walker
+    (a one-tile context's launch shape)."""
+    with M.Context([M.slater_jastrow(system, 64 // lanes)], seed=0, step=0.6, block_size=2,
+                   precision=M.PREC_MIXED, lu_period=64) as c:
+        c.set_walker_lanes(lanes)
+        return c.launch_info()[1]
+
+
 def test_heterogeneous_batch_at_workload_shape():
-    # the heterogeneous workload's one launch: 256 systems of 2..40 electrons,
-    # 2^20 walkers split in proportion to N, the lanes its autotuning picked
-    # (longest tiles first, one LDS footprint for every tile), mixed
-    # precision; the smallest, a middle and the largest system of the launch
-    # against their own oracle runs (systems are independent streams)
+    # the heterogeneous workload's launches: 256 systems of 2..40 electrons,
+    # 2^20 walkers split in proportion to N, the lanes its autotuning picked,
+    # mixed precision.  The tiles are grouped into LDS footprint classes (4 KiB
+    # steps), one launch per class with blockIdx.x restarting at 0 in each,
+    # longest tiles first within a class.  The system with the fewest
+    # electrons, the one with the largest tile footprint (system 52, 11
+    # electrons) and the one with the most electrons -- three different
+    # footprint classes -- against their own oracle runs (systems are
+    # independent streams)
     from mole_amd import autotune
     from mole_amd import workloads as WL
     lanes = {2: 4, 3: 4, 4: 4, 5: 8, 6: 8, 7: 8, 8: 8, 9: 16, 10: 8, 11: 8, 12: 16, 13: 16, 14: 16, 15: 16,
@@ -107,7 +122,10 @@ def test_heterogeneous_batch_at_workload_shape():
     W = np.maximum(1, np.floor(Ns / Ns.sum() * (1 << 20))).astype(int)
     W[np.argmax(Ns)] += (1 << 20) - W.sum()
     order = np.argsort(Ns, kind="stable")
-    picks = [int(order[0]), int(order[len(order) // 2]), int(np.argmax(Ns))]
+    picks = [int(order[0]), 52, int(np.argmax(Ns))]
+    # the picks must sit in different footprint classes, so class boundaries stay covered
+    footprint = {k: tile_footprint(sys_list[k], lanes[int(Ns[k])]) for k in picks}
+    assert len({(footprint[k] + 4095) // 4096 for k in picks}) == 3, footprint
     steps, B = 4, 2
     print(f"heterogeneous batch: {len(sys_list)} systems, {int(W.sum())} walkers, checking {picks}", flush=True)
     dev = [M.slater_jastrow(s, int(w)) for s, w in zip(sys_list, W)]
@@ -159,6 +177,10 @@ def test_heterogeneous_systems_in_one_launch():
 def test_zero_variance_anchors():
     d = run_device([S.hydrogenic(1)], 128, 20, 5)[0]
     assert np.max(np.abs(d["el"] + 0.5)) <= 1e-12
+    # the 2p eigenfunctions x_q exp(-Z r/2): E_L = -Z^2/8 for every sample
+    for axis in range(3):
+        d = run_device([S.hydrogenic_2p(2, axis)], 128, 20, 5)[0]
+        assert np.max(np.abs(d["el"] + 0.5)) <= 1e-11
     d = run_device([S.helium_product(2.0, 0.0)], 128, 20, 5, hamiltonian=M.H_IONIC)[0]
     assert np.max(np.abs(d["el"] + 4.0)) <= 1e-12
 
@@ -278,3 +300,69 @@ def test_diffusion_helium_product_anchor():
     nb = st["n_blocks"]
     err = np.sqrt((st["sum_block_mean2"] / nb - (st["sum_block_mean"] / nb) ** 2) / (nb - 1))
     assert abs(mean - (zeta**2 - 27 * zeta / 8)) < 4 * err
+
+
+# ------------------------------------------------ log-determinants, singular blocks
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+def test_logdet_carried_through_updates_matches_oracle(precision):
+    # det A_s carried through every accepted move as mantissa x 2^exponent:
+    # the drift against each re-inversion (the FP32 chain's error in mixed
+    # precision) bit for bit as the oracle's, the final log-determinants as
+    # the oracle carries them, and (FP64) as the LU of the final configuration
+    W, steps, B = 64, 60, 20
+    system = S.atom(9)
+    dev = [M.slater_jastrow(system, W)]
+    with M.Context(dev, seed=2, step=0.6, block_size=B, precision=precision, lu_period=64) as ctx:
+        ctx.init_walkers()
+        ctx.run(steps)
+        st = ctx.stats()
+        ld, sg = ctx.logdet()
+        cfg = ctx.get_configs()
+    wv = SJ.SjWave(system)
+    o = SJ.run(wv, seed=2, walkers=W, steps=steps, block_size=B, box_side=0.6, precision=precision, lu_period=64,
+               n_threads=8, record=False)
+    assert np.array_equal(bits(cfg), bits(o["cfg"]))
+    assert st["max_logdet_drift"] == o["max_ld_drift"]
+    assert st["n_singular"] == 0 == o["n_singular"]
+    want = np.array([[SJ.logdet_value(o["ld"][w, 2 * q], o["ld"][w, 2 * q + 1]) for q in (0, 1)] for w in range(W)])
+    assert np.max(np.abs(ld - want)) <= 1e-12 * np.max(np.abs(want))
+    assert np.array_equal(sg, np.sign(o["ld"][:, [0, 2]]))
+    if precision == M.PREC_FP64:
+        assert 0.0 < st["max_logdet_drift"] < 1e-10
+        for w in range(W):
+            for q in (0, 1):
+                lu, s_lu = SJ.logdet(wv, cfg[w], q)
+                assert abs(ld[w, q] - lu) < 1e-10 and sg[w, q] == s_lu
+    else:
+        assert st["max_logdet_drift"] > 1e-8
+
+
+def test_planted_singular_configuration_is_counted():
+    # walkers 0..7 start with spin-up electrons 0 and 1 at the same point
+    # (identical rows, det A_up = 0): the re-inversion counts the block and
+    # skips its division instead of writing Inf/NaN into the walker; the
+    # guarded path then follows the oracle's bit for bit, and the other
+    # walkers' determinants stay finite
+    W, steps, B = 16, 20, 10
+    system = S.atom(6)
+    cfg = np.random.default_rng(7).normal(scale=1.2, size=(W, 3 * system.n_electrons))
+    cfg[:8, 3:6] = cfg[:8, 0:3]
+    dev = [M.slater_jastrow(system, W)]
+    with M.Context(dev, seed=4, step=0.6, block_size=B, flags=M.FLAG_TRACE, record_sweeps=steps, lu_period=8) as ctx:
+        ctx.set_configs(cfg)
+        ld0, sg0 = ctx.logdet()
+        n0 = ctx.stats()["n_singular"]
+        ctx.run(steps)
+        masks, _ = ctx.trace(steps, 0)
+        st = ctx.stats()
+        out = ctx.get_configs()
+    assert n0 == 8
+    assert np.all(ld0[:8, 0] == -np.inf) and np.all(sg0[:8, 0] == 0.0)
+    assert np.all(np.isfinite(ld0[8:])) and np.all(np.isfinite(ld0[:, 1]))
+    o = SJ.run(SJ.SjWave(system), seed=4, walkers=W, steps=steps, block_size=B, box_side=0.6, lu_period=8,
+               cfg=cfg, n_threads=8)
+    assert np.array_equal(masks, o["accept_mask"])
+    assert np.array_equal(bits(out), bits(o["cfg"]))
+    assert st["n_singular"] == o["n_singular"] >= 8
+    assert np.all(np.isfinite(out))

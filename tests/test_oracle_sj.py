@@ -1,8 +1,10 @@
 """Pins the oracle's Slater-Jastrow restatement: LU determinants against the
 Leibniz expansion, Gauss-Jordan inverses against A B = I, the local energy
-against finite differences and zero-variance eigenfunctions, the Jastrow
-parameter derivatives against finite differences, and the Sherman-Morrison
-runner against the from-scratch (reference-faithful) runner."""
+against finite differences and zero-variance eigenfunctions (1s and 2p), the
+Jastrow parameter derivatives against finite differences, the Sherman-Morrison
+runner against the from-scratch (reference-faithful) runner, the pivot
+determinants and the carried log|det| against LU and Leibniz, and the
+singular-pivot guard on planted singular configurations."""
 import ctypes as C
 
 import numpy as np
@@ -17,14 +19,24 @@ def cfgs(N, n, seed, scale=1.2):
     return np.random.default_rng(seed).normal(scale=scale, size=(n, 3 * N))
 
 
+def basis_numpy(system, xe):
+    """basis values at the point xe, written from the definitions (numpy)"""
+    d = xe[None, :] - system.nuclei[system.center]
+    r = np.linalg.norm(d, axis=1)
+    kind = np.zeros(system.n_basis, int) if system.kind is None else system.kind
+    v = np.where(kind == S.GTO, np.exp(-system.zeta * r * r), np.exp(-system.zeta * r))
+    for q in range(3):
+        v = np.where(kind == S.STO_PX + q, d[:, q] * np.exp(-system.zeta * r), v)
+    return v
+
+
 def slater_numpy(system, x, spin):
     n = system.n_up if spin == 0 else system.n_dn
     first = 0 if spin == 0 else system.n_up
     A = np.zeros((n, n))
     for c in range(n):
         xe = x[3 * (first + c):3 * (first + c) + 3]
-        r = np.linalg.norm(xe[None, :] - system.nuclei[system.center], axis=1)
-        A[c] = system.coef[:n] @ np.exp(-system.zeta * r)
+        A[c] = system.coef[:n] @ basis_numpy(system, xe)
     return A
 
 
@@ -91,6 +103,14 @@ def test_zero_variance_eigenfunctions():
         for x in cfgs(1, 50, Z):
             e, _ = SJ.local_energy(wv, x)
             assert abs(e + Z * Z / 2) <= 1e-12
+    # 2p: x_q exp(-Z r/2) gives E_L = -Z^2/8 everywhere (each axis); a p
+    # laplacian with 2 z/r in place of 4 z/r would leave E_L = -Z^2/8 - Z/(2r)
+    for Z in (1, 4):
+        for axis in range(3):
+            wv = SJ.SjWave(S.hydrogenic_2p(Z, axis))
+            for x in cfgs(1, 30, 10 * Z + axis):
+                e, _ = SJ.local_energy(wv, x)
+                assert abs(e + Z * Z / 8) <= 1e-11 * max(1, Z * Z)
     # He, zeta = Z = 2, no Jastrow, IonicHamiltonian (operator.rs:130-149): E_L == -4
     wv = SJ.SjWave(S.helium_product(2.0, 0.0), hamiltonian=1)
     for x in cfgs(2, 50, 9):
@@ -207,3 +227,91 @@ def test_diffusion_mixed_precision_runner_stays_close():
     b = SJ.run(wv, precision=SJ.PREC_MIXED, **kw)
     # FP32 Sherman-Morrison and Jastrow flip only rare near-threshold decisions
     assert np.mean(a["accept_mask"] == b["accept_mask"]) > 0.95
+
+
+def test_basis_values_match_definitions():
+    # the oracle's matrix entries against numpy's evaluation of every kind
+    mixed = S.SJSystem([(0.0, 0.0, 0.0), (0.3, -0.2, 1.1)], [3, 1], 2, 1, [1.3, 0.7, 2.1, 2.1, 2.1, 0.9],
+                       [0, 0, 0, 0, 0, 1], np.random.default_rng(0).normal(size=(2, 6)),
+                       kind=[S.STO, S.GTO, S.STO_PX, S.STO_PY, S.STO_PZ, S.STO])
+    wv = SJ.SjWave(mixed)
+    for x in cfgs(3, 10, 21):
+        for spin in (0, 1):
+            B, rc = SJ.inverse(wv, x, spin)
+            A = slater_numpy(mixed, x, spin)
+            assert rc == 0
+            assert np.allclose(B @ A, np.eye(A.shape[0]), atol=1e-10 * np.linalg.cond(A))
+
+
+def wv_sys(Z):
+    return S.atom(Z)
+
+
+@pytest.mark.parametrize("Z", [3, 5, 8, 10])
+def test_pivot_determinant_matches_lu_and_leibniz(Z):
+    # the re-inversion's determinant (mant 2^ex from the Gauss-Jordan pivots)
+    # against the independent LU and the long-double Leibniz expansion
+    wv = SJ.SjWave(S.atom(Z))
+    for x in cfgs(wv.N, 10, 40 + Z):
+        for spin in (0, 1):
+            n = wv.n_up if spin == 0 else wv.n_dn
+            if n == 0:
+                continue
+            _, mant, ex, rc = SJ.inverse_det(wv, x, spin)
+            assert rc == 0 and 0.5 <= abs(mant) < 1.0
+            ld = SJ.logdet_value(mant, ex)
+            lu, sg = SJ.logdet(wv, x, spin)
+            # two backward-stable eliminations: |d log det| <~ n eps cond(A)
+            cond = np.linalg.cond(slater_numpy(wv_sys(Z), x, spin))
+            assert abs(ld - lu) <= 1e-15 * n * cond + 1e-13 * max(1.0, abs(lu))
+            assert np.sign(mant) == sg
+            lz = float(SJ.lib().orc_sj_det_leibniz(wv.ref(), OF.dptr(np.ascontiguousarray(x)), spin))
+            # error scale of the expansion: Hadamard's bound prod_c |row c|
+            hb = np.prod(np.linalg.norm(slater_numpy(wv_sys(Z), x, spin), axis=1))
+            assert abs(mant * 2.0 ** ex - lz) <= 1e-13 * hb
+
+
+def test_carried_logdet_of_parity_runner():
+    # FP64: the determinant carried through the accepted moves since the last
+    # re-inversion equals the LU determinant of the final configuration; the
+    # drift recorded at re-inversions is at rounding level.  Mixed: the FP32
+    # ratios leave a drift far above FP64's (what the device reports).
+    wv = SJ.SjWave(S.atom(9))
+    kw = dict(seed=5, walkers=16, steps=60, block_size=20, box_side=0.6, lu_period=64, record=False)
+    a = SJ.run(wv, precision=SJ.PREC_FP64, **kw)
+    b = SJ.run(wv, precision=SJ.PREC_MIXED, **kw)
+    for w in range(16):
+        for spin in (0, 1):
+            lu, sg = SJ.logdet(wv, a["cfg"][w], spin)
+            mant, ex = a["ld"][w, 2 * spin], a["ld"][w, 2 * spin + 1]
+            assert abs(SJ.logdet_value(mant, ex) - lu) < 1e-10 and np.sign(mant) == sg
+    assert 0.0 < a["max_ld_drift"] < 1e-10
+    assert b["max_ld_drift"] > 1e3 * a["max_ld_drift"]
+    assert a["n_singular"] == 0 and b["n_singular"] == 0
+
+
+def test_singular_block_is_flagged_not_divided():
+    # two spin-up electrons at the same point: identical rows, det A_up = 0.
+    # The re-inversion flags the block (rc -1, mantissa 0) and its inverse
+    # stays finite; the spin-down block is unaffected.
+    wv = SJ.SjWave(S.atom(6))
+    x = cfgs(6, 1, 3)[0]
+    x[3:6] = x[0:3]
+    B, mant, ex, rc = SJ.inverse_det(wv, x, 0)
+    assert rc == -1 and mant == 0.0 and np.all(np.isfinite(B))
+    assert SJ.logdet_value(mant, ex) == -np.inf
+    _, mant_d, _, rc_d = SJ.inverse_det(wv, x, 1)
+    assert rc_d == 0 and mant_d != 0.0
+    # s functions only: two electrons at the same distance from the nucleus
+    # are identical rows as well
+    s_only = S.SJSystem([(0.0, 0.0, 0.0)], [4], 2, 2, [1.1, 3.0], [0, 0],
+                        np.random.default_rng(1).normal(size=(2, 2)))
+    wv2 = SJ.SjWave(s_only)
+    y = cfgs(4, 1, 4)[0]
+    y[3:6] = y[0:3][[1, 2, 0]]  # a rotation: same |r|
+    _, m2, _, rc2 = SJ.inverse_det(wv2, y, 0)
+    assert rc2 == -1 and m2 == 0.0
+    # the runner counts such blocks at its re-inversions (here the first one)
+    cfg = np.tile(x, (4, 1))
+    r = SJ.run(wv, seed=1, walkers=4, steps=4, block_size=2, box_side=0.5, lu_period=64, cfg=cfg, record=False)
+    assert r["n_singular"] == 4
