@@ -1,12 +1,18 @@
-"""Walker-moves/s of the VMC hot path on one MI355X.
+"""Walker-moves/s of the VMC hot path on one MI355X, and effective samples/s
+of the learned proposal against the plain one.
 
 Workload: H2 molecule (nuclei at z = +-0.7 bohr) with the LCAO sigma_g^2 x
 Pade-Jastrow wavefunction, optimised by stochastic reconfiguration: every
 step is one SR iteration = 16,384 walkers x 200 box-Metropolis sweeps
 (block 20, first block discarded), local energy + parameter log-derivatives
-sampled after every sweep, device-reduced SR sums, host 3x3 solve, step
-0.05, diagonal shift 1e-3, FP64 throughout.  Walkers persist across
-iterations.  The default run times the full 50 iterations.
+sampled after every sweep, device-reduced SR sums, 3x3 solve, step 0.05,
+diagonal shift 1e-3, FP64 throughout.  Walkers persist across iterations.
+The default run times the full 50 iterations.
+
+The line's "learned" record is the metric's second half on the Ne atom
+(10 electrons, 65,536 walkers): the MLP proposal trained on the device on
+2M configurations of the box sampler, then effective samples/s and
+kappa = 1/(sigma(E)^2 t) against the best box side, tau_int on the device.
 
 Prints one JSON line (rank 0, N = 1).
 """
@@ -46,12 +52,45 @@ def flops_per_launch(walkers, sweeps, sampled):
 
 VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
 
-# HBM bytes per sweep launch of the timed optimisation loop: rocprofv3
-# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (factors from the plain copy kernel of
-# mole_amd/copyprobe.py), recorded in profiles/sweep2_ws_kernel_h2_sr_16384w.md.
-# Those launches start the accumulators from zero, so the algorithmic state
-# traffic is positions in and out plus accumulators out: 3.93e6 B.
+# HBM bytes per sweep launch of the timed optimisation loop, as measured by
+# rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy
+# kernel of known size reads) and recorded with the command that made it in
+# TRAFFIC_SOURCE; a counter run cannot share this process, so the line carries
+# that measurement.  The launches start the accumulators from zero, so the
+# algorithmic state traffic is positions in and out plus accumulators out:
+# 3.93e6 B.
 TRAFFIC_BYTES_PER_LAUNCH = (680.3 * 2.0 + 3328.0) * 1024
+TRAFFIC_SOURCE = "profiles/sweep2_ws_kernel_h2_sr_16384w.md (rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE)"
+METRIC = "walker-moves/s incl. E_L; effective samples/s learned vs plain proposal at matched sigma(E)"
+
+
+def host_cpus():
+    """The host's logical CPUs, this process's share of them (affinity mask
+    and cgroup quota) and the CPU model; the baseline runs one thread per CPU
+    of the share, at most 16 (a GPU's share of its box)."""
+    nproc = os.cpu_count() or 1
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        aff = nproc
+    quota = None
+    try:
+        q, per = open("/sys/fs/cgroup/cpu.max").read().split()
+        if q != "max":
+            quota = float(q) / float(per)
+    except (OSError, ValueError):
+        pass
+    model = platform.processor() or platform.machine()
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    share = aff if quota is None else min(aff, max(1, int(quota)))
+    return dict(nproc=nproc, affinity=aff, cgroup_quota=quota, share=share, model=model, threads=min(share, 16))
 
 
 def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
@@ -132,7 +171,7 @@ def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
                 params=[float(v) for v in p])
 
 
-def cpu_baseline(threads, budget_s=20.0):
+def cpu_baseline(threads, budget_s=20.0, host=None):
     """The oracle's reference-faithful VMC (two full psi evaluations per move,
     generic act_on/psi local energy, per-sample observables, SR from the stored
     samples) on the host, over whole SR iterations of the same workload."""
@@ -162,10 +201,14 @@ def cpu_baseline(threads, budget_s=20.0):
         if iters > 1:
             t1 = run(iters)
     moves = iters * WALKERS * SWEEPS * N_ELEC
+    host = host or {}
     return dict(value=moves / t1, unit="walker-moves/s", cores=threads, kind="port",
                 sample=f"{iters} SR iteration(s) of the same workload ({WALKERS} walkers x {SWEEPS} "
-                       f"sweeps each) on the oracle in reference-faithful mode, {threads} threads, "
-                       f"{t1:.1f} s; host {platform.processor() or platform.machine()}")
+                       f"sweeps each) on the C oracle in reference-faithful mode (two full psi evaluations "
+                       f"per move, the reference's generic local energy; the reference is Rust and cannot be "
+                       f"built here), {threads} threads, {t1:.1f} s",
+                nproc=host.get("nproc"), cpu_share=host.get("share"), cpu_model=host.get("model"),
+                build="gcc -O3 -ffp-contract=off -fno-fast-math")
 
 
 def cpu_atom_sample(threads, walkers=1024, sweeps=40, block=10):
@@ -188,7 +231,7 @@ def cpu_atom_sample(threads, walkers=1024, sweeps=40, block=10):
 def atoms_main(args):
     from mole_amd import workloads
 
-    threads = args.cpu_threads or min(16, os.cpu_count() or 1)
+    threads = args.cpu_threads or host_cpus()["threads"]
     cpu = None if args.no_cpu_baseline else cpu_atom_sample(threads)
     workloads.atoms(walkers=args.walkers or 65536, sweeps=args.sweeps or 1000, cpu_sample=cpu,
                     log=lambda r: print(json.dumps(r), flush=True))
@@ -211,6 +254,30 @@ def learned_main(args):
     for Z in (3, 4, 10):
         workloads.learned(Z=Z, walkers=args.walkers or 65536, sweeps=args.sweeps or 400,
                           log=lambda r: print(json.dumps(r), flush=True))
+
+
+def learned_record(log_runs=False):
+    """The metric's second half for the line: learned vs box on the Ne atom
+    (training included, all on the device)."""
+    from mole_amd import workloads
+
+    def log(r):
+        if log_runs:
+            print(json.dumps(r), file=sys.stderr, flush=True)
+
+    r = workloads.learned(Z=10, log=log)
+
+    def brief(x):
+        return {k: x[k] for k in ("step", "energy", "error", "acceptance", "tau_int_mean", "moves_per_s",
+                                  "ess_per_s", "kappa")}
+    return dict(system="Ne atom (10 electrons; 1s/2s/2p STO basis, seed 1)", walkers=r["walkers"],
+                sweeps=r["sweeps"], precision=r["precision"], training=r["training"],
+                box=[brief(b) for b in r["box"]], learned=brief(r["learned"]),
+                ess_ratio_vs_best_box=r["ess_ratio_vs_best_box"],
+                kappa_ratio_vs_best_box=r["kappa_ratio_vs_best_box"],
+                tau_ratio_vs_best_box=r["tau_ratio_vs_best_box"],
+                energy_diff_sigma=r["energy_diff_sigma"],
+                tau_estimator="scripts/statfor.rs:30-54 on the device (mole_tau_int)")
 
 
 def cpu_dmc_sample(walkers=4096, steps=200):
@@ -238,9 +305,54 @@ def dmc_main(args):
                   log=lambda r: print(json.dumps(r), flush=True))
 
 
+def launch_main(args):
+    """One fixed launch of the Slater-Jastrow sweep kernel, for profilers
+    (kernel traces and counter passes need a short program whose last
+    dispatch of the kernel is the measured one): the atom Z of the atoms
+    workload, 4 warm-up sweeps, then one launch of --sweeps sampled sweeps.
+    The learned proposal uses an untrained network (seed 2): its cost per move
+    does not depend on the weights."""
+    import mole_amd as M
+    from mole_amd import systems
+
+    prop = {"box": M.PROPOSAL_BOX, "diffuse": M.PROPOSAL_DIFFUSE, "learned": M.PROPOSAL_LEARNED}[args.proposal]
+    prec = M.PREC_MIXED if args.precision == "mixed" else M.PREC_FP64
+    sj = systems.atom(args.Z)
+    W = args.walkers or 65536
+    sweeps = args.sweeps or 20
+    with M.Context([M.slater_jastrow(sj, W)], seed=1, step=args.side, block_size=2, precision=prec,
+                   lu_period=64, proposal=prop) as ctx, M.Mlp(2) as net:
+        if prop == M.PROPOSAL_LEARNED:
+            net.attach(ctx)
+        elif args.lanes:
+            ctx.set_walker_lanes(args.lanes)
+        ctx.init_walkers()
+        ctx.sweep(4)
+        ctx.synchronize()
+        ctx.sweep(sweeps, sample=True)
+        ctx.synchronize()
+        ms, _ = ctx.last_timing()
+        lanes = ctx.walker_lanes()
+    print(json.dumps(dict(workload="launch", proposal=args.proposal, Z=args.Z, walkers=W, sweeps=sweeps, lanes=lanes,
+                          precision=args.precision, ms=ms,
+                          moves_per_s=W * sj.n_electrons * sweeps / (ms * 1e-3))), flush=True)
+
+
+def copy_main(args):
+    """The plain device copy (copy_probe_kernel) that calibrates the HBM byte
+    counters: --bytes from one device buffer to another, --repeats times."""
+    import mole_amd as M
+
+    n = args.bytes // 8
+    ms = M.debug_copy(n, args.repeats)
+    print(json.dumps(dict(workload="copy", bytes_read=n * 8, bytes_written=n * 8, ms_per_copy=ms,
+                          gb_per_s=2 * n * 8 / ms / 1e6)), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero", "learned", "dmc"], default="h2-sr")
+    ap.add_argument("--workload", choices=["h2-sr", "atoms", "hetero", "learned", "dmc", "launch", "copy"],
+                    default="h2-sr")
     ap.add_argument("--walkers", type=int, default=0)
     ap.add_argument("--sweeps", type=int, default=0)
     ap.add_argument("--gpus", type=int, default=1)
@@ -248,10 +360,19 @@ def main():
     ap.add_argument("--warmup", type=int, default=20,
                     help="untimed SR iterations on a separate context (clocks settle)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-learned", action="store_true", help="leave out the learned-vs-box record")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", choices=sorted(VARIANTS), default="auto")
     ap.add_argument("--host-loop", action="store_true",
                     help="one host round trip per SR iteration instead of the device-side loop")
+    # --workload launch / copy (profiler runs)
+    ap.add_argument("--proposal", choices=["box", "diffuse", "learned"], default="box")
+    ap.add_argument("--Z", type=int, default=3)
+    ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
+    ap.add_argument("--lanes", type=int, default=0)
+    ap.add_argument("--side", type=float, default=0.6)
+    ap.add_argument("--bytes", type=int, default=256 << 20)
+    ap.add_argument("--repeats", type=int, default=2)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: walkers are independent Markov chains and this "
@@ -269,6 +390,12 @@ def main():
     if args.workload == "dmc":
         dmc_main(args)
         return
+    if args.workload == "launch":
+        launch_main(args)
+        return
+    if args.workload == "copy":
+        copy_main(args)
+        return
 
     r = gpu_run(args.steps, args.warmup, variant=args.variant, host_loop=args.host_loop)
     moves = args.steps * WALKERS * SWEEPS * N_ELEC
@@ -278,22 +405,24 @@ def main():
     avg_kernel_s = 1e-3 * float(np.mean(r["kernel_ms"]))
     fl = flops_per_launch(WALKERS, SWEEPS, SWEEPS - BLOCK)
     achieved = fl / avg_kernel_s / 1e12
-    # compute roofline: the FP64 work runs on the vector ALUs, whose MI355X
-    # peak equals the FP64 matrix-core figure (78.6 TFLOP/s)
-    roofline = dict(bound="mfma", pipe="fp64 VALU", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
+    # compute roofline: the kernel issues no MFMA; its FP64 work runs on the
+    # vector ALUs (MI355X FP64 vector peak 78.6 TFLOP/s)
+    roofline = dict(bound="fp64-valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS,
                     traffic=None if args.variant == "lane" else TRAFFIC_BYTES_PER_LAUNCH,
+                    traffic_source=None if args.variant == "lane" else TRAFFIC_SOURCE,
                     kernel=("sweep2_lane_kernel<LCAO2>" if args.variant == "lane"
                             else "sweep2_ws_kernel<LCAO2,2,6>"),
                     kernel_ms=1e3 * avg_kernel_s,
                     flop_per_launch=fl)
     cpu = None
     if not args.no_cpu_baseline:
-        threads = args.cpu_threads or min(16, os.cpu_count() or 1)
-        cpu = cpu_baseline(threads)
+        host = host_cpus()
+        cpu = cpu_baseline(args.cpu_threads or host["threads"], host=host)
+    learned = None if args.no_learned else learned_record()
     e_last = r["energies"][-1]
     out = {
-        "metric": "walker-moves/s incl. E_L",
+        "metric": METRIC,
         "value": value,
         "unit": "walker-moves/s",
         "n_gpus": 1,
@@ -317,6 +446,7 @@ def main():
         "gpu_vs_cpu": (value / cpu["value"]) if cpu else None,
         "result": {"energy": e_last[0], "error": e_last[1], "acceptance": e_last[2],
                    "params": r["params"], "energy_first": r["energies"][0][0]},
+        "learned": learned,
     }
     print(json.dumps(out))
 

@@ -270,6 +270,16 @@ typedef struct {
  * MOLE_FLAG_TRACE (records up to record_sweeps samples). */
 mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_block_record *out, size_t cap,
                              size_t *n);
+/* The integrated autocorrelation time of every walker's local-energy
+ * series, computed on the device from the trace (the series stay there):
+ * correlation() of scripts/statfor.rs:30-54 over samples [first, first + n)
+ * since the last reset,
+ *   tau = max(1, 1 + 2 sum_{i=1..min(200, n-1)} rho_i), the sum cut at the
+ *   first negative rho_i, rho_i = sum_t (x_t - m)(x_{t+i} - m)/((n - i) var),
+ *   m the mean and var the variance with n - 1 (statfor.rs:86-88);
+ * every sum in sample order, as the reference folds it.  tau[n_walkers].
+ * Effective samples = sum_w n / tau_w.  Needs MOLE_FLAG_TRACE, n >= 2. */
+mole_status mole_tau_int(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau, size_t n_walkers);
 /* MonteCarloSampler::data (src/montecarlo/src/traits.rs:27) in compat
  * form: "Energy", "Wavefunction value" and "Parameter gradient" samples as
  * [walker][sample] (and [walker][sample][param]), the order in which

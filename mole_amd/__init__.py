@@ -246,6 +246,7 @@ _SIGS = [
     ("mole_read_trace", C.c_int32, [_P, C.c_uint32, _pu64, _pd, C.c_size_t, C.c_size_t, C.c_size_t]),
     ("mole_read_samples", C.c_int32, [_P, C.c_uint32, _pd, _pd, _pd, C.c_size_t, C.c_size_t]),
     ("mole_read_logdet", C.c_int32, [_P, C.c_uint32, _pd, _pd, C.c_size_t]),
+    ("mole_tau_int", C.c_int32, [_P, C.c_uint32, C.c_size_t, C.c_size_t, _pd, C.c_size_t]),
     ("mole_eval", C.c_int32, [_P, C.c_uint32, C.c_size_t, _pd, _pd, _pd, _pd, _pd, _pd]),
     ("mole_set_launch_config", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_set_walker_lanes", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
@@ -498,6 +499,14 @@ class Context:
         sg = np.zeros((W, 2))
         check(lib().mole_read_logdet(self.h, sys, _dp(ld), _dp(sg), W))
         return ld, sg
+
+    def tau_int(self, n, first=0, sys=0):
+        """mole_tau_int: per-walker integrated autocorrelation time of the
+        local-energy samples [first, first + n) of the trace, on the device."""
+        W = self.systems[sys].n_walkers
+        t = np.zeros(W)
+        check(lib().mole_tau_int(self.h, sys, first, n, _dp(t), W))
+        return t
 
     def samples(self, samples, sys=0):
         W = self.systems[sys].n_walkers

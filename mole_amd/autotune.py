@@ -58,12 +58,18 @@ def tune(systems, walkers_of, *, candidates=CANDIDATES, sweeps=6, precision=M.PR
             timings[f"{N}:{L}"] = ms
             if ms is not None and (best_ms is None or ms < best_ms):
                 best, best_ms = L, ms
-        if best is None:  # nothing under the budget: the smallest footprint
-            best = max(c for c in candidates if time_lanes(s, W, c, sweeps=1, precision=precision,
-                                                           lu_period=lu_period) is not None)
+        if best is None:  # nothing under the budget: the smallest footprint that fits the device
+            fits = [c for c in candidates
+                    if time_lanes(s, W, c, sweeps=1, precision=precision, lu_period=lu_period) is not None]
+            if not fits:
+                raise M.MoleError(M.ERR_UNSUPPORTED,
+                                  f"autotune: no lanes per walker in {tuple(candidates)} fits the walker state of "
+                                  f"the {N}-electron system in the device's LDS")
+            best = max(fits)
         table[N] = best
         if verbose:
-            print(f"N={N:2d}: L={best} ({best_ms:.3f} ms/sweep for {W} walkers)", flush=True)
+            timing = f"{best_ms:.3f} ms/sweep" if best_ms is not None else "over the LDS budget, smallest footprint"
+            print(f"N={N:2d}: L={best} ({timing} for {W} walkers)", flush=True)
     return table, timings
 
 

@@ -12,7 +12,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 BUILD = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libmole_amd.so")
-SOURCES = ["api.cpp", "two_electron.hip", "slater_jastrow.hip", "mlp.hip", "dmc.hip", "debug.hip"]
+SOURCES = ["api.cpp", "two_electron.hip", "slater_jastrow.hip", "mlp.hip", "dmc.hip", "statfor.hip", "debug.hip"]
 ARCH = os.environ.get("MOLE_OFFLOAD_ARCH", "gfx950")
 
 CXXFLAGS = [

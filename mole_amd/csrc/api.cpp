@@ -1066,6 +1066,27 @@ mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_blo
   return ok();
 }
 
+mole_status mole_tau_int(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau, size_t n_walkers) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!tau) return fail(MOLE_ERR_INVALID_ARGUMENT, "null tau");
+  if (!(ctx->run.flags & MOLE_FLAG_TRACE)) return fail(MOLE_ERR_DATA_ACCESS, "context has no trace");
+  SysState &s = ctx->sys[sys];
+  if (n_walkers != (size_t)s.W) return fail(MOLE_ERR_SHAPE, "expected %d walkers", s.W);
+  const size_t rec = std::min((size_t)ctx->trace_samples, (size_t)ctx->run.record_sweeps);
+  if (n < 2 || first > rec || n > rec - first)
+    return fail(MOLE_ERR_SHAPE, "samples [%zu, %zu) outside the %zu recorded (n >= 2)", first, first + n, rec);
+  HIP_TRY(hipSetDevice(ctx->device));
+  double *d = nullptr;
+  HIP_TRY(hipMalloc(&d, sizeof(double) * (size_t)s.W));
+  hipError_t e = launch_tau_corr(s.d_trace_e, s.W, ctx->run.record_sweeps, first, (int)n, d, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(tau, d, sizeof(double) * (size_t)s.W, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_tau_int: %s", hipGetErrorString(e));
+  return ok();
+}
+
 mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, double *psi, double *pgrad,
                               size_t walkers, size_t samples) {
   mole_status st = check_ctx(ctx, sys);

@@ -92,5 +92,9 @@ hipError_t launch_eval2(const Eval2Args &a, hipStream_t stream);
 hipError_t launch_reduce_columns(const double *acc, int W, int K, double *out,
                                  int max_first, int max_count, hipStream_t stream);
 hipError_t launch_fill(double *p, size_t n, double v, hipStream_t stream);
+// tau_out[w] = integrated autocorrelation time (scripts/statfor.rs:30-54) of
+// samples [first, first + n) of walker w's series in trace_e [W][cap], n >= 2
+hipError_t launch_tau_corr(const double *trace_e, int W, size_t cap, size_t first, int n, double *tau_out,
+                           hipStream_t stream);
 
 }  // namespace mole

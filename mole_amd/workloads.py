@@ -152,16 +152,16 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
 
 
 def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=8000,
-            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.6, 1.0, 1.5, 2.0, 3.0), precision=M.PREC_FP64,
-            lu_period=64, seed=1, log=print):
+            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.3, 0.45, 0.6, 0.9, 1.2),
+            precision=M.PREC_FP64, lu_period=64, seed=1, log=print):
     """Learned proposal against the box proposal on one atom: configurations
     from the box sampler, Adam training on the device, then runs of each
-    proposal with per-walker E_L series.  Efficiency per proposal: ESS/s =
-    W S / (tau_int t) with the statfor estimator (scripts/statfor.rs:30-54)
-    averaged over walkers, and kappa = 1 / (sigma(E)^2 t) with the blocking
-    error, t the device time of the S sampled sweeps."""
-    from mole_amd.stats import tau_int
-
+    proposal from the same ensemble.  Efficiency per proposal: ESS/s =
+    W S / (tau_int t) with tau_int of each walker's E_L series by the statfor
+    estimator (scripts/statfor.rs:30-54) on the device, averaged over walkers,
+    and kappa = 1 / (sigma(E)^2 t) with the blocking error; t is the device
+    time of the S sampled sweeps.  Learned against the box side of the
+    largest kappa (matched sigma(E): kappa is 1/(sigma^2 t))."""
     sj = systems.atom(Z)
     N = sj.n_electrons
 
@@ -174,12 +174,13 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
     # 1. training configurations from the plain sampler; its final ensemble
     #    is also the common start of every measured run (a slowly mixing
     #    proposal started from U(-1,1) would otherwise carry its burn-in)
-    with context(M.PROPOSAL_BOX, 0.6) as ctx:
+    with context(M.PROPOSAL_BOX, box_sides[len(box_sides) // 2]) as ctx:
         ctx.init_walkers()
         ctx.sweep(equil)
         n_snap = max(1, -(-n_configs // walkers))
         t0 = time.perf_counter()
         mlp.collect(ctx, n_snap, collect_every)
+        ctx.synchronize()
         t_collect = time.perf_counter() - t0
         start = ctx.get_configs()
         # 2. training on the device
@@ -187,14 +188,15 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
         obj = mlp.train(ctx, train_steps, batch=batch, lr=lr, seed=seed + 10)
         t_train = time.perf_counter() - t0
     n_data = mlp.data_size()
+    box_lanes = autotune.tune([sj], lambda s: walkers, candidates=(4, 8, 16), sweeps=2, precision=precision,
+                              lu_period=lu_period)[0]
 
     def measure(proposal, step):
         with context(proposal, step, M.FLAG_TRACE, sweeps + block) as ctx:
             if proposal == M.PROPOSAL_LEARNED:
                 mlp.attach(ctx)
             else:
-                autotune.apply(ctx, [sj], autotune.tune([sj], lambda s: walkers, candidates=(4, 8, 16),
-                                                          sweeps=2, precision=precision, lu_period=lu_period)[0])
+                autotune.apply(ctx, [sj], box_lanes)
             ctx.set_configs(start)
             ctx.sweep(equil // 2)  # the proposal's own equilibration
             ctx.synchronize()
@@ -204,17 +206,16 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
             wall = time.perf_counter() - t0
             dev_ms, _ = ctx.last_timing()
             st = ctx.stats()
-            _, el = ctx.trace(sweeps + block, sweeps)
+            tau = ctx.tau_int(sweeps)  # on the device, from the trace
             lanes = ctx.walker_lanes()
         t = dev_ms * 1e-3 * sweeps / (sweeps + block)  # device time of the sampled sweeps
         e, err = energy_and_error(st)
-        tau = tau_int(el)
         tau_mean = float(np.mean(tau))
         return dict(proposal={M.PROPOSAL_BOX: "box", M.PROPOSAL_LEARNED: "learned"}[proposal], step=step,
                     lanes=lanes, energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
                     tau_int_mean=tau_mean, tau_int_median=float(np.median(tau)), seconds=t, wall=wall,
                     moves_per_s=walkers * sweeps * N / t, ess_per_s=walkers * sweeps / (tau_mean * t),
-                    kappa=1.0 / (err * err * t))
+                    kappa=1.0 / (err * err * t), max_condition=st["max_condition"])
 
     runs = [measure(M.PROPOSAL_BOX, s) for s in box_sides]
     for r in runs:
@@ -231,8 +232,10 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
                              objective_first=float(np.mean(obj[:20])), objective_last=float(np.mean(obj[-20:])),
                              collect_seconds=t_collect, train_seconds=t_train),
                box=runs, learned=lrn,
+               best_box_side_ess=best_ess["step"], best_box_side_kappa=best_kappa["step"],
                ess_ratio_vs_best_box=lrn["ess_per_s"] / best_ess["ess_per_s"],
                kappa_ratio_vs_best_box=lrn["kappa"] / best_kappa["kappa"],
+               tau_ratio_vs_best_box=best_ess["tau_int_mean"] / lrn["tau_int_mean"],
                energy_diff_vs_box=diff,
                energy_diff_sigma=diff / float(np.hypot(lrn["error"], best_kappa["error"])))
     log(rec)

@@ -148,3 +148,62 @@ def test_training_raises_the_objective():
         obj = m.train(ctx, 150, batch=2048, lr=2e-3, seed=5)
     assert np.all(np.isfinite(obj))
     assert obj[-20:].mean() > 1.2 * obj[:10].mean(), (obj[:10].mean(), obj[-20:].mean())
+
+
+def _adam_f32(w, g_sum, m, v, step, batch, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8):
+    """One Adam step in float32, the operations of the device step: the loss
+    is minus the batch mean of the objective, step counts from 1."""
+    f = np.float32
+    w, m, v = w.astype(np.float32), m.astype(np.float32), v.astype(np.float32)
+    gr = -g_sum.astype(np.float32) * (f(1.0) / f(batch))
+    b1, b2, lr, eps = f(b1), f(b2), f(lr), f(eps)
+    m = b1 * m + (f(1.0) - b1) * gr
+    v = b2 * v + (f(1.0) - b2) * gr * gr
+    mh = m / (f(1.0) - b1 ** f(step))
+    vh = v / (f(1.0) - b2 ** f(step))
+    return w - lr * mh / (np.sqrt(vh) + eps), m, v
+
+
+def test_adam_steps_match_float32_restatement():
+    # two device training steps against Adam restated in numpy float32 from
+    # the device's own gradients of the same samples (step k uses samples
+    # [k batch, (k+1) batch)): bias correction, the betas and the moments
+    # carried between steps all enter; the repacked bf16 network follows
+    W, batch, seed = 256, 512, 21
+    rng = np.random.default_rng(3)
+    cfg = rng.normal(scale=0.9, size=(32, 30))
+    el = rng.integers(0, 10, 32).astype(np.int32)
+    with _ne_context(W, seed=2) as ctx, M.Mlp(seed=2) as m:
+        ctx.init_walkers()
+        ctx.sweep(10)
+        m.collect(ctx, 2, 3)
+        w0 = _random_weights(4)
+        m.set_weights(w0)
+        out0 = m.eval(cfg, [0.0, 0.0, 0.0], el)
+        g0, _ = m.gradient(ctx, batch, first=0, seed=seed)
+        m.train(ctx, 1, batch=batch, lr=1e-3, seed=seed)
+        w1_dev = m.weights()
+        g1, _ = m.gradient(ctx, batch, first=batch, seed=seed)
+        m.train(ctx, 1, batch=batch, lr=1e-3, seed=seed)
+        w2_dev = m.weights()
+        out2 = m.eval(cfg, [0.0, 0.0, 0.0], el)
+        with M.Mlp(seed=2) as fresh:
+            fresh.set_weights(w2_dev)
+            out2_fresh = fresh.eval(cfg, [0.0, 0.0, 0.0], el)
+    z = np.zeros_like(w0, dtype=np.float32)
+    w1, m1, v1 = _adam_f32(w0, g0, z, z, 1, batch)
+    w2, _, _ = _adam_f32(w1_dev, g1, m1, v1, 2, batch)
+    # the first Adam step moves every parameter whose gradient is well above
+    # eps by lr (m/sqrt(v) = sign(g) at step 1)
+    moved = np.abs(g0) / batch > 1e-5
+    assert moved.sum() > 0.5 * w0.size
+    assert np.allclose(np.abs(w1_dev - w0)[moved], 1e-3, rtol=1e-2)
+    assert np.max(np.abs(w1_dev - w1)) <= 1e-6 * 1e-3 + 4 * np.spacing(np.abs(w0).astype(np.float32)).max()
+    assert np.max(np.abs(w2_dev - w2)) <= 1e-6 * 1e-3 + 4 * np.spacing(np.abs(w1_dev)).max()
+    # a wrong bias correction (none at step 2) would miss by far more
+    w2_wrong = w1_dev - np.float32(1e-3) * (np.float32(0.9) * m1 + np.float32(0.1) * (-g1 / batch)) / (
+        np.sqrt(np.float32(0.999) * v1 + np.float32(0.001) * (g1 / batch) ** 2) + np.float32(1e-8))
+    assert np.max(np.abs(w2_dev - w2_wrong)) > 100 * np.max(np.abs(w2_dev - w2))
+    # the bf16 fragments were rebuilt from the updated master weights
+    assert not np.array_equal(out2, out0)
+    assert np.array_equal(out2, out2_fresh)

@@ -81,3 +81,40 @@ def test_blocks_need_a_trace():
         with pytest.raises(M.MoleError) as ei:
             ctx.read_blocks()
         assert ei.value.status == M.ERR_DATA_ACCESS
+
+
+# ---- integrated autocorrelation times on the device (scripts/statfor.rs:30-54)
+
+def test_tau_int_on_device_matches_oracle_fold():
+    # every walker's tau from the trace on the device, against the oracle's
+    # sequential fold over the same series (bit for bit) and the numpy FFT
+    # estimator; a window of the series as well; short series (n - 1 < 200
+    # lags) and the first-negative cut are both exercised
+    from mole_amd.stats import tau_int
+    from oracle import oracle_ffi as O
+
+    W, steps, B = 96, 260, 10
+    sj = S.atom(4)
+    with M.Context([M.slater_jastrow(sj, W)], seed=3, step=0.6, block_size=B, flags=M.FLAG_TRACE,
+                   record_sweeps=steps, lu_period=16) as ctx:
+        ctx.init_walkers()
+        ctx.run(steps)
+        n = steps - B
+        _, el = ctx.trace(n, n)
+        tau = ctx.tau_int(n)
+        tau_w = ctx.tau_int(40, first=100)
+        with pytest.raises(M.MoleError):
+            ctx.tau_int(n + 1)
+        with pytest.raises(M.MoleError):
+            ctx.tau_int(1)
+    L = O.lib()
+    want = np.array([L.orc_tau_corr(O.dptr(np.ascontiguousarray(el[w])), n) for w in range(W)])
+    want_w = np.array([L.orc_tau_corr(O.dptr(np.ascontiguousarray(el[w, 100:140])), 40) for w in range(W)])
+    assert np.array_equal(bits(tau), bits(want))
+    assert np.array_equal(bits(tau_w), bits(want_w))
+    assert np.allclose(tau, tau_int(el), rtol=1e-9)
+    assert np.all(tau >= 1.0) and np.any(tau > 1.5)
+
+
+def bits(a):
+    return np.asarray(a, dtype=np.float64).view(np.uint64)
