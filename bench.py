@@ -50,7 +50,7 @@ def flops_per_launch(walkers, sweeps, sampled):
     return walkers * (sweeps * N_ELEC * FLOP_MOVE + sampled * FLOP_SAMPLE)
 
 
-VARIANTS = {"auto": 0, "lane": 1, "ws": 2}
+VARIANTS = {"auto": 0, "lane": 1, "ws": 2, "quad": 3}
 
 # HBM bytes per sweep launch of the timed optimisation loop, as measured by
 # rocprofv3 (FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0, the factors a plain copy
@@ -409,10 +409,10 @@ def main():
     # vector ALUs (MI355X FP64 vector peak 78.6 TFLOP/s)
     roofline = dict(bound="fp64-valu", achieved=achieved, peak=PEAK_FP64_TFLOPS, unit="TFLOP/s",
                     frac=achieved / PEAK_FP64_TFLOPS,
-                    traffic=None if args.variant == "lane" else TRAFFIC_BYTES_PER_LAUNCH,
-                    traffic_source=None if args.variant == "lane" else TRAFFIC_SOURCE,
-                    kernel=("sweep2_lane_kernel<LCAO2>" if args.variant == "lane"
-                            else "sweep2_ws_kernel<LCAO2,2,6>"),
+                    traffic=TRAFFIC_BYTES_PER_LAUNCH if args.variant in ("auto", "ws") else None,
+                    traffic_source=TRAFFIC_SOURCE if args.variant in ("auto", "ws") else None,
+                    kernel={"lane": "sweep2_lane_kernel<LCAO2>",
+                            "quad": "sweep2_quad_kernel<LCAO2,2>"}.get(args.variant, "sweep2_ws_kernel<LCAO2,2,6>"),
                     kernel_ms=1e3 * avg_kernel_s,
                     flop_per_launch=fl)
     cpu = None

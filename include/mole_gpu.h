@@ -280,6 +280,7 @@ mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_blo
  * every sum in sample order, as the reference folds it.  tau[n_walkers].
  * Effective samples = sum_w n / tau_w.  Needs MOLE_FLAG_TRACE, n >= 2. */
 mole_status mole_tau_int(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau, size_t n_walkers);
+
 /* MonteCarloSampler::data (src/montecarlo/src/traits.rs:27) in compat
  * form: "Energy", "Wavefunction value" and "Parameter gradient" samples as
  * [walker][sample] (and [walker][sample][param]), the order in which
@@ -494,13 +495,18 @@ mole_status mole_dmc_read_walkers(mole_ctx *ctx, uint32_t sys, double *weights, 
 
 /* ---- launch configuration ----------------------------------------------
  * No counterpart in the reference (rayon's fan-out, src/vmc/src/vmc.rs:63-76,
- * is the closest).  Selects the sweep kernel: AUTO picks the
- * wave-specialised kernel where it applies; LANE runs one lane per walker
- * with block_threads threads per workgroup (0 keeps the current value).
- * Every variant produces bit-identical walker trajectories. */
+ * is the closest).  Selects the two-electron sweep kernel: AUTO picks
+ * WAVESPEC where it applies (He, or LCAO with at most 2 nuclei); WAVESPEC
+ * gives each 64 walkers a workgroup of four specialised waves (chain, two
+ * draw producers, sampler); QUAD spreads each walker over four lanes of a
+ * wave (same eligibility);
+ * LANE runs one lane per walker with block_threads threads per workgroup
+ * (0 keeps the current value).  Every variant produces bit-identical walker
+ * trajectories. */
 #define MOLE_VARIANT_AUTO 0
 #define MOLE_VARIANT_LANE 1
 #define MOLE_VARIANT_WAVESPEC 2
+#define MOLE_VARIANT_QUAD 3
 mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t block_threads);
 /* Slater-Jastrow: lanes of a wave that share one walker of system sys
  * (power of two, 1..64; 0 = the library's choice).  Results do not depend
@@ -511,6 +517,17 @@ mole_status mole_get_walker_lanes(mole_ctx *ctx, uint32_t sys, uint32_t *lanes);
  * launches (one launch per LDS footprint class of the tiles) and the
  * largest dynamic LDS bytes per workgroup among them. */
 mole_status mole_get_launch_info(mole_ctx *ctx, uint32_t *workgroups, size_t *lds_bytes);
+/* Slater-Jastrow tile schedule (no counterpart in the reference).  CLASSES:
+ * one launch per LDS footprint class (4 KiB steps) with that class's dynamic
+ * LDS, longest tiles first within a class.  PERSISTENT (box proposal, plain
+ * kernel; other launches keep CLASSES): one launch of `workgroups` resident
+ * workgroups (0: CUs x the occupancy the largest footprint allows) that take
+ * every tile of the context, longest first, from a device ticket -- the
+ * footprint classes overlap instead of draining one after another, at the
+ * largest footprint's LDS.  Results do not depend on the schedule. */
+#define MOLE_SCHEDULE_CLASSES 0
+#define MOLE_SCHEDULE_PERSISTENT 1
+mole_status mole_set_tile_schedule(mole_ctx *ctx, uint32_t schedule, uint32_t workgroups);
 
 /* ---- captured sweeps ---------------------------------------------------
  * No counterpart in the reference (its sweeps are a host loop,
@@ -546,11 +563,21 @@ mole_status mole_last_timing(mole_ctx *ctx, double *ms, uint32_t *launches);
 #define MOLE_MLP_PARAMS 9476
 typedef struct mole_mlp mole_mlp;
 
+/* Training objectives: the expected squared jump distance times acceptance,
+ * mean of A |x' - x_i|^2 (ESJD), or the same jump measured against the
+ * electron's distance to its nearest nucleus, mean of
+ * A |x' - x_i|^2 / (d^2 + 0.01) with d the larger of that distance at x_i
+ * and at x' (ESJD_RELATIVE: core and valence electrons weigh alike instead
+ * of the valence electrons' large jumps dominating). */
+#define MOLE_MLP_OBJ_ESJD 0
+#define MOLE_MLP_OBJ_ESJD_RELATIVE 1
+
 typedef struct {
   uint32_t steps;  /* Adam steps */
   uint32_t batch;  /* samples per step, a multiple of 16 */
   float lr, beta1, beta2, eps;
   uint64_t seed;   /* Philox key of the training proposals' normals */
+  uint32_t objective; /* MOLE_MLP_OBJ_* */
 } mole_train_desc;
 
 /* Xavier-uniform weights from the Philox stream of seed, zero biases. */

@@ -44,6 +44,8 @@ SWEEP_SAMPLE = 1
 VARIANT_AUTO = 0
 VARIANT_LANE = 1
 VARIANT_WAVESPEC = 2
+VARIANT_QUAD = 3
+SCHEDULE_CLASSES, SCHEDULE_PERSISTENT = 0, 1
 MAX_PARAMS = 16
 
 BASIS_STO = 0
@@ -90,7 +92,10 @@ class MoleError(RuntimeError):
 
 class TrainDesc(C.Structure):
     _fields_ = [("steps", C.c_uint32), ("batch", C.c_uint32), ("lr", C.c_float), ("beta1", C.c_float),
-                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64)]
+                ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("objective", C.c_uint32)]
+
+
+MLP_OBJ_ESJD, MLP_OBJ_ESJD_RELATIVE = 0, 1
 
 
 class SystemDesc(C.Structure):
@@ -252,6 +257,7 @@ _SIGS = [
     ("mole_set_walker_lanes", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_get_walker_lanes", C.c_int32, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
     ("mole_get_launch_info", C.c_int32, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_size_t)]),
+    ("mole_set_tile_schedule", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_graph_capture", C.c_int32, [_P, C.c_uint32, C.c_uint32, C.POINTER(_P)]),
     ("mole_graph_launch", C.c_int32, [_P, _P, C.c_uint32]),
     ("mole_graph_destroy", None, [_P]),
@@ -642,6 +648,10 @@ class Context:
         check(lib().mole_get_launch_info(self.h, C.byref(wg), C.byref(lds)))
         return wg.value, lds.value
 
+
+    def set_tile_schedule(self, schedule, workgroups=0):
+        """mole_set_tile_schedule: SCHEDULE_CLASSES or SCHEDULE_PERSISTENT."""
+        check(lib().mole_set_tile_schedule(self.h, schedule, workgroups))
     def capture(self, n_sweeps, sample=False):
         """HIP graph of one launch of n_sweeps sweeps (see mole_graph_capture)."""
         g = C.c_void_p()
@@ -745,8 +755,10 @@ class Mlp:
         check(lib().mole_mlp_data_size(self.h, C.byref(n)))
         return n.value
 
-    def train(self, ctx, steps, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=11, sys=0):
-        d = TrainDesc(steps=steps, batch=batch, lr=lr, beta1=beta1, beta2=beta2, eps=eps, seed=seed)
+    def train(self, ctx, steps, batch=4096, lr=1e-3, beta1=0.9, beta2=0.999, eps=1e-8, seed=11, sys=0,
+              objective=MLP_OBJ_ESJD):
+        d = TrainDesc(steps=steps, batch=batch, lr=lr, beta1=beta1, beta2=beta2, eps=eps, seed=seed,
+                      objective=objective)
         obj = np.zeros(max(1, steps))
         check(lib().mole_mlp_train(self.h, ctx.h, sys, C.byref(d), _dp(obj)))
         return obj[:steps]

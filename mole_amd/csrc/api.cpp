@@ -147,6 +147,14 @@ struct mole_ctx {
     size_t lds;
   };
   std::vector<TileGroup> tile_groups;
+  // persistent schedule: every tile in one launch, longest first, taken from
+  // a device ticket by `persist_wg` resident workgroups (0: CUs x occupancy)
+  int schedule = MOLE_SCHEDULE_CLASSES;
+  uint32_t persist_wg_req = 0;
+  int persist_wg = 0;
+  SJTile *d_tiles_lpt = nullptr;
+  uint32_t *d_ticket = nullptr;
+  int cu_count = 0;
   bool sysdev_dirty = true;
   mole_mlp *mlp = nullptr;  // network of the learned proposal (not owned)
   // device-side optimisation: parameters read by the kernels from d_params
@@ -315,6 +323,8 @@ void free_ctx(mole_ctx *ctx) {
   if (ctx->d_sysdev) (void)hipFree(ctx->d_sysdev);
   if (ctx->d_state) (void)hipFree(ctx->d_state);
   if (ctx->d_tiles) (void)hipFree(ctx->d_tiles);
+  if (ctx->d_tiles_lpt) (void)hipFree(ctx->d_tiles_lpt);
+  if (ctx->d_ticket) (void)hipFree(ctx->d_ticket);
   if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
   if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
   if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -374,7 +384,27 @@ mole_status sj_build_tiles(mole_ctx *ctx) {
   HIP_TRY(hipMemcpy(ctx->d_tiles, h.data(), sizeof(SJTile) * h.size(), hipMemcpyHostToDevice));
   ctx->n_tiles = (int)h.size();
   ctx->lds_bytes = lds;
+  // the persistent schedule's order: all tiles by cost, longest first
+  std::stable_sort(tiles.begin(), tiles.end(), [](const T &a, const T &b) { return a.cost > b.cost; });
+  for (size_t i = 0; i < tiles.size(); ++i) h[i] = tiles[i].t;
+  if (ctx->d_tiles_lpt) (void)hipFree(ctx->d_tiles_lpt);
+  ctx->d_tiles_lpt = nullptr;
+  HIP_TRY(hipMalloc(&ctx->d_tiles_lpt, sizeof(SJTile) * h.size()));
+  HIP_TRY(hipMemcpy(ctx->d_tiles_lpt, h.data(), sizeof(SJTile) * h.size(), hipMemcpyHostToDevice));
+  if (!ctx->d_ticket) {
+    HIP_TRY(hipMalloc(&ctx->d_ticket, sizeof(uint32_t)));
+    HIP_TRY(hipMemset(ctx->d_ticket, 0, sizeof(uint32_t)));
+  }
+  const int occ = sj_persistent_occupancy(sj_tsize(ctx) == 4, lds);
+  const int fill = ctx->cu_count * (occ > 0 ? occ : 1);
+  ctx->persist_wg = (int)std::min<int64_t>(ctx->n_tiles, ctx->persist_wg_req ? (int64_t)ctx->persist_wg_req : fill);
   return MOLE_OK;
+}
+
+// The persistent schedule applies to plain box-proposal launches.
+bool sj_persistent_eligible(const mole_ctx *ctx, const SJArgs &a) {
+  return ctx->schedule == MOLE_SCHEDULE_PERSISTENT && ctx->run.proposal == MOLE_PROPOSAL_BOX && !a.ext &&
+         a.dmc == kSJDmcOff;
 }
 
 mole_status sj_upload_sysdev(mole_ctx *ctx) {
@@ -428,9 +458,16 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
   return MOLE_OK;
 }
 
-// One launch per tile group, each with its group's dynamic LDS.
+// One launch per tile group, each with its group's dynamic LDS; or, on the
+// persistent schedule, one launch over every tile.
 hipError_t sj_launch_groups(const mole_ctx *ctx, SJArgs a) {
   const int fp32 = ctx->run.precision == MOLE_PREC_MIXED;
+  if (sj_persistent_eligible(ctx, a)) {
+    a.tiles = ctx->d_tiles_lpt;
+    a.n_tiles = ctx->n_tiles;
+    a.ticket = ctx->d_ticket;
+    return launch_sj_persistent(a, ctx->persist_wg, fp32, ctx->lds_bytes, ctx->stream);
+  }
   for (const mole_ctx::TileGroup &g : ctx->tile_groups) {
     a.tiles = ctx->d_tiles + g.first;
     const hipError_t e = launch_sj(a, g.count, fp32, sj_prop(ctx), g.lds, ctx->stream);
@@ -615,6 +652,22 @@ mole_status reduce_sys(mole_ctx *ctx, uint32_t sys, const double **cols) {
   return MOLE_OK;
 }
 
+// Arguments of the tau entry points; *out the system.
+mole_status tau_check(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, const double *tau, size_t n_walkers,
+                      SysState **out) {
+  mole_status st = check_ctx(ctx, sys);
+  if (st != MOLE_OK) return st;
+  if (!tau) return fail(MOLE_ERR_INVALID_ARGUMENT, "null tau");
+  if (!(ctx->run.flags & MOLE_FLAG_TRACE)) return fail(MOLE_ERR_DATA_ACCESS, "context has no trace");
+  SysState &s = ctx->sys[sys];
+  if (n_walkers != (size_t)s.W) return fail(MOLE_ERR_SHAPE, "expected %d walkers", s.W);
+  const size_t rec = std::min((size_t)ctx->trace_samples, (size_t)ctx->run.record_sweeps);
+  if (n < 2 || first > rec || n > rec - first)
+    return fail(MOLE_ERR_SHAPE, "samples [%zu, %zu) outside the %zu recorded (n >= 2)", first, first + n, rec);
+  *out = &s;
+  return MOLE_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -701,6 +754,7 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
     // opt-in LDS per workgroup (160 KiB on gfx950), granted to the
     // Slater-Jastrow kernel through hipFuncAttributeMaxDynamicSharedMemorySize
     ctx->lds_limit = std::max<size_t>(prop.sharedMemPerBlock, prop.sharedMemPerBlockOptin);
+    ctx->cu_count = prop.multiProcessorCount;
     if (ctx->lds_limit == 0) ctx->lds_limit = 65536;
   }
   CTX_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
@@ -1067,15 +1121,10 @@ mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_blo
 }
 
 mole_status mole_tau_int(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau, size_t n_walkers) {
-  mole_status st = check_ctx(ctx, sys);
+  SysState *sp = nullptr;
+  mole_status st = tau_check(ctx, sys, first, n, tau, n_walkers, &sp);
   if (st != MOLE_OK) return st;
-  if (!tau) return fail(MOLE_ERR_INVALID_ARGUMENT, "null tau");
-  if (!(ctx->run.flags & MOLE_FLAG_TRACE)) return fail(MOLE_ERR_DATA_ACCESS, "context has no trace");
-  SysState &s = ctx->sys[sys];
-  if (n_walkers != (size_t)s.W) return fail(MOLE_ERR_SHAPE, "expected %d walkers", s.W);
-  const size_t rec = std::min((size_t)ctx->trace_samples, (size_t)ctx->run.record_sweeps);
-  if (n < 2 || first > rec || n > rec - first)
-    return fail(MOLE_ERR_SHAPE, "samples [%zu, %zu) outside the %zu recorded (n >= 2)", first, first + n, rec);
+  SysState &s = *sp;
   HIP_TRY(hipSetDevice(ctx->device));
   double *d = nullptr;
   HIP_TRY(hipMalloc(&d, sizeof(double) * (size_t)s.W));
@@ -1243,13 +1292,13 @@ mole_status mole_optimizer_update(mole_optimizer *opt, const mole_sr_sums *sums,
 mole_status mole_set_launch_config(mole_ctx *ctx, uint32_t variant, uint32_t block_threads) {
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
-  if (variant > MOLE_VARIANT_WAVESPEC) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown variant %u", variant);
+  if (variant > MOLE_VARIANT_QUAD) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown variant %u", variant);
   if (block_threads != 0 && (block_threads % 64 != 0 || block_threads > 256))
     return fail(MOLE_ERR_INVALID_ARGUMENT, "block_threads must be 0 or a multiple of 64 up to 256");
-  if (ctx->family == kFamTwo && variant == MOLE_VARIANT_WAVESPEC)
+  if (ctx->family == kFamTwo && (variant == MOLE_VARIANT_WAVESPEC || variant == MOLE_VARIANT_QUAD))
     for (const SysState &s : ctx->sys)
       if (!sweep2_wavespec_eligible(s.sp))
-        return fail(MOLE_ERR_UNSUPPORTED, "wave-specialised kernel needs <= 2 nuclei (1 for helium)");
+        return fail(MOLE_ERR_UNSUPPORTED, "the wave-specialised and quad kernels need <= 2 nuclei (1 for helium)");
   ctx->variant = (int)variant;
   if (block_threads) ctx->block_threads = (int)block_threads;
   return ok();
@@ -1367,8 +1416,24 @@ mole_status mole_get_launch_info(mole_ctx *ctx, uint32_t *workgroups, size_t *ld
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
   if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "launch shape applies to Slater-Jastrow contexts");
-  if (workgroups) *workgroups = (uint32_t)ctx->n_tiles;
+  const bool persist = ctx->schedule == MOLE_SCHEDULE_PERSISTENT && ctx->run.proposal == MOLE_PROPOSAL_BOX;
+  if (workgroups) *workgroups = (uint32_t)(persist ? ctx->persist_wg : ctx->n_tiles);
   if (lds_bytes) *lds_bytes = ctx->lds_bytes;
+  return ok();
+}
+
+mole_status mole_set_tile_schedule(mole_ctx *ctx, uint32_t schedule, uint32_t workgroups) {
+  mole_status st = check_ctx(ctx, 0);
+  if (st != MOLE_OK) return st;
+  if (ctx->family != kFamSJ) return fail(MOLE_ERR_UNSUPPORTED, "tile schedules apply to Slater-Jastrow contexts");
+  if (schedule > MOLE_SCHEDULE_PERSISTENT) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown schedule %u", schedule);
+  HIP_TRY(hipSetDevice(ctx->device));
+  HIP_TRY(hipStreamSynchronize(ctx->stream));
+  ctx->schedule = (int)schedule;
+  ctx->persist_wg_req = workgroups;
+  st = sj_build_tiles(ctx);
+  if (st != MOLE_OK) return st;
+  ++ctx->generation;
   return ok();
 }
 
@@ -1564,7 +1629,8 @@ namespace {
 // One gradient evaluation of `batch` samples starting at global sample
 // `first` into m->d_grad and *obj_dev (device); returns after enqueueing.
 mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
-                                 uint64_t seed, double *obj_dev, const MlpAdamStep *adam = nullptr) {
+                                 uint64_t seed, double *obj_dev, const MlpAdamStep *adam = nullptr,
+                                 uint32_t objective = MOLE_MLP_OBJ_ESJD) {
   const int n_mb = (int)(batch / 16);
   const int G = std::min(n_mb, 1024);
   if (G > m->partial_groups) {
@@ -1586,6 +1652,7 @@ mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint3
   a.seed = seed;
   a.partial = m->d_partial;
   a.obj = m->d_obj;
+  a.relative = objective == MOLE_MLP_OBJ_ESJD_RELATIVE ? 1 : 0;
   HIP_TRY(launch_mlp_train(a, G, m->stream));
   HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_grad, obj_dev, adam, m->stream));
   return MOLE_OK;
@@ -1638,6 +1705,8 @@ mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_
   if (!(d->lr > 0.0f) || !(d->beta1 >= 0.0f && d->beta1 < 1.0f) || !(d->beta2 >= 0.0f && d->beta2 < 1.0f) ||
       !(d->eps > 0.0f))
     return fail(MOLE_ERR_INVALID_ARGUMENT, "Adam needs lr > 0, 0 <= beta < 1, eps > 0");
+  if (d->objective > MOLE_MLP_OBJ_ESJD_RELATIVE)
+    return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown training objective %u", d->objective);
   if (d->steps == 0) return ok();
   if ((int)d->steps > m->obj_steps_cap) {
     if (m->d_obj_steps) (void)hipFree(m->d_obj_steps);
@@ -1650,7 +1719,7 @@ mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_
     // the gradient reduction applies the Adam step (step counts from 1)
     const MlpAdamStep adam{m->d_w, m->d_m, m->d_v, 1.0f / (float)d->batch, d->lr, d->beta1, d->beta2, d->eps,
                            (int)m->adam_step + 1};
-    st = mlp_gradient_enqueue(m, ctx, sys, d->batch, first, d->seed, m->d_obj_steps + k, &adam);
+    st = mlp_gradient_enqueue(m, ctx, sys, d->batch, first, d->seed, m->d_obj_steps + k, &adam, d->objective);
     if (st != MOLE_OK) return st;
     ++m->adam_step;
     HIP_TRY(launch_mlp_pack(m->d_w, m->d_fwd1, m->d_fwd, m->d_bwd, m->d_bias, m->stream));

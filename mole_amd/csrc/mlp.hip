@@ -319,6 +319,8 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
     // forward at x
     float fv[4];
     mlp_features(cfg, N, S.nuc, S.n_nuc, i, r, q, fv);
+    // nearest-nucleus distance at x (feature 0, held by the lanes q == 0)
+    const double dn = (double)fv[0];
     const bf16x4 x1 = features_bf16(fv);
     f32x4 h[12];
     const f32x4 o1 = mlp_forward(a.net, x1, h);
@@ -355,6 +357,11 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
     const f32x4 o2 = mlp_forward(a.net, x2, h);
     f32x4 dO1 = {0.f, 0.f, 0.f, 0.f}, dO2 = {0.f, 0.f, 0.f, 0.f};
     if (q == 0) {
+      // the relative objective's weight 1/(d^2 + d0^2), d the larger of the
+      // nearest-nucleus distances at x and x' (a jump is measured on the
+      // scale of its wider end); a constant of the gradient
+      const double dm = dn > (double)fv[0] ? dn : (double)fv[0];
+      const double wgt = a.relative ? 1.0 / (dm * dm + kMlpRelD0 * kMlpRelD0) : 1.0;
       const double ls = sp[9], sg = sp[10], ls2 = mlp_log_sigma((double)o2[3]), sg2 = exp_m(ls2);
       double e[3], dx[3], z[3], gx[3] = {sp[13], sp[14], sp[15]};
       double e2 = 0.0, z2 = 0.0, d2 = 0.0;
@@ -370,11 +377,13 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
       const double lq_fwd = -3.0 * ls - 0.5 * z2;
       const double logr = sp[12] + lq_rev - lq_fwd;
       const double A = logr < 0.0 ? exp_m(logr) : 1.0;
-      obj_acc += A * d2;
-      const double Aact = logr < 0.0 ? A : 0.0;
+      obj_acc += wgt * (A * d2);
+      // the weight is a constant of the gradient: scale every output's
+      // derivative by it
+      const double Aact = (logr < 0.0 ? A : 0.0) * wgt;
       const double is2 = 1.0 / (sg2 * sg2);
       double GD[3];
-      for (int k = 0; k < 3; ++k) GD[k] = 2.0 * A * dx[k] + d2 * Aact * (2.0 * gx[k] + e[k] * is2);
+      for (int k = 0; k < 3; ++k) GD[k] = 2.0 * (A * wgt) * dx[k] + d2 * Aact * (2.0 * gx[k] + e[k] * is2);
       const double dls = (GD[0] * z[0] + GD[1] * z[1] + GD[2] * z[2]) * sg + d2 * Aact * 3.0;
       const double o13 = sp[11], o23 = (double)o2[3];
       for (int k = 0; k < 3; ++k) {

@@ -18,8 +18,12 @@ struct MlpTrainArgs {
   uint32_t batch;       // samples, a multiple of 16
   uint64_t seed;        // Philox key of the proposal normals
   float *partial;       // [gridDim][kMlpParams], zero on entry
-  double *obj;          // [gridDim] sum of A |dx|^2, zero on entry
+  double *obj;          // [gridDim] sum of the objective, zero on entry
+  int relative;         // 1: each sample's A |dx|^2 divided by d^2 + kMlpRelD0^2
+                        // (d the larger nearest-nucleus distance of x and x')
 };
+// length scale below which the relative objective stops growing (bohr)
+constexpr double kMlpRelD0 = 0.1;
 
 // Philox system tag of the training normals: sample g draws 0..3 of the
 // stream (seed, kMlpTrainSys, lo32(g), hi32(g)).

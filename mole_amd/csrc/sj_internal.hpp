@@ -127,6 +127,8 @@ struct SJArgs {
   double dmc_tau;          // DMC: the weights' time step
   int ext;                 // 1: DMC, local operators or Gaussian bases in use (the extended kernel)
   int one_wave;            // learned proposal: the kernel fitted to one wave per SIMD
+  uint32_t *ticket;        // persistent launch: next tile index (zero on entry)
+  int n_tiles;             // persistent launch: tiles in `tiles`
 };
 
 size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);
@@ -136,6 +138,11 @@ size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);
 // s of a launch is sampled when s >= sample_start.
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_t lds_bytes,
                      hipStream_t stream);
+// One persistent launch of the box-proposal plain kernel over a.n_tiles tiles
+// (a.tiles in the order they are taken), then the ticket's reset.
+hipError_t launch_sj_persistent(const SJArgs &a, int workgroups, int fp32, size_t lds_bytes, hipStream_t stream);
+// Resident workgroups of the persistent kernel per CU at lds_bytes of LDS.
+int sj_persistent_occupancy(int fp32, size_t lds_bytes);
 // state->sweeps_done += n_sweeps; block fill and trace cursors advance by the
 // launch's samples.
 hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n_samples,

@@ -641,11 +641,11 @@ __device__ __forceinline__ void drift_sum(const double *og, const T *B, int n, i
 // MINW: minimum waves per SIMD the register budget is fitted to (2: up to
 // 256 registers; 1: 512, for the learned proposal's larger systems, where
 // spills cost more than the lost occupancy).
-template <typename T, int PROP, bool EXT, int MINW>
-__global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
+// One tile's launch: its walkers' state into the workgroup's LDS, the sweeps,
+// the state out.
+template <typename T, int PROP, bool EXT>
+__device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsigned char *lds) {
   constexpr int DPM = sj_draws_per_move(PROP);
-  extern __shared__ __align__(16) unsigned char lds[];
-  const SJTile tile = a.tiles[blockIdx.x];
   const SJSysDev &S = a.systems[tile.sys];
   const int N = S.N, nu = S.n_up, nd = S.n_dn, L = S.L, W = S.W;
   const int lane = threadIdx.x;
@@ -1029,6 +1029,34 @@ __global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
   }
 }
 
+// One workgroup per tile (the tiles of one LDS footprint class).
+template <typename T, int PROP, bool EXT, int MINW>
+__global__ void __launch_bounds__(64, MINW) sj_sweep_kernel(SJArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  sj_tile<T, PROP, EXT>(a, a.tiles[blockIdx.x], lds);
+}
+
+// Persistent: a grid sized to the resident workgroups takes every tile of the
+// context, longest first, each workgroup drawing the next tile index from a
+// device ticket until the list is exhausted (every wave reaches the exit).
+// The ticket is zero on entry; sj_ticket_reset_kernel zeroes it after.
+template <typename T>
+__global__ void __launch_bounds__(64, 2) sj_persistent_kernel(SJArgs a) {
+  extern __shared__ __align__(16) unsigned char lds[];
+  for (;;) {
+    int t = 0;
+    if (threadIdx.x == 0) t = (int)atomicAdd(a.ticket, 1u);
+    t = __shfl(t, 0);
+    if (t >= a.n_tiles) break;
+    sj_tile<T, kSJPropBox, false>(a, a.tiles[t], lds);
+    __syncthreads();  // the next tile reuses the LDS
+  }
+}
+
+__global__ void sj_ticket_reset_kernel(uint32_t *ticket) {
+  if (threadIdx.x == 0) *ticket = 0u;
+}
+
 __global__ void sj_advance_kernel(SJLaunchState *st, uint32_t n_sweeps, uint32_t n_samples,
                                   uint32_t block_size) {
   if (threadIdx.x != 0) return;
@@ -1077,6 +1105,34 @@ template <typename T, int PROP>
 static hipError_t launch_sj_t(const SJArgs &a, int n_tiles, size_t lds_bytes, hipStream_t stream) {
   return a.ext ? launch_sj_e<T, PROP, true>(a, n_tiles, lds_bytes, stream)
                : launch_sj_e<T, PROP, false>(a, n_tiles, lds_bytes, stream);
+}
+
+template <typename T>
+static hipError_t launch_sj_p(const SJArgs &a, int workgroups, size_t lds_bytes, hipStream_t stream) {
+  if (lds_bytes > 65536) {
+    const hipError_t e = hipFuncSetAttribute((const void *)sj_persistent_kernel<T>,
+                                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes);
+    if (e != hipSuccess) return e;
+  }
+  hipLaunchKernelGGL(sj_persistent_kernel<T>, dim3(workgroups), dim3(64), lds_bytes, stream, a);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(sj_ticket_reset_kernel, dim3(1), dim3(64), 0, stream, a.ticket);
+  return hipGetLastError();
+}
+
+hipError_t launch_sj_persistent(const SJArgs &a, int workgroups, int fp32, size_t lds_bytes, hipStream_t stream) {
+  if (a.n_tiles <= 0 || workgroups <= 0) return hipSuccess;
+  return fp32 ? launch_sj_p<float>(a, workgroups, lds_bytes, stream)
+              : launch_sj_p<double>(a, workgroups, lds_bytes, stream);
+}
+
+int sj_persistent_occupancy(int fp32, size_t lds_bytes) {
+  int n = 0;
+  const hipError_t e =
+      fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sj_persistent_kernel<float>, 64, lds_bytes)
+           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sj_persistent_kernel<double>, 64, lds_bytes);
+  return e == hipSuccess ? n : 0;
 }
 
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_t lds_bytes,

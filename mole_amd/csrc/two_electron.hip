@@ -142,8 +142,10 @@ struct SampleAcc {
     for (int k = 0; k < kAccColumns2; ++k)
       if (k != kAccAccepted) a.acc[(size_t)k * a.W + w] = acc[k];
   }
+  // write: this lane records the sample's trace and compat entries (the
+  // sums are kept either way)
   __device__ __forceinline__ void add(const Sweep2Args &a, int w, double E,
-                                      const double *dlog, double psi, double inv_block) {
+                                      const double *dlog, double psi, double inv_block, bool write = true) {
     acc[kAccSumE] += E;
     acc[kAccSumE2] = fma(E, E, acc[kAccSumE2]);
     acc[kAccBlockSum] += E;
@@ -174,7 +176,7 @@ struct SampleAcc {
       for (int l = k; l < kNParam2; ++l, ++t)
         acc[kAccSumOO + t] = fma(O[k], O[l], acc[kAccSumOO + t]);
     }
-    if (sample_idx < a.cap_samples) {
+    if (write && sample_idx < a.cap_samples) {
       const size_t si = (size_t)w * a.cap_samples + sample_idx;
       const double psi_s = (psi * psi) / psi;
       if (a.trace_e) a.trace_e[si] = E;
@@ -525,6 +527,294 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
   }
 }
 
+// ---- four lanes per walker ---------------------------------------------------
+//
+// A quad of lanes owns one walker; every lane holds the walker's state.  Per
+// sweep, lane q
+//   - draws u64s p+2q and p+2q+1 (one Philox block when p is even): lanes 0-1
+//     hold electron 0's (dx, dy | dz, u), lanes 2-3 electron 1's;
+//   - forms electron q/2's trial position and its distance and orbital
+//     exponential to nucleus q%2 (the electron-local part of the moves);
+//   - forms one coupled part: lane 0 electron 0's (r12', Jastrow / psi'),
+//     lane 1 electron 1's if electron 0's move is rejected, lanes 2-3 if it
+//     is accepted;
+// and the lanes exchange these by DPP quad permutations (register moves, no
+// LDS, no barrier).  The remaining chain -- two ratios and the selects of
+// the state update -- runs on all four lanes alike.  Every value is the
+// same operation on the same operands as in the one-lane chain, so the
+// trajectories are bit for bit the other kernels'.  After a sampled sweep
+// the local energy is split the same way: lane q divides by r_{q/2, q%2},
+// then by phi_0 (lane 0), r12 (lane 1), phi_1 (lane 2), 1 + c r12 (lane 3),
+// with local2_cached's operations in its order.  With 4 lanes per walker,
+// 16,384 walkers occupy all 1,024 SIMDs of the chip.
+
+constexpr int kQuadXor1 = 0xB1;  // quad_perm [1, 0, 3, 2]
+constexpr int kQuadXor2 = 0x4E;  // quad_perm [2, 3, 0, 1]
+constexpr int kQuadLane0 = 0x00, kQuadLane1 = 0x55, kQuadLane2 = 0xAA, kQuadLane3 = 0xFF;
+
+// v of the lane the quad permutation CTRL names (two 32-bit DPP moves)
+template <int CTRL>
+__device__ __forceinline__ double quad_mov(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)b, CTRL, 0xF, 0xF, false);
+  const unsigned hi = (unsigned)__builtin_amdgcn_mov_dpp((int)(unsigned)(b >> 32), CTRL, 0xF, 0xF, false);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// coordinate k of electron 1 (e1) or electron 0
+__device__ __forceinline__ double xe_sel(bool e1, const double *x, int k) { return e1 ? x[3 + k] : x[k]; }
+
+template <int KIND, int NNUC>
+__global__ void __launch_bounds__(64) sweep2_quad_kernel(Sweep2Args a) {
+  static_assert(NNUC == 1 || NNUC == 2, "one lane per (electron, nucleus) pair");
+  const int q = threadIdx.x & 3;
+  const int w = (int)((blockIdx.x * 64u + threadIdx.x) >> 2);
+  const bool live = w < a.W;
+  const bool lead = live && q == 0;  // writes the walker's results
+  const int W = a.W;
+  Sys2 sp = a.sp;
+  if (a.dparams) {  // parameters updated on the device between launches
+    sp.zeta = a.dparams[0];
+    sp.b = a.dparams[1];
+    sp.c = a.dparams[2];
+  }
+  const bool me = (q >> 1) != 0;                 // this lane's electron is 1
+  const bool mi = NNUC == 2 && (q & 1) != 0;     // this lane's nucleus is 1
+  const double *Rq = mi ? sp.nuc[NNUC - 1] : sp.nuc[0];
+  const double inv_block = 1.0 / (double)a.block_size;
+
+  // the walker's state, on every lane of the quad
+  double x[6];
+#pragma unroll
+  for (int k = 0; k < 6; ++k) x[k] = live ? a.pos[(size_t)k * W + w] : 0.0;
+  double r[2][NNUC], ee[2][NNUC], phi[2];
+#pragma unroll
+  for (int e = 0; e < 2; ++e) {
+    phi[e] = 0.0;
+#pragma unroll
+    for (int I = 0; I < NNUC; ++I) {
+      r[e][I] = norm3(x[3 * e] - sp.nuc[I][0], x[3 * e + 1] - sp.nuc[I][1], x[3 * e + 2] - sp.nuc[I][2]);
+      ee[e][I] = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : exp_m(-sp.zeta * r[e][I]);
+      phi[e] = (I == 0) ? ee[e][I] : phi[e] + ee[e][I];
+    }
+  }
+  double r12 = norm3(x[0] - x[3], x[1] - x[4], x[2] - x[5]);
+  double psi = (KIND == MOLE_WF_HELIUM_PADE) ? exp_m(-sp.zeta * (r[0][0] + r[1][0]) + pade(sp.b, sp.c, r12))
+                                             : (phi[0] * phi[1]) * exp_m(pade(sp.b, sp.c, r12));
+  double accepted = (live && !a.fresh) ? a.acc[(size_t)kAccAccepted * W + w] : 0.0;
+  SampleAcc sa;
+  if (live) {
+    sa.load(a, w);
+  } else {
+#pragma unroll
+    for (int k = 0; k < kAccColumns2; ++k) sa.acc[k] = 0.0;
+    sa.fill = a.block_fill;
+    sa.sample_idx = a.trace_sample0;
+  }
+  Stream rng;
+  rng.init(a.seed, a.sys, (uint32_t)w, a.epoch);
+  uint64_t p = a.draw0;
+
+  for (uint32_t s = 0; s < a.n_sweeps; ++s) {
+    // ---- draws: electron q/2's (dx, dy) on its even lane, (dz, u) on its odd one
+    const uint64_t da = rng.draw(p + 2 * (uint64_t)q), db = rng.draw(p + 2 * (uint64_t)q + 1);
+    p += 8;
+    const bool odd = (q & 1) != 0;
+    const double va = u64_range(da, a.scale, a.offset);
+    const double vb = odd ? u64_standard(db) : u64_range(db, a.scale, a.offset);
+    const double pa = quad_mov<kQuadXor1>(va), pb = quad_mov<kQuadXor1>(vb);
+    const double dxe = odd ? pa : va, dye = odd ? pb : vb, dze = odd ? va : pa, ue = odd ? vb : pb;
+    // ---- electron-local part of this lane's (electron, nucleus)
+    const double xe0 = me ? x[3] : x[0], xe1 = me ? x[4] : x[1], xe2 = me ? x[5] : x[2];
+    const double nxe0 = xe0 + dxe, nxe1 = xe1 + dye, nxe2 = xe2 + dze;
+    const double nrq = norm3(nxe0 - Rq[0], nxe1 - Rq[1], nxe2 - Rq[2]);
+    const double neq = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : exp_m(-sp.zeta * nrq);
+    // this electron's nuclei (the partner lane holds the other nucleus)
+    double nre[NNUC], nee[NNUC];
+    if constexpr (NNUC == 2) {
+      const double nrp = quad_mov<kQuadXor1>(nrq), nep = quad_mov<kQuadXor1>(neq);
+      nre[0] = mi ? nrp : nrq;
+      nre[1] = mi ? nrq : nrp;
+      nee[0] = mi ? nep : neq;
+      nee[1] = mi ? neq : nep;
+    } else {
+      nre[0] = nrq;
+      nee[0] = neq;
+    }
+    double nphie = nee[0];
+    if constexpr (NNUC == 2) nphie = nphie + nee[1];
+    // the other electron's, from the lane pair (q ^ 2)
+    double nx[2][3], nr[2][NNUC], ne[2][NNUC], nphi[2], uu[2];
+    {
+      const double o0 = quad_mov<kQuadXor2>(nxe0), o1 = quad_mov<kQuadXor2>(nxe1), o2 = quad_mov<kQuadXor2>(nxe2);
+      const double ou = quad_mov<kQuadXor2>(ue), ophi = quad_mov<kQuadXor2>(nphie);
+      nx[0][0] = me ? o0 : nxe0;
+      nx[0][1] = me ? o1 : nxe1;
+      nx[0][2] = me ? o2 : nxe2;
+      nx[1][0] = me ? nxe0 : o0;
+      nx[1][1] = me ? nxe1 : o1;
+      nx[1][2] = me ? nxe2 : o2;
+      uu[0] = me ? ou : ue;
+      uu[1] = me ? ue : ou;
+      nphi[0] = me ? ophi : nphie;
+      nphi[1] = me ? nphie : ophi;
+#pragma unroll
+      for (int I = 0; I < NNUC; ++I) {
+        const double orr = quad_mov<kQuadXor2>(nre[I]), oee = quad_mov<kQuadXor2>(nee[I]);
+        nr[0][I] = me ? orr : nre[I];
+        nr[1][I] = me ? nre[I] : orr;
+        ne[0][I] = me ? oee : nee[I];
+        ne[1][I] = me ? nee[I] : oee;
+      }
+    }
+    // ---- coupled parts: lane 0 electron 0's move; lane 1 electron 1's after
+    // a rejected, lanes 2-3 after an accepted move of electron 0
+    const bool c0 = q == 0, c1 = q == 1;
+    const double A0 = c1 ? x[0] : nx[0][0], A1 = c1 ? x[1] : nx[0][1], A2 = c1 ? x[2] : nx[0][2];
+    const double B0 = c0 ? x[3] : nx[1][0], B1 = c0 ? x[4] : nx[1][1], B2 = c0 ? x[5] : nx[1][2];
+    const double n12c = norm3(A0 - B0, A1 - B1, A2 - B2);
+    double valc;
+    if constexpr (KIND == MOLE_WF_HELIUM_PADE) {
+      const double s12 = c0 ? (nr[0][0] + r[1][0]) : (c1 ? (r[0][0] + nr[1][0]) : (nr[0][0] + nr[1][0]));
+      valc = exp_m(-sp.zeta * s12 + pade(sp.b, sp.c, n12c));
+    } else {
+      valc = exp_m(pade(sp.b, sp.c, n12c));
+    }
+    const double n12_0 = quad_mov<kQuadLane0>(n12c), n12_r = quad_mov<kQuadLane1>(n12c),
+                 n12_a = quad_mov<kQuadLane2>(n12c);
+    const double v_0 = quad_mov<kQuadLane0>(valc), v_r = quad_mov<kQuadLane1>(valc), v_a = quad_mov<kQuadLane2>(valc);
+    // ---- the chain: electron 0, then electron 1 (every lane)
+    uint64_t mask = 0;
+    {
+      const double psin = (KIND == MOLE_WF_HELIUM_PADE) ? v_0 : (nphi[0] * phi[1]) * v_0;
+      const double ratio = min_one((psin * psin) / (psi * psi));
+      const bool acc = ratio > uu[0];
+      x[0] = acc ? nx[0][0] : x[0];
+      x[1] = acc ? nx[0][1] : x[1];
+      x[2] = acc ? nx[0][2] : x[2];
+#pragma unroll
+      for (int I = 0; I < NNUC; ++I) {
+        r[0][I] = acc ? nr[0][I] : r[0][I];
+        ee[0][I] = acc ? ne[0][I] : ee[0][I];
+      }
+      phi[0] = acc ? nphi[0] : phi[0];
+      r12 = acc ? n12_0 : r12;
+      psi = acc ? psin : psi;
+      accepted += acc ? 1.0 : 0.0;
+      mask |= acc ? 1ull : 0ull;
+      // electron 1's coupled part for this outcome
+      const double n12_1 = acc ? n12_a : n12_r;
+      const double v_1 = acc ? v_a : v_r;
+      const double psin1 = (KIND == MOLE_WF_HELIUM_PADE) ? v_1 : (phi[0] * nphi[1]) * v_1;
+      const double ratio1 = min_one((psin1 * psin1) / (psi * psi));
+      const bool acc1 = ratio1 > uu[1];
+      x[3] = acc1 ? nx[1][0] : x[3];
+      x[4] = acc1 ? nx[1][1] : x[4];
+      x[5] = acc1 ? nx[1][2] : x[5];
+#pragma unroll
+      for (int I = 0; I < NNUC; ++I) {
+        r[1][I] = acc1 ? nr[1][I] : r[1][I];
+        ee[1][I] = acc1 ? ne[1][I] : ee[1][I];
+      }
+      phi[1] = acc1 ? nphi[1] : phi[1];
+      r12 = acc1 ? n12_1 : r12;
+      psi = acc1 ? psin1 : psi;
+      accepted += acc1 ? 1.0 : 0.0;
+      mask |= acc1 ? 2ull : 0ull;
+    }
+    if (lead && a.trace_mask && a.trace_sweep0 + s < a.cap_sweeps)
+      a.trace_mask[(size_t)w * a.cap_sweeps + a.trace_sweep0 + s] = mask;
+    if (s < a.sample_start) continue;
+    // ---- local energy and d ln psi/dp (local2_cached's operations, split over the quad)
+    const double zeta = sp.zeta, b = sp.b, c = sp.c;
+    double inv[2][NNUC];
+    {
+      const double rq = me ? (mi ? r[1][NNUC - 1] : r[1][0]) : (mi ? r[0][NNUC - 1] : r[0][0]);
+      const double iq = 1.0 / rq;
+      double ie[NNUC];
+      if constexpr (NNUC == 2) {
+        const double ip_ = quad_mov<kQuadXor1>(iq);
+        ie[0] = mi ? ip_ : iq;
+        ie[1] = mi ? iq : ip_;
+      } else {
+        ie[0] = iq;
+      }
+#pragma unroll
+      for (int I = 0; I < NNUC; ++I) {
+        const double oi = quad_mov<kQuadXor2>(ie[I]);
+        inv[0][I] = me ? oi : ie[I];
+        inv[1][I] = me ? ie[I] : oi;
+      }
+    }
+    // this lane's electron: the sums over nuclei
+    double phis = 0.0, gs0 = 0.0, gs1 = 0.0, gs2 = 0.0, ls = 0.0, zs = 0.0;
+#pragma unroll
+    for (int I = 0; I < NNUC; ++I) {
+      const double d0 = xe_sel(me, x, 0) - sp.nuc[I][0], d1 = xe_sel(me, x, 1) - sp.nuc[I][1],
+                   d2 = xe_sel(me, x, 2) - sp.nuc[I][2];
+      const double iv = me ? inv[1][I] : inv[0][I];
+      const double eI = (KIND == MOLE_WF_HELIUM_PADE) ? 1.0 : (me ? ee[1][I] : ee[0][I]);
+      phis += eI;
+      const double f = -zeta * eI * iv;
+      gs0 = fma(f, d0, gs0);
+      gs1 = fma(f, d1, gs1);
+      gs2 = fma(f, d2, gs2);
+      ls = fma(eI, zeta * zeta - 2.0 * zeta * iv, ls);
+      zs = fma(-(me ? r[1][I] : r[0][I]), eI, zs);
+    }
+    // one division per lane: 1/phi_0, 1/r12, 1/phi_1, 1/(1 + c r12)
+    const double dv = (q & 1) ? ((q == 1) ? r12 : (1.0 + c * r12)) : phis;
+    const double qv = 1.0 / dv;
+    const double ip0 = quad_mov<kQuadLane0>(qv), i12 = quad_mov<kQuadLane1>(qv), ip1 = quad_mov<kQuadLane2>(qv),
+                 iden = quad_mov<kQuadLane3>(qv);
+    const double ogs0 = quad_mov<kQuadXor2>(gs0), ogs1 = quad_mov<kQuadXor2>(gs1), ogs2 = quad_mov<kQuadXor2>(gs2);
+    const double ols = quad_mov<kQuadXor2>(ls), ozs = quad_mov<kQuadXor2>(zs);
+    double E, dlog[kNParam2];
+    {
+      double V = sp.v_nn;
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+#pragma unroll
+        for (int I = 0; I < NNUC; ++I) V = V - sp.charge[I] * inv[e][I];
+      double gi[2][3], li[2], dz = 0.0;
+      const double g0[3] = {me ? ogs0 : gs0, me ? ogs1 : gs1, me ? ogs2 : gs2};
+      const double g1[3] = {me ? gs0 : ogs0, me ? gs1 : ogs1, me ? gs2 : ogs2};
+      const double ls0 = me ? ols : ls, ls1 = me ? ls : ols, zs0 = me ? ozs : zs, zs1 = me ? zs : ozs;
+      gi[0][0] = g0[0] * ip0;
+      gi[0][1] = g0[1] * ip0;
+      gi[0][2] = g0[2] * ip0;
+      li[0] = ls0 * ip0;
+      dz += zs0 * ip0;
+      gi[1][0] = g1[0] * ip1;
+      gi[1][1] = g1[1] * ip1;
+      gi[1][2] = g1[2] * ip1;
+      li[1] = ls1 * ip1;
+      dz += zs1 * ip1;
+      const double d0 = x[0] - x[3], d1 = x[1] - x[4], d2 = x[2] - x[5];
+      const double up = b * iden * iden;
+      const double upp = -2.0 * b * c * iden * iden * iden;
+      const double J = upp + 2.0 * up * i12 + up * up;
+      const double h0 = d0 * i12, h1 = d1 * i12, h2 = d2 * i12;
+      const double gd1 = gi[0][0] * h0 + gi[0][1] * h1 + gi[0][2] * h2;
+      const double gd2 = gi[1][0] * h0 + gi[1][1] * h1 + gi[1][2] * h2;
+      const double k1 = li[0] + J + 2.0 * up * gd1;
+      const double k2 = li[1] + J - 2.0 * up * gd2;
+      if (sp.hamiltonian == MOLE_H_ELECTRONIC) V = V + i12;
+      E = -0.5 * (k1 + k2) + V;
+      dlog[0] = dz;
+      dlog[1] = r12 * iden;
+      dlog[2] = -b * r12 * r12 * iden * iden;
+    }
+    sa.add(a, w, E, dlog, psi, inv_block, lead);
+  }
+  if (lead) {
+#pragma unroll
+    for (int k = 0; k < 6; ++k) a.pos[(size_t)k * W + w] = x[k];
+    sa.store(a, w);
+    a.acc[(size_t)kAccAccepted * W + w] = accepted;
+  }
+}
+
 __global__ void init2_kernel(double *pos, int W, uint64_t seed, uint32_t sys,
                              uint32_t epoch) {
   const int w = blockIdx.x * blockDim.x + threadIdx.x;
@@ -627,7 +917,18 @@ bool sweep2_wavespec_eligible(const Sys2 &sp) {
 hipError_t launch_sweep2(const Sweep2Args &a, int variant, int block_threads, hipStream_t stream) {
   const bool he = a.sp.kind == MOLE_WF_HELIUM_PADE;
   const int nn = a.sp.n_nuc;
-  if (variant != 1 && sweep2_wavespec_eligible(a.sp)) {
+  if (variant == MOLE_VARIANT_QUAD && sweep2_wavespec_eligible(a.sp)) {
+    // four lanes per walker, 16 walkers per one-wave workgroup
+    const dim3 grid((unsigned)((4 * (size_t)a.W + 63) / 64)), blk(64);
+    if (he)
+      hipLaunchKernelGGL((sweep2_quad_kernel<MOLE_WF_HELIUM_PADE, 1>), grid, blk, 0, stream, a);
+    else if (nn == 1)
+      hipLaunchKernelGGL((sweep2_quad_kernel<MOLE_WF_LCAO2_PADE, 1>), grid, blk, 0, stream, a);
+    else
+      hipLaunchKernelGGL((sweep2_quad_kernel<MOLE_WF_LCAO2_PADE, 2>), grid, blk, 0, stream, a);
+    return hipGetLastError();
+  }
+  if (variant != MOLE_VARIANT_LANE && sweep2_wavespec_eligible(a.sp)) {
     // wave-specialised path: 64 walkers per 256-thread workgroup
     const dim3 grid((a.W + 63) / 64), blk(256);
     constexpr int T = 6;

@@ -103,6 +103,16 @@ def h2_lcao(R=1.4, zeta=1.0, b=0.5, c=0.5):
                     (b, 0.0, c, 0.0, 1.0), name="H2 LCAO")
 
 
+def h2_ion_product(R=2.5, alpha=1.0):
+    """H2+ as tests/hydrogen_molecular_ion_lcao.rs:60-77 writes it: one
+    electron in exp(-alpha |r - A|) exp(-alpha |r - B|), nuclei at (+-R/2, 0,
+    0).  As a Slater-Jastrow function: a one-by-one determinant of a constant
+    orbital (one STO with zeta 0) times the e-n Pade factor with a = alpha,
+    d = 0, exp(-alpha r_A - alpha r_B)."""
+    return SJSystem([(-0.5 * R, 0.0, 0.0), (0.5 * R, 0.0, 0.0)], [1, 1], 1, 0, [0.0], [0], [[1.0]],
+                    (0.0, 0.0, 1.0, alpha, 0.0), name="H2+ product")
+
+
 def poisson_disk(n, min_sep, rng, box=None, max_tries=10000):
     """n points with pairwise distance >= min_sep by dart throwing in a cube
     whose side grows with n (seeded)."""

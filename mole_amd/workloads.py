@@ -152,8 +152,8 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
 
 
 def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=8000,
-            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.3, 0.45, 0.6, 0.9, 1.2),
-            precision=M.PREC_FP64, lu_period=64, seed=1, log=print):
+            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.2, 1.5, 2.0),
+            precision=M.PREC_FP64, lu_period=64, seed=1, objective=M.MLP_OBJ_ESJD, collect_side=0.3, log=print):
     """Learned proposal against the box proposal on one atom: configurations
     from the box sampler, Adam training on the device, then runs of each
     proposal from the same ensemble.  Efficiency per proposal: ESS/s =
@@ -174,7 +174,7 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
     # 1. training configurations from the plain sampler; its final ensemble
     #    is also the common start of every measured run (a slowly mixing
     #    proposal started from U(-1,1) would otherwise carry its burn-in)
-    with context(M.PROPOSAL_BOX, box_sides[len(box_sides) // 2]) as ctx:
+    with context(M.PROPOSAL_BOX, collect_side) as ctx:
         ctx.init_walkers()
         ctx.sweep(equil)
         n_snap = max(1, -(-n_configs // walkers))
@@ -185,7 +185,7 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
         start = ctx.get_configs()
         # 2. training on the device
         t0 = time.perf_counter()
-        obj = mlp.train(ctx, train_steps, batch=batch, lr=lr, seed=seed + 10)
+        obj = mlp.train(ctx, train_steps, batch=batch, lr=lr, seed=seed + 10, objective=objective)
         t_train = time.perf_counter() - t0
     n_data = mlp.data_size()
     box_lanes = autotune.tune([sj], lambda s: walkers, candidates=(4, 8, 16), sweeps=2, precision=precision,
@@ -229,6 +229,7 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
     rec = dict(workload="learned", Z=Z, element=sj.name, walkers=walkers, sweeps=sweeps, block=block,
                precision="mixed" if precision == M.PREC_MIXED else "fp64",
                training=dict(configs=n_data, steps=train_steps, batch=batch, lr=lr,
+                             objective="esjd-relative" if objective == M.MLP_OBJ_ESJD_RELATIVE else "esjd",
                              objective_first=float(np.mean(obj[:20])), objective_last=float(np.mean(obj[-20:])),
                              collect_seconds=t_collect, train_seconds=t_train),
                box=runs, learned=lrn,

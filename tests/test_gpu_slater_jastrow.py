@@ -366,3 +366,74 @@ def test_planted_singular_configuration_is_counted():
     assert np.array_equal(bits(out), bits(o["cfg"]))
     assert st["n_singular"] == o["n_singular"] >= 8
     assert np.all(np.isfinite(out))
+
+
+# ------------------------------------------------ reference anchors
+
+H2ION_E = -0.43720930232581723   # quadrature of <E> of the product function (R = 2.5)
+H2ION_STD = 0.5052200457935744
+
+
+def test_h2_ion_product_reference_anchor():
+    # tests/hydrogen_molecular_ion_lcao.rs:100-142 on the device: H2+ at
+    # R = 2.5 with psi = exp(-r_A) exp(-r_B) (a constant orbital times the e-n
+    # Pade factor, a = 1, d = 0), MetropolisBox(1.0), Runner::run(.., 100);
+    # the reference asserts |E + 0.565| < std of the samples.  The device's
+    # short run also follows the oracle bit for bit, and its long run's mean
+    # sits on the quadrature value within the blocking error.
+    system = S.h2_ion_product()
+    d = run_device([system], 64, 40, 10, side=1.0, seed=4)[0]
+    o = run_oracle(system, 64, 40, 10, side=1.0, seed=4)
+    assert np.array_equal(d["masks"], o["accept_mask"])
+    assert np.array_equal(bits(d["cfg"]), bits(o["cfg"]))
+    with M.Context([M.slater_jastrow(system, 8192)], seed=0, step=1.0, block_size=100, lu_period=64) as ctx:
+        ctx.init_walkers()
+        ctx.run(3000)
+        st = ctx.stats()
+    n = st["n_samples"]
+    mean = st["sum_e"] / n
+    std = np.sqrt(st["sum_e2"] / n - mean * mean)
+    nb = st["n_blocks"]
+    err = np.sqrt((st["sum_block_mean2"] / nb - (st["sum_block_mean"] / nb) ** 2) / (nb - 1))
+    assert abs(mean - (-0.565)) < std
+    assert abs(mean - H2ION_E) < 4 * err, (mean, err)
+    assert abs(std - H2ION_STD) < 0.02 * H2ION_STD
+
+
+@pytest.mark.parametrize("workgroups", [0, 3])
+def test_persistent_schedule_matches_class_launches(workgroups):
+    # every tile of the context in one persistent launch (the tiles drawn from
+    # a device ticket, longest first; 3 workgroups: each takes many tiles)
+    # against one launch per footprint class: the same walkers, bit for bit,
+    # eagerly and replayed from a captured graph (the ticket's reset is part
+    # of the capture)
+    systems = [S.atom(4), S.random_molecule(9, seed=2), S.hydrogen_chain(6), S.atom(10)]
+    W, B = 70, 5
+
+    def run(schedule):
+        dev = [M.slater_jastrow(s, W) for s in systems]
+        with M.Context(dev, seed=6, step=0.6, block_size=B, flags=M.FLAG_TRACE, record_sweeps=30,
+                       precision=M.PREC_MIXED, lu_period=16) as ctx:
+            for k, L in enumerate((4, 8, 16, 8)):
+                ctx.set_walker_lanes(L, sys=k)
+            ctx.set_tile_schedule(schedule, workgroups)
+            if schedule == M.SCHEDULE_PERSISTENT:
+                wg, _ = ctx.launch_info()
+                assert wg == (workgroups or wg) and wg >= 1
+            ctx.init_walkers()
+            ctx.run(10)
+            g = ctx.capture(5, sample=True)
+            for _ in range(3):
+                g.launch(1)
+                ctx.synchronize()
+                ctx.sweep(1)
+            g.close()
+            return [(ctx.get_configs(k), ctx.trace(28, 24, sys=k), ctx.stats(k)) for k in range(len(systems))]
+
+    a = run(M.SCHEDULE_CLASSES)
+    b = run(M.SCHEDULE_PERSISTENT)
+    for k in range(len(systems)):
+        assert np.array_equal(bits(a[k][0]), bits(b[k][0])), k
+        assert np.array_equal(a[k][1][0], b[k][1][0]), k
+        assert np.array_equal(bits(a[k][1][1]), bits(b[k][1][1])), k
+        assert a[k][2] == b[k][2], k

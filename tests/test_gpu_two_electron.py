@@ -84,7 +84,7 @@ TRAJ_CASES = [
 ]
 
 
-@pytest.mark.parametrize("variant", [M.VARIANT_LANE, M.VARIANT_WAVESPEC])
+@pytest.mark.parametrize("variant", [M.VARIANT_LANE, M.VARIANT_WAVESPEC, M.VARIANT_QUAD])
 @pytest.mark.parametrize("kind,params,W,steps,B", TRAJ_CASES)
 def test_trajectory_parity_with_oracle(kind, params, W, steps, B, variant):
     side, seed = 0.5, 0
@@ -136,25 +136,29 @@ def test_three_nuclei_general_path_parity():
     assert np.max(np.abs(el - ref["e_local"]) / np.maximum(1, np.abs(ref["e_local"]))) <= E_TOL
 
 
-def test_variants_agree_on_split_sweeps():
+@pytest.mark.parametrize("make", [M.h2, M.helium], ids=["h2", "he"])
+def test_variants_agree_on_split_sweeps(make):
     # arbitrary sweep splits (not multiples of the chunk), sampling switched
-    # on mid-stream: both kernels leave identical state and statistics
+    # on mid-stream, a walker count that leaves a partial workgroup: every
+    # kernel (one lane, wave-specialised, four lanes per walker) leaves
+    # identical state and statistics
     res = []
-    for variant in (M.VARIANT_LANE, M.VARIANT_WAVESPEC):
-        with M.Context([M.h2(130)], seed=8, step=0.5, block_size=6) as ctx:
+    for variant in (M.VARIANT_LANE, M.VARIANT_WAVESPEC, M.VARIANT_QUAD):
+        with M.Context([make(130)], seed=8, step=0.5, block_size=6) as ctx:
             ctx.set_launch_config(variant)
             ctx.init_walkers()
             ctx.sweep(7)
             ctx.sweep(13, sample=True)
             ctx.sweep(5, sample=True)
             res.append((ctx.get_configs(), ctx.stats(), ctx.sr_sums()))
-    (c0, s0, r0), (c1, s1, r1) = res
-    assert np.array_equal(bits(c0), bits(c1))
-    assert s0["n_accepted"] == s1["n_accepted"]
-    for k in ("sum_e", "sum_e2", "sum_block_mean", "sum_block_mean2"):
-        assert s0[k] == s1[k], k
-    for k in ("sum_o", "sum_oe", "sum_oo"):
-        assert np.array_equal(r0[k], r1[k]), k
+    c0, s0, r0 = res[0]
+    for c1, s1, r1 in res[1:]:
+        assert np.array_equal(bits(c0), bits(c1))
+        assert s0["n_accepted"] == s1["n_accepted"]
+        for k in ("sum_e", "sum_e2", "sum_block_mean", "sum_block_mean2"):
+            assert s0[k] == s1[k], k
+        for k in ("sum_o", "sum_oe", "sum_oo"):
+            assert np.array_equal(r0[k], r1[k]), k
 
 
 @pytest.mark.parametrize("conv", [M.O_LOGDERIV, M.O_REFERENCE])

@@ -315,3 +315,40 @@ def test_singular_block_is_flagged_not_divided():
     cfg = np.tile(x, (4, 1))
     r = SJ.run(wv, seed=1, walkers=4, steps=4, block_size=2, box_side=0.5, lu_period=64, cfg=cfg, record=False)
     assert r["n_singular"] == 4
+
+
+# exact <E> of the H2+ product function at R = 2.5 and its local-energy
+# standard deviation, by quadrature in cylindrical coordinates (scipy dblquad,
+# 1e-11 relative): the anchor of tests/hydrogen_molecular_ion_lcao.rs:100-142
+H2ION_E = -0.43720930232581723
+H2ION_STD = 0.5052200457935744
+
+
+def test_h2_ion_product_is_the_reference_function():
+    # psi and E_L of the Slater-Jastrow form against the reference's H2WF
+    # (tests/hydrogen_molecular_ion_lcao.rs:60-98) written out in numpy
+    wv = SJ.SjWave(S.h2_ion_product())
+    A, B = np.array([-1.25, 0, 0]), np.array([1.25, 0, 0])
+    for x in cfgs(1, 20, 31, scale=1.5):
+        r1, r2 = np.linalg.norm(x - A), np.linalg.norm(x - B)
+        assert abs(SJ.value(wv, x) - np.exp(-r1) * np.exp(-r2)) <= 1e-15 * np.exp(-r1 - r2) * 4
+        g = -(x - A) / r1 - (x - B) / r2
+        want = -0.5 * (g @ g - 2 / r1 - 2 / r2) - 1 / r1 - 1 / r2 + 1 / 2.5
+        e, _ = SJ.local_energy(wv, x)
+        assert abs(e - want) <= 1e-12 * max(1, abs(want))
+
+
+def test_h2_ion_product_energy_anchor():
+    # the reference asserts |E + 0.565| < std of the samples; the oracle's mean
+    # also sits on the quadrature value within its blocking error
+    wv = SJ.SjWave(S.h2_ion_product())
+    r = SJ.run(wv, seed=1, walkers=256, steps=1000, block_size=100, box_side=1.0, lu_period=64, record=False,
+               n_threads=8)
+    n = r["n_samples"]
+    mean = r["sum_e"] / n
+    std = np.sqrt(r["sum_e2"] / n - mean * mean)
+    nb = r["n_blocks"]
+    err = np.sqrt((r["sum_bm2"] / nb - (r["sum_bm"] / nb) ** 2) / (nb - 1))
+    assert abs(mean - (-0.565)) < std
+    assert abs(mean - H2ION_E) < 4 * err
+    assert abs(std - H2ION_STD) < 0.05 * H2ION_STD
