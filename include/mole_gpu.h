@@ -280,7 +280,18 @@ mole_status mole_read_blocks(mole_ctx *ctx, uint32_t sys, size_t first, mole_blo
  * every sum in sample order, as the reference folds it.  tau[n_walkers].
  * Effective samples = sum_w n / tau_w.  Needs MOLE_FLAG_TRACE, n >= 2. */
 mole_status mole_tau_int(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau, size_t n_walkers);
-
+/* mole_tau_int without any allocation: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * serving loops, harnesses that forbid hipMalloc): the per-walker values are
+ * staged in a library-owned device array of MOLE_TAU_NOALLOC_MAX_WALKERS
+ * doubles per GPU (held until the process ends), so the call makes no
+ * hipMalloc/hipFree.  Same arguments and results as mole_tau_int; a system
+ * with more walkers returns MOLE_ERR_UNSUPPORTED.  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other threads, other
+ * contexts' streams) race. */
+#define MOLE_TAU_NOALLOC_MAX_WALKERS 262144
+mole_status mole_tau_int_noalloc(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau,
+                                 size_t n_walkers);
 /* MonteCarloSampler::data (src/montecarlo/src/traits.rs:27) in compat
  * form: "Energy", "Wavefunction value" and "Parameter gradient" samples as
  * [walker][sample] (and [walker][sample][param]), the order in which

@@ -36,6 +36,7 @@ PROPOSAL_BOX = 0
 PROPOSAL_DIFFUSE = 1
 PROPOSAL_LEARNED = 2
 MLP_PARAMS = 9476
+TAU_NOALLOC_MAX_WALKERS = 262144  # mole_tau_int_noalloc's scratch (MOLE_TAU_NOALLOC_MAX_WALKERS)
 O_LOGDERIV = 0
 O_REFERENCE = 1
 FLAG_TRACE = 1
@@ -252,6 +253,7 @@ _SIGS = [
     ("mole_read_samples", C.c_int32, [_P, C.c_uint32, _pd, _pd, _pd, C.c_size_t, C.c_size_t]),
     ("mole_read_logdet", C.c_int32, [_P, C.c_uint32, _pd, _pd, C.c_size_t]),
     ("mole_tau_int", C.c_int32, [_P, C.c_uint32, C.c_size_t, C.c_size_t, _pd, C.c_size_t]),
+    ("mole_tau_int_noalloc", C.c_int32, [_P, C.c_uint32, C.c_size_t, C.c_size_t, _pd, C.c_size_t]),
     ("mole_eval", C.c_int32, [_P, C.c_uint32, C.c_size_t, _pd, _pd, _pd, _pd, _pd, _pd]),
     ("mole_set_launch_config", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
     ("mole_set_walker_lanes", C.c_int32, [_P, C.c_uint32, C.c_uint32]),
@@ -512,6 +514,14 @@ class Context:
         W = self.systems[sys].n_walkers
         t = np.zeros(W)
         check(lib().mole_tau_int(self.h, sys, first, n, _dp(t), W))
+        return t
+
+    def tau_int_noalloc(self, n, first=0, sys=0):
+        """mole_tau_int_noalloc: as tau_int, staged in the library's static
+        device scratch instead of an allocation."""
+        W = self.systems[sys].n_walkers
+        t = np.zeros(W)
+        check(lib().mole_tau_int_noalloc(self.h, sys, first, n, _dp(t), W))
         return t
 
     def samples(self, samples, sys=0):

@@ -15,6 +15,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -670,6 +671,42 @@ mole_status tau_check(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, const
 
 }  // namespace
 
+// ---- the allocation-free tau entry point's scratch ---------------------------
+// mole_tau_int_noalloc stages the per-walker values in this file-scope device
+// array instead of a buffer allocated per call: one copy per GPU, held until
+// the process ends, holding the last call's values.  In host code the name
+// `scratch` is a host-side stand-in, not a device address: each device's
+// address comes from hipGetSymbolAddress, taken once per device with that
+// device current, and only that address reaches kernels and copies.
+constexpr size_t kTauScratchWalkers = (size_t)1 << 18;  // walkers (2 MiB of doubles)
+static_assert(kTauScratchWalkers == MOLE_TAU_NOALLOC_MAX_WALKERS, "header and library limits differ");
+static __device__ __align__(256) double scratch[kTauScratchWalkers];
+
+namespace {
+
+constexpr int kMaxScratchDevices = 64;
+std::mutex g_scratch_mu;
+double *g_scratch_ptr[kMaxScratchDevices] = {};
+
+// This device's address of `scratch`, looked up on its first use (the
+// device must be current).
+mole_status scratch_address(int device, double **out) {
+  if (device < 0 || device >= kMaxScratchDevices)
+    return fail(MOLE_ERR_DEVICE, "device %d beyond the %d the scratch table holds", device, kMaxScratchDevices);
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[device]) {
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p)
+      return fail(MOLE_ERR_DEVICE, "hipGetSymbolAddress(scratch): %s", hipGetErrorString(e));
+    g_scratch_ptr[device] = (double *)p;
+  }
+  *out = g_scratch_ptr[device];
+  return MOLE_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *mole_status_string(mole_status s) {
@@ -1133,6 +1170,26 @@ mole_status mole_tau_int(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, do
   if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
   (void)hipFree(d);
   if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_tau_int: %s", hipGetErrorString(e));
+  return ok();
+}
+
+mole_status mole_tau_int_noalloc(mole_ctx *ctx, uint32_t sys, size_t first, size_t n, double *tau,
+                                 size_t n_walkers) {
+  SysState *sp = nullptr;
+  mole_status st = tau_check(ctx, sys, first, n, tau, n_walkers, &sp);
+  if (st != MOLE_OK) return st;
+  SysState &s = *sp;
+  if ((size_t)s.W > kTauScratchWalkers)
+    return fail(MOLE_ERR_UNSUPPORTED, "mole_tau_int_noalloc: %d walkers exceed its scratch of %zu (use mole_tau_int)",
+                s.W, kTauScratchWalkers);
+  HIP_TRY(hipSetDevice(ctx->device));
+  double *d = nullptr;
+  st = scratch_address(ctx->device, &d);
+  if (st != MOLE_OK) return st;
+  hipError_t e = launch_tau_corr(s.d_trace_e, s.W, ctx->run.record_sweeps, first, (int)n, d, ctx->stream);
+  if (e == hipSuccess) e = hipMemcpyAsync(tau, d, sizeof(double) * (size_t)s.W, hipMemcpyDeviceToHost, ctx->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_tau_int_noalloc: %s", hipGetErrorString(e));
   return ok();
 }
 
