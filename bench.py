@@ -268,13 +268,14 @@ def learned_record(log_runs=False):
     r = workloads.learned(Z=10, log=log)
 
     def brief(x):
-        return {k: x[k] for k in ("step", "energy", "error", "acceptance", "tau_int_mean", "moves_per_s",
-                                  "ess_per_s", "kappa")}
+        return {k: x[k] for k in ("step", "energy", "error", "error_tau", "acceptance", "tau_int_mean",
+                                  "moves_per_s", "ess_per_s", "kappa")}
     return dict(system="Ne atom (10 electrons; 1s/2s/2p STO basis, seed 1)", walkers=r["walkers"],
                 sweeps=r["sweeps"], precision=r["precision"], training=r["training"],
                 box=[brief(b) for b in r["box"]], learned=brief(r["learned"]),
                 ess_ratio_vs_best_box=r["ess_ratio_vs_best_box"],
                 kappa_ratio_vs_best_box=r["kappa_ratio_vs_best_box"],
+                best_box_side_ess=r["best_box_side_ess"], best_box_side_tau=r["best_box_side_tau"],
                 tau_ratio_vs_best_box=r["tau_ratio_vs_best_box"],
                 energy_diff_sigma=r["energy_diff_sigma"],
                 tau_estimator="scripts/statfor.rs:30-54 on the device (mole_tau_int)")

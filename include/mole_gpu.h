@@ -565,7 +565,7 @@ mole_status mole_last_timing(mole_ctx *ctx, double *ms, uint32_t *launches);
  * No counterpart in the reference.  The network's inputs for electron i at
  * r: distance to the nearest nucleus, x_j - r for the three nearest other
  * electrons (nearest first; zeros when missing), r.  Outputs: mu = out[0..2],
- * ln sigma = ln 0.25 + clamp(out[3], -7, 1).  Parameters are FP32 in the
+ * ln sigma = ln 0.25 + 3 tanh(out[3] / 3).  Parameters are FP32 in the
  * order W1[13][64] b1[64] W2[64][64] b2[64] W3[64][64] b3[64] W4[64][4]
  * b4[4] (z = b + x W); the kernels run bf16 operands with FP32 accumulation.
  * Training maximises the mean of A |x' - x_i|^2 (expected squared jump
@@ -579,9 +579,12 @@ typedef struct mole_mlp mole_mlp;
  * electron's distance to its nearest nucleus, mean of
  * A |x' - x_i|^2 / (d^2 + 0.01) with d the larger of that distance at x_i
  * and at x' (ESJD_RELATIVE: core and valence electrons weigh alike instead
- * of the valence electrons' large jumps dominating). */
+ * of the valence electrons' large jumps dominating), or the mean of
+ * log(1e-4 + A |x' - x_i|^2) (LOG_ESJD: every sample counts by its relative
+ * gain, and a rejected move scores as a jump of 0.01 bohr). */
 #define MOLE_MLP_OBJ_ESJD 0
 #define MOLE_MLP_OBJ_ESJD_RELATIVE 1
+#define MOLE_MLP_OBJ_LOG_ESJD 2
 
 typedef struct {
   uint32_t steps;  /* Adam steps */
@@ -609,13 +612,15 @@ mole_status mole_mlp_data_size(mole_mlp *m, size_t *n_cfg);
 /* Adam steps on the training set; sample g (g = step * batch + b) moves
  * electron g mod N of configuration (g / N) mod n_cfg with normals from draws
  * 0..3 of the Philox stream (seed, system 0x6D6C70, lo32(g), hi32(g)).
- * objective (optional, [steps]) receives the batch mean of A |dx|^2. */
+ * objective (optional, [steps]) receives the batch mean of the training
+ * objective d->objective. */
 mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_train_desc *d,
                            double *objective);
 /* One batch without an update: grad[MOLE_MLP_PARAMS] = sum over the samples
- * first .. first+batch-1 of d(A |dx|^2)/dtheta; objective = their sum. */
+ * first .. first+batch-1 of the derivative of training objective obj_kind
+ * (MOLE_MLP_OBJ_*); objective = the sum of the samples' objective values. */
 mole_status mole_mlp_gradient(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
-                              uint64_t seed, float *grad, double *objective);
+                              uint64_t seed, uint32_t obj_kind, float *grad, double *objective);
 /* Network outputs out[4 e .. 4 e + 3] for electron electron[e] of
  * configuration e of cfg[n][3N] (nuclei nuc_xyz[n_nuc][3]). */
 mole_status mole_mlp_eval(mole_mlp *m, const double *cfg, uint32_t N, const double *nuc_xyz, uint32_t n_nuc,

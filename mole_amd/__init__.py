@@ -96,7 +96,7 @@ class TrainDesc(C.Structure):
                 ("beta2", C.c_float), ("eps", C.c_float), ("seed", C.c_uint64), ("objective", C.c_uint32)]
 
 
-MLP_OBJ_ESJD, MLP_OBJ_ESJD_RELATIVE = 0, 1
+MLP_OBJ_ESJD, MLP_OBJ_ESJD_RELATIVE, MLP_OBJ_LOG_ESJD = 0, 1, 2
 
 
 class SystemDesc(C.Structure):
@@ -297,7 +297,7 @@ _SIGS = [
     ("mole_mlp_clear_data", C.c_int32, [_P]),
     ("mole_mlp_data_size", C.c_int32, [_P, C.POINTER(C.c_size_t)]),
     ("mole_mlp_train", C.c_int32, [_P, _P, C.c_uint32, C.POINTER(TrainDesc), _pd]),
-    ("mole_mlp_gradient", C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, _pf, _pd]),
+    ("mole_mlp_gradient", C.c_int32, [_P, _P, C.c_uint32, C.c_uint32, C.c_uint64, C.c_uint64, C.c_uint32, _pf, _pd]),
     ("mole_mlp_eval", C.c_int32, [_P, _pd, C.c_uint32, _pd, C.c_uint32, C.POINTER(C.c_int32), C.c_uint32, _pf]),
 ]
 
@@ -773,10 +773,10 @@ class Mlp:
         check(lib().mole_mlp_train(self.h, ctx.h, sys, C.byref(d), _dp(obj)))
         return obj[:steps]
 
-    def gradient(self, ctx, batch, first=0, seed=11, sys=0):
+    def gradient(self, ctx, batch, first=0, seed=11, sys=0, objective=MLP_OBJ_ESJD):
         g = np.zeros(MLP_PARAMS, np.float32)
         o = C.c_double()
-        check(lib().mole_mlp_gradient(self.h, ctx.h, sys, batch, first, seed, _fp(g), C.byref(o)))
+        check(lib().mole_mlp_gradient(self.h, ctx.h, sys, batch, first, seed, objective, _fp(g), C.byref(o)))
         return g, o.value
 
     def eval(self, cfg, nuclei, electron):

@@ -1709,7 +1709,7 @@ mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint3
   a.seed = seed;
   a.partial = m->d_partial;
   a.obj = m->d_obj;
-  a.relative = objective == MOLE_MLP_OBJ_ESJD_RELATIVE ? 1 : 0;
+  a.objective = (int)objective;
   HIP_TRY(launch_mlp_train(a, G, m->stream));
   HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_grad, obj_dev, adam, m->stream));
   return MOLE_OK;
@@ -1737,14 +1737,15 @@ mole_status mlp_train_prepare(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t
 }  // namespace
 
 mole_status mole_mlp_gradient(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint32_t batch, uint64_t first,
-                              uint64_t seed, float *grad, double *objective) {
+                              uint64_t seed, uint32_t obj_kind, float *grad, double *objective) {
+  if (obj_kind > MOLE_MLP_OBJ_LOG_ESJD) return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown training objective %u", obj_kind);
   mole_status st = mlp_train_prepare(m, ctx, sys, batch);
   if (st != MOLE_OK) return st;
   if (!m->d_obj_steps) {
     HIP_TRY(hipMalloc(&m->d_obj_steps, sizeof(double) * 1024));
     m->obj_steps_cap = 1024;
   }
-  st = mlp_gradient_enqueue(m, ctx, sys, batch, first, seed, m->d_obj_steps);
+  st = mlp_gradient_enqueue(m, ctx, sys, batch, first, seed, m->d_obj_steps, nullptr, obj_kind);
   if (st != MOLE_OK) return st;
   if (grad) HIP_TRY(hipMemcpyAsync(grad, m->d_grad, sizeof(float) * kMlpParams, hipMemcpyDeviceToHost, m->stream));
   double o = 0.0;
@@ -1762,7 +1763,7 @@ mole_status mole_mlp_train(mole_mlp *m, mole_ctx *ctx, uint32_t sys, const mole_
   if (!(d->lr > 0.0f) || !(d->beta1 >= 0.0f && d->beta1 < 1.0f) || !(d->beta2 >= 0.0f && d->beta2 < 1.0f) ||
       !(d->eps > 0.0f))
     return fail(MOLE_ERR_INVALID_ARGUMENT, "Adam needs lr > 0, 0 <= beta < 1, eps > 0");
-  if (d->objective > MOLE_MLP_OBJ_ESJD_RELATIVE)
+  if (d->objective > MOLE_MLP_OBJ_LOG_ESJD)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "unknown training objective %u", d->objective);
   if (d->steps == 0) return ok();
   if ((int)d->steps > m->obj_steps_cap) {

@@ -361,7 +361,7 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
       // nearest-nucleus distances at x and x' (a jump is measured on the
       // scale of its wider end); a constant of the gradient
       const double dm = dn > (double)fv[0] ? dn : (double)fv[0];
-      const double wgt = a.relative ? 1.0 / (dm * dm + kMlpRelD0 * kMlpRelD0) : 1.0;
+      double wgt = a.objective == MOLE_MLP_OBJ_ESJD_RELATIVE ? 1.0 / (dm * dm + kMlpRelD0 * kMlpRelD0) : 1.0;
       const double ls = sp[9], sg = sp[10], ls2 = mlp_log_sigma((double)o2[3]), sg2 = exp_m(ls2);
       double e[3], dx[3], z[3], gx[3] = {sp[13], sp[14], sp[15]};
       double e2 = 0.0, z2 = 0.0, d2 = 0.0;
@@ -377,7 +377,15 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
       const double lq_fwd = -3.0 * ls - 0.5 * z2;
       const double logr = sp[12] + lq_rev - lq_fwd;
       const double A = logr < 0.0 ? exp_m(logr) : 1.0;
-      obj_acc += wgt * (A * d2);
+      if (a.objective == MOLE_MLP_OBJ_LOG_ESJD) {
+        // d log(eps + O) = dO / (eps + O): the ESJD derivatives scaled by
+        // this sample's constant 1 / (eps + O)
+        const double f = kMlpLogEps + A * d2;
+        wgt = 1.0 / f;
+        obj_acc += log_m(f);
+      } else {
+        obj_acc += wgt * (A * d2);
+      }
       // the weight is a constant of the gradient: scale every output's
       // derivative by it
       const double Aact = (logr < 0.0 ? A : 0.0) * wgt;
@@ -390,9 +398,9 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
         dO1[k] = (float)GD[k];
         dO2[k] = (float)(d2 * Aact * (e[k] * is2));
       }
-      dO1[3] = (o13 > kMlpLSLo && o13 < kMlpLSHi) ? (float)dls : 0.0f;
+      dO1[3] = (float)(dls * mlp_log_sigma_slope(o13));
       const double dls2 = d2 * Aact * (-3.0 + e2 * is2);
-      dO2[3] = (o23 > kMlpLSLo && o23 < kMlpLSHi) ? (float)dls2 : 0.0f;
+      dO2[3] = (float)(dls2 * mlp_log_sigma_slope(o23));
     }
     mlp_backward(a.net, dO2, h, x2, s.stage, g);
     for (int k = 0; k < 12; ++k) h[k] = s.h1[k][l];

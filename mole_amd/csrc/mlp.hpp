@@ -27,7 +27,7 @@ constexpr int kMlpParams = 9476;
 constexpr int kMlpW1 = 0, kMlpB1 = 832, kMlpW2 = 896, kMlpB2 = 4992, kMlpW3 = 5056, kMlpB3 = 9152,
               kMlpW4 = 9216, kMlpB4 = 9472;
 constexpr double kMlpLS0 = -1.3862943611198906;  // ln 0.25
-constexpr double kMlpLSLo = -7.0, kMlpLSHi = 1.0;
+constexpr double kMlpLSHalf = 3.0;                // ln sigma within ln 0.25 +- 3
 // forward fragments: layer 1 on 16x16x16 [4 m-tiles]; layers 2-4 on
 // 16x16x32: layers 2-3 [4 m][2 k], layer 4 [2 k]
 constexpr int kFwd1Frags = 4;
@@ -148,10 +148,24 @@ __device__ __forceinline__ f32x4 mlp_forward(const MlpDev &m, bf16x4 x, f32x4 *h
   return out;
 }
 
-// ln sigma from the raw output (clamped)
+// tanh(out3 / 3) as 1 - 2 / (e^{2y} + 1) on exp_m (absolute error ~1e-16;
+// +-1 beyond |y| = 20)
+__device__ __forceinline__ double mlp_sigma_tanh(double out3) {
+  const double y = fmin(fmax(out3 / kMlpLSHalf, -20.0), 20.0);
+  return 1.0 - 2.0 / (exp_m(2.0 * y) + 1.0);
+}
+
+// ln sigma from the raw output: ln 0.25 + 3 tanh(out3 / 3), a smooth bound
+// (sigma in [0.0124, 5.02]) whose slope never vanishes, so a proposal
+// pushed towards either end can still be trained back
 __device__ __forceinline__ double mlp_log_sigma(double out3) {
-  const double c = out3 < kMlpLSLo ? kMlpLSLo : (out3 > kMlpLSHi ? kMlpLSHi : out3);
-  return kMlpLS0 + c;
+  return kMlpLS0 + kMlpLSHalf * mlp_sigma_tanh(out3);
+}
+
+// d ln sigma / d out3
+__device__ __forceinline__ double mlp_log_sigma_slope(double out3) {
+  const double t = mlp_sigma_tanh(out3);
+  return 1.0 - t * t;
 }
 
 // The 13 inputs of electron i at r (FP64; rounded to bf16 by the caller):

@@ -19,11 +19,16 @@ struct MlpTrainArgs {
   uint64_t seed;        // Philox key of the proposal normals
   float *partial;       // [gridDim][kMlpParams], zero on entry
   double *obj;          // [gridDim] sum of the objective, zero on entry
-  int relative;         // 1: each sample's A |dx|^2 divided by d^2 + kMlpRelD0^2
-                        // (d the larger nearest-nucleus distance of x and x')
+  int objective;        // MOLE_MLP_OBJ_*: ESJD; RELATIVE, each sample's
+                        // A |dx|^2 divided by d^2 + kMlpRelD0^2 (d the larger
+                        // nearest-nucleus distance of x and x'); LOG, each
+                        // sample's log(kMlpLogEps + A |dx|^2)
 };
 // length scale below which the relative objective stops growing (bohr)
 constexpr double kMlpRelD0 = 0.1;
+// floor of the log objective: a rejected move scores log(1e-4), a jump of
+// 0.01 bohr
+constexpr double kMlpLogEps = 1e-4;
 
 // Philox system tag of the training normals: sample g draws 0..3 of the
 // stream (seed, kMlpTrainSys, lo32(g), hi32(g)).

@@ -75,10 +75,11 @@ def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, s
 
 def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, chunk=10, block=10,
            precision=M.PREC_MIXED, lu_period=64, seed=3, tune=True, table=None, lds_budget=40 << 10,
-           log=print):
+           schedule=M.SCHEDULE_CLASSES, workgroups=0, log=print):
     """All systems in one context: per `chunk` sweeps one launch per LDS
-    footprint class of the tiles (most expensive first within a class),
-    replayed from a HIP graph."""
+    footprint class of the tiles (most expensive first within a class), or
+    one persistent launch over every tile (schedule), replayed from a HIP
+    graph."""
     sys_list = hetero_systems(n_systems, seed)
     Ns = np.array([s.n_electrons for s in sys_list])
     W = np.maximum(1, np.floor(Ns / Ns.sum() * total_walkers)).astype(int)
@@ -92,6 +93,7 @@ def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, ch
     dev = [M.slater_jastrow(s, int(w)) for s, w in zip(sys_list, W)]
     ctx = M.Context(dev, seed=seed, step=0.6, block_size=block, precision=precision, lu_period=lu_period)
     autotune.apply(ctx, sys_list, table)
+    ctx.set_tile_schedule(schedule, workgroups)
     t0 = time.perf_counter()
     ctx.init_walkers()
     ctx.synchronize()
@@ -126,7 +128,8 @@ def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, ch
     g.close()
     geq.close()
     ctx.close()
-    rec = dict(workload="hetero", systems=n_systems, walkers=int(W.sum()),
+    rec = dict(workload="hetero", schedule="persistent" if schedule == M.SCHEDULE_PERSISTENT else "classes",
+               systems=n_systems, walkers=int(W.sum()),
                electrons_min=int(Ns.min()), electrons_max=int(Ns.max()), sweeps=done, chunk=chunk,
                precision="mixed" if precision == M.PREC_MIXED else "fp64", lu_period=lu_period,
                seconds=wall, graph_device_ms=graph_ms, moves_per_s=moves / wall,
@@ -152,8 +155,9 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
 
 
 def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=8000,
-            batch=4096, lr=1e-3, sweeps=400, block=20, box_sides=(0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.2, 1.5, 2.0),
-            precision=M.PREC_FP64, lu_period=64, seed=1, objective=M.MLP_OBJ_ESJD, collect_side=0.3, log=print):
+            batch=4096, lr=1e-3, sweeps=400, block=50, box_sides=(0.1, 0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.5),
+            precision=M.PREC_FP64, lu_period=64, seed=1, objective=M.MLP_OBJ_LOG_ESJD, collect_side=0.3,
+            log=print):
     """Learned proposal against the box proposal on one atom: configurations
     from the box sampler, Adam training on the device, then runs of each
     proposal from the same ensemble.  Efficiency per proposal: ESS/s =
@@ -211,8 +215,12 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
         t = dev_ms * 1e-3 * sweeps / (sweeps + block)  # device time of the sampled sweeps
         e, err = energy_and_error(st)
         tau_mean = float(np.mean(tau))
+        # error from the variance and tau_int (blocks shorter than the
+        # correlation time understate the blocking error)
+        var = max(st["sum_e2"] / st["n_samples"] - e * e, 0.0)
+        err_tau = float(np.sqrt(var * tau_mean / st["n_samples"]))
         return dict(proposal={M.PROPOSAL_BOX: "box", M.PROPOSAL_LEARNED: "learned"}[proposal], step=step,
-                    lanes=lanes, energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
+                    lanes=lanes, energy=e, error=err, error_tau=err_tau, acceptance=st["n_accepted"] / st["n_moves"],
                     tau_int_mean=tau_mean, tau_int_median=float(np.median(tau)), seconds=t, wall=wall,
                     moves_per_s=walkers * sweeps * N / t, ess_per_s=walkers * sweeps / (tau_mean * t),
                     kappa=1.0 / (err * err * t), max_condition=st["max_condition"])
@@ -225,20 +233,24 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
     mlp.close()
     best_ess = max(runs, key=lambda r: r["ess_per_s"])
     best_kappa = max(runs, key=lambda r: r["kappa"])
+    best_tau = min(runs, key=lambda r: r["tau_int_mean"])
     diff = lrn["energy"] - best_kappa["energy"]
+    sig = lambda r: max(r["error"], r["error_tau"])
     rec = dict(workload="learned", Z=Z, element=sj.name, walkers=walkers, sweeps=sweeps, block=block,
                precision="mixed" if precision == M.PREC_MIXED else "fp64",
                training=dict(configs=n_data, steps=train_steps, batch=batch, lr=lr,
-                             objective="esjd-relative" if objective == M.MLP_OBJ_ESJD_RELATIVE else "esjd",
+                             objective={M.MLP_OBJ_ESJD: "esjd", M.MLP_OBJ_ESJD_RELATIVE: "esjd-relative",
+                                        M.MLP_OBJ_LOG_ESJD: "log-esjd"}[objective],
                              objective_first=float(np.mean(obj[:20])), objective_last=float(np.mean(obj[-20:])),
                              collect_seconds=t_collect, train_seconds=t_train),
                box=runs, learned=lrn,
                best_box_side_ess=best_ess["step"], best_box_side_kappa=best_kappa["step"],
                ess_ratio_vs_best_box=lrn["ess_per_s"] / best_ess["ess_per_s"],
                kappa_ratio_vs_best_box=lrn["kappa"] / best_kappa["kappa"],
-               tau_ratio_vs_best_box=best_ess["tau_int_mean"] / lrn["tau_int_mean"],
+               best_box_side_tau=best_tau["step"],
+               tau_ratio_vs_best_box=best_tau["tau_int_mean"] / lrn["tau_int_mean"],
                energy_diff_vs_box=diff,
-               energy_diff_sigma=diff / float(np.hypot(lrn["error"], best_kappa["error"])))
+               energy_diff_sigma=diff / float(np.hypot(sig(lrn), sig(best_kappa))))
     log(rec)
     return rec
 

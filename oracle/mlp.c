@@ -88,9 +88,17 @@ void orc_mlp_forward(const float *w, const double *f, int em, double *out, doubl
   }
 }
 
-double orc_mlp_log_sigma(double out3) {
-  double c = out3 < ORC_MLP_LS_LO ? ORC_MLP_LS_LO : (out3 > ORC_MLP_LS_HI ? ORC_MLP_LS_HI : out3);
-  return ORC_MLP_LS0 + c;
+/* tanh(out3 / 3) as 1 - 2 / (e^{2y} + 1), the device's formula */
+static double sigma_tanh(double out3) {
+  double y = fmin(fmax(out3 / ORC_MLP_LS_HALF, -20.0), 20.0);
+  return 1.0 - 2.0 / (exp(2.0 * y) + 1.0);
+}
+
+double orc_mlp_log_sigma(double out3) { return ORC_MLP_LS0 + ORC_MLP_LS_HALF * sigma_tanh(out3); }
+
+double orc_mlp_log_sigma_slope(double out3) {
+  double t = sigma_tanh(out3);
+  return 1.0 - t * t;
 }
 
 /* backprop of dO/dout through one forward evaluation (FP64), adding into g */
@@ -132,7 +140,7 @@ static void backward(const float *w, const double *f, const double *act, const d
 }
 
 double orc_mlp_sample(const orc_sj *wf, const float *w, const double *cfg, int i, const double *z, int em,
-                      const double *frozen_f2, double *grad, double *diag) {
+                      const double *frozen_f2, double *grad, double *diag, int objective) {
   int N = wf->n_up + wf->n_dn;
   const double *r = cfg + 3 * i;
   double f1[NI], f2[NI], o1[NO], o2[NO], a1[3 * H], a2[3 * H];
@@ -167,7 +175,20 @@ double orc_mlp_sample(const orc_sj *wf, const float *w, const double *cfg, int i
   double lq_fwd = -3.0 * ls - 0.5 * z2;
   double logr = lR2 + lq_rev - lq_fwd;
   double A = logr < 0.0 ? exp(logr) : 1.0;
-  double O = A * d2;
+  /* the objective's weight, a constant of the gradient (mlp.h) */
+  double wgt = 1.0, O;
+  if (objective == 1) {
+    double da = (double)(float)f1[0], db = (double)(float)f2[0];
+    double dm = da > db ? da : db;
+    wgt = 1.0 / (dm * dm + ORC_MLP_REL_D0 * ORC_MLP_REL_D0);
+  }
+  if (objective == 2) {
+    double f = ORC_MLP_LOG_EPS + A * d2;
+    wgt = 1.0 / f;
+    O = log(f);
+  } else {
+    O = wgt * (A * d2);
+  }
   if (diag) {
     diag[0] = logr;
     diag[1] = A;
@@ -178,16 +199,16 @@ double orc_mlp_sample(const orc_sj *wf, const float *w, const double *cfg, int i
     for (int q = 0; q < 3; ++q) diag[6 + q] = xn[q];
   }
   if (grad) {
-    double Aact = logr < 0.0 ? A : 0.0;
+    double Aact = (logr < 0.0 ? A : 0.0) * wgt;
     double is2 = 1.0 / (sg2 * sg2);
     double GD[3], dout1[NO], dout2[NO];
-    for (int q = 0; q < 3; ++q) GD[q] = 2.0 * A * dx[q] + d2 * Aact * (2.0 * gx[q] + e[q] * is2);
+    for (int q = 0; q < 3; ++q) GD[q] = 2.0 * (A * wgt) * dx[q] + d2 * Aact * (2.0 * gx[q] + e[q] * is2);
     for (int q = 0; q < 3; ++q) dout1[q] = GD[q];
     double dls = (GD[0] * z[0] + GD[1] * z[1] + GD[2] * z[2]) * sg + d2 * Aact * 3.0;
-    dout1[3] = (o1[3] > ORC_MLP_LS_LO && o1[3] < ORC_MLP_LS_HI) ? dls : 0.0;
+    dout1[3] = dls * orc_mlp_log_sigma_slope(o1[3]);
     for (int q = 0; q < 3; ++q) dout2[q] = d2 * Aact * (e[q] * is2);
     double dls2 = d2 * Aact * (-3.0 + e2 * is2);
-    dout2[3] = (o2[3] > ORC_MLP_LS_LO && o2[3] < ORC_MLP_LS_HI) ? dls2 : 0.0;
+    dout2[3] = dls2 * orc_mlp_log_sigma_slope(o2[3]);
     backward(w, f1, a1, dout1, em, grad);
     backward(w, f2, a2, dout2, em, grad);
   }

@@ -27,13 +27,23 @@ def lib():
         L.orc_mlp_forward.argtypes = [_pf, _pd, C.c_int, _pd, _pd]
         L.orc_mlp_log_sigma.restype = C.c_double
         L.orc_mlp_log_sigma.argtypes = [C.c_double]
+        L.orc_mlp_log_sigma_slope.restype = C.c_double
+        L.orc_mlp_log_sigma_slope.argtypes = [C.c_double]
         L.orc_mlp_sample.restype = C.c_double
-        L.orc_mlp_sample.argtypes = [ps, _pf, _pd, C.c_int, _pd, C.c_int, _pd, _pd, _pd]
+        L.orc_mlp_sample.argtypes = [ps, _pf, _pd, C.c_int, _pd, C.c_int, _pd, _pd, _pd, C.c_int]
         L.orc_mlp_log_q_ratio.restype = C.c_double
         L.orc_mlp_log_q_ratio.argtypes = [ps, _pf, _pd, C.c_int, _pd, C.c_int]
         L.orc_sj_move_terms.argtypes = [ps, _pd, C.c_int, _pd, _pd, _pd]
         _bound = True
     return L
+
+
+def log_sigma(out3):
+    return lib().orc_mlp_log_sigma(float(out3))
+
+
+def log_sigma_slope(out3):
+    return lib().orc_mlp_log_sigma_slope(float(out3))
 
 
 def _d(a):
@@ -61,7 +71,10 @@ def forward(w, f, emulate_bf16=False):
     return out, act.reshape(3, HID)
 
 
-def sample(wv, w, cfg, i, z, emulate_bf16=False, frozen_f2=None, grad=False):
+OBJ_ESJD, OBJ_ESJD_RELATIVE, OBJ_LOG_ESJD = 0, 1, 2
+
+
+def sample(wv, w, cfg, i, z, emulate_bf16=False, frozen_f2=None, grad=False, objective=OBJ_ESJD):
     """(O, dO/dtheta or None, diag) of one training sample."""
     w = np.ascontiguousarray(w, dtype=np.float32)
     cfg = np.ascontiguousarray(cfg, dtype=np.float64)
@@ -70,7 +83,8 @@ def sample(wv, w, cfg, i, z, emulate_bf16=False, frozen_f2=None, grad=False):
     diag = np.zeros(9)
     ff = None if frozen_f2 is None else np.ascontiguousarray(frozen_f2, dtype=np.float64)
     O = lib().orc_mlp_sample(wv.ref(), _f(w), _d(cfg), int(i), _d(z), int(emulate_bf16),
-                             None if ff is None else _d(ff), None if g is None else _d(g), _d(diag))
+                             None if ff is None else _d(ff), None if g is None else _d(g), _d(diag),
+                             int(objective))
     return O, g, dict(logr=diag[0], A=diag[1], d2=diag[2], ls=diag[3], ls2=diag[4], lR2=diag[5],
                       xn=diag[6:9].copy())
 

@@ -64,15 +64,32 @@ def test_forward_matches_numpy():
     assert np.max(np.abs(got16 - out)) < 0.05
 
 
-def test_training_gradient_matches_finite_differences(ne):
+def test_log_sigma_is_a_smooth_bound():
+    # ln sigma = ln 0.25 + 3 tanh(out / 3): the textbook tanh, the bounds, and
+    # a slope that stays positive far outside the old hard clamp [-7, 1]
+    for o in [-40.0, -9.0, -7.5, -1.0, 0.0, 0.4, 1.0, 1.5, 4.0, 60.0]:
+        assert ML.log_sigma(o) == pytest.approx(ML.LS0 + 3.0 * np.tanh(o / 3.0), abs=2e-15)
+        h = 1e-6
+        fd = (ML.log_sigma(o + h) - ML.log_sigma(o - h)) / (2 * h)
+        assert ML.log_sigma_slope(o) == pytest.approx(fd, rel=1e-6, abs=1e-9)
+    assert ML.log_sigma(0.0) == ML.LS0 and ML.log_sigma_slope(0.0) == 1.0
+    assert np.exp(ML.log_sigma(1e6)) == pytest.approx(0.25 * np.exp(3.0))
+    assert np.exp(ML.log_sigma(-1e6)) == pytest.approx(0.25 * np.exp(-3.0))
+    assert ML.log_sigma_slope(1.5) > 0.7 and ML.log_sigma_slope(-9.0) > 0.009
+
+
+@pytest.mark.parametrize("b43,objective", [(0.3, ML.OBJ_ESJD), (2.5, ML.OBJ_ESJD),
+                                           (0.3, ML.OBJ_ESJD_RELATIVE), (0.3, ML.OBJ_LOG_ESJD)])
+def test_training_gradient_matches_finite_differences(ne, b43, objective):
+    # b4[3] = 2.5 puts ln sigma's output where a hard clamp would give no slope
     w = ML.xavier(2)
     rng = np.random.default_rng(4)
-    w[ML.OFF["b4"] + 3] = 0.3
+    w[ML.OFF["b4"] + 3] = b43
     samples = []
     for x in cfgs(10, 12, 5, scale=0.8):
         i = int(rng.integers(0, 10))
         z = rng.normal(size=3)
-        O, g, d = ML.sample(ne, w, x, i, z, grad=True)
+        O, g, d = ML.sample(ne, w, x, i, z, grad=True, objective=objective)
         if abs(d["logr"]) < 0.05:  # keep away from the kink of min(1, r)
             continue
         cfg2 = x.copy()
@@ -90,10 +107,34 @@ def test_training_gradient_matches_finite_differences(ne):
             # float32 weights: the perturbation itself is rounded, use the rounded step
             wpf, wmf = wp.astype(np.float32), wm.astype(np.float32)
             hh = (float(wpf[p]) - float(wmf[p])) / 2
-            Op, _, _ = ML.sample(ne, wpf, x, i, z, frozen_f2=f2)
-            Om, _, _ = ML.sample(ne, wmf, x, i, z, frozen_f2=f2)
+            Op, _, _ = ML.sample(ne, wpf, x, i, z, frozen_f2=f2, objective=objective)
+            Om, _, _ = ML.sample(ne, wmf, x, i, z, frozen_f2=f2, objective=objective)
             fd = (Op - Om) / (2 * hh)
             assert abs(fd - g[p]) <= 2e-4 * max(1.0, abs(fd)) + 1e-6, (p, fd, g[p])
+
+
+def test_objectives_relate_as_documented(ne):
+    # one sample under the three objectives: RELATIVE is ESJD over
+    # d^2 + 0.01 (d the larger float-rounded nearest-nucleus distance of x
+    # and x'), LOG is log(1e-4 + ESJD); the gradients scale the same way
+    w = ML.xavier(2)
+    rng = np.random.default_rng(21)
+    for x in cfgs(10, 4, 22, scale=0.8):
+        i = int(rng.integers(0, 10))
+        z = rng.normal(size=3)
+        O0, g0, d = ML.sample(ne, w, x, i, z, grad=True)
+        assert O0 == pytest.approx(d["A"] * d["d2"], rel=1e-15)
+        cfg2 = x.copy()
+        cfg2[3 * i:3 * i + 3] = d["xn"]
+        da = float(np.float32(np.linalg.norm(x[3 * i:3 * i + 3])))
+        db = float(np.float32(np.linalg.norm(d["xn"])))
+        wgt = 1.0 / (max(da, db) ** 2 + 0.01)
+        O1, g1, _ = ML.sample(ne, w, x, i, z, grad=True, objective=ML.OBJ_ESJD_RELATIVE)
+        assert O1 == pytest.approx(O0 * wgt, rel=1e-12)
+        assert np.allclose(g1, g0 * wgt, rtol=1e-9, atol=1e-14)
+        O2, g2, _ = ML.sample(ne, w, x, i, z, grad=True, objective=ML.OBJ_LOG_ESJD)
+        assert O2 == pytest.approx(np.log(1e-4 + O0), rel=1e-14)
+        assert np.allclose(g2, g0 / (1e-4 + O0), rtol=1e-9, atol=1e-12)
 
 
 def test_mh_reversibility_identity(ne):
