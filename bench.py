@@ -211,6 +211,54 @@ def cpu_baseline(threads, budget_s=20.0, host=None):
                 build="gcc -O3 -ffp-contract=off -fno-fast-math")
 
 
+def helium_record(threads=None, cpu=True):
+    """The plain-VMC helium singlet run in full: exp(-zeta (r1 + r2)) with the
+    Pade e-e factor (zeta 1.6875, b 0.5, c 0.3), box moves of side 0.5,
+    walkers uniform in [-1, 1]^3 per electron, 4,096 walkers x 10,000 sweeps,
+    FP64, Philox seed 0 (blocks of 100 sweeps): E +- sigma and acceptance on
+    the device, and the same chains on the oracle's reference-faithful runner
+    (two full psi evaluations per move) on the host cores, whose energy must
+    equal the device's (same streams, same decisions)."""
+    import mole_amd as M
+
+    walkers, sweeps, block = 4096, 10000, 100
+    with M.Context([M.helium(walkers)], seed=0, step=0.5, block_size=block) as ctx:
+        ctx.init_walkers()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.run(sweeps, block)
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        dev_ms, launches = ctx.last_timing()
+        st = ctx.stats()
+    moves = walkers * sweeps * 2
+
+    def energy(sum_e, n, sum_bm, sum_bm2, nb):
+        mean = sum_e / n
+        return mean, float(np.sqrt(max(sum_bm2 / nb - (sum_bm / nb) ** 2, 0.0) / (nb - 1)))
+
+    e, err = energy(st["sum_e"], st["n_samples"], st["sum_block_mean"], st["sum_block_mean2"], st["n_blocks"])
+    rec = dict(workload="He singlet plain VMC", walkers=walkers, sweeps=sweeps, block=block, box_side=0.5,
+               params=[1.6875, 0.5, 0.3], energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
+               device_seconds=dev_ms * 1e-3, wall_seconds=wall, launches=launches,
+               moves_per_s=moves / (dev_ms * 1e-3), moves_per_s_wall=moves / wall)
+    if cpu:
+        from oracle import oracle_ffi as O
+
+        threads = threads or host_cpus()["threads"]
+        t0 = time.perf_counter()
+        r = O.run_box(O.helium(), seed=0, walkers=walkers, steps=sweeps, block_size=block, box_side=0.5,
+                      n_threads=threads, record=False)
+        dt = time.perf_counter() - t0
+        ec, errc = energy(r["sum_e"], r["n_samples"], r["sum_bm"], r["sum_bm2"], r["n_blocks"])
+        rec.update(cpu=dict(seconds=dt, threads=threads, moves_per_s=moves / dt, energy=ec, error=errc,
+                            acceptance=r["n_accepted"] / r["n_moves"], kind="port",
+                            runner="oracle reference-faithful (two full psi evaluations per move)"),
+                   gpu_vs_cpu=(moves / (dev_ms * 1e-3)) / (moves / dt), energy_gpu_minus_cpu=e - ec,
+                   accepted_gpu_minus_cpu=int(st["n_accepted"]) - int(r["n_accepted"]))
+    return rec
+
+
 def cpu_atom_sample(threads, walkers=1024, sweeps=40, block=10):
     """Reference-faithful oracle (two full LU evaluations per move, fresh-inverse
     local energy) on a bounded sample of an atom's workload."""
@@ -362,6 +410,7 @@ def main():
                     help="untimed SR iterations on a separate context (clocks settle)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-learned", action="store_true", help="leave out the learned-vs-box record")
+    ap.add_argument("--no-helium", action="store_true", help="leave out the helium singlet record")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--variant", choices=sorted(VARIANTS), default="auto")
     ap.add_argument("--host-loop", action="store_true",
@@ -421,6 +470,7 @@ def main():
         host = host_cpus()
         cpu = cpu_baseline(args.cpu_threads or host["threads"], host=host)
     learned = None if args.no_learned else learned_record()
+    helium = None if args.no_helium else helium_record(args.cpu_threads, cpu=not args.no_cpu_baseline)
     e_last = r["energies"][-1]
     out = {
         "metric": METRIC,
@@ -448,6 +498,7 @@ def main():
         "result": {"energy": e_last[0], "error": e_last[1], "acceptance": e_last[2],
                    "params": r["params"], "energy_first": r["energies"][0][0]},
         "learned": learned,
+        "helium": helium,
     }
     print(json.dumps(out))
 

@@ -42,9 +42,14 @@ def run_atom(sj, walkers, sweeps, block, precision, lu_period, lanes, seed):
         lanes = ctx.walker_lanes()
     e, err = energy_and_error(st)
     moves = walkers * ((sweeps // block) * block) * sj.n_electrons
-    return dict(energy=e, error=err, acceptance=st["n_accepted"] / st["n_moves"],
-                max_condition=st["max_condition"], seconds=wall, kernel_ms=kernel_ms,
-                moves_per_s=moves / wall, lanes=lanes)
+    acc = st["n_accepted"] / st["n_moves"]
+    tflops = moves * sj_flop_per_move(sj, acc) / (kernel_ms * 1e-3) / 1e12 if kernel_ms > 0 else None
+    return dict(energy=e, error=err, acceptance=acc,
+                max_condition=st["max_condition"], n_singular=st.get("n_singular", 0),
+                max_logdet_drift=st.get("max_logdet_drift", 0.0), seconds=wall, kernel_ms=kernel_ms,
+                moves_per_s=moves / wall, lanes=lanes,
+                roofline=dict(bound="fp64-valu", flop_per_move=sj_flop_per_move(sj, acc), achieved=tflops,
+                              peak=78.6, unit="TFLOP/s", frac=None if tflops is None else tflops / 78.6))
 
 
 def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, seed=1, tune=True,
@@ -75,21 +80,27 @@ def atoms(Zs=range(3, 11), walkers=65536, sweeps=1000, block=50, lu_period=64, s
 
 def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, chunk=10, block=10,
            precision=M.PREC_MIXED, lu_period=64, seed=3, tune=True, table=None, lds_budget=40 << 10,
-           schedule=M.SCHEDULE_CLASSES, workgroups=0, log=print):
+           schedule=M.SCHEDULE_CLASSES, workgroups=0, tune_batch=True, log=print):
     """All systems in one context: per `chunk` sweeps one launch per LDS
     footprint class of the tiles (most expensive first within a class), or
     one persistent launch over every tile (schedule), replayed from a HIP
-    graph."""
+    graph.  tune: lanes per walker per electron count from single-system
+    timings; tune_batch (with tune): the LDS budget, the schedule and the
+    sweeps per launch from timings of the whole batch (overrides
+    lds_budget, schedule and chunk)."""
     sys_list = hetero_systems(n_systems, seed)
     Ns = np.array([s.n_electrons for s in sys_list])
     W = np.maximum(1, np.floor(Ns / Ns.sum() * total_walkers)).astype(int)
     W[np.argmax(Ns)] += total_walkers - W.sum()
     timings = {}
+    t_tune = 0.0
     if table is None:
         table = {}
         if tune:
+            t0 = time.perf_counter()
             table, timings = autotune.tune(sys_list, lambda s: 16384, sweeps=4, precision=precision,
-                                           lu_period=lu_period, lds_budget=lds_budget)
+                                           lu_period=lu_period, lds_budget=None if tune_batch else lds_budget)
+            t_tune = time.perf_counter() - t0
     dev = [M.slater_jastrow(s, int(w)) for s, w in zip(sys_list, W)]
     ctx = M.Context(dev, seed=seed, step=0.6, block_size=block, precision=precision, lu_period=lu_period)
     autotune.apply(ctx, sys_list, table)
@@ -98,6 +109,13 @@ def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, ch
     ctx.init_walkers()
     ctx.synchronize()
     t_init = time.perf_counter() - t0
+    batch = None
+    if tune and tune_batch and timings:
+        t0 = time.perf_counter()
+        best, trials = autotune.tune_batch(ctx, sys_list, timings)
+        batch = dict(seconds=time.perf_counter() - t0, best={k: v for k, v in best.items() if k != "table"},
+                     trials=trials)
+        table, schedule, chunk = best["table"], best["schedule"], best["chunk"]
     geq = ctx.capture(chunk, sample=False)
     geq.launch(max(1, equil_sweeps // chunk))
     ctx.synchronize()
@@ -128,6 +146,8 @@ def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, ch
     g.close()
     geq.close()
     ctx.close()
+    flop = sweeps_flop(sys_list, W, done, per_sys)
+    achieved = flop / (graph_ms * 1e-3) / 1e12 if graph_ms > 0 else None  # TFLOP/s over the graph replays
     rec = dict(workload="hetero", schedule="persistent" if schedule == M.SCHEDULE_PERSISTENT else "classes",
                systems=n_systems, walkers=int(W.sum()),
                electrons_min=int(Ns.min()), electrons_max=int(Ns.max()), sweeps=done, chunk=chunk,
@@ -135,9 +155,40 @@ def hetero(n_systems=256, total_walkers=1 << 20, equil_sweeps=20, sweeps=100, ch
                seconds=wall, graph_device_ms=graph_ms, moves_per_s=moves / wall,
                eager_seconds=wall_eager, eager_moves_per_s=moves / wall_eager, init_seconds=t_init,
                workgroups_per_launch=n_wg, lds_per_workgroup=lds,
-               lanes_table={int(k): int(v) for k, v in table.items()}, per_system=per_sys)
+               lanes_table={int(k): int(v) for k, v in table.items()}, per_system=per_sys,
+               tune_seconds=t_tune, autotune_batch=batch,
+               roofline=dict(bound="fp64-valu", flop_per_sweep=flop / done, achieved=achieved, peak=78.6,
+                             unit="TFLOP/s", frac=None if achieved is None else achieved / 78.6,
+                             model="F(N, n, K, a) per electron-move, sj_flop_per_move"))
     log(rec)
     return rec
+
+
+def sj_flop_per_move(system, acceptance):
+    """Algorithmic FLOP of one electron-move of a Slater-Jastrow walker plus
+    its share of the sampled sweep's local energy (N electrons, K basis
+    functions, n = the moved electron's spin-block size, a = acceptance):
+    F = 33 + 2K + 2nK + 2n + 14(N-1) + a(4n^2 + 2n)
+        + [N(7K + 10nK + 8n + 36) + 21 N(N-1)/2] / N + 8
+    (trial position and distance 33; basis values 2K and the new row of the
+    orbital matrix 2nK; ratio 2n; Jastrow pairs 14(N-1); Sherman-Morrison
+    4n^2 + 2n when accepted; the sampled sweep's gradients, laplacians and
+    pair terms per electron; the FP64 re-inversion every 64 moves ~8)."""
+    N = system.n_electrons
+    K = len(system.zeta)
+    F = 0.0
+    for n in (system.n_up, system.n_dn):
+        if n == 0:
+            continue
+        F += n * (33 + 2 * K + 2 * n * K + 2 * n + 14 * (N - 1) + acceptance * (4 * n * n + 2 * n)
+                  + (N * (7 * K + 10 * n * K + 8 * n + 36) + 21 * N * (N - 1) / 2) / N + 8)
+    return F / N
+
+
+def sweeps_flop(sys_list, W, sweeps, per_sys):
+    """Algorithmic FLOP of `sweeps` sampled sweeps of the batch."""
+    return float(sum(w * s.n_electrons * sweeps * sj_flop_per_move(s, p["acceptance"])
+                     for s, w, p in zip(sys_list, W, per_sys)))
 
 
 def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2):
