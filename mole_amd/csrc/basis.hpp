@@ -8,7 +8,8 @@
 //   GTO    grad = (-2 z e) d,                       lap = e (4 z^2 r^2 - 6 z)
 //   STO_P  grad = ((-z e / r) d_q) d + e u_q,       lap = d_q e (z^2 - 4 z / r)
 //          (each component adds e or +0)
-// (u_q the unit vector; lap(x f(r)) = x (f'' + 4 f'/r)).  The oracle (sj.c)
+// (u_q the unit vector; lap(x f(r)) = x (f'' + 4 f'/r)), with 1/r formed once
+// and multiplied.  The oracle (sj.c)
 // evaluates every form with the same operations in the same order, so
 // values and gradients agree bit for bit.
 #pragma once
@@ -47,7 +48,8 @@ __device__ __forceinline__ void basis_vgl(int kind, double z, double d0, double 
     return;
   }
   const double e = exp_m(-z * r);
-  const double gf = -z * e / r;
+  const double ir = 1.0 / r;  // one division
+  const double gf = (-z * e) * ir;
   if (kind >= MOLE_BASIS_STO_PX) {
     const int q = kind - MOLE_BASIS_STO_PX;
     const double dq = pick_axis(q, d0, d1, d2);
@@ -58,13 +60,13 @@ __device__ __forceinline__ void basis_vgl(int kind, double z, double d0, double 
     g0 = gq * d0 + (q == 0 ? e : 0.0);
     g1 = gq * d1 + (q == 1 ? e : 0.0);
     g2 = gq * d2 + (q == 2 ? e : 0.0);
-    l = v * (z * z - 4.0 * z / r);
+    l = v * (z * z - (4.0 * z) * ir);
   } else {
     v = e;
     g0 = gf * d0;
     g1 = gf * d1;
     g2 = gf * d2;
-    l = e * (z * z - 2.0 * z / r);
+    l = e * (z * z - (2.0 * z) * ir);
   }
 }
 

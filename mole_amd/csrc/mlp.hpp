@@ -183,26 +183,43 @@ __device__ __forceinline__ void mlp_features(const double *pos, int N, const dou
   }
   int b0 = -1, b1 = -1, b2 = -1;
   double d0 = kInf, d1 = kInf, d2 = kInf;
-  for (int j = 0; j < N; ++j) {
-    if (j == i) continue;
-    const double *x = pos + 3 * j;
-    // squared distance (same order as the distance, no square root); ties
-    // of the squared values are ties of the distances
+  // insertion into the sorted three by selects (a later equal distance does
+  // not displace an earlier one: ties go to the lower index)
+  auto insert = [&](double d, int j) {
+    const bool c0 = d < d0, c1 = d < d1, c2 = d < d2;
+    d2 = c1 ? d1 : (c2 ? d : d2);
+    b2 = c1 ? b1 : (c2 ? j : b2);
+    d1 = c0 ? d0 : (c1 ? d : d1);
+    b1 = c0 ? b0 : (c1 ? j : b1);
+    d0 = c0 ? d : d0;
+    b0 = c0 ? j : b0;
+  };
+  // squared distance (same order as the distance, no square root); ties of
+  // the squared values are ties of the distances.  Positions are loaded four
+  // partners at a time ahead of the arithmetic (the scan order is unchanged).
+  auto dist2 = [&](const double *x) {
     const double e0 = x[0] - r[0], e1 = x[1] - r[1], e2 = x[2] - r[2];
     double d = e0 * e0;
     d = d + e1 * e1;
     d = d + e2 * e2;
-    if (d < d0) {
-      d2 = d1; b2 = b1;
-      d1 = d0; b1 = b0;
-      d0 = d; b0 = j;
-    } else if (d < d1) {
-      d2 = d1; b2 = b1;
-      d1 = d; b1 = j;
-    } else if (d < d2) {
-      d2 = d; b2 = j;
+    return d;
+  };
+  int j = 0;
+  for (; j + 4 <= N; j += 4) {
+    double xv[4][3];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+      for (int q = 0; q < 3; ++q) xv[u][q] = pos[3 * (j + u) + q];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double d = dist2(xv[u]);
+      // the moved electron itself never enters
+      insert(j + u == i ? kInf : d, j + u == i ? -1 : j + u);
     }
   }
+  for (; j < N; ++j)
+    if (j != i) insert(dist2(pos + 3 * j), j);
   auto vec = [&](int b, int q) { return b >= 0 ? pos[3 * b + q] - r[q] : 0.0; };
   double v0, v1, v2, v3;
   if (sub == 0) {

@@ -448,39 +448,45 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
       const double r = norm3(dv0, dv1, dv2);
       const bool same = (j < S.n_up ? 0 : 1) == s;
       const double b = same ? bP : bA;
-      const double den = 1.0 + c * r;
-      const double up = b / (den * den);
-      const double upp = -2.0 * b * c / (den * den * den);
-      gJ0 += up * dv0 / r;
-      gJ1 += up * dv1 / r;
-      gJ2 += up * dv2 / r;
-      lJ += upp + 2.0 * up / r;
+      // two divisions per pair: 1/r and 1/(1 + c r)
+      const double ir = 1.0 / r;
+      const double iden = 1.0 / (1.0 + c * r);
+      const double iden2 = iden * iden;
+      const double up = b * iden2;
+      const double upp = (-2.0 * b * c) * (iden2 * iden);
+      const double ur = up * ir;
+      gJ0 += ur * dv0;
+      gJ1 += ur * dv1;
+      gJ2 += ur * dv2;
+      lJ += upp + 2.0 * ur;
       if (j > i) {
-        const double inv = 1.0 / r;
-        Vi += inv;
-        if constexpr (EXT) Vee += inv;
+        Vi += ir;
+        if constexpr (EXT) Vee += ir;
         if (same)
-          e1 += r / den;
+          e1 += r * iden;
         else
-          e0 += r / den;
-        e2 += -b * r * r / (den * den);
+          e0 += r * iden;
+        e2 += (-b * r * r) * iden2;
       }
     }
     for (int I = 0; I < nn; ++I) {
       const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
       const double r = norm3(d0, d1, d2);
       const double Z = S.charge[I];
-      const double den = 1.0 + d * r;
+      const double ir = 1.0 / r;
+      const double iden = 1.0 / (1.0 + d * r);
+      const double iden2 = iden * iden;
       const double aZ = a * Z;
-      const double vp = -aZ / (den * den);
-      const double vpp = 2.0 * aZ * d / (den * den * den);
-      gJ0 += vp * d0 / r;
-      gJ1 += vp * d1 / r;
-      gJ2 += vp * d2 / r;
-      lJ += vpp + 2.0 * vp / r;
-      Vi += -Z / r;
-      e3 += -Z * r / den;
-      e4 += aZ * r * r / (den * den);
+      const double vp = -aZ * iden2;
+      const double vpp = (2.0 * aZ * d) * (iden2 * iden);
+      const double vr = vp * ir;
+      gJ0 += vr * d0;
+      gJ1 += vr * d1;
+      gJ2 += vr * d2;
+      lJ += vpp + 2.0 * vr;
+      Vi += -Z * ir;
+      e3 += (-Z * r) * iden;
+      e4 += (aZ * r * r) * iden2;
     }
     const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
     const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
@@ -545,8 +551,8 @@ __device__ __forceinline__ void drift_terms(const SJSysDev &S, const double *pos
     const double d0 = r[0] - S.nuc[3 * I], d1 = r[1] - S.nuc[3 * I + 1], d2 = r[2] - S.nuc[3 * I + 2];
     const int kd = (gto || pk) ? S.kind[m] : MOLE_BASIS_STO;
     if (kd == MOLE_BASIS_STO) {
-      // chi[m] = exp(-z r) is in LDS
-      const double gf = -S.zeta[m] * chi[m] / rn[I];
+      // chi[m] = exp(-z r) is in LDS; basis_vgl's operations
+      const double gf = (-S.zeta[m] * chi[m]) * (1.0 / rn[I]);
       gch[3 * m] = gf * d0;
       gch[3 * m + 1] = gf * d1;
       gch[3 * m + 2] = gf * d2;

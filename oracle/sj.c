@@ -31,7 +31,8 @@ static int kind_of(const orc_sj *wf, int m) { return wf->kind ? wf->kind[m] : OR
  *   STO    e = exp(-z r), g = (-z e / r) d, l = e (z^2 - 2 z/r)
  *   GTO    v = exp(-z r^2), g = (-2 z v) d, l = v (4 z^2 r^2 - 6 z)
  *   STO_P  v = d_q e, g_k = ((-z e / r) d_q) d_k + (k == q ? e : 0),
- *          l = v (z^2 - 4 z/r)  (lap(x f(r)) = x (f'' + 4 f'/r)) */
+ *          l = v (z^2 - 4 z/r)  (lap(x f(r)) = x (f'' + 4 f'/r))
+ * with 1/r formed once and multiplied. */
 static void basis_vgl(int kind, double z, const double d[3], double r, double *v, double g[3], double *l) {
   if (kind == ORC_BASIS_GTO) {
     const double r2 = r * r;
@@ -42,17 +43,18 @@ static void basis_vgl(int kind, double z, const double d[3], double r, double *v
     return;
   }
   const double e = orc_exp(-z * r);
-  const double gf = -z * e / r;
+  const double ir = 1.0 / r; /* one division, the device's */
+  const double gf = (-z * e) * ir;
   if (kind >= ORC_BASIS_STO_PX) {
     const int q = kind - ORC_BASIS_STO_PX;
     const double gq = gf * d[q];
     *v = d[q] * e;
     for (int k = 0; k < 3; ++k) g[k] = gq * d[k] + (k == q ? e : 0.0);
-    *l = *v * (z * z - 4.0 * z / r);
+    *l = *v * (z * z - (4.0 * z) * ir);
   } else {
     *v = e;
     for (int k = 0; k < 3; ++k) g[k] = gf * d[k];
-    *l = e * (z * z - 2.0 * z / r);
+    *l = e * (z * z - (2.0 * z) * ir);
   }
 }
 
@@ -372,35 +374,45 @@ static double sj_local_parts(const orc_sj *wf, const double *cfg, const double *
       double dv[3] = {xi[0] - xj[0], xi[1] - xj[1], xi[2] - xj[2]};
       double r = orc_norm3(dv[0], dv[1], dv[2]);
       double b = (spin_of(wf, j) == s) ? bP : bA;
-      double den = 1.0 + c * r;
-      double up = b / (den * den);
-      double upp = -2.0 * b * c / (den * den * den);
-      for (int q = 0; q < 3; ++q) gJ[q] += up * dv[q] / r;
-      lJ += upp + 2.0 * up / r;
+      /* u = b r / (1 + c r): u' = b iden^2, u'' = -2 b c iden^3 with
+       * iden = 1 / (1 + c r); grad_i u = u' dv / r, lap_i u = u'' + 2 u' / r;
+       * d u / d b = r iden, d u / d c = -b r^2 iden^2 (two divisions: 1/r
+       * and iden, the device's operations) */
+      double ir = 1.0 / r;
+      double iden = 1.0 / (1.0 + c * r);
+      double iden2 = iden * iden;
+      double up = b * iden2;
+      double upp = (-2.0 * b * c) * (iden2 * iden);
+      double ur = up * ir;
+      for (int q = 0; q < 3; ++q) gJ[q] += ur * dv[q];
+      lJ += upp + 2.0 * ur;
       if (j > i) {
-        const double inv = 1.0 / r;
-        Vi += inv;
-        Vee += inv;
-        double t = r / den;
+        Vi += ir;
+        Vee += ir;
+        double t = r * iden;
         if (spin_of(wf, j) == s)
           dli[1] += t;
         else
           dli[0] += t;
-        dli[2] += -b * r * r / (den * den);
+        dli[2] += (-b * r * r) * iden2;
       }
     }
     for (int I = 0; I < wf->n_nuc; ++I) {
       double Z = (double)wf->charge[I];
       double r = rI[I];
-      double den = 1.0 + d * r;
+      /* v = -a Z r / (1 + d r), the same forms with iden = 1 / (1 + d r) */
+      double ir = 1.0 / r;
+      double iden = 1.0 / (1.0 + d * r);
+      double iden2 = iden * iden;
       double aZ = a * Z;
-      double vp = -aZ / (den * den);
-      double vpp = 2.0 * aZ * d / (den * den * den);
-      for (int q = 0; q < 3; ++q) gJ[q] += vp * dI[I][q] / r;
-      lJ += vpp + 2.0 * vp / r;
-      Vi += -Z / r;
-      dli[3] += -Z * r / den;
-      dli[4] += aZ * r * r / (den * den);
+      double vp = -aZ * iden2;
+      double vpp = (2.0 * aZ * d) * (iden2 * iden);
+      double vr = vp * ir;
+      for (int q = 0; q < 3; ++q) gJ[q] += vr * dI[I][q];
+      lJ += vpp + 2.0 * vr;
+      Vi += -Z * ir;
+      dli[3] += (-Z * r) * iden;
+      dli[4] += (aZ * r * r) * iden2;
     }
     double dot = Dg[0] * gJ[0] + Dg[1] * gJ[1] + Dg[2] * gJ[2];
     double gg = gJ[0] * gJ[0] + gJ[1] * gJ[1] + gJ[2] * gJ[2];
