@@ -44,11 +44,17 @@ typedef int32_t mole_status;
  * phi(r) = sum_I exp(-zeta |r - R_I|), params (zeta, b, c). */
 #define MOLE_WF_LCAO2_PADE 2
 /* psi = det A_up det A_dn exp(J), A_s[c][k] = phi_k(r of the c-th electron
- * of spin s), phi_k(r) = sum_m coef[k][m] exp(-zeta_m |r - R_center(m)|),
+ * of spin s), phi_k(r) = sum_m coef[k][m] chi_m(r - R_center(m)) with basis
+ * functions chi_m of exponent zeta_m and kind MOLE_BASIS_*,
  * J = sum_{i<j} b_ij r_ij/(1 + c r_ij) - sum_{i,I} a Z_I r_iI/(1 + d r_iI),
  * b_ij = b_anti (opposite spins) or b_para (same spin); params
- * (b_anti, b_para, c, a, d).  Electrons 0..n_up-1 are spin up.  A context
- * holds either two-electron kinds or Slater-Jastrow systems, not both. */
+ * (b_anti, b_para, c, a, d), or (b_anti, b_para, c, a, d, s) with the
+ * exponent scale s as a sixth variational parameter: the basis functions
+ * then take exponents s basis_exponent[m] (the orbital exponents the
+ * reference's example wavefunctions optimise, e.g.
+ * examples/helium_atom_singlet.rs:102-111, tests/sho_optimize.rs:90-106).
+ * Electrons 0..n_up-1 are spin up.  A context holds either two-electron
+ * kinds or Slater-Jastrow systems, not both. */
 #define MOLE_WF_SLATER_JASTROW 3
 
 /* Slater-Jastrow precision: FP64 throughout, or MIXED = Sherman-Morrison
@@ -113,7 +119,7 @@ typedef struct {
   uint32_t n_walkers;
   /* MOLE_WF_SLATER_JASTROW only (ignored otherwise) */
   uint32_t n_basis;
-  const double *basis_exponent;  /* [n_basis] zeta_m */
+  const double *basis_exponent;  /* [n_basis] zeta_m (times the exponent scale, if any) */
   const uint32_t *basis_center;  /* [n_basis] nucleus index */
   uint32_t n_orbitals;           /* >= max(n_up, n_down) */
   const double *orbital_coeff;   /* [n_orbitals][n_basis] */
@@ -353,9 +359,11 @@ mole_status mole_compute_parameter_update(const mole_sr_sums *sums, uint32_t opt
  * between iterations.  log (optional, [iters][8]): sum_e, sum_e2, sum of
  * block means, sum of squared block means, accepted moves, status,
  * n_samples, 0.  params_out (optional): the final parameters, which the
- * context keeps.  One two-electron system, no trace/sample flags.  A
- * singular SR matrix leaves the parameters unchanged at that iteration and
- * returns MOLE_ERR_LINALG after the loop. */
+ * context keeps.  One system (two-electron, or Slater-Jastrow with its 5
+ * or 6 parameters: the step rewrites the device system table the sweep
+ * kernel reads, and with the exponent scale the exponents s zeta0_m), no
+ * trace/sample flags.  A singular SR matrix leaves the parameters unchanged
+ * at that iteration and returns MOLE_ERR_LINALG after the loop. */
 mole_status mole_optimize(mole_ctx *ctx, uint32_t iters, uint32_t steps, uint32_t block_size,
                           uint32_t optimizer, double step, uint32_t sr_mode, double sr_eps,
                           double *log, double *params_out);

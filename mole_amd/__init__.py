@@ -378,9 +378,14 @@ class System:
         return self
 
 
-def slater_jastrow(sj, n_walkers, hamiltonian=H_ELECTRONIC, params=None):
-    """Device system from a mole_amd.systems.SJSystem."""
+def slater_jastrow(sj, n_walkers, hamiltonian=H_ELECTRONIC, params=None, exponent_scale=None):
+    """Device system from a mole_amd.systems.SJSystem.  params: the five
+    Jastrow parameters, or six with the exponent scale s (basis exponents s
+    zeta_m) as a variational parameter; exponent_scale appends s to the
+    system's Jastrow parameters."""
     p = sj.jastrow if params is None else params
+    if exponent_scale is not None:
+        p = list(p)[:5] + [float(exponent_scale)]
     s = System(WF_SLATER_JASTROW, sj.nuclei, sj.charges, p, n_walkers, n_up=sj.n_up,
                n_down=sj.n_dn, hamiltonian=hamiltonian)
     return s.with_basis(sj.zeta, sj.center, sj.coef, getattr(sj, "kind", None))

@@ -83,6 +83,8 @@ struct SysState {
   double v_nn = 0.0;
   uint64_t draw_base = 0, move_base = 0;
   bool any_gto = false, any_p = false;
+  std::vector<double> zeta0;  // basis exponents as given (scaled by params[5] with six parameters)
+  double *d_zeta0 = nullptr;  // the same on the device (the device optimisation loop rescales)
   double *d_ld = nullptr;  // [kSJLdColumns][W] log-determinant state
   // local operators: ops[0] the custom Hamiltonian, ops[1..] the observables
   std::vector<mole_operator> ops = std::vector<mole_operator>(1);
@@ -254,8 +256,10 @@ mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run
                     kSJMaxSpin, kSJMaxSpin, kSJMaxElectrons);
       if (d.n_nuclei < 1 || d.n_nuclei > MOLE_MAX_NUCLEI)
         return fail(MOLE_ERR_SHAPE, "system %u: need 1..%d nuclei", s, MOLE_MAX_NUCLEI);
-      if (d.n_params != (uint32_t)kSJNParam)
-        return fail(MOLE_ERR_SHAPE, "system %u: expected 5 Jastrow parameters", s);
+      if (d.n_params != 5 && d.n_params != (uint32_t)kSJNParam)
+        return fail(MOLE_ERR_SHAPE, "system %u: expected 5 Jastrow parameters, or 6 with the exponent scale", s);
+      if (d.n_params == (uint32_t)kSJNParam && !(d.params[5] > 0.0 && d.params[5] < 1e300))
+        return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: the exponent scale must be positive and finite", s);
       if (d.n_basis < 1 || d.n_basis > 1024 || !d.basis_exponent || !d.basis_center || !d.orbital_coeff)
         return fail(MOLE_ERR_INVALID_ARGUMENT, "system %u: basis/orbital tables missing", s);
       if (d.n_orbitals < std::max(d.n_up, d.n_down))
@@ -310,7 +314,7 @@ void free_ctx(mole_ctx *ctx) {
   if (!ctx) return;
   for (SysState &st : ctx->sys) {
     void *ptrs[] = {st.d_pos, st.d_acc, st.d_trace_mask, st.d_trace_e, st.d_samp_psi,
-                    st.d_samp_g, st.d_tab, st.d_center, st.d_B, st.d_ld};
+                    st.d_samp_g, st.d_tab, st.d_center, st.d_B, st.d_ld, st.d_zeta0};
     for (void *p : ptrs)
       if (p) (void)hipFree(p);
     free_dmc(st.dmc);
@@ -427,7 +431,8 @@ mole_status sj_upload_sysdev(mole_ctx *ctx) {
     d.center = st.d_center;
     d.coef_t = st.d_tab + 4 * st.n_nuc + st.n_basis;
     d.n_orb = st.n_orb;
-    for (int k = 0; k < kSJNParam; ++k) d.p[k] = st.params[k];
+    for (int k = 0; k < kSJNParam; ++k) d.p[k] = k < (int)st.params.size() ? st.params[k] : 1.0;
+    d.opt_exp = st.params.size() == (size_t)kSJNParam ? 1 : 0;
     d.v_nn = st.v_nn;
     d.pos = st.d_pos;
     d.B = st.d_B;
@@ -827,7 +832,9 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       std::vector<double> tab;
       tab.insert(tab.end(), d.nuclei_xyz, d.nuclei_xyz + 3 * d.n_nuclei);
       for (uint32_t I = 0; I < d.n_nuclei; ++I) tab.push_back((double)d.nuclei_charge[I]);
-      tab.insert(tab.end(), d.basis_exponent, d.basis_exponent + d.n_basis);
+      S.zeta0.assign(d.basis_exponent, d.basis_exponent + d.n_basis);
+      for (uint32_t m = 0; m < d.n_basis; ++m)
+        tab.push_back(d.n_params == (uint32_t)kSJNParam ? d.params[5] * d.basis_exponent[m] : d.basis_exponent[m]);
       for (uint32_t m = 0; m < d.n_basis; ++m)
         for (uint32_t k = 0; k < d.n_orbitals; ++k) tab.push_back(d.orbital_coeff[(size_t)k * d.n_basis + m]);
       // centres, then the basis kinds (MOLE_BASIS_*)
@@ -837,6 +844,8 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       S.any_p = std::any_of(cen.begin() + d.n_basis, cen.end(), [](int k) { return k >= MOLE_BASIS_STO_PX; });
       CTX_TRY(hipMalloc(&S.d_tab, sizeof(double) * tab.size()));
       CTX_TRY(hipMemcpy(S.d_tab, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice));
+      CTX_TRY(hipMalloc(&S.d_zeta0, sizeof(double) * d.n_basis));
+      CTX_TRY(hipMemcpy(S.d_zeta0, d.basis_exponent, sizeof(double) * d.n_basis, hipMemcpyHostToDevice));
       CTX_TRY(hipMalloc(&S.d_center, sizeof(int) * cen.size()));
       CTX_TRY(hipMemcpy(S.d_center, cen.data(), sizeof(int) * cen.size(), hipMemcpyHostToDevice));
       const size_t nB = (size_t)S.n_up * S.n_up + (size_t)S.n_down * S.n_down;
@@ -979,6 +988,15 @@ mole_status mole_set_params(mole_ctx *ctx, uint32_t sys, const double *p, uint32
   if (st != MOLE_OK) return st;
   SysState &s = ctx->sys[sys];
   if (!p || n_params != s.n_params) return fail(MOLE_ERR_SHAPE, "expected %u parameters", s.n_params);
+  if (ctx->family == kFamSJ && n_params == (uint32_t)kSJNParam) {
+    if (!(p[5] > 0.0 && p[5] < 1e300)) return fail(MOLE_ERR_INVALID_ARGUMENT, "the exponent scale must be positive");
+    // the exponents the kernels read: s zeta0_m, after the queued sweeps
+    std::vector<double> z(s.zeta0.size());
+    for (size_t m = 0; m < z.size(); ++m) z[m] = p[5] * s.zeta0[m];
+    HIP_TRY(hipSetDevice(ctx->device));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    HIP_TRY(hipMemcpy(s.d_tab + 4 * s.n_nuc, z.data(), sizeof(double) * z.size(), hipMemcpyHostToDevice));
+  }
   s.params.assign(p, p + n_params);
   if (ctx->family == kFamTwo) {
     s.sp.zeta = p[0];
@@ -1082,16 +1100,19 @@ mole_status mole_read_sr_sums(mole_ctx *ctx, uint32_t sys, mole_sr_sums *out) {
   if (st != MOLE_OK) return st;
   std::memset(out, 0, sizeof *out);
   const bool sj = ctx->family == kFamSJ;
-  const int np = sj ? kSJNParam : kNParam2;
+  // the system's parameters (an SJ system without the exponent scale
+  // reports the first five of the six accumulated)
+  const int np = sj ? (int)ctx->sys[sys].n_params : kNParam2;
+  const int nacc = sj ? kSJNParam : kNParam2;  // the accumulator triangle's order
   const int cO = sj ? kSJSumO : kAccSumO, cOE = sj ? kSJSumOE : kAccSumOE, cOO = sj ? kSJSumOO : kAccSumOO;
   out->n_params = np;
   out->n = ctx->sys[sys].n_samples;
   out->sum_e = cols[sj ? kSJSumE : kAccSumE];
-  int t = 0;
   for (int k = 0; k < np; ++k) {
     out->sum_o[k] = cols[cO + k];
     out->sum_oe[k] = cols[cOE + k];
-    for (int l = k; l < np; ++l, ++t) {
+    for (int l = k; l < np; ++l) {
+      const int t = k * nacc - k * (k - 1) / 2 + (l - k);  // upper-triangle position of (k, l)
       out->sum_oo[k * np + l] = cols[cOO + t];
       out->sum_oo[l * np + k] = cols[cOO + t];
     }
@@ -1831,8 +1852,8 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
                                mole_optimizer *opt, double *log, double *params_out) {
   mole_status st = check_ctx(ctx, 0);
   if (st != MOLE_OK) return st;
-  if (ctx->family != kFamTwo || ctx->sys.size() != 1)
-    return fail(MOLE_ERR_UNSUPPORTED, "device-side optimisation covers one two-electron system");
+  if (ctx->sys.size() != 1) return fail(MOLE_ERR_UNSUPPORTED, "device-side optimisation covers one system");
+  const bool sj = ctx->family == kFamSJ;
   if (block_size != ctx->run.block_size)
     return fail(MOLE_ERR_INVALID_ARGUMENT, "block_size %u differs from the context's %u", block_size,
                 ctx->run.block_size);
@@ -1859,7 +1880,13 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
   HIP_TRY(hipMemcpyAsync(ctx->d_params, s.params.data(), sizeof(double) * s.params.size(), hipMemcpyHostToDevice,
                          ctx->stream));
   HIP_TRY(hipMemcpyAsync(ctx->d_opt, opt, sizeof(mole_optimizer), hipMemcpyHostToDevice, ctx->stream));
-  ctx->dev_params = true;
+  if (sj && ctx->sysdev_dirty) {
+    // the sweep kernels read the parameters from the device system table,
+    // which the step kernel updates in place: upload it once, before the loop
+    st = sj_upload_sysdev(ctx);
+    if (st != MOLE_OK) return st;
+  }
+  ctx->dev_params = !sj;
   HIP_TRY(hipEventRecord(ctx->ev0, ctx->stream));
   const uint32_t blocks = steps / block_size;
   uint32_t launches = 0;
@@ -1867,19 +1894,35 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
     // reset statistics (as mole_reset_stats: the first sweep launch starts
     // the accumulators from zero; steps >= 2 blocks, so there is one),
     // Runner::run, reduce, step
-    ctx->fresh_acc = true;
     s.n_samples = s.n_blocks = s.n_moves = 0;
     ctx->block_fill = 0;
     ctx->trace_sweeps = ctx->trace_samples = 0;
+    if (sj) {
+      // as mole_reset_stats: zero accumulators and the launch state's block cursor
+      HIP_TRY(launch_fill(s.d_acc, (size_t)s.n_cols * s.W, 0.0, ctx->stream));
+      if (s.d_obs_acc) HIP_TRY(launch_fill(s.d_obs_acc, 2 * (size_t)MOLE_MAX_OBSERVABLES * s.W, 0.0, ctx->stream));
+      st = sj_push_state(ctx);
+      if (st != MOLE_OK) return st;
+      launches += 2;
+    } else {
+      ctx->fresh_acc = true;
+    }
     st = enqueue_sweeps(ctx, blocks * block_size, block_size, &launches);
     ctx->fresh_acc = false;
     if (st != MOLE_OK) {
       ctx->dev_params = false;
       return st;
     }
-    HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, 0, ctx->stream));
-    HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, ctx->d_opt, ctx->d_optlog + 8 * (size_t)it,
-                            ctx->stream));
+    if (sj) {
+      HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, kSJMaxFirst, kSJMaxCount, ctx->stream));
+      HIP_TRY(launch_sj_vmc_step(ctx->d_red, (double)s.n_samples, (int)s.params.size(), ctx->d_params, ctx->d_opt,
+                                 ctx->d_optlog + 8 * (size_t)it, ctx->d_sysdev, s.d_zeta0, s.d_tab + 4 * s.n_nuc,
+                                 s.n_basis, ctx->stream));
+    } else {
+      HIP_TRY(launch_reduce_columns(s.d_acc, s.W, s.n_cols, ctx->d_red, -1, 0, ctx->stream));
+      HIP_TRY(launch_vmc_step(ctx->d_red, (double)s.n_samples, ctx->d_params, ctx->d_opt,
+                              ctx->d_optlog + 8 * (size_t)it, ctx->stream));
+    }
     launches += 2;
   }
   HIP_TRY(hipEventRecord(ctx->ev1, ctx->stream));
@@ -1892,9 +1935,13 @@ mole_status mole_optimize_with(mole_ctx *ctx, uint32_t iters, uint32_t steps, ui
   HIP_TRY(hipMemcpyAsync(opt, ctx->d_opt, sizeof(mole_optimizer), hipMemcpyDeviceToHost, ctx->stream));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   s.params = p;
-  s.sp.zeta = p[0];
-  s.sp.b = p[1];
-  s.sp.c = p[2];
+  if (sj) {
+    ctx->sysdev_dirty = true;  // the host copy of the system table follows
+  } else {
+    s.sp.zeta = p[0];
+    s.sp.b = p[1];
+    s.sp.c = p[2];
+  }
   s.cols_valid = false;
   if (params_out) std::memcpy(params_out, p.data(), sizeof(double) * p.size());
   if (log) std::memcpy(log, lg.data(), sizeof(double) * lg.size());

@@ -10,7 +10,9 @@
 
 namespace mole {
 
-constexpr int kSJNParam = 5;  // b_anti, b_para, c, a, d
+// b_anti, b_para, c, a, d, and s: the basis exponents are s zeta0_m (the
+// sixth parameter is live when the system declares six parameters)
+constexpr int kSJNParam = 6;
 
 // Proposals of the Slater-Jastrow kernel and their draws per move.
 enum SJProposal : int { kSJPropBox = 0, kSJPropDiffuse = 1, kSJPropLearned = 2 };
@@ -31,7 +33,7 @@ enum SJAccColumn : int {
   kSJSingular = 8,   // spin blocks found singular at a refresh
   kSJSumO = 9,
   kSJSumOE = 9 + kSJNParam,
-  kSJSumOO = 9 + 2 * kSJNParam,  // upper triangle, 15 columns
+  kSJSumOO = 9 + 2 * kSJNParam,  // upper triangle, 21 columns
   kSJColumns = 9 + 2 * kSJNParam + kSJNParam * (kSJNParam + 1) / 2
 };
 // The columns reduced by their maximum (the rest by their sum).
@@ -55,6 +57,7 @@ struct SJSysDev {
   const int *kind;       // [n_basis] MOLE_BASIS_*
   int any_gto;           // 0: no Gaussian basis function
   int any_p;             // 0: no p-type STO (with any_gto == 0, kind is not read)
+  int opt_exp;           // 1: p[5] = s is a parameter (zeta holds s zeta0)
   const double *coef_t;  // [n_basis][n_orb]: lanes over orbitals read consecutive words
   double p[kSJNParam];
   double v_nn;
@@ -143,6 +146,13 @@ hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_
 hipError_t launch_sj_persistent(const SJArgs &a, int workgroups, int fp32, size_t lds_bytes, hipStream_t stream);
 // Resident workgroups of the persistent kernel per CU at lds_bytes of LDS.
 int sj_persistent_occupancy(int fp32, size_t lds_bytes);
+// One device optimiser step of a Slater-Jastrow system from its reduced
+// accumulator columns: params (n_params = 5, or 6 with the exponent scale)
+// and sys->p updated in place, zeta = s zeta0 with the scale; log[8] as the
+// two-electron step's.
+hipError_t launch_sj_vmc_step(const double *red, double n_samples, int n_params, double *params, mole_optimizer *opt,
+                              double *log, SJSysDev *sys, const double *zeta0, double *zeta, int n_basis,
+                              hipStream_t stream);
 // state->sweeps_done += n_sweeps; block fill and trace cursors advance by the
 // launch's samples.
 hipError_t launch_sj_advance(SJLaunchState *state, uint32_t n_sweeps, uint32_t n_samples,

@@ -23,6 +23,7 @@
 #include "device_common.hpp"
 #include "mlp.hpp"
 #include "sj_internal.hpp"
+#include "sr_math.hpp"
 
 namespace mole {
 
@@ -47,9 +48,9 @@ struct SJLayout {
     jo = take(8u * (N + nn));
     f = take(8u * nmax);
     // scratch: Gauss-Jordan (2 n^2 + n), per-electron local-energy results
-    // (8 N), drift pieces (3K + 3n + 3(N + n_nuc))
+    // (9 N), drift pieces (3K + 3n + 3(N + n_nuc))
     const uint32_t lu = 8u * (2u * nmax * nmax + nmax);
-    const uint32_t el = 8u * (8u * N);
+    const uint32_t el = 8u * (9u * N);
     const uint32_t dr = 8u * (3u * K + 3u * nmax + 3u * (N + nn));
     const uint32_t sc = lu > el ? lu : el;
     scr = take(sc > dr ? sc : dr);
@@ -396,7 +397,8 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
                                SJParts *parts = nullptr) {
   const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
   const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
-  double *res = wk.scr;  // [N][8]: kinetic term, potential, five derivatives, e-e potential
+  double *res = wk.scr;  // [N][9]: kinetic term, potential, five Jastrow derivatives, e-e potential,
+                        // exponent-scale derivative
   SJParts own;
   for (int i = t; i < N; i += L) {
     const int s = i < S.n_up ? 0 : 1;
@@ -438,6 +440,33 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
           Dg2 = fma(og[k][2], bkc, Dg2);
           Dl = fma(og[k][3], bkc, Dl);
         }
+    }
+    // exponent scale s (zeta_m = s zeta0_m): sum_m zeta_m d ln psi / d zeta_m
+    // of electron i's row, with d chi_m / d zeta_m = -r chi_m (s- and
+    // p-type STO) or -r^2 chi_m (GTO); fma chains in m, then k order
+    double ez = 0.0;
+    if (S.opt_exp) {
+      for (int k0 = 0; k0 < n; k0 += kSJLanesNMax) {
+        double oz[kSJLanesNMax];
+#pragma unroll
+        for (int k = 0; k < kSJLanesNMax; ++k) oz[k] = 0.0;
+        for (int m = 0; m < K; ++m) {
+          const int I = S.center[m];
+          const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
+          const double r = norm3(d0, d1, d2);
+          const int kd = ((EXT && S.any_gto) || S.any_p) ? S.kind[m] : MOLE_BASIS_STO;
+          const double ch = basis_value<EXT>(kd, S.zeta[m], d0, d1, d2, r);
+          const double rp = (EXT && kd == MOLE_BASIS_GTO) ? r * r : r;
+          const double tz = (-(S.zeta[m] * rp)) * ch;
+          const double *cm = S.coef_t + (size_t)m * no + k0;
+#pragma unroll
+          for (int k = 0; k < kSJLanesNMax; ++k)
+            if (k0 + k < n) oz[k] = fma(cm[k], tz, oz[k]);
+        }
+#pragma unroll
+        for (int k = 0; k < kSJLanesNMax; ++k)
+          if (k0 + k < n) ez = fma(oz[k], (double)B[(k0 + k) * n + cc], ez);
+      }
     }
     double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0;
     double Vi = 0.0, Vee = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
@@ -490,7 +519,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     }
     const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
     const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
-    double *o = res + 8 * i;
+    double *o = res + 9 * i;
     o[0] = Dl + 2.0 * dot + lJ + gg;
     o[1] = Vi;
     o[2] = e0;
@@ -499,17 +528,21 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     o[5] = e3;
     o[6] = e4;
     if constexpr (EXT) o[7] = Vee;
+    o[8] = ez;
   }
   __syncthreads();
   double kin = 0.0, V = 0.0;
   for (int q = 0; q < kSJNParam; ++q) dl[q] = 0.0;
   if (t == 0) {
     for (int i = 0; i < N; ++i) {
-      const double *o = res + 8 * i;
+      const double *o = res + 9 * i;
       kin += o[0];
       V += o[1];
-      for (int q = 0; q < kSJNParam; ++q) dl[q] += o[2 + q];
+      for (int q = 0; q < 5; ++q) dl[q] += o[2 + q];
+      dl[5] += o[8];
     }
+    // d ln psi / d s = (1/s) sum_m zeta_m d ln psi / d zeta_m
+    if (S.opt_exp) dl[5] = dl[5] / S.p[5];
     if (S.hamiltonian != MOLE_H_ELECTRONIC) {
       V = 0.0;
       for (int i = 0; i < N; ++i)
@@ -522,7 +555,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
       if (!parts) parts = &own;
       parts->kin = kin;
       double vee = 0.0, ven = 0.0;
-      for (int i = 0; i < N; ++i) vee += res[8 * i + 7];
+      for (int i = 0; i < N; ++i) vee += res[9 * i + 7];
       // IonicPotential::value's order: ions outer, electrons inner
       for (int I = 0; I < nn; ++I)
         for (int i = 0; i < N; ++i) {
@@ -1139,6 +1172,65 @@ int sj_persistent_occupancy(int fp32, size_t lds_bytes) {
       fp32 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sj_persistent_kernel<float>, 64, lds_bytes)
            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, sj_persistent_kernel<double>, 64, lds_bytes);
   return e == hipSuccess ? n : 0;
+}
+
+namespace {
+// One optimiser step of a Slater-Jastrow system on the device (the loop of
+// VmcRunner::run_optimization, src/vmc/src/vmc.rs:43-179): the reduced
+// sums of the system's NP parameters (the accumulator holds kSJNParam; an
+// NP = 5 system reads the first five), Optimizer::compute_parameter_update,
+// then the parameters the sweep kernel reads (sys->p) and, with the
+// exponent scale, the exponents s zeta0_m.
+template <int NP>
+__global__ void sj_vmc_step_kernel(const double *red, double n_samples, double *params, mole_optimizer *opt,
+                                   double *log, SJSysDev *sys, const double *zeta0, double *zeta, int n_basis) {
+  __shared__ double scale;
+  __shared__ int updated;
+  if (threadIdx.x == 0) {
+    double so[NP], soe[NP], soo[NP * NP], dp[NP], pars[NP];
+    for (int k = 0; k < NP; ++k) {
+      so[k] = red[kSJSumO + k];
+      soe[k] = red[kSJSumOE + k];
+      pars[k] = params[k];
+      for (int l = k; l < NP; ++l) {
+        const int t = k * kSJNParam - k * (k - 1) / 2 + (l - k);  // upper-triangle position of (k, l)
+        soo[k * NP + l] = red[kSJSumOO + t];
+        soo[l * NP + k] = red[kSJSumOO + t];
+      }
+    }
+    const int rc = optimizer_update<NP>(opt, n_samples, red[kSJSumE], so, soe, soo, pars, dp, nullptr);
+    log[0] = red[kSJSumE];
+    log[1] = red[kSJSumE2];
+    log[2] = red[kSJSumBm];
+    log[3] = red[kSJSumBm2];
+    log[4] = red[kSJAccepted];
+    log[5] = (double)rc;
+    log[6] = n_samples;
+    log[7] = 0.0;
+    updated = rc == 0;
+    if (rc == 0)
+      for (int k = 0; k < NP; ++k) {
+        params[k] = pars[k] + dp[k];
+        sys->p[k] = params[k];
+      }
+    scale = NP == kSJNParam ? params[kSJNParam - 1] : 1.0;
+  }
+  __syncthreads();
+  if (NP == kSJNParam && updated)
+    for (int m = threadIdx.x; m < n_basis; m += blockDim.x) zeta[m] = scale * zeta0[m];
+}
+}  // namespace
+
+hipError_t launch_sj_vmc_step(const double *red, double n_samples, int n_params, double *params, mole_optimizer *opt,
+                              double *log, SJSysDev *sys, const double *zeta0, double *zeta, int n_basis,
+                              hipStream_t stream) {
+  if (n_params == kSJNParam)
+    hipLaunchKernelGGL(sj_vmc_step_kernel<kSJNParam>, dim3(1), dim3(64), 0, stream, red, n_samples, params, opt,
+                       log, sys, zeta0, zeta, n_basis);
+  else
+    hipLaunchKernelGGL(sj_vmc_step_kernel<kSJNParam - 1>, dim3(1), dim3(64), 0, stream, red, n_samples, params,
+                       opt, log, sys, zeta0, zeta, n_basis);
+  return hipGetLastError();
 }
 
 hipError_t launch_sj(const SJArgs &a, int n_tiles, int fp32, int proposal, size_t lds_bytes,

@@ -338,7 +338,7 @@ static double sj_local_parts(const orc_sj *wf, const double *cfg, const double *
   int N = wf->n_up + wf->n_dn;
   const double bA = wf->p[0], bP = wf->p[1], c = wf->p[2], a = wf->p[3], d = wf->p[4];
   double kin = 0.0, V = 0.0, vee = 0.0;
-  double dl[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
+  double dl[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0, 0};
   for (int i = 0; i < N; ++i) {
     int s = spin_of(wf, i), cc = col_of(wf, i), n = nspin(wf, s);
     const double *B = s == 0 ? Bup : Bdn;
@@ -367,7 +367,24 @@ static double sj_local_parts(const orc_sj *wf, const double *cfg, const double *
     }
     /* electron i's parameter derivatives are summed on their own and added
      * to the totals in electron order (the device's order) */
-    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0, Vee = 0.0, dli[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0};
+    double gJ[3] = {0, 0, 0}, lJ = 0.0, Vi = 0.0, Vee = 0.0, dli[ORC_SJ_NPARAM] = {0, 0, 0, 0, 0, 0};
+    if (wf->opt_exp) {
+      /* sum_m zeta_m d ln psi / d zeta_m over electron i's row: d chi_m /
+       * d zeta_m = -r chi_m (s- and p-type STO) or -r^2 chi_m (GTO); fma
+       * chains in m, then k order */
+      double ez = 0.0;
+      for (int k = 0; k < n; ++k) {
+        const double *cf = wf->coef + (size_t)k * wf->n_basis;
+        double oz = 0.0;
+        for (int m = 0; m < wf->n_basis; ++m) {
+          const double r = rI[wf->center[m]];
+          const double rp = kind_of(wf, m) == ORC_BASIS_GTO ? r * r : r;
+          oz = fma(cf[m], (-(wf->zeta[m] * rp)) * chi[m], oz);
+        }
+        ez = fma(oz, B[k * n + cc], ez);
+      }
+      dli[5] = ez;
+    }
     for (int j = 0; j < N; ++j) {
       if (j == i) continue;
       const double *xj = cfg + 3 * j;
@@ -438,6 +455,8 @@ static double sj_local_parts(const orc_sj *wf, const double *cfg, const double *
              orc_norm3(xi[0] - wf->nuc[3 * I], xi[1] - wf->nuc[3 * I + 1], xi[2] - wf->nuc[3 * I + 2]);
       }
   }
+  /* d ln psi / d s = (1/s) sum_m zeta_m d ln psi / d zeta_m */
+  if (wf->opt_exp) dl[5] = dl[5] / wf->p[5];
   if (dlog) memcpy(dlog, dl, sizeof dl);
   if (kin_out) *kin_out = kin;
   if (vee_out) *vee_out = vee;

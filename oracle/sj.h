@@ -32,7 +32,9 @@
 extern "C" {
 #endif
 
-#define ORC_SJ_NPARAM 5 /* b_anti, b_para, c, a, d */
+/* b_anti, b_para, c, a, d, and the exponent scale s (live when opt_exp:
+ * zeta then holds s zeta0 and d ln psi / d s is formed) */
+#define ORC_SJ_NPARAM 6
 
 typedef struct {
   int n_up, n_dn, n_nuc, n_basis, n_orb, hamiltonian;
@@ -43,6 +45,7 @@ typedef struct {
   const double *coef;   /* [n_orb][n_basis] */
   double p[ORC_SJ_NPARAM];
   const int *kind;      /* [n_basis] ORC_BASIS_*; NULL: all STO */
+  int opt_exp;          /* 1: p[5] is the exponent scale */
 } orc_sj;
 
 /* basis functions about R, d = x - R, r = |d|: exp(-zeta r) (STO),

@@ -9,7 +9,7 @@ from . import oracle_ffi as OF
 
 PREC_FP64 = 0
 PREC_MIXED = 1
-NPARAM = 5
+NPARAM = 6  # five Jastrow parameters and the exponent scale
 
 _pd = C.POINTER(C.c_double)
 _pi = C.POINTER(C.c_int)
@@ -22,6 +22,7 @@ class Sj(C.Structure):
         ("nuc", _pd), ("charge", _pi), ("zeta", _pd), ("center", _pi), ("coef", _pd),
         ("p", C.c_double * NPARAM),
         ("kind", _pi),
+        ("opt_exp", C.c_int),
     ]
 
 
@@ -109,18 +110,24 @@ class SjWave:
     """Keeps the arrays of a systems.SJSystem alive behind an orc_sj."""
 
     def __init__(self, system, hamiltonian=0, params=None):
+        """params: the five Jastrow parameters, or six with the exponent
+        scale s (the basis exponents become s zeta_m, the device's
+        products)."""
         s = system
         kind = getattr(s, "kind", None)
         kind = np.zeros(s.zeta.size, np.intc) if kind is None else np.ascontiguousarray(kind, dtype=np.intc)
-        self._keep = [s.nuclei, s.charges.astype(np.intc), s.zeta, s.center.astype(np.intc), s.coef, kind]
+        p = s.jastrow if params is None else params
+        opt_exp = len(p) == NPARAM
+        zeta = np.ascontiguousarray(float(p[5]) * s.zeta if opt_exp else s.zeta, dtype=np.float64)
+        self._keep = [s.nuclei, s.charges.astype(np.intc), zeta, s.center.astype(np.intc), s.coef, kind]
         nuc, ch, ze, ce, co, kd = self._keep
         self.st = Sj(n_up=s.n_up, n_dn=s.n_dn, n_nuc=nuc.shape[0], n_basis=ze.size, n_orb=co.shape[0],
                      hamiltonian=hamiltonian, nuc=nuc.ctypes.data_as(_pd), charge=ch.ctypes.data_as(_pi),
                      zeta=ze.ctypes.data_as(_pd), center=ce.ctypes.data_as(_pi),
                      coef=co.ctypes.data_as(_pd), kind=kd.ctypes.data_as(_pi))
-        p = s.jastrow if params is None else params
         for k in range(NPARAM):
-            self.st.p[k] = float(p[k])
+            self.st.p[k] = float(p[k]) if k < len(p) else 1.0
+        self.st.opt_exp = int(opt_exp)
         self.n_up, self.n_dn = s.n_up, s.n_dn
         self.N = s.n_up + s.n_dn
 

@@ -134,6 +134,49 @@ def test_jastrow_parameter_log_derivatives():
             assert abs(dl[k] - fd) <= 1e-6 * max(1.0, abs(fd))
 
 
+@pytest.mark.parametrize("make", [lambda: S.random_molecule(6, seed=5), lambda: S.atom(8),
+                                  lambda: S.harmonic_oscillator(1.0, a=1.0)])
+def test_exponent_scale_log_derivative(make):
+    # d ln psi / d s with the basis exponents s zeta_m: central differences of
+    # ln|psi| over s (s-, p-type STO and GTO bases); the five Jastrow
+    # derivatives are unchanged by the sixth parameter
+    system = make()
+    h = 1e-6
+    for s0 in (1.0, 1.3):
+        p6 = list(system.jastrow) + [s0]
+        for x in cfgs(system.n_electrons, 4, 13, scale=0.9):
+            e6, dl6 = SJ.local_energy(SJ.SjWave(system, params=p6), x)
+            vp = SJ.value(SJ.SjWave(system, params=p6[:5] + [s0 + h]), x)
+            vm = SJ.value(SJ.SjWave(system, params=p6[:5] + [s0 - h]), x)
+            fd = (np.log(abs(vp)) - np.log(abs(vm))) / (2 * h)
+            assert abs(dl6[5] - fd) <= 1e-6 * max(1.0, abs(fd)), (dl6[5], fd)
+            if s0 == 1.0:
+                e5, dl5 = SJ.local_energy(SJ.SjWave(system), x)
+                assert e6 == e5 and np.array_equal(dl6[:5], dl5[:5]) and dl5[5] == 0.0
+
+
+def test_exponent_scale_gaussian_closed_form():
+    # psi = exp(-s zeta0 r^2): d ln psi / d s = -zeta0 r^2 exactly, and under
+    # T + r^2/2 the local energy is 3 s zeta0 + r^2 (1/2 - 2 (s zeta0)^2)
+    system = S.harmonic_oscillator(1.0, a=1.0)  # zeta0 = 1
+    h = M_harmonic()
+    rng = np.random.default_rng(3)
+    for s0 in (0.3, 0.5, 1.0):
+        wv = SJ.SjWave(system, hamiltonian=0, params=list(system.jastrow) + [s0])
+        for _ in range(5):
+            x = rng.normal(size=3)
+            r2 = float(x @ x)
+            _, dl = SJ.local_energy(wv, x)
+            assert dl[5] == pytest.approx(-r2, rel=1e-14, abs=1e-15)
+            e = SJ.operator_value(wv, x, h)
+            z = s0
+            assert e == pytest.approx(3 * z + r2 * (0.5 - 2 * z * z), rel=1e-12, abs=1e-12)
+
+
+def M_harmonic():
+    return [(SJ.TERM_KINETIC, 1.0), (SJ.TERM_HARMONIC, 1.0, 1.0, 0.0, 0.0, 0.0)]
+
+
 @pytest.mark.parametrize("Z", [3, 7, 10])
 def test_sherman_morrison_runner_follows_faithful_runner(Z):
     # same streams: SM ratios/updates (FP64, LU refresh every 16 moves) give the
