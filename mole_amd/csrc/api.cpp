@@ -357,7 +357,7 @@ mole_status sj_build_tiles(mole_ctx *ctx) {
   for (size_t s = 0; s < ctx->sys.size(); ++s) {
     SysState &st = ctx->sys[s];
     st.lds_walker = sj_lds_per_walker((int)st.N, (int)st.n_up, (int)st.n_down, st.n_nuc, st.n_basis,
-                                      sj_tsize(ctx));
+                                      sj_tsize(ctx), (int)st.params.size());
     const int wpt = 64 / st.L;
     lds = std::max(lds, (size_t)wpt * st.lds_walker);
     const int nmax = (int)std::max(st.n_up, st.n_down);
@@ -773,7 +773,7 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
       S.n_orb = (int)d.n_orbitals;
       S.v_nn = nuclear_repulsion(d.nuclei_xyz, d.nuclei_charge, d.n_nuclei);
       S.L = run->proposal == MOLE_PROPOSAL_LEARNED ? 4 : default_lanes((int)S.N);
-      S.n_cols = kSJColumns;
+      S.n_cols = sj_columns((int)d.n_params);
     }
   }
   auto bail = [&](mole_status s) {
@@ -802,7 +802,7 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
   CTX_TRY(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
   CTX_TRY(hipEventCreate(&ctx->ev0));
   CTX_TRY(hipEventCreate(&ctx->ev1));
-  CTX_TRY(hipMalloc(&ctx->d_red, sizeof(double) * std::max<int>(kAccColumns2, kSJColumns)));
+  CTX_TRY(hipMalloc(&ctx->d_red, sizeof(double) * std::max<int>(kAccColumns2, kSJColumnsMax)));
   for (uint32_t s = 0; s < n_sys; ++s) {
     SysState &S = ctx->sys[s];
     const mole_system_desc &d = sys[s];
@@ -1100,11 +1100,10 @@ mole_status mole_read_sr_sums(mole_ctx *ctx, uint32_t sys, mole_sr_sums *out) {
   if (st != MOLE_OK) return st;
   std::memset(out, 0, sizeof *out);
   const bool sj = ctx->family == kFamSJ;
-  // the system's parameters (an SJ system without the exponent scale
-  // reports the first five of the six accumulated)
+  // the system's parameters (a Slater-Jastrow system accumulates its own 5 or 6)
   const int np = sj ? (int)ctx->sys[sys].n_params : kNParam2;
-  const int nacc = sj ? kSJNParam : kNParam2;  // the accumulator triangle's order
-  const int cO = sj ? kSJSumO : kAccSumO, cOE = sj ? kSJSumOE : kAccSumOE, cOO = sj ? kSJSumOO : kAccSumOO;
+  const int nacc = np;  // the accumulator triangle's order
+  const int cO = sj ? kSJSumO : kAccSumO, cOE = sj ? sj_sum_oe(np) : kAccSumOE, cOO = sj ? sj_sum_oo(np) : kAccSumOO;
   out->n_params = np;
   out->n = ctx->sys[sys].n_samples;
   out->sum_e = cols[sj ? kSJSumE : kAccSumE];

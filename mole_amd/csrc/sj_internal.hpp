@@ -10,8 +10,8 @@
 
 namespace mole {
 
-// b_anti, b_para, c, a, d, and s: the basis exponents are s zeta0_m (the
-// sixth parameter is live when the system declares six parameters)
+// At most six parameters: b_anti, b_para, c, a, d, and s (the basis
+// exponents are s zeta0_m), the sixth only when the system declares it.
 constexpr int kSJNParam = 6;
 
 // Proposals of the Slater-Jastrow kernel and their draws per move.
@@ -31,11 +31,14 @@ enum SJAccColumn : int {
   kSJCondMax = 6,    // largest ||A||_1 ||A^-1||_1 seen at an LU refresh
   kSJLdDrift = 7,    // largest |log|det| accumulated - log|det| refreshed| (FP64)
   kSJSingular = 8,   // spin blocks found singular at a refresh
-  kSJSumO = 9,
-  kSJSumOE = 9 + kSJNParam,
-  kSJSumOO = 9 + 2 * kSJNParam,  // upper triangle, 21 columns
-  kSJColumns = 9 + 2 * kSJNParam + kSJNParam * (kSJNParam + 1) / 2
+  kSJSumO = 9,  // then sum O [np], sum O E [np], sum O O^T [np (np + 1) / 2]
 };
+// Column layout of a system with np parameters (5, or 6 with the exponent
+// scale): only the system's own parameters take columns, in HBM and in LDS.
+__host__ __device__ constexpr int sj_sum_oe(int np) { return kSJSumO + np; }
+__host__ __device__ constexpr int sj_sum_oo(int np) { return kSJSumO + 2 * np; }  // upper triangle, row-major
+__host__ __device__ constexpr int sj_columns(int np) { return kSJSumO + 2 * np + np * (np + 1) / 2; }
+constexpr int kSJColumnsMax = sj_columns(kSJNParam);
 // The columns reduced by their maximum (the rest by their sum).
 constexpr int kSJMaxFirst = kSJCondMax, kSJMaxCount = 2;
 
@@ -63,7 +66,7 @@ struct SJSysDev {
   double v_nn;
   double *pos;          // [3N][W]
   void *B;              // [n_up^2 + n_dn^2][W], element type of the launch
-  double *acc;          // [kSJColumns][W]
+  double *acc;          // [sj_columns(5 + opt_exp)][W]
   double *ld;           // [kSJLdColumns][W] log-determinant state
   uint64_t *trace_mask; // [W][cap_sweeps] or null
   double *trace_e;      // [W][cap_samples] or null
@@ -134,7 +137,7 @@ struct SJArgs {
   int n_tiles;             // persistent launch: tiles in `tiles`
 };
 
-size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize);
+size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize, int np);
 
 // Draws start at draw_base + D N sweeps_done (D draws per move of the
 // proposal) and moves at move_base + N sweeps_done for every system; sample

@@ -30,7 +30,7 @@ namespace mole {
 // LDS layout of one walker (bytes); identical on host and device.
 struct SJLayout {
   uint32_t pos, B, rng, rn, chi, phin, jn, jo, f, scr, flag, mv, acc, ld, bytes;
-  __host__ __device__ SJLayout(int N, int nu, int nd, int nn, int K, int tsize) {
+  __host__ __device__ SJLayout(int N, int nu, int nd, int nn, int K, int tsize, int np) {
     const int nmax = nu > nd ? nu : nd;
     uint32_t o = 0;
     auto take = [&](uint32_t n_bytes) {
@@ -48,22 +48,25 @@ struct SJLayout {
     jo = take(8u * (N + nn));
     f = take(8u * nmax);
     // scratch: Gauss-Jordan (2 n^2 + n), per-electron local-energy results
-    // (9 N), drift pieces (3K + 3n + 3(N + n_nuc))
+    // ((3 + np) N), drift pieces (3K + 3n + 3(N + n_nuc))
     const uint32_t lu = 8u * (2u * nmax * nmax + nmax);
-    const uint32_t el = 8u * (9u * N);
+    const uint32_t el = 8u * ((3u + (uint32_t)np) * N);
     const uint32_t dr = 8u * (3u * K + 3u * nmax + 3u * (N + nn));
     const uint32_t sc = lu > el ? lu : el;
     scr = take(sc > dr ? sc : dr);
     flag = take(16u);
     mv = take(64u);  // proposed position and drift (Gaussian proposals)
-    acc = take(8u * kSJColumns);
-    ld = take(8u * kSJLdColumns);  // directly after acc: addressed as acc + kSJColumns
-    bytes = o;
+    acc = take(8u * sj_columns(np));
+    ld = take(8u * kSJLdColumns);  // directly after acc: addressed as acc + sj_columns(np)
+    // the walkers of a tile sit bytes apart; an odd number of 8-byte words
+    // puts the same field of consecutive walkers in different LDS banks
+    // (an even count maps many walkers of a wave onto one bank)
+    bytes = ((o / 8u) % 2u == 0u) ? o + 8u : o;
   }
 };
 
-__host__ size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize) {
-  return SJLayout(N, nu, nd, nn, K, tsize).bytes;
+__host__ size_t sj_lds_per_walker(int N, int nu, int nd, int nn, int K, int tsize, int np) {
+  return SJLayout(N, nu, nd, nn, K, tsize, np).bytes;
 }
 
 namespace {
@@ -310,7 +313,7 @@ __device__ SJRefresh refresh_spin(const SJSysDev &S, SJWalker<T> &wk, int s, int
 template <typename T>
 __device__ void refresh_walker(const SJSysDev &S, SJWalker<T> &wk, double *acc, int t, int L, bool gto, bool pk,
                                bool write_B, bool count, int ld_mode) {
-  double *ld = acc + kSJColumns;
+  double *ld = acc + sj_columns(5 + S.opt_exp);
 #pragma unroll 1
   for (int q = 0; q < 2; ++q) {
     const SJRefresh r = refresh_spin<T>(S, wk, q, t, L, gto, pk, write_B);
@@ -392,13 +395,48 @@ __device__ double operator_value(const mole_operator &op, const SJParts &P, cons
   return v;
 }
 
+// Exponent scale s (zeta_m = s zeta0_m): sum_m zeta_m d ln psi / d zeta_m
+// of electron i (at xi, column cc of its n x n inverse B), with d chi_m /
+// d zeta_m = -r chi_m (s- and p-type STO) or -r^2 chi_m (GTO); fma chains in
+// m, then k order.  Out of line: only systems with the scale call it, and the
+// sweep kernels keep their code size.
+template <typename T, bool EXT>
+__device__ __noinline__ double exponent_scale_term(const SJSysDev &S, const T *B, int n, int cc, const double *xi) {
+  const int K = S.n_basis, no = S.n_orb;
+  double ez = 0.0;
+  for (int k0 = 0; k0 < n; k0 += kSJLanesNMax) {
+    double oz[kSJLanesNMax];
+#pragma unroll
+    for (int k = 0; k < kSJLanesNMax; ++k) oz[k] = 0.0;
+    for (int m = 0; m < K; ++m) {
+      const int I = S.center[m];
+      const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
+      const double r = norm3(d0, d1, d2);
+      const int kd = ((EXT && S.any_gto) || S.any_p) ? S.kind[m] : MOLE_BASIS_STO;
+      const double ch = basis_value<EXT>(kd, S.zeta[m], d0, d1, d2, r);
+      const double rp = (EXT && kd == MOLE_BASIS_GTO) ? r * r : r;
+      const double tz = (-(S.zeta[m] * rp)) * ch;
+      const double *cm = S.coef_t + (size_t)m * no + k0;
+#pragma unroll
+      for (int k = 0; k < kSJLanesNMax; ++k)
+        if (k0 + k < n) oz[k] = fma(cm[k], tz, oz[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < kSJLanesNMax; ++k)
+      if (k0 + k < n) ez = fma(oz[k], (double)B[(k0 + k) * n + cc], ez);
+  }
+  return ez;
+}
+
 template <typename T, bool EXT>
 __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl,
                                SJParts *parts = nullptr) {
   const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
   const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
-  double *res = wk.scr;  // [N][9]: kinetic term, potential, five Jastrow derivatives, e-e potential,
-                        // exponent-scale derivative
+  // [N][RS]: kinetic term, potential, five Jastrow derivatives, e-e potential
+  // (EXT), exponent-scale derivative (with the scale); RS = 3 + np
+  double *res = wk.scr;
+  const int RS = 8 + S.opt_exp;
   SJParts own;
   for (int i = t; i < N; i += L) {
     const int s = i < S.n_up ? 0 : 1;
@@ -441,33 +479,8 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
           Dl = fma(og[k][3], bkc, Dl);
         }
     }
-    // exponent scale s (zeta_m = s zeta0_m): sum_m zeta_m d ln psi / d zeta_m
-    // of electron i's row, with d chi_m / d zeta_m = -r chi_m (s- and
-    // p-type STO) or -r^2 chi_m (GTO); fma chains in m, then k order
-    double ez = 0.0;
-    if (S.opt_exp) {
-      for (int k0 = 0; k0 < n; k0 += kSJLanesNMax) {
-        double oz[kSJLanesNMax];
-#pragma unroll
-        for (int k = 0; k < kSJLanesNMax; ++k) oz[k] = 0.0;
-        for (int m = 0; m < K; ++m) {
-          const int I = S.center[m];
-          const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
-          const double r = norm3(d0, d1, d2);
-          const int kd = ((EXT && S.any_gto) || S.any_p) ? S.kind[m] : MOLE_BASIS_STO;
-          const double ch = basis_value<EXT>(kd, S.zeta[m], d0, d1, d2, r);
-          const double rp = (EXT && kd == MOLE_BASIS_GTO) ? r * r : r;
-          const double tz = (-(S.zeta[m] * rp)) * ch;
-          const double *cm = S.coef_t + (size_t)m * no + k0;
-#pragma unroll
-          for (int k = 0; k < kSJLanesNMax; ++k)
-            if (k0 + k < n) oz[k] = fma(cm[k], tz, oz[k]);
-        }
-#pragma unroll
-        for (int k = 0; k < kSJLanesNMax; ++k)
-          if (k0 + k < n) ez = fma(oz[k], (double)B[(k0 + k) * n + cc], ez);
-      }
-    }
+    // exponent scale s: sum_m zeta_m d ln psi / d zeta_m of electron i's row
+    const double ez = S.opt_exp ? exponent_scale_term<T, EXT>(S, B, n, cc, xi) : 0.0;
     double gJ0 = 0.0, gJ1 = 0.0, gJ2 = 0.0, lJ = 0.0;
     double Vi = 0.0, Vee = 0.0, e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0, e4 = 0.0;
     for (int j = 0; j < N; ++j) {
@@ -519,7 +532,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     }
     const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
     const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
-    double *o = res + 9 * i;
+    double *o = res + RS * i;
     o[0] = Dl + 2.0 * dot + lJ + gg;
     o[1] = Vi;
     o[2] = e0;
@@ -528,18 +541,18 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     o[5] = e3;
     o[6] = e4;
     if constexpr (EXT) o[7] = Vee;
-    o[8] = ez;
+    if (S.opt_exp) o[8] = ez;
   }
   __syncthreads();
   double kin = 0.0, V = 0.0;
   for (int q = 0; q < kSJNParam; ++q) dl[q] = 0.0;
   if (t == 0) {
     for (int i = 0; i < N; ++i) {
-      const double *o = res + 9 * i;
+      const double *o = res + RS * i;
       kin += o[0];
       V += o[1];
       for (int q = 0; q < 5; ++q) dl[q] += o[2 + q];
-      dl[5] += o[8];
+      if (S.opt_exp) dl[5] += o[8];
     }
     // d ln psi / d s = (1/s) sum_m zeta_m d ln psi / d zeta_m
     if (S.opt_exp) dl[5] = dl[5] / S.p[5];
@@ -555,7 +568,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
       if (!parts) parts = &own;
       parts->kin = kin;
       double vee = 0.0, ven = 0.0;
-      for (int i = 0; i < N; ++i) vee += res[9 * i + 7];
+      for (int i = 0; i < N; ++i) vee += res[RS * i + 7];
       // IonicPotential::value's order: ions outer, electrons inner
       for (int I = 0; I < nn; ++I)
         for (int i = 0; i < N; ++i) {
@@ -697,7 +710,8 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
   const int lg_slots = __ffs(nslots) - 1;  // nslots is a power of two
   const bool live = slot < tile.nw;
   const int w = tile.w0 + slot;
-  const SJLayout lay(N, nu, nd, S.n_nuc, S.n_basis, (int)sizeof(T));
+  const int np = 5 + S.opt_exp, ncol = sj_columns(np);
+  const SJLayout lay(N, nu, nd, S.n_nuc, S.n_basis, (int)sizeof(T), np);
   unsigned char *base = lds + (size_t)slot * lay.bytes;
   SJWalker<T> wk;
   wk.pos = (double *)(base + lay.pos);
@@ -739,11 +753,11 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
   if (t == 0) {
     // DMC steps touch only the accepted-move count and the condition maximum
     const bool dmc_step = EXT && a.dmc == kSJDmcStep;
-    for (int k = 0; k < kSJColumns; ++k)
+    for (int k = 0; k < ncol; ++k)
       if (!dmc_step || k == kSJAccepted || k == kSJCondMax || k == kSJSingular)
         acc[k] = live ? S.acc[(size_t)k * W + w] : 0.0;
     if (track_ld)
-      for (int k = 0; k < kSJLdColumns; ++k) acc[kSJColumns + k] = live ? S.ld[(size_t)k * W + w] : 1.0;
+      for (int k = 0; k < kSJLdColumns; ++k) acc[ncol + k] = live ? S.ld[(size_t)k * W + w] : 1.0;
   }
   __syncthreads();
 
@@ -830,40 +844,40 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
         xn[1] = xi[1] + wk.rng[4 * i + 1];
         xn[2] = xi[2] + wk.rng[4 * i + 2];
       } else if constexpr (PROP == kSJPropLearned) {
-        // learned proposal: network at x -> (mu, ln sigma), x' = (x_i + mu) +
-        // sigma z; network at x' for the reverse density (all lanes run the
-        // MFMA passes)
-        float fv[4];
-        mlp_features(wk.pos, N, S.nuc, S.n_nuc, i, xi, t, fv);
-        const f32x4 o1 = mlp_forward(a.net, features_bf16(fv), nullptr);
-        if (t == 0) {
-          const double ls = mlp_log_sigma((double)o1[3]);
-          const double sg = exp_m(ls);
-          double z2 = 0.0;
-          for (int q = 0; q < 3; ++q) {
-            const double z = wk.rng[4 * i + q];
-            wk.mv[q] = (xi[q] + (double)o1[q]) + sg * z;
-            z2 += z * z;
+        // learned proposal: pass 0, the network at x -> (mu, ln sigma), x' =
+        // (x_i + mu) + sigma z; pass 1, the network at x' for the reverse
+        // density (all lanes run the MFMA passes).  One loop body keeps a
+        // single copy of the features and the network in the kernel's code.
+#pragma unroll 1
+        for (int pass = 0; pass < 2; ++pass) {
+          float fv[4];
+          mlp_features(wk.pos, N, S.nuc, S.n_nuc, i, pass == 0 ? xi : wk.mv, t, fv);
+          const f32x4 o = mlp_forward(a.net, features_bf16(fv), nullptr);
+          if (t == 0) {
+            const double ls = mlp_log_sigma((double)o[3]);
+            const double sg = exp_m(ls);
+            if (pass == 0) {
+              double z2 = 0.0;
+              for (int q = 0; q < 3; ++q) {
+                const double z = wk.rng[4 * i + q];
+                wk.mv[q] = (xi[q] + (double)o[q]) + sg * z;
+                z2 += z * z;
+              }
+              wk.mv[3] = -3.0 * ls - 0.5 * z2;  // log q(x'|x) + const
+            } else {
+              double e2 = 0.0;
+              for (int q = 0; q < 3; ++q) {
+                const double e = (xi[q] - wk.mv[q]) - (double)o[q];
+                e2 += e * e;
+              }
+              wk.mv[4] = (-3.0 * ls - e2 / (2.0 * sg * sg)) - wk.mv[3];  // log q(x|x') - log q(x'|x)
+            }
           }
-          wk.mv[3] = -3.0 * ls - 0.5 * z2;  // log q(x'|x) + const
+          __syncthreads();
         }
-        __syncthreads();
         xn[0] = wk.mv[0];
         xn[1] = wk.mv[1];
         xn[2] = wk.mv[2];
-        mlp_features(wk.pos, N, S.nuc, S.n_nuc, i, xn, t, fv);
-        const f32x4 o2 = mlp_forward(a.net, features_bf16(fv), nullptr);
-        if (t == 0) {
-          const double ls2 = mlp_log_sigma((double)o2[3]);
-          const double sg2 = exp_m(ls2);
-          double e2 = 0.0;
-          for (int q = 0; q < 3; ++q) {
-            const double e = (xi[q] - xn[q]) - (double)o2[q];
-            e2 += e * e;
-          }
-          wk.mv[4] = (-3.0 * ls2 - e2 / (2.0 * sg2 * sg2)) - wk.mv[3];  // log q(x|x') - log q(x'|x)
-        }
-        __syncthreads();
       } else {
         // drift at x (metrop.rs:150-163): x' = (x_i + tau v) + sqrt(tau) z
         basis_to_lds(S, xi, wk.rn, wk.chi, t, L, gto, pk);
@@ -942,7 +956,7 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
         wk.flag[0] = accept ? 1.0 : 0.0;
         wk.flag[1] = (double)((T)1 / rd);
         // det A_s' = R_D det A_s, carried in FP64
-        if (accept && track_ld) logdet_mul(acc[kSJColumns + 2 * s], acc[kSJColumns + 2 * s + 1], rdd);
+        if (accept && track_ld) logdet_mul(acc[ncol + 2 * s], acc[ncol + 2 * s + 1], rdd);
       }
       __syncthreads();
       const bool accept = wk.flag[0] != 0.0;
@@ -1028,10 +1042,10 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
         acc[kSJSumE2] = fma(E, E, acc[kSJSumE2]);
         acc[kSJBlockSum] += E;
         int q = 0;
-        for (int k = 0; k < kSJNParam; ++k) {
+        for (int k = 0; k < np; ++k) {
           acc[kSJSumO + k] += dlg[k];
-          acc[kSJSumOE + k] = fma(dlg[k], E, acc[kSJSumOE + k]);
-          for (int l = k; l < kSJNParam; ++l, ++q) acc[kSJSumOO + q] = fma(dlg[k], dlg[l], acc[kSJSumOO + q]);
+          acc[sj_sum_oe(np) + k] = fma(dlg[k], E, acc[sj_sum_oe(np) + k]);
+          for (int l = k; l < np; ++l, ++q) acc[sj_sum_oo(np) + q] = fma(dlg[k], dlg[l], acc[sj_sum_oo(np) + q]);
         }
         if (live && S.trace_e && sidx < a.cap_samples) S.trace_e[(size_t)w * a.cap_samples + sidx] = E;
       }
@@ -1060,11 +1074,11 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
     if (sl < tile.nw) Bo[(size_t)k * W + tile.w0 + sl] = ((T *)(lds + (size_t)sl * lay.bytes + lay.B))[k];
   }
   if (t == 0 && live) {
-    for (int k = 0; k < kSJColumns; ++k)
+    for (int k = 0; k < ncol; ++k)
       if (!(EXT && a.dmc == kSJDmcStep) || k == kSJAccepted || k == kSJCondMax || k == kSJSingular)
         S.acc[(size_t)k * W + w] = acc[k];
     if (track_ld)
-      for (int k = 0; k < kSJLdColumns; ++k) S.ld[(size_t)k * W + w] = acc[kSJColumns + k];
+      for (int k = 0; k < kSJLdColumns; ++k) S.ld[(size_t)k * W + w] = acc[ncol + k];
   }
 }
 
@@ -1177,8 +1191,8 @@ int sj_persistent_occupancy(int fp32, size_t lds_bytes) {
 namespace {
 // One optimiser step of a Slater-Jastrow system on the device (the loop of
 // VmcRunner::run_optimization, src/vmc/src/vmc.rs:43-179): the reduced
-// sums of the system's NP parameters (the accumulator holds kSJNParam; an
-// NP = 5 system reads the first five), Optimizer::compute_parameter_update,
+// sums of the system's NP parameters (columns sj_columns(NP)),
+// Optimizer::compute_parameter_update,
 // then the parameters the sweep kernel reads (sys->p) and, with the
 // exponent scale, the exponents s zeta0_m.
 template <int NP>
@@ -1190,12 +1204,12 @@ __global__ void sj_vmc_step_kernel(const double *red, double n_samples, double *
     double so[NP], soe[NP], soo[NP * NP], dp[NP], pars[NP];
     for (int k = 0; k < NP; ++k) {
       so[k] = red[kSJSumO + k];
-      soe[k] = red[kSJSumOE + k];
+      soe[k] = red[sj_sum_oe(NP) + k];
       pars[k] = params[k];
       for (int l = k; l < NP; ++l) {
-        const int t = k * kSJNParam - k * (k - 1) / 2 + (l - k);  // upper-triangle position of (k, l)
-        soo[k * NP + l] = red[kSJSumOO + t];
-        soo[l * NP + k] = red[kSJSumOO + t];
+        const int t = k * NP - k * (k - 1) / 2 + (l - k);  // upper-triangle position of (k, l)
+        soo[k * NP + l] = red[sj_sum_oo(NP) + t];
+        soo[l * NP + k] = red[sj_sum_oo(NP) + t];
       }
     }
     const int rc = optimizer_update<NP>(opt, n_samples, red[kSJSumE], so, soe, soo, pars, dp, nullptr);
