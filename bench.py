@@ -326,6 +326,7 @@ def learned_record(log_runs=False):
                 best_box_side_ess=r["best_box_side_ess"], best_box_side_tau=r["best_box_side_tau"],
                 tau_ratio_vs_best_box=r["tau_ratio_vs_best_box"],
                 energy_diff_sigma=r["energy_diff_sigma"],
+                energy_diff_sigma_vs_best_ess_box=r["energy_diff_sigma_vs_best_ess_box"],
                 tau_estimator="scripts/statfor.rs:30-54 on the device (mole_tau_int)")
 
 

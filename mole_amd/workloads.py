@@ -205,7 +205,7 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
     return out
 
 
-def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200, train_steps=8000,
+def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=1000, train_steps=8000,
             batch=4096, lr=1e-3, sweeps=400, block=50, box_sides=(0.1, 0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.5),
             precision=M.PREC_FP64, lu_period=64, seed=1, objective=M.MLP_OBJ_LOG_ESJD, collect_side=0.3,
             log=print):
@@ -215,8 +215,11 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
     W S / (tau_int t) with tau_int of each walker's E_L series by the statfor
     estimator (scripts/statfor.rs:30-54) on the device, averaged over walkers,
     and kappa = 1 / (sigma(E)^2 t) with the blocking error; t is the device
-    time of the S sampled sweeps.  Learned against the box side of the
-    largest kappa (matched sigma(E): kappa is 1/(sigma^2 t))."""
+    time of the S sampled sweeps.  Every run first equilibrates for `equil`
+    sweeps of its own proposal.  Energies: the learned run against the box
+    sides of the largest kappa (matched sigma(E): kappa is 1/(sigma^2 t))
+    and of the largest ESS/s, in units of the combined error (the larger of
+    the blocking and the tau_int-based error of each)."""
     sj = systems.atom(Z)
     N = sj.n_electrons
 
@@ -253,7 +256,7 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
             else:
                 autotune.apply(ctx, [sj], box_lanes)
             ctx.set_configs(start)
-            ctx.sweep(equil // 2)  # the proposal's own equilibration
+            ctx.sweep(equil)  # the proposal's own equilibration (small box sides mix slowly)
             ctx.synchronize()
             t0 = time.perf_counter()
             ctx.run(sweeps + block)
@@ -287,6 +290,7 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
     best_tau = min(runs, key=lambda r: r["tau_int_mean"])
     diff = lrn["energy"] - best_kappa["energy"]
     sig = lambda r: max(r["error"], r["error_tau"])
+    diff_ess = lrn["energy"] - best_ess["energy"]
     rec = dict(workload="learned", Z=Z, element=sj.name, walkers=walkers, sweeps=sweeps, block=block,
                precision="mixed" if precision == M.PREC_MIXED else "fp64",
                training=dict(configs=n_data, steps=train_steps, batch=batch, lr=lr,
@@ -301,7 +305,9 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=200
                best_box_side_tau=best_tau["step"],
                tau_ratio_vs_best_box=best_tau["tau_int_mean"] / lrn["tau_int_mean"],
                energy_diff_vs_box=diff,
-               energy_diff_sigma=diff / float(np.hypot(sig(lrn), sig(best_kappa))))
+               energy_diff_sigma=diff / float(np.hypot(sig(lrn), sig(best_kappa))),
+               energy_diff_vs_best_ess_box=diff_ess,
+               energy_diff_sigma_vs_best_ess_box=diff_ess / float(np.hypot(sig(lrn), sig(best_ess))))
     log(rec)
     return rec
 

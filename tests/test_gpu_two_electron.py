@@ -3,6 +3,7 @@ CPU oracle on the same Philox streams: device primitives bit for bit, accept
 decisions bit for bit, local energies within 1e-10, reductions within
 rounding, and the reference's statistical anchors."""
 import ctypes as C
+import os
 import struct
 
 import numpy as np
@@ -398,6 +399,38 @@ def test_benchmark_shape_parity_and_device_loop():
     assert np.all(log[:, 5] == 0.0)
     assert np.array_equal(p.view(np.uint64), host_p.view(np.uint64))
     assert np.array_equal(cfg_dev.view(np.uint64), cfg_host.view(np.uint64))
+
+
+def test_benchmark_sr_trajectory_matches_oracle():
+    # the benchmark's whole optimisation: 50 SR iterations of 16,384
+    # persistent walkers x 200 sweeps (block 20, box side 0.5, step 0.05,
+    # shift 1e-3, parameters from (1.0, 0.5, 0.3), seed 0) on the device
+    # loop, against the oracle's VmcRunner with persistent walkers (two full
+    # psi evaluations per move, gradient and SR matrix from the stored
+    # samples) on the host cores.  Accept decisions are bit-exact; the sums
+    # are formed in different orders, so energies and parameters agree to
+    # rounding, iteration by iteration.
+    iters, W, S, B, side, p0, seed = 50, 16384, 200, 20, 0.5, (1.0, 0.5, 0.3), 0
+    print(f"benchmark SR trajectory: {iters} iterations of W={W} x {S} sweeps", flush=True)
+    owf = O.h2(*p0)
+    op = O.OptParams(optimizer=O.OPT_SR, step=0.05, sr_mode=O.SR_TEXTBOOK, sr_eps=1e-3,
+                     o_convention=O.O_LOGDERIV, persistent=1)
+    en, er, ac, gr, pa = (np.zeros(iters), np.zeros(iters), np.zeros(iters), np.zeros(iters * 3),
+                          np.zeros(iters * 3))
+    threads = max(1, min(16, len(os.sched_getaffinity(0))))
+    rc = O.lib().orc_vmc_optimize(C.byref(owf), seed, side, iters, W * S, B, W, C.byref(op), threads,
+                                  O.dptr(en), O.dptr(er), O.dptr(ac), O.dptr(gr), O.dptr(pa))
+    assert rc == 0
+    with M.Context([M.h2(W, *p0)], seed=seed, step=side, block_size=B) as ctx:
+        ctx.init_walkers()
+        p, log = ctx.optimize(iters, S, B, M.OPT_SR, 0.05, M.SR_TEXTBOOK, 1e-3)
+    assert np.all(log[:, 5] == 0.0)
+    e_dev = log[:, 0] / log[:, 6]
+    assert np.allclose(e_dev, en, rtol=1e-11, atol=0.0), np.max(np.abs(e_dev - en))
+    assert np.allclose(log[:, 4] / (2.0 * W * S), ac, rtol=0.0, atol=1e-15)
+    assert np.allclose(p, pa[-3:], rtol=1e-9, atol=0.0), (p, pa[-3:])
+    # the parameters moved (zeta from 1.0 to ~1.26)
+    assert abs(pa[-3] - 1.0) > 0.1
 
 
 @pytest.mark.parametrize("kind,kw", [(M.OPT_MOMENTUM, dict(momentum=0.5)), (M.OPT_NESTEROV, dict(momentum=0.5)),
