@@ -9,7 +9,9 @@ batch, on its own context, is timed under the tables of several LDS budgets
 (the budget caps the footprint the largest systems run at, hence the
 occupancy of their launches), both tile schedules (one launch per footprint
 class, or one persistent launch pulling longest-first tiles from a device
-ticket) and several sweeps per graph-captured launch; the fastest
+ticket), tables that take the fewest lanes within a tolerance of each
+count's best single-system time, and several sweeps per graph-captured
+launch; the fastest
 combination is applied.  Results do not depend on any of these choices
 (every sum has a fixed order), so tuning only moves time.  Tables can be
 saved as JSON and reused.
@@ -22,6 +24,7 @@ import mole_amd as M
 
 CANDIDATES = (1, 2, 4, 8, 16, 32, 64)
 BUDGETS = (16 << 10, 24 << 10, 32 << 10, 40 << 10, 64 << 10, None)
+TOLERANCES = (0.05, 0.15)
 
 
 def time_lanes(system, walkers, lanes, *, sweeps=6, precision=M.PREC_MIXED, lu_period=64, seed=7,
@@ -109,6 +112,24 @@ def table_for_budget(timings, budget):
     return out
 
 
+def table_within(timings, tol):
+    """Per electron count, the fewest lanes whose single-system time is
+    within (1 + tol) of that count's best.  Fewer lanes put more walkers in a
+    wave; in a full batch, where every CU already has work, that can beat the
+    lane count that wins on one system alone."""
+    runs = {}
+    for key, ms in timings.items():
+        if key.startswith("_") or ms is None:
+            continue
+        N, L = (int(v) for v in key.split(":"))
+        runs.setdefault(N, []).append((L, ms))
+    out = {}
+    for N, rs in runs.items():
+        best = min(ms for _, ms in rs)
+        out[N] = min(L for L, ms in rs if ms <= (1.0 + tol) * best)
+    return out
+
+
 def _time_config(ctx, systems, table, schedule, chunk, reps):
     apply(ctx, systems, table)
     ctx.set_tile_schedule(schedule)
@@ -126,12 +147,14 @@ def _time_config(ctx, systems, table, schedule, chunk, reps):
 
 
 def tune_batch(ctx, systems, timings, *, budgets=BUDGETS, schedules=(M.SCHEDULE_CLASSES, M.SCHEDULE_PERSISTENT),
-               chunks=(5, 10, 20), reps=2, base_chunk=5, log=None):
+               tolerances=TOLERANCES, chunks=(5, 10, 20), reps=2, base_chunk=5, log=None):
     """Whole-batch tuning on an initialised context: seconds per sweep under
     every (LDS budget, schedule) pair at base_chunk sweeps per launch, then
-    the best pair at each chunk.  Applies the winner (lanes and schedule) to
-    ctx and returns (best, trials) with best = dict(budget, schedule, chunk,
-    table, s_per_sweep)."""
+    the fewer-lanes tables of table_within(tolerance) under the best
+    schedule, then the best configuration at each chunk.  Applies the winner
+    (lanes and schedule) to ctx and returns (best, trials) with best =
+    dict(budget, schedule, chunk, table, s_per_sweep) (budget None and a
+    tolerance key for a fewer-lanes table)."""
     trials = []
     tables = {}
     for b in budgets:
@@ -155,11 +178,25 @@ def tune_batch(ctx, systems, timings, *, budgets=BUDGETS, schedules=(M.SCHEDULE_
                 log(rec)
             if best is None or sps < best["s_per_sweep"]:
                 best = dict(rec, table=table)
+    for tol in tolerances:
+        table = table_within(timings, tol)
+        key = tuple(sorted(table.items()))
+        if key in tables.values():
+            continue
+        tables[f"tol{tol}"] = key
+        sps = _time_config(ctx, systems, table, best["schedule"], base_chunk, reps)
+        rec = dict(budget=None, tolerance=tol, schedule=best["schedule"], chunk=base_chunk, s_per_sweep=sps)
+        trials.append(rec)
+        if log:
+            log(rec)
+        if sps < best["s_per_sweep"]:
+            best = dict(rec, table=table)
     for c in chunks:
         if c == base_chunk:
             continue
         sps = _time_config(ctx, systems, best["table"], best["schedule"], c, max(1, reps * base_chunk // c))
-        rec = dict(budget=best["budget"], schedule=best["schedule"], chunk=c, s_per_sweep=sps)
+        rec = {k: best[k] for k in ("budget", "tolerance") if k in best}
+        rec.update(schedule=best["schedule"], chunk=c, s_per_sweep=sps)
         trials.append(rec)
         if log:
             log(rec)
