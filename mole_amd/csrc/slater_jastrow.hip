@@ -444,13 +444,18 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     const int n = s == 0 ? S.n_up : S.n_dn;
     const T *B = s == 0 ? wk.Bup : wk.Bdn;
     const double *xi = wk.pos + 3 * i;
-    // orbitals in register chunks of kSJLanesNMax; the basis values are
-    // recomputed per chunk (bit-identical each time)
+    // orbitals in register chunks of kCh; the basis values are recomputed
+    // per chunk (bit-identical each time), and the chunks add in orbital
+    // order, so the width does not change the sums.  The extended
+    // instantiations (DMC, operators, Gaussians) carry more live state and
+    // take 6, which keeps their spills down (scratch 240 -> 176 bytes per
+    // lane in the mixed-precision DMC step).
+    constexpr int kCh = EXT ? 6 : kSJLanesNMax;
     double Dg0 = 0.0, Dg1 = 0.0, Dg2 = 0.0, Dl = 0.0;
-    for (int k0 = 0; k0 < n; k0 += kSJLanesNMax) {
-      double og[kSJLanesNMax][4];
+    for (int k0 = 0; k0 < n; k0 += kCh) {
+      double og[kCh][4];
 #pragma unroll
-      for (int k = 0; k < kSJLanesNMax; ++k) og[k][0] = og[k][1] = og[k][2] = og[k][3] = 0.0;
+      for (int k = 0; k < kCh; ++k) og[k][0] = og[k][1] = og[k][2] = og[k][3] = 0.0;
       for (int m = 0; m < K; ++m) {
         const int I = S.center[m];
         const double d0 = xi[0] - S.nuc[3 * I], d1 = xi[1] - S.nuc[3 * I + 1], d2 = xi[2] - S.nuc[3 * I + 2];
@@ -460,7 +465,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
         basis_vgl<EXT>(kd, S.zeta[m], d0, d1, d2, r, ch, g0, g1, g2, lc);
         const double *cm = S.coef_t + (size_t)m * no + k0;
 #pragma unroll
-        for (int k = 0; k < kSJLanesNMax; ++k)
+        for (int k = 0; k < kCh; ++k)
           if (k0 + k < n) {
             const double cv = cm[k];
             og[k][0] = fma(cv, g0, og[k][0]);
@@ -470,7 +475,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
           }
       }
 #pragma unroll
-      for (int k = 0; k < kSJLanesNMax; ++k)
+      for (int k = 0; k < kCh; ++k)
         if (k0 + k < n) {
           const double bkc = (double)B[(k0 + k) * n + cc];
           Dg0 = fma(og[k][0], bkc, Dg0);
