@@ -300,9 +300,11 @@ mole_status mole_tau_int_noalloc(mole_ctx *ctx, uint32_t sys, size_t first, size
                                  size_t n_walkers);
 /* MonteCarloSampler::data (src/montecarlo/src/traits.rs:27) in compat
  * form: "Energy", "Wavefunction value" and "Parameter gradient" samples as
- * [walker][sample] (and [walker][sample][param]), the order in which
- * VmcRunner::concatenate_worker_data appends worker data (vmc.rs:108-130).
- * Needs MOLE_FLAG_SAMPLES. */
+ * [walker][sample] (and [walker][sample][param], the system's n_params), the
+ * order in which VmcRunner::concatenate_worker_data appends worker data
+ * (vmc.rs:108-130).  Needs MOLE_FLAG_SAMPLES.  Slater-Jastrow systems: psi_s
+ * = (psi psi)/psi with psi = sign exp(log|det up| + log|det dn| + J) from
+ * the carried FP64 determinants, and O_k psi_s with O = d ln psi/dp. */
 mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local,
                               double *psi, double *pgrad, size_t walkers,
                               size_t samples);
@@ -326,7 +328,11 @@ mole_status mole_read_logdet(mole_ctx *ctx, uint32_t sys, double *logdet, double
  * ElectronicHamiltonian::act_on / psi (operator.rs:177-183) for n
  * configurations [n][N][3].  grad is d psi (not d ln psi) as
  * [n][N][3]; pgrad is d psi/dp as [n][n_params].  Output pointers may be
- * NULL. */
+ * NULL.  Slater-Jastrow systems are evaluated in FP64 whatever the
+ * context's precision, from a Gauss-Jordan re-inversion at each
+ * configuration (psi = sign exp(log|det up| + log|det dn| + J)), by the
+ * sweep kernel on walkers of their own: the context's walkers are not
+ * touched. */
 mole_status mole_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg,
                       double *value, double *grad, double *lap,
                       double *e_local, double *pgrad);

@@ -275,8 +275,6 @@ mole_status validate(const mole_system_desc *sys, uint32_t n_sys, const mole_run
           if (d.basis_kind[m] == MOLE_BASIS_GTO && run->proposal == MOLE_PROPOSAL_LEARNED)
             return fail(MOLE_ERR_UNSUPPORTED, "system %u: the learned proposal takes STO bases only", s);
         }
-      if (run->flags & MOLE_FLAG_SAMPLES)
-        return fail(MOLE_ERR_UNSUPPORTED, "per-sample compat data is for the two-electron kinds");
       if (run->o_convention != MOLE_O_LOGDERIV)
         return fail(MOLE_ERR_UNSUPPORTED, "Slater-Jastrow systems report O = d ln psi / dp");
     }
@@ -412,52 +410,58 @@ bool sj_persistent_eligible(const mole_ctx *ctx, const SJArgs &a) {
          a.dmc == kSJDmcOff;
 }
 
+// The device view of system s.
+SJSysDev sj_sysdev_of(const mole_ctx *ctx, size_t s) {
+  const SysState &st = ctx->sys[s];
+  SJSysDev d{};
+  d.N = (int)st.N;
+  d.n_up = (int)st.n_up;
+  d.n_dn = (int)st.n_down;
+  d.n_nuc = st.n_nuc;
+  d.n_basis = st.n_basis;
+  d.L = st.L;
+  d.hamiltonian = (int)st.hamiltonian;
+  d.W = st.W;
+  d.nuc = st.d_tab;
+  d.charge = st.d_tab + 3 * st.n_nuc;
+  d.zeta = st.d_tab + 4 * st.n_nuc;
+  d.center = st.d_center;
+  d.coef_t = st.d_tab + 4 * st.n_nuc + st.n_basis;
+  d.n_orb = st.n_orb;
+  for (int k = 0; k < kSJNParam; ++k) d.p[k] = k < (int)st.params.size() ? st.params[k] : 1.0;
+  d.opt_exp = st.params.size() == (size_t)kSJNParam ? 1 : 0;
+  d.v_nn = st.v_nn;
+  d.pos = st.d_pos;
+  d.B = st.d_B;
+  d.acc = st.d_acc;
+  d.ld = st.d_ld;
+  d.trace_mask = st.d_trace_mask;
+  d.trace_e = st.d_trace_e;
+  d.samp_psi = st.d_samp_psi;
+  d.samp_g = st.d_samp_g;
+  d.draw_base = st.draw_base;
+  d.move_base = st.move_base;
+  d.sys_index = (uint32_t)s;
+  d.lds_per_walker = (uint32_t)st.lds_walker;
+  d.dmc = st.dmc.dmc;
+  d.dmc_eref = st.dmc.st ? &st.dmc.st->eref : nullptr;
+  d.dmc_pos_in = st.dmc.pos_s;
+  d.dmc_B_in = st.dmc.B_s;
+  d.dmc_el_in = st.dmc.el_s;
+  d.dmc_w_in = st.dmc.wnew;
+  d.kind = st.d_center + st.n_basis;
+  d.any_gto = st.any_gto ? 1 : 0;
+  d.any_p = st.any_p ? 1 : 0;
+  d.ops = st.d_ops;
+  d.custom_h = st.custom_h ? 1 : 0;
+  d.n_obs = st.n_obs;
+  d.obs_acc = st.d_obs_acc;
+  return d;
+}
+
 mole_status sj_upload_sysdev(mole_ctx *ctx) {
   std::vector<SJSysDev> h(ctx->sys.size());
-  for (size_t s = 0; s < ctx->sys.size(); ++s) {
-    const SysState &st = ctx->sys[s];
-    SJSysDev &d = h[s];
-    d.N = (int)st.N;
-    d.n_up = (int)st.n_up;
-    d.n_dn = (int)st.n_down;
-    d.n_nuc = st.n_nuc;
-    d.n_basis = st.n_basis;
-    d.L = st.L;
-    d.hamiltonian = (int)st.hamiltonian;
-    d.W = st.W;
-    d.nuc = st.d_tab;
-    d.charge = st.d_tab + 3 * st.n_nuc;
-    d.zeta = st.d_tab + 4 * st.n_nuc;
-    d.center = st.d_center;
-    d.coef_t = st.d_tab + 4 * st.n_nuc + st.n_basis;
-    d.n_orb = st.n_orb;
-    for (int k = 0; k < kSJNParam; ++k) d.p[k] = k < (int)st.params.size() ? st.params[k] : 1.0;
-    d.opt_exp = st.params.size() == (size_t)kSJNParam ? 1 : 0;
-    d.v_nn = st.v_nn;
-    d.pos = st.d_pos;
-    d.B = st.d_B;
-    d.acc = st.d_acc;
-    d.ld = st.d_ld;
-    d.trace_mask = st.d_trace_mask;
-    d.trace_e = st.d_trace_e;
-    d.draw_base = st.draw_base;
-    d.move_base = st.move_base;
-    d.sys_index = (uint32_t)s;
-    d.lds_per_walker = (uint32_t)st.lds_walker;
-    d.dmc = st.dmc.dmc;
-    d.dmc_eref = st.dmc.st ? &st.dmc.st->eref : nullptr;
-    d.dmc_pos_in = st.dmc.pos_s;
-    d.dmc_B_in = st.dmc.B_s;
-    d.dmc_el_in = st.dmc.el_s;
-    d.dmc_w_in = st.dmc.wnew;
-    d.kind = st.d_center + st.n_basis;
-    d.any_gto = st.any_gto ? 1 : 0;
-    d.any_p = st.any_p ? 1 : 0;
-    d.ops = st.d_ops;
-    d.custom_h = st.custom_h ? 1 : 0;
-    d.n_obs = st.n_obs;
-    d.obs_acc = st.d_obs_acc;
-  }
+  for (size_t s = 0; s < ctx->sys.size(); ++s) h[s] = sj_sysdev_of(ctx, s);
   HIP_TRY(hipStreamSynchronize(ctx->stream));
   HIP_TRY(hipMemcpy(ctx->d_sysdev, h.data(), sizeof(SJSysDev) * h.size(), hipMemcpyHostToDevice));
   ctx->sysdev_dirty = false;
@@ -497,14 +501,16 @@ SJArgs sj_args(const mole_ctx *ctx) {
   a.lu_period = ctx->run.lu_period;
   a.state = ctx->d_state;
   a.cap_sweeps = (ctx->run.flags & MOLE_FLAG_TRACE) ? ctx->run.record_sweeps : 0;
-  a.cap_samples = a.cap_sweeps;
+  a.cap_samples = (ctx->run.flags & (MOLE_FLAG_TRACE | MOLE_FLAG_SAMPLES)) ? ctx->run.record_sweeps : 0;
   if (ctx->mlp) a.net = ctx->mlp->net();
   for (const SysState &s : ctx->sys) {
-    if (s.any_gto || s.custom_h || s.n_obs) a.ext = 1;
-    // learned proposal: from 6 electrons on, the register budget of one
-    // wave per SIMD (no spills) beats two waves (measured: Li..B faster
-    // with two, C even, O and Ne faster with one)
-    if (s.N >= 6) a.one_wave = 1;
+    // compat samples are written by the extended instantiation only
+    if (s.any_gto || s.custom_h || s.n_obs || (ctx->run.flags & MOLE_FLAG_SAMPLES)) a.ext = 1;
+    // learned proposal: from 8 electrons on, the register budget of one
+    // wave per SIMD (no spills) beats two waves (FP64 atoms, 65,536 walkers:
+    // Li, Be 1.4x and B..N 1.1x faster with two; O 1.08x, F and Ne 1.19x
+    // faster with one)
+    if (s.N >= 8) a.one_wave = 1;
   }
   return a;
 }
@@ -820,7 +826,7 @@ mole_status mole_ctx_create(const mole_system_desc *sys, uint32_t n_sys, const m
     }
     if (run->flags & MOLE_FLAG_SAMPLES) {
       CTX_TRY(hipMalloc(&S.d_samp_psi, sizeof(double) * cap * W));
-      CTX_TRY(hipMalloc(&S.d_samp_g, sizeof(double) * cap * W * kNParam2));
+      CTX_TRY(hipMalloc(&S.d_samp_g, sizeof(double) * cap * W * S.n_params));
       if (!S.d_trace_e) {
         CTX_TRY(hipMalloc(&S.d_trace_e, sizeof(double) * cap * W));
         CTX_TRY(launch_fill(S.d_trace_e, cap * W, 0.0, ctx->stream));
@@ -1224,7 +1230,8 @@ mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, doub
     return fail(MOLE_ERR_SHAPE, "sample request exceeds %d walkers x %zu records", s.W, cap);
   HIP_TRY(hipSetDevice(ctx->device));
   HIP_TRY(hipStreamSynchronize(ctx->stream));
-  std::vector<double> h(cap * s.W * kNParam2);
+  const size_t np = s.n_params;  // parameters per "Parameter gradient" sample
+  std::vector<double> h(cap * s.W * np);
   if (e_local) {
     HIP_TRY(hipMemcpy(h.data(), s.d_trace_e, cap * s.W * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t w = 0; w < walkers; ++w)
@@ -1236,10 +1243,9 @@ mole_status mole_read_samples(mole_ctx *ctx, uint32_t sys, double *e_local, doub
       std::memcpy(psi + w * samples, h.data() + w * cap, samples * sizeof(double));
   }
   if (pgrad) {
-    HIP_TRY(hipMemcpy(h.data(), s.d_samp_g, cap * s.W * kNParam2 * sizeof(double), hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(h.data(), s.d_samp_g, cap * s.W * np * sizeof(double), hipMemcpyDeviceToHost));
     for (size_t w = 0; w < walkers; ++w)
-      std::memcpy(pgrad + w * samples * kNParam2, h.data() + w * cap * kNParam2,
-                  samples * kNParam2 * sizeof(double));
+      std::memcpy(pgrad + w * samples * np, h.data() + w * cap * np, samples * np * sizeof(double));
   }
   return ok();
 }
@@ -1266,14 +1272,112 @@ mole_status mole_read_logdet(mole_ctx *ctx, uint32_t sys, double *logdet, double
   return ok();
 }
 
+// mole_eval of a Slater-Jastrow system: the configurations become the
+// walkers of a temporary copy of the system (positions [3N][n], inverses,
+// accumulators and determinants of its own), evaluated by one launch of the
+// extended kernel in its evaluation mode (re-inversion, then the local
+// energy); the context's walkers are not touched.
+static mole_status sj_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg, double *value, double *grad,
+                           double *lap, double *e_local, double *pgrad) {
+  const SysState &s = ctx->sys[sys];
+  const size_t D = 3 * (size_t)s.N, np = s.params.size();
+  const size_t nB = (size_t)s.n_up * s.n_up + (size_t)s.n_down * s.n_down;
+  const size_t ncol = (size_t)sj_columns((int)np);
+  const int wpt = 64 / s.L;
+  if (n > ((size_t)1 << 28)) return fail(MOLE_ERR_SHAPE, "mole_eval: at most 2^28 configurations per call");
+  // evaluated in FP64 whatever the context's precision (Function::value and
+  // its derivatives at given configurations, no update chain to save)
+  const size_t lds_walker = sj_lds_per_walker((int)s.N, (int)s.n_up, (int)s.n_down, s.n_nuc, s.n_basis, 8, (int)np);
+  if ((size_t)wpt * lds_walker > ctx->lds_limit)
+    return fail(MOLE_ERR_UNSUPPORTED, "mole_eval: %zu B of LDS per workgroup at %d lanes per walker (limit %zu)",
+                (size_t)wpt * lds_walker, s.L, ctx->lds_limit);
+  const size_t n_tiles = (n + wpt - 1) / wpt;
+  // one allocation of doubles: positions, inverses (8 bytes per element for
+  // either precision), accumulators, determinants, then the outputs
+  const size_t o_pos = 0, o_B = o_pos + D * n, o_acc = o_B + nB * n, o_ld = o_acc + ncol * n,
+               o_val = o_ld + kSJLdColumns * n, o_grad = o_val + n, o_lap = o_grad + D * n, o_el = o_lap + n,
+               o_pg = o_el + n, total = o_pg + np * n;
+  double *d = nullptr;
+  SJSysDev *d_sys = nullptr;
+  SJTile *d_tiles = nullptr;
+  HIP_TRY(hipSetDevice(ctx->device));
+  hipError_t e = hipMalloc(&d, sizeof(double) * total);
+  if (e == hipSuccess) e = hipMalloc(&d_sys, sizeof(SJSysDev));
+  if (e == hipSuccess) e = hipMalloc(&d_tiles, sizeof(SJTile) * n_tiles);
+  if (e == hipSuccess) {
+    std::vector<double> soa(D * n);
+    for (size_t w = 0; w < n; ++w)
+      for (size_t k = 0; k < D; ++k) soa[k * n + w] = cfg[w * D + k];
+    SJSysDev v = sj_sysdev_of(ctx, sys);
+    v.W = (int)n;
+    v.lds_per_walker = (uint32_t)lds_walker;
+    v.pos = d + o_pos;
+    v.B = d + o_B;
+    v.acc = d + o_acc;
+    v.ld = d + o_ld;
+    v.trace_mask = nullptr;
+    v.trace_e = nullptr;
+    v.samp_psi = v.samp_g = nullptr;
+    v.dmc = nullptr;
+    v.dmc_eref = nullptr;
+    v.dmc_pos_in = nullptr;
+    v.dmc_B_in = nullptr;
+    v.dmc_el_in = v.dmc_w_in = nullptr;
+    v.n_obs = 0;
+    v.obs_acc = nullptr;
+    v.ev_value = value ? d + o_val : nullptr;
+    v.ev_grad = grad ? d + o_grad : nullptr;
+    v.ev_lap = lap ? d + o_lap : nullptr;
+    v.ev_el = e_local ? d + o_el : nullptr;
+    v.ev_pgrad = pgrad ? d + o_pg : nullptr;
+    std::vector<SJTile> tiles(n_tiles);
+    for (size_t k = 0; k < n_tiles; ++k)
+      tiles[k] = SJTile{0, (int)(k * wpt), (int)std::min<size_t>(wpt, n - k * wpt)};
+    e = hipStreamSynchronize(ctx->stream);
+    if (e == hipSuccess) e = hipMemcpy(d_sys, &v, sizeof(SJSysDev), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(d_tiles, tiles.data(), sizeof(SJTile) * n_tiles, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = launch_fill(d + o_B, o_val - o_B, 0.0, ctx->stream);
+    if (e == hipSuccess)
+      e = hipMemcpyAsync(d + o_pos, soa.data(), sizeof(double) * D * n, hipMemcpyHostToDevice, ctx->stream);
+    if (e == hipSuccess) {
+      SJArgs a = sj_args(ctx);
+      a.systems = d_sys;
+      a.tiles = d_tiles;
+      a.n_sweeps = 0;
+      a.sample_start = 0;
+      a.init = 0;
+      a.refresh = 1;
+      a.dmc = kSJEval;
+      a.ext = 1;
+      a.one_wave = 0;
+      a.cap_sweeps = a.cap_samples = 0;
+      e = launch_sj(a, (int)n_tiles, 0, kSJPropBox, (size_t)wpt * lds_walker, ctx->stream);
+    }
+    auto back = [&](double *host, size_t off, size_t cnt) {
+      if (e == hipSuccess && host)
+        e = hipMemcpyAsync(host, d + off, sizeof(double) * cnt, hipMemcpyDeviceToHost, ctx->stream);
+    };
+    back(value, o_val, n);
+    back(grad, o_grad, D * n);
+    back(lap, o_lap, n);
+    back(e_local, o_el, n);
+    back(pgrad, o_pg, np * n);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+  }
+  (void)hipFree(d_tiles);
+  (void)hipFree(d_sys);
+  (void)hipFree(d);
+  if (e != hipSuccess) return fail(MOLE_ERR_DEVICE, "mole_eval: %s", hipGetErrorString(e));
+  return ok();
+}
+
 mole_status mole_eval(mole_ctx *ctx, uint32_t sys, size_t n, const double *cfg, double *value,
                       double *grad, double *lap, double *e_local, double *pgrad) {
   mole_status st = check_ctx(ctx, sys);
   if (st != MOLE_OK) return st;
   if (!cfg) return fail(MOLE_ERR_INVALID_ARGUMENT, "null cfg");
-  if (ctx->family != kFamTwo)
-    return fail(MOLE_ERR_UNSUPPORTED, "mole_eval covers the two-electron kinds");
   if (n == 0) return ok();
+  if (ctx->family == kFamSJ) return sj_eval(ctx, sys, n, cfg, value, grad, lap, e_local, pgrad);
   SysState &s = ctx->sys[sys];
   const size_t D = 3 * s.N;
   HIP_TRY(hipSetDevice(ctx->device));

@@ -70,6 +70,9 @@ struct SJSysDev {
   double *ld;           // [kSJLdColumns][W] log-determinant state
   uint64_t *trace_mask; // [W][cap_sweeps] or null
   double *trace_e;      // [W][cap_samples] or null
+  // compat samples (extended instantiation): psi_s [W][cap_samples] and
+  // O psi_s [W][cap_samples][np], or null
+  double *samp_psi, *samp_g;
   uint64_t draw_base;   // draw index of the first move after the stream origin
   uint64_t move_base;   // moves made before the stream origin (LU refresh phase)
   uint32_t sys_index;   // mixed into the Philox key
@@ -90,11 +93,17 @@ struct SJSysDev {
   const mole_operator *ops;
   int custom_h, n_obs;
   double *obs_acc;
+  // kSJEval outputs per walker (each optional): psi [W], d psi [W][N][3],
+  // lap psi [W], E_L [W], d psi/dp [W][np]
+  double *ev_value, *ev_grad, *ev_lap, *ev_el, *ev_pgrad;
 };
 
 // What a launch does besides the sweeps: nothing (VMC), DMC's start (local
 // energies and unit weights of the configurations), or DMC steps.
-enum SJDmcMode : int { kSJDmcOff = 0, kSJDmcInit = 1, kSJDmcStep = 2 };
+// kSJEval (mole_eval, extended kernel, no sweeps): the value, gradient,
+// laplacian, local energy and parameter gradient of each walker's
+// configuration into the ev_* arrays.
+enum SJDmcMode : int { kSJDmcOff = 0, kSJDmcInit = 1, kSJDmcStep = 2, kSJEval = 3 };
 
 // A workgroup's work: walkers [w0, w0 + nw) of one system.
 struct SJTile {

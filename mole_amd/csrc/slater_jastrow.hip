@@ -428,9 +428,34 @@ __device__ __noinline__ double exponent_scale_term(const SJSysDev &S, const T *B
   return ez;
 }
 
+// One compat sample of MonteCarloSampler::data (the extended
+// instantiation only): "Wavefunction value" as Sampler::sample stores it,
+// psi_s = (psi psi)/psi (src/vmc/src/operators.rs:19-23, samplers.rs:89-90),
+// and "Parameter gradient" O_k psi_s with O = d ln psi/dp.  psi = sign
+// exp(log|det up| + log|det dn| + J) from the carried determinants ld =
+// (mant_up, ex_up, mant_dn, ex_dn) and J = b_anti O_0 + b_para O_1 + a O_3
+// (the Pade factor is linear in b_anti, b_para and a).  Out of line: only
+// contexts that keep compat samples call it.
+__device__ __forceinline__ double sj_psi(const SJSysDev &S, const double *ld, const double *dl) {
+  const double J = (S.p[0] * dl[0] + S.p[1] * dl[1]) + S.p[3] * dl[3];
+  const double lg = (logdet_value(ld[0], ld[1]) + logdet_value(ld[2], ld[3])) + J;
+  const double sg = ((ld[0] < 0.0) != (ld[2] < 0.0)) ? -1.0 : 1.0;
+  return sg * exp_m(lg);
+}
+
+__device__ __noinline__ void sj_compat_sample(const SJSysDev &S, const double *ld, const double *dl, int np,
+                                              size_t at) {
+  const double psi = sj_psi(S, ld, dl);
+  const double ps = (psi * psi) / psi;
+  S.samp_psi[at] = ps;
+  for (int k = 0; k < np; ++k) S.samp_g[at * (size_t)np + k] = dl[k] * ps;
+}
+
+// g_out (extended kernel, optional): grad_i ln psi of every electron, written
+// by the lane that takes the electron as g_out[3 i + q].
 template <typename T, bool EXT>
 __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L, double *dl,
-                               SJParts *parts = nullptr) {
+                               SJParts *parts = nullptr, double *g_out = nullptr) {
   const int N = S.N, K = S.n_basis, nn = S.n_nuc, no = S.n_orb;
   const double bA = S.p[0], bP = S.p[1], c = S.p[2], a = S.p[3], d = S.p[4];
   // [N][RS]: kinetic term, potential, five Jastrow derivatives, e-e potential
@@ -537,6 +562,11 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
     }
     const double dot = Dg0 * gJ0 + Dg1 * gJ1 + Dg2 * gJ2;
     const double gg = gJ0 * gJ0 + gJ1 * gJ1 + gJ2 * gJ2;
+    if (EXT && g_out) {
+      g_out[3 * i] = Dg0 + gJ0;
+      g_out[3 * i + 1] = Dg1 + gJ1;
+      g_out[3 * i + 2] = Dg2 + gJ2;
+    }
     double *o = res + RS * i;
     o[0] = Dl + 2.0 * dot + lJ + gg;
     o[1] = Vi;
@@ -569,6 +599,7 @@ __device__ double local_energy(const SJSysDev &S, SJWalker<T> &wk, int t, int L,
           V += -S.charge[I] / norm3(xi[0] - S.nuc[3 * I], xi[1] - S.nuc[3 * I + 1], xi[2] - S.nuc[3 * I + 2]);
         }
     }
+    if (EXT && parts) parts->kin = kin;
     if (EXT && (S.custom_h || S.n_obs)) {
       if (!parts) parts = &own;
       parts->kin = kin;
@@ -791,6 +822,26 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
     if (t == 0 && live) {
       S.dmc[w] = 1.0;
       S.dmc[(size_t)W + w] = E;
+    }
+  }
+  if (EXT && a.dmc == kSJEval) {
+    // mole_eval: psi = sign exp(log|det up| + log|det dn| + J) from the
+    // re-inversion's determinants, d psi = psi grad ln psi, lap psi = psi
+    // sum_i lap_i psi/psi, E_L and d psi/dp = psi d ln psi/dp
+    double dlg[kSJNParam];
+    SJParts parts;
+    double *gw = (live && S.ev_grad) ? S.ev_grad + (size_t)w * 3 * N : nullptr;
+    const double E = local_energy<T, EXT>(S, wk, t, L, dlg, &parts, gw);
+    if (t == 0 && live) {
+      const double psi = sj_psi(S, acc + ncol, dlg);
+      if (S.ev_value) S.ev_value[w] = psi;
+      if (S.ev_lap) S.ev_lap[w] = psi * parts.kin;
+      if (S.ev_el) S.ev_el[w] = E;
+      if (S.ev_pgrad)
+        for (int k = 0; k < np; ++k) S.ev_pgrad[(size_t)w * np + k] = psi * dlg[k];
+      // the lanes' gradient stores precede local_energy's closing barrier
+      if (gw)
+        for (int k = 0; k < 3 * N; ++k) gw[k] = psi * gw[k];
     }
   }
   const uint32_t lu_period = a.lu_period;
@@ -1053,6 +1104,8 @@ __device__ __forceinline__ void sj_tile(const SJArgs &a, const SJTile tile, unsi
           for (int l = k; l < np; ++l, ++q) acc[sj_sum_oo(np) + q] = fma(dlg[k], dlg[l], acc[sj_sum_oo(np) + q]);
         }
         if (live && S.trace_e && sidx < a.cap_samples) S.trace_e[(size_t)w * a.cap_samples + sidx] = E;
+        if (EXT && live && S.samp_psi && sidx < a.cap_samples)
+          sj_compat_sample(S, acc + ncol, dlg, np, (size_t)w * a.cap_samples + sidx);
       }
       if (++fill == a.block_size) {
         if (t == 0) {

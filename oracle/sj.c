@@ -180,6 +180,14 @@ double orc_sj_logdet_value(double mant, double ex) {
   return fma(ex, ln2_hi, fma(ex, ln2_lo, orc_log(fabs(mant))));
 }
 
+double orc_sj_compat_psi(const orc_sj *wf, const double *ld, const double *dl) {
+  const double J = (wf->p[0] * dl[0] + wf->p[1] * dl[1]) + wf->p[3] * dl[3];
+  const double lg = (orc_sj_logdet_value(ld[0], ld[1]) + orc_sj_logdet_value(ld[2], ld[3])) + J;
+  const double sg = ((ld[0] < 0.0) != (ld[2] < 0.0)) ? -1.0 : 1.0;
+  const double psi = sg * orc_exp(lg);
+  return (psi * psi) / psi;
+}
+
 void orc_sj_logdet_mul(double *mant, double *ex, double f) {
   int e;
   *mant = frexp(*mant * f, &e);
@@ -907,6 +915,7 @@ static void sj_walker(sj_task *t, uint32_t w) {
         size_t si = (size_t)w * nsw + sidx;
         if (out->e_local) out->e_local[si] = el;
         if (out->dlog) memcpy(out->dlog + si * ORC_SJ_NPARAM, dl, sizeof dl);
+        if (out->psi && !t->faithful) out->psi[si] = orc_sj_compat_psi(wf, ld, dl);
         t->sum_e += el;
         t->sum_e2 += el * el;
         bsum += el;

@@ -170,7 +170,18 @@ typedef struct {
   double *ld;
   double max_ld_drift;
   uint64_t n_singular;
+  /* parity runner, optional: the compat "Wavefunction value" samples
+   * [W][samples] (orc_sj_compat_psi of the carried determinants) */
+  double *psi;
 } orc_sj_run_out;
+
+/* "Wavefunction value" of a compat sample as Sampler::sample stores it:
+ * (psi psi)/psi (src/vmc/src/operators.rs:19-23, samplers.rs:89-90), psi =
+ * sign exp(log|det up| + log|det dn| + J) from the carried determinants ld =
+ * (mant_up, ex_up, mant_dn, ex_dn) and J = b_anti dl[0] + b_para dl[1] +
+ * a dl[3] from the Jastrow log-derivatives dl (the Pade factor is linear in
+ * those three parameters). */
+double orc_sj_compat_psi(const orc_sj *wf, const double *ld, const double *dl);
 
 /* Parity runner: walkers [w0, w1), each its own Runner::run
  * (src/montecarlo/src/montecarlo.rs:24-46), Sherman-Morrison ratios and

@@ -65,7 +65,7 @@ class SjRunOut(C.Structure):
         ("sum_e", C.c_double), ("sum_e2", C.c_double), ("sum_bm", C.c_double), ("sum_bm2", C.c_double),
         ("n_samples", C.c_uint64), ("n_blocks", C.c_uint64), ("n_accepted", C.c_uint64),
         ("n_moves", C.c_uint64), ("obs_sum", _pd), ("obs_sum2", _pd),
-        ("ld", _pd), ("max_ld_drift", C.c_double), ("n_singular", C.c_uint64),
+        ("ld", _pd), ("max_ld_drift", C.c_double), ("n_singular", C.c_uint64), ("psi", _pd),
     ]
 
 
@@ -244,6 +244,10 @@ def run(wv, *, seed, walkers, steps, block_size, box_side=0.0, faithful=False, p
         out.e_local = OF.dptr(el)
         out.dlog = OF.dptr(dl)
         res.update(accept_mask=masks, e_local=el, dlog=dl)
+        if not faithful:
+            ps = np.zeros((walkers, ns))
+            out.psi = OF.dptr(ps)
+            res.update(psi=ps)
     f = L.orc_sj_run_faithful if faithful else L.orc_sj_run_parity
     rc = f(wv.ref(), C.byref(rp), 0, walkers, n_threads, C.byref(out))
     if rc != 0:

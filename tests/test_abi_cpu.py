@@ -119,8 +119,8 @@ def test_slater_jastrow_validation_before_device():
     good = M.slater_jastrow(ne, 8)
     # mixing families in one context
     assert _create([good, M.helium(8)]) == M.ERR_UNSUPPORTED
-    # compat samples and the reference O convention are two-electron only
-    assert _create([good], flags=M.FLAG_SAMPLES, record_sweeps=4) == M.ERR_UNSUPPORTED
+    # the reference O convention is two-electron only (compat samples of
+    # Slater-Jastrow systems report O = d ln psi/dp)
     assert _create([good], o_convention=M.O_REFERENCE) == M.ERR_UNSUPPORTED
     assert _create([good], precision=7) == M.ERR_INVALID_ARGUMENT
     bad = M.slater_jastrow(ne, 8)

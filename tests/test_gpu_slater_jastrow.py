@@ -566,3 +566,73 @@ def test_device_optimisation_loop_equals_host_loop(make, params, optimizer, step
         assert log[it, 4] == host_log[it][2] and log[it, 5] == 0.0
     assert np.array_equal(bits(cfg_dev), bits(cfg_host))
     assert not np.array_equal(p_dev, p0)
+
+
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+def test_compat_samples_match_oracle(precision):
+    # MOLE_FLAG_SAMPLES on a Slater-Jastrow system: per-sample "Energy",
+    # "Wavefunction value" (psi psi)/psi from the carried determinants and
+    # "Parameter gradient" O psi_s, against the oracle's parity runner
+    system = S.random_molecule(6, seed=5)
+    W, steps, B = 70, 30, 10
+    ns = (steps // B - 1) * B
+    with M.Context([M.slater_jastrow(system, W)], seed=3, step=0.6, block_size=B, flags=M.FLAG_SAMPLES,
+                   record_sweeps=steps, precision=precision, lu_period=16) as ctx:
+        ctx.init_walkers()
+        ctx.run(steps)
+        e, p, g = ctx.samples(ns)
+    o = run_oracle(system, W, steps, B, precision=SJ.PREC_FP64 if precision == M.PREC_FP64 else SJ.PREC_MIXED,
+                   seed=3)
+    rel = np.abs(e - o["e_local"]) / np.maximum(1.0, np.abs(o["e_local"]))
+    assert np.max(rel) <= E_TOL
+    assert np.all(np.isfinite(p)) and np.all(p != 0.0)
+    assert np.array_equal(np.sign(p), np.sign(o["psi"]))
+    assert np.max(np.abs(p - o["psi"]) / np.abs(o["psi"])) <= 1e-9
+    want = o["dlog"][:, :, :5] * o["psi"][:, :, None]
+    assert g.shape == (W, ns, 5)
+    scale = np.abs(o["psi"])[:, :, None] * np.maximum(1.0, np.abs(o["dlog"][:, :, :5]))
+    assert np.max(np.abs(g - want) / scale) <= 1e-9
+
+
+def _configs_near_nuclei(system, n, seed):
+    rng = np.random.default_rng(seed)
+    N = system.n_electrons
+    at = rng.integers(0, system.nuclei.shape[0], (n, N))
+    return system.nuclei[at] + rng.normal(0.0, 0.8, (n, N, 3))
+
+
+@pytest.mark.parametrize("make", [lambda: S.atom(4), lambda: S.atom(10), lambda: S.random_molecule(6, seed=5)])
+@pytest.mark.parametrize("precision", [M.PREC_FP64, M.PREC_MIXED])
+def test_eval_matches_oracle(make, precision):
+    # mole_eval on a Slater-Jastrow system (FP64 whatever the context's
+    # precision): psi against the oracle's LU determinants and full Jastrow
+    # sum, grad psi = psi drift, lap psi = psi kin with kin from E_L and the
+    # potential, E_L and d psi/dp against the oracle's fresh-inverse local
+    # energy; the context's own walkers are left as they were
+    system = make()
+    n, N = 37, system.n_electrons
+    x = _configs_near_nuclei(system, n, 4)
+    with M.Context([M.slater_jastrow(system, 8)], seed=1, step=0.6, block_size=2, precision=precision,
+                   lu_period=16) as ctx:
+        ctx.init_walkers()
+        ctx.run(4)
+        before = ctx.get_configs()
+        out = ctx.eval(x)
+        assert np.array_equal(bits(ctx.get_configs()), bits(before))
+    wv = SJ.SjWave(system)
+    R, Z = system.nuclei, system.charges.astype(float)
+    v_nn = sum(Z[I] * Z[J] / np.linalg.norm(R[I] - R[J]) for I in range(len(Z)) for J in range(I + 1, len(Z)))
+    for j in range(n):
+        xj = x[j].ravel()
+        v = SJ.value(wv, xj)
+        e, dl = SJ.local_energy(wv, xj)
+        assert abs(out["value"][j] - v) <= 1e-10 * abs(v), j
+        assert abs(out["e_local"][j] - e) <= 1e-9 * max(1.0, abs(e)), j
+        assert np.all(np.abs(out["pgrad"][j] - v * dl[:5]) <= 1e-9 * abs(v) * np.maximum(1.0, np.abs(dl[:5]))), j
+        g = np.concatenate([SJ.drift(wv, xj, i) for i in range(N)])
+        assert np.all(np.abs(out["grad"][j] - v * g) <= 1e-9 * abs(v) * np.maximum(1.0, np.abs(g))), j
+        pos = x[j]
+        V = -sum(Z[I] / np.linalg.norm(pos[i] - R[I]) for i in range(N) for I in range(len(Z)))
+        V += sum(1.0 / np.linalg.norm(pos[i] - pos[k]) for i in range(N) for k in range(i + 1, N))
+        kin = -2.0 * (e - V - v_nn)
+        assert abs(out["lap"][j] / v - kin) <= 1e-8 * max(1.0, abs(kin)), j

@@ -395,3 +395,20 @@ def test_h2_ion_product_energy_anchor():
     assert abs(mean - (-0.565)) < std
     assert abs(mean - H2ION_E) < 4 * err
     assert abs(std - H2ION_STD) < 0.05 * H2ION_STD
+
+
+@pytest.mark.parametrize("params", [None, (0.5, 0.25, 1.0, 0.3, 1.0, 1.07)])
+def test_compat_psi_samples_are_the_wavefunction_value(params):
+    # the parity runner's "Wavefunction value" samples, (psi psi)/psi from the
+    # carried determinants and J = b_anti O_0 + b_para O_1 + a O_3: the last
+    # sample of a walker is taken at its final configuration, where the LU
+    # determinants and the full Jastrow sum of orc_sj_value give psi afresh
+    system = S.random_molecule(6, seed=5)
+    wv = SJ.SjWave(system) if params is None else SJ.SjWave(system, params=np.array(params))
+    r = SJ.run(wv, seed=3, walkers=12, steps=40, block_size=10, box_side=0.6, lu_period=64,
+               precision=SJ.PREC_FP64)
+    assert r["psi"].shape == (12, 30) and np.all(np.isfinite(r["psi"]))
+    for w in range(12):
+        v = SJ.value(wv, r["cfg"][w])
+        assert np.sign(r["psi"][w, -1]) == np.sign(v)
+        assert abs(r["psi"][w, -1] - v) <= 1e-9 * abs(v)
