@@ -2,11 +2,12 @@
 //
 // GpuSlaterJastrow is the wave function of a Slater-Jastrow system as plain
 // data (nuclei, spins, s-type STO/GTO basis, orbital coefficients, the five
-// Jastrow parameters b_anti, b_para, c, a, d): the WaveFunction and Optimize
-// traits (src/wavefunction_traits/src/lib.rs:26-28, src/optimize/src/
-// traits.rs:8-16) for GpuSampler and GpuDmcRunner.  Its per-configuration
-// Function/Differentiate evaluations are not offered on the device (the
-// sampler evaluates every walker in place) and return FuncError.
+// Jastrow parameters b_anti, b_para, c, a, d, and optionally the exponent
+// scale): the WaveFunction and Optimize traits (src/wavefunction_traits/src/
+// lib.rs:26-28, src/optimize/src/traits.rs:8-16) for GpuSampler and
+// GpuDmcRunner, and the per-configuration Function/Differentiate
+// evaluations through mole_eval (FP64, on a one-walker context of its own
+// that copies do not share).
 //
 // The operator builders make the terms of mole_operator: KineticEnergy,
 // IonicPotential, ElectronicPotential (src/operator/src/operator.rs:12-121),
@@ -16,9 +17,11 @@
 #pragma once
 
 #include <cstring>
+#include <memory>
 #include <vector>
 
 #include "errors.hpp"
+#include "gpu.hpp"
 #include "traits.hpp"
 
 namespace mole {
@@ -71,20 +74,72 @@ class GpuSlaterJastrow : public Function, public Differentiate, public WaveFunct
     return d;
   }
 
-  Result<double> value(const Array2 &) const override { return unsupported(); }
-  Result<Array2> gradient(const Array2 &) const override { return unsupported(); }
-  Result<double> laplacian(const Array2 &) const override { return unsupported(); }
+  // Function::value
+  Result<double> value(const Array2 &cfg) const override {
+    double v = 0.0;
+    const Status st = eval(cfg, &v, nullptr, nullptr, nullptr, nullptr);
+    if (st.is_err()) return st.error();
+    return v;
+  }
+  // Differentiate::gradient (d psi, rows = electrons)
+  Result<Array2> gradient(const Array2 &cfg) const override {
+    Array2 g(num_electrons(), 3);
+    const Status st = eval(cfg, nullptr, g.data.data(), nullptr, nullptr, nullptr);
+    if (st.is_err()) return st.error();
+    return g;
+  }
+  Result<double> laplacian(const Array2 &cfg) const override {
+    double l = 0.0;
+    const Status st = eval(cfg, nullptr, nullptr, &l, nullptr, nullptr);
+    if (st.is_err()) return st.error();
+    return l;
+  }
+  // the system's Hamiltonian acting on psi, divided by psi (operator.rs:177-183)
+  Result<double> local_energy(const Array2 &cfg) const {
+    double e = 0.0;
+    const Status st = eval(cfg, nullptr, nullptr, nullptr, &e, nullptr);
+    if (st.is_err()) return st.error();
+    return e;
+  }
   size_t num_electrons() const override { return n_up_ + n_down_; }
-  Result<Array1> parameter_gradient(const Array2 &) const override { return unsupported(); }
+  // Optimize::parameter_gradient: d psi / dp
+  Result<Array1> parameter_gradient(const Array2 &cfg) const override {
+    Array1 g(params_.size());
+    const Status st = eval(cfg, nullptr, nullptr, nullptr, nullptr, g.data());
+    if (st.is_err()) return st.error();
+    return g;
+  }
   void update_parameters(const Array1 &deltap) override {
     for (size_t k = 0; k < params_.size() && k < deltap.size(); ++k) params_[k] += deltap[k];
+    cache_.ctx.reset();
   }
   const Array1 &parameters() const override { return params_; }
   size_t num_parameters() const override { return params_.size(); }
 
  private:
-  static Error unsupported() {
-    return Error{ErrorKind::FuncError, "Slater-Jastrow systems are evaluated in place by the device samplers"};
+  // the evaluation context: created on first use, dropped when the
+  // parameters change, never shared by copies
+  struct EvalCache {
+    mutable std::shared_ptr<gpu::Context> ctx;
+    EvalCache() = default;
+    EvalCache(const EvalCache &) {}
+    EvalCache &operator=(const EvalCache &) {
+      ctx.reset();
+      return *this;
+    }
+  };
+  Status eval(const Array2 &cfg, double *v, double *g, double *l, double *e, double *pg) const {
+    if (cfg.rows != num_electrons() || cfg.cols != 3)
+      return Error{ErrorKind::ShapeError, "expected an N x 3 configuration"};
+    if (!cache_.ctx) {
+      cache_.ctx = std::make_shared<gpu::Context>(std::vector<mole_system_desc>{desc(1)}, gpu::run_desc(0, 0.5, 1));
+      if (!cache_.ctx->ok()) {
+        const Error err = *cache_.ctx->error();
+        cache_.ctx.reset();
+        return err;
+      }
+    }
+    return status(mole_eval(cache_.ctx->get(), 0, 1, cfg.data.data(), v, g, l, e, pg));
   }
   std::vector<double> nuc_;
   std::vector<int32_t> z_;
@@ -95,6 +150,7 @@ class GpuSlaterJastrow : public Function, public Differentiate, public WaveFunct
   Array1 params_;
   std::vector<uint32_t> kind_;
   uint32_t hamiltonian_;
+  EvalCache cache_;
 };
 
 // ---- operators as device terms ----

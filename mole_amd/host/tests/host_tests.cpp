@@ -238,12 +238,28 @@ static int run_gpu(int iters, int walkers, int sweeps, int block, double box, do
   x(0, 0) = 0.3; x(0, 1) = -0.2; x(0, 2) = 0.5; x(1, 0) = -0.4; x(1, 1) = 0.1; x(1, 2) = -0.6;
   const double v = he.value(x).value(), lap = he.laplacian(x).value(), el = he.local_energy(x).value();
   const Array2 g = he.gradient(x).value();
+  // the same through GpuSlaterJastrow (mole_eval of a Slater-Jastrow system)
+  const GpuSlaterJastrow hs_sj = GpuSlaterJastrow::helium(1.6875, 0.5, 0.3);
+  const auto sj_v = hs_sj.value(x), sj_l = hs_sj.laplacian(x), sj_e = hs_sj.local_energy(x);
+  const auto sj_g = hs_sj.gradient(x);
+  const auto sj_p = hs_sj.parameter_gradient(x);
+  if (sj_v.is_err() || sj_l.is_err() || sj_e.is_err() || sj_g.is_err() || sj_p.is_err()) {
+    std::fprintf(stderr, "GpuSlaterJastrow evaluation failed\n");
+    return 7;
+  }
+  const GpuSlaterJastrow copy = hs_sj;  // a copy evaluates on its own context
+  const bool copy_same = copy.value(x).value() == sj_v.value();
   std::printf("{\"params\": [%.17g, %.17g, %.17g], \"energies\": [", r.parameters[0], r.parameters[1], r.parameters[2]);
   for (size_t k = 0; k < r.energies.size(); ++k) std::printf("%s%.17g", k ? ", " : "", r.energies[k]);
   std::printf("], \"errors\": [");
   for (size_t k = 0; k < r.energy_errors.size(); ++k) std::printf("%s%.17g", k ? ", " : "", r.energy_errors[k]);
   std::printf("], \"he_value\": %.17g, \"he_laplacian\": %.17g, \"he_local_energy\": %.17g, \"he_grad\": [", v, lap, el);
   for (size_t k = 0; k < 6; ++k) std::printf("%s%.17g", k ? ", " : "", g.data[k]);
+  std::printf("], \"sj_value\": %.17g, \"sj_laplacian\": %.17g, \"sj_local_energy\": %.17g, \"sj_copy_same\": %d, "
+              "\"sj_grad\": [", sj_v.value(), sj_l.value(), sj_e.value(), copy_same ? 1 : 0);
+  for (size_t k = 0; k < 6; ++k) std::printf("%s%.17g", k ? ", " : "", sj_g.value().data[k]);
+  std::printf("], \"sj_pgrad\": [");
+  for (size_t k = 0; k < sj_p.value().size(); ++k) std::printf("%s%.17g", k ? ", " : "", sj_p.value()[k]);
   std::printf("], \"stream_calls\": %zu, \"stream_blocks\": %zu, \"result_blocks\": %zu, \"sho_energy\": %.17g, "
               "\"dmc_energy\": %.17g, \"dmc_blocks\": %zu, \"dmc_panics\": %d}\n",
               counter.calls, counter.last_blocks, n_bm, sho_energy, dmc_energy, de.first.size(), dmc_panics ? 1 : 0);

@@ -1,8 +1,8 @@
 """The C++ host layer on the device: an H2 stochastic-reconfiguration run
 through GpuSampler + StochasticReconfiguration + VmcRunner (C++) gives the
 same parameters and energies, bit for bit, as the same schedule driven
-from Python over the same C ABI; GpuWaveFunction evaluations equal
-mole_eval's."""
+from Python over the same C ABI; GpuWaveFunction and GpuSlaterJastrow
+evaluations equal mole_eval's."""
 import json
 import os
 import subprocess
@@ -45,6 +45,17 @@ def test_cpp_vmc_runner_matches_python_schedule():
     assert out["he_laplacian"] == e["lap"][0]
     assert out["he_local_energy"] == e["e_local"][0]
     assert np.array_equal(np.array(out["he_grad"]), e["grad"].reshape(-1))
+    # GpuSlaterJastrow (traits) against mole_eval of the same Slater-Jastrow
+    # system through Python, and a copy evaluating on its own context
+    from mole_amd import systems as S
+    hs = S.SJSystem([(0.0, 0.0, 0.0)], [2], 1, 1, [1.6875], [0], [[1.0]], (0.5, 0.0, 0.3, 0.0, 1.0))
+    with M.Context([M.slater_jastrow(hs, 1)], seed=0, step=0.5, block_size=1) as ctx:
+        e = ctx.eval(x.reshape(1, 6))
+    assert out["sj_value"] == e["value"][0] and out["sj_laplacian"] == e["lap"][0]
+    assert out["sj_local_energy"] == e["e_local"][0]
+    assert np.array_equal(np.array(out["sj_grad"]), e["grad"].reshape(-1))
+    assert np.array_equal(np.array(out["sj_pgrad"]), e["pgrad"].reshape(-1))
+    assert out["sj_copy_same"] == 1
     # the streaming Runner::run of 6 blocks: Log::log after each of the 5
     # sampled blocks, the last call seeing all 5 block means
     assert out["stream_calls"] == 5 and out["stream_blocks"] == 5 and out["result_blocks"] == 5
