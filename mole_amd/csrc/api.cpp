@@ -187,8 +187,8 @@ struct mole_mlp {
   size_t n_cfg = 0, cap_cfg = 0;
   uint32_t data_N = 0;
   // per-step scratch
-  float *d_partial = nullptr;
-  double *d_obj = nullptr, *d_obj_steps = nullptr;
+  float *d_partial = nullptr, *d_mid = nullptr;
+  double *d_obj = nullptr, *d_obj_steps = nullptr, *d_mid_obj = nullptr;
   int partial_groups = 0, obj_steps_cap = 0;
   MlpDev net() const { return MlpDev{d_fwd1, d_fwd, d_bwd, d_bias}; }
 };
@@ -1648,7 +1648,7 @@ namespace {
 void free_mlp(mole_mlp *m) {
   if (!m) return;
   void *ptrs[] = {m->d_w, m->d_m, m->d_v, m->d_grad, m->d_fwd1, m->d_fwd, m->d_bwd, m->d_bias,
-                  m->d_cfg, m->d_partial, m->d_obj, m->d_obj_steps};
+                  m->d_cfg, m->d_partial, m->d_obj, m->d_obj_steps, m->d_mid, m->d_mid_obj};
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (m->stream) (void)hipStreamDestroy(m->stream);
@@ -1815,12 +1815,16 @@ mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint3
   const int n_mb = (int)(batch / 16);
   const int G = std::min(n_mb, 1024);
   if (G > m->partial_groups) {
-    if (m->d_partial) (void)hipFree(m->d_partial);
-    if (m->d_obj) (void)hipFree(m->d_obj);
-    m->d_partial = nullptr;
-    m->d_obj = nullptr;
+    for (void *p : {(void *)m->d_partial, (void *)m->d_obj, (void *)m->d_mid, (void *)m->d_mid_obj})
+      if (p) (void)hipFree(p);
+    m->d_partial = m->d_mid = nullptr;
+    m->d_obj = m->d_mid_obj = nullptr;
+    m->partial_groups = 0;
+    const size_t H = ((size_t)G + kMlpSlice - 1) / kMlpSlice;
     HIP_TRY(hipMalloc(&m->d_partial, sizeof(float) * kMlpParams * (size_t)G));
     HIP_TRY(hipMalloc(&m->d_obj, sizeof(double) * (size_t)G));
+    HIP_TRY(hipMalloc(&m->d_mid, sizeof(float) * kMlpParams * H));
+    HIP_TRY(hipMalloc(&m->d_mid_obj, sizeof(double) * H));
     m->partial_groups = G;
   }
   MlpTrainArgs a{};
@@ -1835,7 +1839,7 @@ mole_status mlp_gradient_enqueue(mole_mlp *m, mole_ctx *ctx, uint32_t sys, uint3
   a.obj = m->d_obj;
   a.objective = (int)objective;
   HIP_TRY(launch_mlp_train(a, G, m->stream));
-  HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_grad, obj_dev, adam, m->stream));
+  HIP_TRY(launch_mlp_reduce(m->d_partial, m->d_obj, G, m->d_mid, m->d_mid_obj, m->d_grad, obj_dev, adam, m->stream));
   return MOLE_OK;
 }
 

@@ -47,10 +47,14 @@ struct MlpAdamStep {
   float inv_batch, lr, beta1, beta2, eps;
   int step;
 };
-// grad[p] = sum_g partial[g][p], obj_sum = sum_g obj[g] (fixed order); with
-// adam, the same thread then applies the Adam step to parameter p
-hipError_t launch_mlp_reduce(const float *partial, const double *obj, int n_groups, float *grad,
-                             double *obj_sum, const MlpAdamStep *adam, hipStream_t stream);
+// grad[p] = sum_g partial[g][p], obj_sum = sum_g obj[g] (fixed order: with
+// more than kMlpSlice groups, sums of consecutive slices of kMlpSlice rows
+// into mid [ceil(G / kMlpSlice)][kMlpParams] and mid_obj first, then those in
+// slice order); with adam, the same thread then applies the Adam step to
+// parameter p
+constexpr int kMlpSlice = 32;
+hipError_t launch_mlp_reduce(const float *partial, const double *obj, int n_groups, float *mid, double *mid_obj,
+                             float *grad, double *obj_sum, const MlpAdamStep *adam, hipStream_t stream);
 // features and network outputs of given configurations (debug / tests)
 hipError_t launch_mlp_eval(const MlpDev &net, const double *cfg, int N, const double *nuc, int n_nuc,
                            const int *electron, int n, float *out, hipStream_t stream);

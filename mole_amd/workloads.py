@@ -205,8 +205,8 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
     return out
 
 
-def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=1000, train_steps=8000,
-            batch=4096, lr=1e-3, sweeps=400, block=50, box_sides=(0.1, 0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.5),
+def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=1000, train_steps=24000,
+            batch=16384, lr=1e-3, sweeps=400, block=50, box_sides=(0.1, 0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.5),
             precision=M.PREC_FP64, lu_period=64, seed=1, objective=M.MLP_OBJ_LOG_ESJD, collect_side=0.3,
             log=print):
     """Learned proposal against the box proposal on one atom: configurations

@@ -74,7 +74,8 @@ def test_mixed_precision_parity_with_oracle(Z):
     assert np.max(rel) <= E_TOL
 
 
-@pytest.mark.parametrize("Z,lanes,steps", [(3, 4, 30), (10, 8, 10)])
+@pytest.mark.parametrize("Z,lanes,steps", [(3, 4, 30), (4, 4, 12), (5, 4, 10), (6, 4, 10), (7, 4, 8), (8, 8, 8),
+                                           (9, 8, 8), (10, 8, 10)])
 def test_mixed_precision_parity_at_workload_width(Z, lanes, steps):
     # the atoms workload's width (65,536 walkers, LU refresh every 64 moves)
     # with the lanes per walker its autotuning picks: every accept decision,
