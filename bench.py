@@ -327,6 +327,8 @@ def learned_record(log_runs=False):
                 tau_ratio_vs_best_box=r["tau_ratio_vs_best_box"],
                 energy_diff_sigma=r["energy_diff_sigma"],
                 energy_diff_sigma_vs_best_ess_box=r["energy_diff_sigma_vs_best_ess_box"],
+                reference=r.get("reference"),
+                energy_diff_sigma_vs_reference=r.get("energy_diff_sigma_vs_reference"),
                 tau_estimator="scripts/statfor.rs:30-54 on the device (mole_tau_int)")
 
 

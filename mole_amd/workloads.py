@@ -208,7 +208,7 @@ def hetero_systems(n_systems=256, seed=3, max_electrons=40, basis_per_nucleus=2)
 def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=1000, train_steps=24000,
             batch=16384, lr=1e-3, sweeps=400, block=50, box_sides=(0.1, 0.15, 0.2, 0.3, 0.45, 0.6, 0.9, 1.5),
             precision=M.PREC_FP64, lu_period=64, seed=1, objective=M.MLP_OBJ_LOG_ESJD, collect_side=0.3,
-            log=print):
+            ref_sweeps=20000, ref_block=500, log=print):
     """Learned proposal against the box proposal on one atom: configurations
     from the box sampler, Adam training on the device, then runs of each
     proposal from the same ensemble.  Efficiency per proposal: ESS/s =
@@ -219,7 +219,11 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=100
     sweeps of its own proposal.  Energies: the learned run against the box
     sides of the largest kappa (matched sigma(E): kappa is 1/(sigma^2 t))
     and of the largest ESS/s, in units of the combined error (the larger of
-    the blocking and the tau_int-based error of each)."""
+    the blocking and the tau_int-based error of each), and against a long
+    reference run of the box side with the shortest tau_int (5 x equil
+    sweeps of equilibration, then ref_sweeps sampled sweeps in blocks of
+    ref_block, far longer than tau_int, so its blocking error holds where
+    the short box chains of small sides have not mixed)."""
     sj = systems.atom(Z)
     N = sj.n_electrons
 
@@ -288,6 +292,17 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=100
     best_ess = max(runs, key=lambda r: r["ess_per_s"])
     best_kappa = max(runs, key=lambda r: r["kappa"])
     best_tau = min(runs, key=lambda r: r["tau_int_mean"])
+    ref = None
+    if ref_sweeps:
+        with M.Context([M.slater_jastrow(sj, walkers)], seed=seed + 1, step=best_tau["step"], block_size=ref_block,
+                       precision=precision, lu_period=lu_period, proposal=M.PROPOSAL_BOX) as ctx:
+            autotune.apply(ctx, [sj], box_lanes)
+            ctx.set_configs(start)
+            ctx.sweep(5 * equil)
+            ctx.run(ref_sweeps + ref_block)
+            e_ref, err_ref = energy_and_error(ctx.stats())
+        ref = dict(step=best_tau["step"], sweeps=ref_sweeps, block=ref_block, energy=e_ref, error=err_ref)
+        log(dict(workload="learned-reference", Z=Z, **ref))
     diff = lrn["energy"] - best_kappa["energy"]
     sig = lambda r: max(r["error"], r["error_tau"])
     diff_ess = lrn["energy"] - best_ess["energy"]
@@ -308,6 +323,9 @@ def learned(Z=10, walkers=65536, n_configs=2_000_000, collect_every=5, equil=100
                energy_diff_sigma=diff / float(np.hypot(sig(lrn), sig(best_kappa))),
                energy_diff_vs_best_ess_box=diff_ess,
                energy_diff_sigma_vs_best_ess_box=diff_ess / float(np.hypot(sig(lrn), sig(best_ess))))
+    if ref is not None:
+        rec.update(reference=ref, energy_diff_sigma_vs_reference=(lrn["energy"] - ref["energy"]) /
+                   float(np.hypot(sig(lrn), ref["error"])))
     log(rec)
     return rec
 
