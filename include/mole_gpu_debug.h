@@ -48,6 +48,12 @@ mole_status mole_debug_mfma_bf16(const float *A, const float *B, float *D);
  * B[8(l/16)+j][l%16], j = 0..7. */
 mole_status mole_debug_mfma_bf16_k32(const float *A, const float *B, float *D);
 
+/* The learned proposal's network inputs (mlp_features) of n (configuration
+ * [N][3], electron) samples as 16 floats each (the 13 features, then zeros):
+ * pins the neighbour search and its tie order against the CPU restatement. */
+mole_status mole_debug_mlp_features(const double *cfg, uint32_t N, const double *nuc_xyz, uint32_t n_nuc,
+                                    const int32_t *electron, size_t n, float *out);
+
 #ifdef __cplusplus
 }
 #endif

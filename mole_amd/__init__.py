@@ -286,6 +286,8 @@ _SIGS = [
     ("mole_debug_copy", C.c_int32, [C.c_size_t, C.c_int32, _pd]),
     ("mole_debug_mfma_bf16", C.c_int32, [_pf, _pf, _pf]),
     ("mole_debug_mfma_bf16_k32", C.c_int32, [_pf, _pf, _pf]),
+    ("mole_debug_mlp_features", C.c_int32, [_pd, C.c_uint32, _pd, C.c_uint32, C.POINTER(C.c_int32), C.c_size_t,
+                                            _pf]),
     ("mole_optimize", C.c_int32, [_P, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, C.c_double, C.c_uint32,
                                   C.c_double, _pd, _pd]),
     ("mole_mlp_create", C.c_int32, [C.c_uint64, C.POINTER(_P)]),
@@ -813,6 +815,20 @@ def debug_mfma_bf16_k32(A, B):
     D = np.zeros((16, 16), np.float32)
     check(lib().mole_debug_mfma_bf16_k32(_fp(A), _fp(B), _fp(D)))
     return D
+
+
+def debug_mlp_features(cfg, nuclei, electron):
+    """mole_debug_mlp_features: the learned proposal's 13 network inputs of
+    each (configuration [N][3], electron) sample, as float32 (n, 16)."""
+    cfg = np.ascontiguousarray(cfg, dtype=np.float64)
+    n = cfg.shape[0]
+    N = cfg.reshape(n, -1).shape[1] // 3
+    nuc = np.ascontiguousarray(nuclei, dtype=np.float64).reshape(-1, 3)
+    el = np.ascontiguousarray(electron, dtype=np.int32)
+    out = np.zeros((n, 16), np.float32)
+    check(lib().mole_debug_mlp_features(_dp(cfg), N, _dp(nuc), nuc.shape[0],
+                                        el.ctypes.data_as(C.POINTER(C.c_int32)), n, _fp(out)))
+    return out
 
 
 def parameter_update(sums, optimizer=OPT_SR, step=0.05, sr_mode=SR_TEXTBOOK, sr_eps=1e-3):

@@ -7,6 +7,7 @@
 
 #include "../../include/mole_gpu_debug.h"
 #include "device_common.hpp"
+#include "mlp.hpp"
 
 using namespace mole;
 
@@ -99,6 +100,21 @@ __global__ void copy_probe_kernel(const double *__restrict__ src, double *__rest
   for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += (size_t)gridDim.x * blockDim.x)
     dst[i] = src[i];
+}
+
+// mlp_features of n (configuration, electron) samples, 16 floats per
+// sample: lane sub of the sample's four writes features 4 sub .. 4 sub + 3
+__global__ void __launch_bounds__(64) mlp_features_probe_kernel(const double *cfg, int N, const double *nuc,
+                                                                int n_nuc, const int *electron, int n,
+                                                                float *out) {
+  const int l = threadIdx.x, sl = l & 15, q = l >> 4;
+  const int e = blockIdx.x * 16 + sl;
+  if (e < n) {
+    const double *c = cfg + (size_t)e * 3 * N;
+    float fv[4];
+    mlp_features(c, N, nuc, n_nuc, electron[e], c + 3 * electron[e], q, fv);
+    for (int k = 0; k < 4; ++k) out[16 * (size_t)e + 4 * q + k] = fv[k];
+  }
 }
 
 __global__ void iota_kernel(double *p, size_t n) {
@@ -198,6 +214,33 @@ extern "C" mole_status mole_debug_mfma_bf16_k32(const float *A, const float *B, 
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipMemcpy(D, d + 1024, 256 * sizeof(float), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
+}
+
+extern "C" mole_status mole_debug_mlp_features(const double *cfg, uint32_t N, const double *nuc_xyz,
+                                               uint32_t n_nuc, const int32_t *electron, size_t n, float *out) {
+  if (!cfg || !electron || !out || (n_nuc && !nuc_xyz)) return MOLE_ERR_INVALID_ARGUMENT;
+  if (N < 1 || N > MOLE_MAX_ELECTRONS || n_nuc > MOLE_MAX_NUCLEI || n > ((size_t)1 << 24)) return MOLE_ERR_SHAPE;
+  for (size_t e = 0; e < n; ++e)
+    if (electron[e] < 0 || electron[e] >= (int32_t)N) return MOLE_ERR_INVALID_ARGUMENT;
+  if (n == 0) return MOLE_OK;
+  const size_t nc = n * 3 * N, nn = 3 * (size_t)n_nuc;
+  void *d = nullptr;
+  const size_t bytes = (nc + nn) * sizeof(double) + n * sizeof(int32_t) + 16 * n * sizeof(float);
+  if (hipMalloc(&d, bytes) != hipSuccess) return MOLE_ERR_OUT_OF_MEMORY;
+  double *dc = (double *)d, *dn = dc + nc;
+  int32_t *de = (int32_t *)(dn + nn);
+  float *dout = (float *)(de + n);
+  hipError_t e = hipMemcpy(dc, cfg, nc * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nn) e = hipMemcpy(dn, nuc_xyz, nn * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(de, electron, n * sizeof(int32_t), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(mlp_features_probe_kernel, dim3((unsigned)((n + 15) / 16)), dim3(64), 0, 0, dc, (int)N, dn,
+                       (int)n_nuc, de, (int)n, dout);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpy(out, dout, 16 * n * sizeof(float), hipMemcpyDeviceToHost);
   (void)hipFree(d);
   return e == hipSuccess ? MOLE_OK : MOLE_ERR_DEVICE;
 }

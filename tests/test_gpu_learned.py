@@ -255,3 +255,33 @@ def test_trained_network_lowers_tau_int_below_every_box_side_on_ne(ne_learned_re
     assert r["tau_ratio_vs_best_box"] > 1.25
     # the training objective rose
     assert r["training"]["objective_last"] > r["training"]["objective_first"] + 0.5
+
+
+@pytest.mark.parametrize("Z", [3, 6, 8, 9, 10])
+def test_features_match_oracle(Z):
+    # the network inputs bit for bit against the oracle, for random
+    # configurations and for ones whose partner distances tie exactly (the
+    # lower index wins)
+    system = S.atom(Z)
+    wv = SJ.SjWave(system)
+    N = Z
+    rng = np.random.default_rng(Z)
+    n = 300
+    cfg = rng.normal(scale=0.9, size=(n, N, 3))
+    el = rng.integers(0, N, n)
+    dirs = np.array([[1, 0, 0], [0, 1, 0], [0, 0, 1], [-1, 0, 0], [0, -1, 0], [0, 0, -1]], dtype=float)
+    for t in range(24):
+        # electron i at the origin, the rest on lattice points at distances
+        # 1, 1, ..., 2, 2, ... in a shuffled order
+        i = t % N
+        others = [k for k in range(N) if k != i]
+        rng.shuffle(others)
+        c = np.zeros((N, 3))
+        for m, k in enumerate(others):
+            c[k] = dirs[m % 6] * (1 + m // 6 // (1 + t % 2))
+        cfg[t] = c
+        el[t] = i
+    got = M.debug_mlp_features(cfg.reshape(n, -1), system.nuclei, el)
+    want = np.array([ML.features(wv, cfg[e].ravel(), int(el[e])) for e in range(n)])
+    assert np.array_equal(got[:, :13], want.astype(np.float32))
+    assert np.all(got[:, 13:] == 0.0)
