@@ -293,7 +293,13 @@ def hetero_main(args):
         print(json.dumps(r), flush=True)
         print(json.dumps({"workload": "hetero-per-system", "systems": per}), flush=True)
 
-    workloads.hetero(sweeps=args.sweeps or 100, total_walkers=args.walkers or (1 << 20), log=log)
+    kw = {}
+    if args.lanes_table:
+        # a fixed lanes-per-walker table (JSON {electrons: lanes}): no tuning,
+        # so a profiler sees the same launches every run
+        kw = dict(tune=False, table={int(k): int(v) for k, v in json.loads(args.lanes_table).items()},
+                  chunk=args.chunk or 20)
+    workloads.hetero(sweeps=args.sweeps or 100, total_walkers=args.walkers or (1 << 20), log=log, **kw)
 
 
 def learned_main(args):
@@ -424,6 +430,8 @@ def main():
     ap.add_argument("--precision", choices=["mixed", "fp64"], default="mixed")
     ap.add_argument("--lanes", type=int, default=0)
     ap.add_argument("--side", type=float, default=0.6)
+    ap.add_argument("--lanes-table", default="", help="hetero: fixed lanes per walker, JSON {electrons: lanes}")
+    ap.add_argument("--chunk", type=int, default=0, help="hetero with --lanes-table: sweeps per graph launch")
     ap.add_argument("--bytes", type=int, default=256 << 20)
     ap.add_argument("--repeats", type=int, default=2)
     args = ap.parse_args()
