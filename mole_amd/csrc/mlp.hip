@@ -117,12 +117,15 @@ __device__ __forceinline__ f32x4 bf16x4_to_f32(bf16x4 b) {
   return r;
 }
 
-// dW[in][out] += X^T dZ over the wave's 16 samples for one layer, with
-// input tiles Xt (operand layout, lane l: X^T[in 16 mi + l%16][s 4q+j]) and
-// output-gradient tiles Dt (operand layout of dZ^T), and db[out] += sum_s dZ.
-__device__ __forceinline__ void weight_grads(const bf16x4 *Xt, int n_in_tiles, const bf16x4 *Dt,
-                                             int n_out_tiles, int woff, int boff, int nin, int nout,
-                                             float *g) {
+// dW[in][out] (+)= Xa^T dZa + Xb^T dZb over the wave's 16 samples of two
+// backward passes for one layer (one MFMA chain per tile: pass a, then pass
+// b, accumulated in the matrix core), and db[out] (+)= sum_s (dZa + dZb).
+// Input tiles X (operand layout, lane l: X^T[in 16 mi + l%16][s 4q+j]) and
+// output-gradient tiles D (operand layout of dZ^T).  store: this
+// workgroup's first contribution to its partial, written without reading it.
+__device__ __forceinline__ void weight_grads2(const bf16x4 *Xa, const bf16x4 *Da, const bf16x4 *Xb,
+                                              const bf16x4 *Db, int n_in_tiles, int n_out_tiles, int woff,
+                                              int boff, int nin, int nout, float *g, bool store) {
   const int l = threadIdx.x & 63, r = l % 16, q = l / 16;
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
   bf16x4 ones;
@@ -130,34 +133,52 @@ __device__ __forceinline__ void weight_grads(const bf16x4 *Xt, int n_in_tiles, c
   for (int no = 0; no < n_out_tiles; ++no) {
     const int out = 16 * no + r;
     for (int mi = 0; mi < n_in_tiles; ++mi) {
-      const f32x4 d = mfma16(Xt[mi], Dt[no], zero);
+      const f32x4 d = mfma16(Xb[mi], Db[no], mfma16(Xa[mi], Da[no], zero));
       for (int j = 0; j < 4; ++j) {
         const int in = 16 * mi + 4 * q + j;
-        if (in < nin && out < nout) g[woff + in * nout + out] += d[j];
+        if (in < nin && out < nout) {
+          float *gp = g + woff + in * nout + out;
+          *gp = store ? d[j] : *gp + d[j];
+        }
       }
     }
-    const f32x4 db = mfma16(ones, Dt[no], zero);
-    if (q == 0 && out < nout) g[boff + out] += db[0];
+    const f32x4 db = mfma16(ones, Db[no], mfma16(ones, Da[no], zero));
+    if (q == 0 && out < nout) {
+      float *gp = g + boff + out;
+      *gp = store ? db[0] : *gp + db[0];
+    }
   }
 }
 
-// Backward pass of one forward evaluation: dO (accumulator layout; lanes
-// 0..15 hold dO/dout[0..3] of their sample), h the 12 saved activation tiles,
-// x the bf16 input operand; adds dO/dtheta into g.
-__device__ void mlp_backward(const MlpDev &net, f32x4 dO, const f32x4 *h, bf16x4 x, float *stage, float *g) {
+// The backward passes of two forward evaluations in lockstep, layer by
+// layer: a with output gradient dOa, activation tiles ha (registers) and
+// bf16 input operand xa; b with dOb, its activation tiles in LDS (hb[k][l])
+// and xb.  Adds (stores, with store) both passes' dO/dtheta into g: each
+// weight-gradient element is read and written once for the two passes.
+__device__ void mlp_backward2(const MlpDev &net, f32x4 dOa, const f32x4 *ha, bf16x4 xa, f32x4 dOb,
+                              const f32x4 (*hb)[64], bf16x4 xb, float *stage, float *g, bool store) {
   const int l = threadIdx.x & 63;
   const f32x4 zero = {0.f, 0.f, 0.f, 0.f};
-  bf16x4 Xt[4], Dt[4];
-  f32x4 dz[4], dh[4];
+  bf16x4 Xa[4], Da[4], Xb[4], Db[4];
+  f32x4 dza[4], dzb[4];
   // layer 4 (64 -> 4)
-  Dt[0] = transpose_tile(dO, stage);
-  for (int mi = 0; mi < 4; ++mi) Xt[mi] = transpose_tile(h[8 + mi], stage);
-  weight_grads(Xt, 4, Dt, 1, kMlpW4, kMlpB4, kMlpHid, kMlpOut, g);
+  Da[0] = transpose_tile(dOa, stage);
+  Db[0] = transpose_tile(dOb, stage);
+  for (int mi = 0; mi < 4; ++mi) {
+    Xa[mi] = transpose_tile(ha[8 + mi], stage);
+    Xb[mi] = transpose_tile(hb[8 + mi][l], stage);
+  }
+  weight_grads2(Xa, Da, Xb, Db, 4, 1, kMlpW4, kMlpB4, kMlpHid, kMlpOut, g, store);
   {
-    const bf16x4 db = to_bf16x4(dO);
+    const bf16x4 oa = to_bf16x4(dOa), ob = to_bf16x4(dOb);
     for (int mt = 0; mt < 4; ++mt) {
-      const f32x4 a = mfma16(net.bwd[(kBwdL4 + mt) * 64 + l], db, zero);
-      for (int j = 0; j < 4; ++j) dz[mt][j] = a[j] * (1.0f - h[8 + mt][j] * h[8 + mt][j]);
+      const f32x4 A = mfma16(net.bwd[(kBwdL4 + mt) * 64 + l], oa, zero);
+      const f32x4 B = mfma16(net.bwd[(kBwdL4 + mt) * 64 + l], ob, zero);
+      const f32x4 tha = ha[8 + mt], thb = hb[8 + mt][l];
+      for (int j = 0; j < 4; ++j) {
+        dza[mt][j] = A[j] * (1.0f - tha[j] * tha[j]);
+        dzb[mt][j] = B[j] * (1.0f - thb[j] * thb[j]);
+      }
     }
   }
   // layers 3 and 2 (64 -> 64)
@@ -165,23 +186,41 @@ __device__ void mlp_backward(const MlpDev &net, f32x4 dO, const f32x4 *h, bf16x4
     const int hin = layer == 2 ? 4 : 0;  // inputs of this layer: h2 (layer 3) or h1 (layer 2)
     const int woff = layer == 2 ? kMlpW3 : kMlpW2, boff = layer == 2 ? kMlpB3 : kMlpB2;
     const int bfr = layer == 2 ? kBwdL3 : kBwdL2;
-    for (int no = 0; no < 4; ++no) Dt[no] = transpose_tile(dz[no], stage);
-    for (int mi = 0; mi < 4; ++mi) Xt[mi] = transpose_tile(h[hin + mi], stage);
-    weight_grads(Xt, 4, Dt, 4, woff, boff, kMlpHid, kMlpHid, g);
-    bf16x4 dzb[4];
-    for (int kc = 0; kc < 4; ++kc) dzb[kc] = to_bf16x4(dz[kc]);
-    for (int mt = 0; mt < 4; ++mt) {
-      f32x4 a = zero;
-      for (int kc = 0; kc < 4; ++kc) a = mfma16(net.bwd[(bfr + mt * 4 + kc) * 64 + l], dzb[kc], a);
-      dh[mt] = a;
+    for (int no = 0; no < 4; ++no) {
+      Da[no] = transpose_tile(dza[no], stage);
+      Db[no] = transpose_tile(dzb[no], stage);
     }
-    for (int mt = 0; mt < 4; ++mt)
-      for (int j = 0; j < 4; ++j) dz[mt][j] = dh[mt][j] * (1.0f - h[hin + mt][j] * h[hin + mt][j]);
+    for (int mi = 0; mi < 4; ++mi) {
+      Xa[mi] = transpose_tile(ha[hin + mi], stage);
+      Xb[mi] = transpose_tile(hb[hin + mi][l], stage);
+    }
+    weight_grads2(Xa, Da, Xb, Db, 4, 4, woff, boff, kMlpHid, kMlpHid, g, store);
+    bf16x4 za[4], zb[4];
+    for (int kc = 0; kc < 4; ++kc) {
+      za[kc] = to_bf16x4(dza[kc]);
+      zb[kc] = to_bf16x4(dzb[kc]);
+    }
+    for (int mt = 0; mt < 4; ++mt) {
+      f32x4 A = zero, B = zero;
+      for (int kc = 0; kc < 4; ++kc) {
+        A = mfma16(net.bwd[(bfr + mt * 4 + kc) * 64 + l], za[kc], A);
+        B = mfma16(net.bwd[(bfr + mt * 4 + kc) * 64 + l], zb[kc], B);
+      }
+      const f32x4 tha = ha[hin + mt], thb = hb[hin + mt][l];
+      for (int j = 0; j < 4; ++j) {
+        dza[mt][j] = A[j] * (1.0f - tha[j] * tha[j]);
+        dzb[mt][j] = B[j] * (1.0f - thb[j] * thb[j]);
+      }
+    }
   }
   // layer 1 (13 -> 64): inputs are the features
-  for (int no = 0; no < 4; ++no) Dt[no] = transpose_tile(dz[no], stage);
-  Xt[0] = transpose_tile(bf16x4_to_f32(x), stage);
-  weight_grads(Xt, 1, Dt, 4, kMlpW1, kMlpB1, kMlpIn, kMlpHid, g);
+  for (int no = 0; no < 4; ++no) {
+    Da[no] = transpose_tile(dza[no], stage);
+    Db[no] = transpose_tile(dzb[no], stage);
+  }
+  Xa[0] = transpose_tile(bf16x4_to_f32(xa), stage);
+  Xb[0] = transpose_tile(bf16x4_to_f32(xb), stage);
+  weight_grads2(Xa, Da, Xb, Db, 1, 4, kMlpW1, kMlpB1, kMlpIn, kMlpHid, g, store);
 }
 
 // ln (psi(x')/psi(x))^2 and grad_{x'} ln|psi(x')| of moving electron i of
@@ -302,10 +341,9 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
   const int l = threadIdx.x, sl = l & 15, q = l >> 4;
   const int N = S.N;
   const int n_mb = (int)(a.batch / 16);
+  // this workgroup's partial gradient: its first minibatch stores every
+  // element (the backward passes cover all kMlpParams), later ones add
   float *g = a.partial + (size_t)blockIdx.x * kMlpParams;
-  // this workgroup's partial gradient starts from zero (the passes below add
-  // to it; the loop's first barrier orders these stores before them)
-  for (int p = l; p < kMlpParams; p += 64) g[p] = 0.0f;
   double obj_acc = 0.0;
   for (int mb = blockIdx.x; mb < n_mb; mb += gridDim.x) {
     const uint64_t gi = a.first + (uint64_t)mb * 16 + sl;
@@ -402,9 +440,7 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
       const double dls2 = d2 * Aact * (-3.0 + e2 * is2);
       dO2[3] = (float)(dls2 * mlp_log_sigma_slope(o23));
     }
-    mlp_backward(a.net, dO2, h, x2, s.stage, g);
-    for (int k = 0; k < 12; ++k) h[k] = s.h1[k][l];
-    mlp_backward(a.net, dO1, h, x1, s.stage, g);
+    mlp_backward2(a.net, dO2, h, x2, dO1, s.h1, x1, s.stage, g, mb == (int)blockIdx.x);
   }
   // objective: fixed-order sum over the wave's sample lanes
   __syncthreads();

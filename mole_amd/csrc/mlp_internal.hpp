@@ -17,8 +17,8 @@ struct MlpTrainArgs {
   uint64_t first;       // global index of the batch's first sample
   uint32_t batch;       // samples, a multiple of 16
   uint64_t seed;        // Philox key of the proposal normals
-  float *partial;       // [gridDim][kMlpParams], zero on entry
-  double *obj;          // [gridDim] sum of the objective, zero on entry
+  float *partial;       // [gridDim][kMlpParams], written by the kernel (no clearing needed)
+  double *obj;          // [gridDim] sum of the objective, written by the kernel
   int objective;        // MOLE_MLP_OBJ_*: ESJD; RELATIVE, each sample's
                         // A |dx|^2 divided by d^2 + kMlpRelD0^2 (d the larger
                         // nearest-nucleus distance of x and x'); LOG, each
