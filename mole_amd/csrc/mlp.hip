@@ -223,16 +223,19 @@ __device__ void mlp_backward2(const MlpDev &net, f32x4 dOa, const f32x4 *ha, bf1
   weight_grads2(Xa, Da, Xb, Db, 1, 4, kMlpW1, kMlpB1, kMlpIn, kMlpHid, g, store);
 }
 
-// ln (psi(x')/psi(x))^2 and grad_{x'} ln|psi(x')| of moving electron i of
-// cfg to xn (FP64, one lane): Gaussian elimination with partial pivoting of
-// A y = e_c gives column c of A^-1.  scr holds n^2 + n doubles.
-__device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const double *xn, double *scr,
-                           double *lR2, double *gx) {
-  const int N = S.N, sp = i < S.n_up ? 0 : 1;
+// Spin blocks of the training systems hold at most this many electrons.
+constexpr int kMlpMaxSpin = 8;
+
+// Rows of the Slater matrix A of electron i's spin block at cfg and the
+// right-hand side e_c of A y = e_c into scr (n^2 + n doubles): the four lanes
+// of a sample (sub = 0..3) build rows sub, sub + 4, ... (each row by the
+// same operations as one lane would).  A barrier must follow.
+__device__ void slater_rows(const SJSysDev &S, const double *cfg, int i, double *scr, int sub) {
+  const int sp = i < S.n_up ? 0 : 1;
   const int n = sp == 0 ? S.n_up : S.n_dn, first = sp == 0 ? 0 : S.n_up, c = i - first;
   const int K = S.n_basis, no = S.n_orb;
   double *A = scr, *y = scr + n * n;
-  for (int row = 0; row < n; ++row) {
+  for (int row = sub; row < n; row += 4) {
     const double *re = cfg + 3 * (first + row);
     for (int k = 0; k < n; ++k) A[row * n + k] = 0.0;
     for (int m = 0; m < K; ++m) {
@@ -244,6 +247,18 @@ __device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const do
     }
     y[row] = row == c ? 1.0 : 0.0;
   }
+}
+
+// ln (psi(x')/psi(x))^2 and grad_{x'} ln|psi(x')| of moving electron i of
+// cfg to xn (FP64, one lane), from the rows slater_rows left in scr:
+// Gaussian elimination with partial pivoting of A y = e_c gives column c of
+// A^-1.
+__device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const double *xn, double *scr,
+                           double *lR2, double *gx) {
+  const int N = S.N, sp = i < S.n_up ? 0 : 1;
+  const int n = sp == 0 ? S.n_up : S.n_dn;
+  const int K = S.n_basis, no = S.n_orb;
+  double *A = scr, *y = scr + n * n;
   for (int p = 0; p < n; ++p) {
     int piv = p;
     for (int r = p + 1; r < n; ++r)
@@ -269,26 +284,33 @@ __device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const do
     for (int k = p + 1; k < n; ++k) s = fma(-A[p * n + k], y[k], s);
     y[p] = s / A[p * n + p];
   }
-  // orbitals and their gradients at x'
+  // orbitals and their gradients at x': each basis function once, the
+  // orbitals' sums over m in m order as before
+  double phi[kMlpMaxSpin], g0[kMlpMaxSpin], g1[kMlpMaxSpin], g2[kMlpMaxSpin];
+#pragma unroll
+  for (int k = 0; k < kMlpMaxSpin; ++k) phi[k] = g0[k] = g1[k] = g2[k] = 0.0;
+  for (int m = 0; m < K; ++m) {
+    const int I = S.center[m];
+    const double d0 = xn[0] - S.nuc[3 * I], d1 = xn[1] - S.nuc[3 * I + 1], d2 = xn[2] - S.nuc[3 * I + 2];
+    const double rr = norm3(d0, d1, d2);
+    double ch, b0, b1, b2, bl;
+    basis_vgl<false>(S.any_p ? S.kind[m] : MOLE_BASIS_STO, S.zeta[m], d0, d1, d2, rr, ch, b0, b1, b2, bl);
+    const double *cm = S.coef_t + (size_t)m * no;
+#pragma unroll
+    for (int k = 0; k < kMlpMaxSpin; ++k)
+      if (k < n) {
+        phi[k] = fma(cm[k], ch, phi[k]);
+        g0[k] = fma(cm[k], b0, g0[k]);
+        g1[k] = fma(cm[k], b1, g1[k]);
+        g2[k] = fma(cm[k], b2, g2[k]);
+      }
+  }
   double rd = 0.0, gd[3] = {0.0, 0.0, 0.0};
   for (int k = 0; k < n; ++k) {
-    double phi = 0.0, g0 = 0.0, g1 = 0.0, g2 = 0.0;
-    for (int m = 0; m < K; ++m) {
-      const int I = S.center[m];
-      const double d0 = xn[0] - S.nuc[3 * I], d1 = xn[1] - S.nuc[3 * I + 1], d2 = xn[2] - S.nuc[3 * I + 2];
-      const double rr = norm3(d0, d1, d2);
-      double ch, b0, b1, b2, bl;
-      basis_vgl<false>(S.any_p ? S.kind[m] : MOLE_BASIS_STO, S.zeta[m], d0, d1, d2, rr, ch, b0, b1, b2, bl);
-      const double cm = S.coef_t[(size_t)m * no + k];
-      phi = fma(cm, ch, phi);
-      g0 = fma(cm, b0, g0);
-      g1 = fma(cm, b1, g1);
-      g2 = fma(cm, b2, g2);
-    }
-    rd = fma(phi, y[k], rd);
-    gd[0] = fma(g0, y[k], gd[0]);
-    gd[1] = fma(g1, y[k], gd[1]);
-    gd[2] = fma(g2, y[k], gd[2]);
+    rd = fma(phi[k], y[k], rd);
+    gd[0] = fma(g0[k], y[k], gd[0]);
+    gd[1] = fma(g1[k], y[k], gd[1]);
+    gd[2] = fma(g2[k], y[k], gd[2]);
   }
   // Jastrow difference and gradient at x'
   const double *r = cfg + 3 * i;
@@ -326,7 +348,7 @@ __device__ void move_terms(const SJSysDev &S, const double *cfg, int i, const do
 
 // Per-wave LDS of the training kernel.
 struct TrainLds {
-  static constexpr int kMaxN = 16, kMaxSpin = 8;
+  static constexpr int kMaxN = 16, kMaxSpin = kMlpMaxSpin;
   double cfg[16][3 * kMaxN];
   double ge[16][kMaxSpin * kMaxSpin + kMaxSpin];
   double smp[16][16];      // per sample: x'[3], z[3], mu[3], ls, out3 ... (see below)
@@ -354,6 +376,10 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
     __syncthreads();
     const double *cfg = s.cfg[sl];
     const double *r = cfg + 3 * i;
+    // the Slater matrix at x for the determinant ratio (rows over the
+    // sample's four lanes; the barrier after the forward pass orders them
+    // before move_terms)
+    slater_rows(S, cfg, i, s.ge[sl], q);
     // forward at x
     float fv[4];
     mlp_features(cfg, N, S.nuc, S.n_nuc, i, r, q, fv);
@@ -363,6 +389,7 @@ __global__ void __launch_bounds__(64) mlp_train_kernel(MlpTrainArgs a) {
     f32x4 h[12];
     const f32x4 o1 = mlp_forward(a.net, x1, h);
     for (int k = 0; k < 12; ++k) s.h1[k][l] = h[k];
+    __syncthreads();
     if (q == 0) {
       Stream st;
       st.init(a.seed, kMlpTrainSys, (uint32_t)gi, (uint32_t)(gi >> 32));
