@@ -138,6 +138,14 @@ __device__ __forceinline__ double min_one(double a) {
   return (a < 1.0) ? a : 1.0;
 }
 
+// c ? a : b on values.  `v[i] = c ? w[i] : v[i]` with array elements is a
+// conditional on lvalues: it is compiled as a branch over the two element
+// addresses and one load, which can leave the arrays in scratch memory (a
+// memory round trip and a full wait per select); the arguments here are
+// loaded first, so the arrays stay in registers and the select is a
+// v_cndmask.
+__device__ __forceinline__ double sel(bool c, double a, double b) { return c ? a : b; }
+
 // ln x for finite x > 0 with a normal exponent: x = 2^k m, m in
 // [sqrt(1/2), sqrt(2)), ln m = 2 atanh((m-1)/(m+1)) by its series to s^20
 // (Horner with fma), ln x = k ln2_hi + (ln m + k ln2_lo), fdlibm split.

@@ -459,17 +459,17 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
           }
           const double ratio = min_one((psin * psin) / (psi * psi));
           const bool acc = ratio > uu[e];
-          x[3 * e] = acc ? nx[e][0] : x[3 * e];
-          x[3 * e + 1] = acc ? nx[e][1] : x[3 * e + 1];
-          x[3 * e + 2] = acc ? nx[e][2] : x[3 * e + 2];
+          x[3 * e] = sel(acc, nx[e][0], x[3 * e]);
+          x[3 * e + 1] = sel(acc, nx[e][1], x[3 * e + 1]);
+          x[3 * e + 2] = sel(acc, nx[e][2], x[3 * e + 2]);
 #pragma unroll
           for (int I = 0; I < NNUC; ++I) {
-            r[e][I] = acc ? nr[e][I] : r[e][I];
-            ee[e][I] = acc ? ne[e][I] : ee[e][I];
+            r[e][I] = sel(acc, nr[e][I], r[e][I]);
+            ee[e][I] = sel(acc, ne[e][I], ee[e][I]);
           }
-          phi[e] = acc ? nphi[e] : phi[e];
-          r12 = acc ? n12 : r12;
-          psi = acc ? psin : psi;
+          phi[e] = sel(acc, nphi[e], phi[e]);
+          r12 = sel(acc, n12, r12);
+          psi = sel(acc, psin, psi);
           accepted += acc ? 1.0 : 0.0;
           mask |= acc ? ((uint64_t)1 << e) : 0;
         }
@@ -689,17 +689,17 @@ __global__ void __launch_bounds__(64) sweep2_quad_kernel(Sweep2Args a) {
       const double psin = (KIND == MOLE_WF_HELIUM_PADE) ? v_0 : (nphi[0] * phi[1]) * v_0;
       const double ratio = min_one((psin * psin) / (psi * psi));
       const bool acc = ratio > uu[0];
-      x[0] = acc ? nx[0][0] : x[0];
-      x[1] = acc ? nx[0][1] : x[1];
-      x[2] = acc ? nx[0][2] : x[2];
+      x[0] = sel(acc, nx[0][0], x[0]);
+      x[1] = sel(acc, nx[0][1], x[1]);
+      x[2] = sel(acc, nx[0][2], x[2]);
 #pragma unroll
       for (int I = 0; I < NNUC; ++I) {
-        r[0][I] = acc ? nr[0][I] : r[0][I];
-        ee[0][I] = acc ? ne[0][I] : ee[0][I];
+        r[0][I] = sel(acc, nr[0][I], r[0][I]);
+        ee[0][I] = sel(acc, ne[0][I], ee[0][I]);
       }
-      phi[0] = acc ? nphi[0] : phi[0];
-      r12 = acc ? n12_0 : r12;
-      psi = acc ? psin : psi;
+      phi[0] = sel(acc, nphi[0], phi[0]);
+      r12 = sel(acc, n12_0, r12);
+      psi = sel(acc, psin, psi);
       accepted += acc ? 1.0 : 0.0;
       mask |= acc ? 1ull : 0ull;
       // electron 1's coupled part for this outcome
@@ -708,17 +708,17 @@ __global__ void __launch_bounds__(64) sweep2_quad_kernel(Sweep2Args a) {
       const double psin1 = (KIND == MOLE_WF_HELIUM_PADE) ? v_1 : (phi[0] * nphi[1]) * v_1;
       const double ratio1 = min_one((psin1 * psin1) / (psi * psi));
       const bool acc1 = ratio1 > uu[1];
-      x[3] = acc1 ? nx[1][0] : x[3];
-      x[4] = acc1 ? nx[1][1] : x[4];
-      x[5] = acc1 ? nx[1][2] : x[5];
+      x[3] = sel(acc1, nx[1][0], x[3]);
+      x[4] = sel(acc1, nx[1][1], x[4]);
+      x[5] = sel(acc1, nx[1][2], x[5]);
 #pragma unroll
       for (int I = 0; I < NNUC; ++I) {
-        r[1][I] = acc1 ? nr[1][I] : r[1][I];
-        ee[1][I] = acc1 ? ne[1][I] : ee[1][I];
+        r[1][I] = sel(acc1, nr[1][I], r[1][I]);
+        ee[1][I] = sel(acc1, ne[1][I], ee[1][I]);
       }
-      phi[1] = acc1 ? nphi[1] : phi[1];
-      r12 = acc1 ? n12_1 : r12;
-      psi = acc1 ? psin1 : psi;
+      phi[1] = sel(acc1, nphi[1], phi[1]);
+      r12 = sel(acc1, n12_1, r12);
+      psi = sel(acc1, psin1, psi);
       accepted += acc1 ? 1.0 : 0.0;
       mask |= acc1 ? 2ull : 0ull;
     }
