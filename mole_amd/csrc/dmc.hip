@@ -53,14 +53,27 @@ __global__ void __launch_bounds__(kRedThreads) reduce1_kernel(const double *dmc,
   __shared__ double sh[4][kRedThreads];
   const int t = threadIdx.x;
   const size_t b0 = (size_t)blockIdx.x * kRedChunk;
-  double s_we = 0.0, s_w = 0.0, s_wn = 0.0, mx = 0.0;
-  for (int k = 0; k < kRedChunk / kRedThreads; ++k) {
+  // every load is issued before the first sum (an index past W reads
+  // element 0 and is not summed), so the 16 rounds of a thread overlap in
+  // memory instead of waiting one after another; the sums keep their order
+  constexpr int kPer = kRedChunk / kRedThreads;
+  double vwe[kPer], vw[kPer], vwn[kPer];
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
     const size_t i = b0 + (size_t)k * kRedThreads + t;
-    if (i < (size_t)W) {
-      s_we += dmc[2 * (size_t)W + i];
-      s_w += dmc[3 * (size_t)W + i];
-      s_wn += dmc[i];
-      mx = fmax(mx, dmc[i]);
+    const size_t ic = (i < (size_t)W) ? i : 0;
+    vwe[k] = dmc[2 * (size_t)W + ic];
+    vw[k] = dmc[3 * (size_t)W + ic];
+    vwn[k] = dmc[ic];
+  }
+  double s_we = 0.0, s_w = 0.0, s_wn = 0.0, mx = 0.0;
+#pragma unroll
+  for (int k = 0; k < kPer; ++k) {
+    if (b0 + (size_t)k * kRedThreads + t < (size_t)W) {
+      s_we += vwe[k];
+      s_w += vw[k];
+      s_wn += vwn[k];
+      mx = fmax(mx, vwn[k]);
     }
   }
   sh[0][t] = s_we;
@@ -290,6 +303,22 @@ __global__ void __launch_bounds__(kScanThreads) sr_select_kernel(DmcArgs a) {
   rs.init(a.seed, a.sys, st->branch, kDmcBranchEpoch);
   const uint64_t u = uniform_below(rs.draw((uint64_t)j), total);
   uint32_t lo = 0, hi = (uint32_t)a.W - 1;
+  if constexpr (LOCAL_OFFSETS) {
+    // the scan block holding the answer first, from the offsets in LDS: the
+    // first block b whose inclusive end off[b + 1] exceeds u (the last block
+    // ends at the total, which exceeds u); before it the cumulative weight is
+    // off[b] <= u, so the first index past u lies in block b and the global
+    // search below covers one block's kScanBlock entries instead of all W
+    const int nb = (a.W + kScanBlock - 1) / kScanBlock;
+    int bl = 0, bh = nb - 1;
+    while (bl < bh) {
+      const int bm = bl + (bh - bl) / 2;
+      if (off[bm + 1] > u) bh = bm;
+      else bl = bm + 1;
+    }
+    lo = (uint32_t)bl * kScanBlock;
+    hi = (uint32_t)min(a.W - 1, (bl + 1) * kScanBlock - 1);
+  }
   while (lo < hi) {
     const uint32_t mid = lo + (hi - lo) / 2;
     if (cum(mid) > u) hi = mid;
