@@ -344,8 +344,15 @@ __device__ __forceinline__ void local2_cached(const Sys2 &sp, const double *x,
 template <int KIND, int NNUC, int T>
 __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
   constexpr int SNAP = 6 + 4 * NNUC + 2;  // x, r[e][I], e[e][I], r12, psi
-  __shared__ double rng_lds[2][T][2][4][64];
-  __shared__ double snap_lds[2][T][SNAP][64];
+  // one block with the draws first: their LDS addresses then start at 0, so
+  // the eight draws of a sweep are one base register plus immediate offsets
+  struct Lds {
+    double rng[2][T][2][4][64];
+    double snap[2][T][SNAP][64];
+  };
+  __shared__ Lds lds;
+  auto &rng_lds = lds.rng;
+  auto &snap_lds = lds.snap;
   const int lane = threadIdx.x & 63;
   const int role = threadIdx.x >> 6;
   const int w = blockIdx.x * 64 + lane;
@@ -423,6 +430,12 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
       for (int t = 0; t < T; ++t) {
         const uint32_t s = s0 + t;
         if (s >= a.n_sweeps) break;
+        // the sweep's eight draws read from LDS together, ahead of the arithmetic
+        double dr[2][4];
+#pragma unroll
+        for (int e = 0; e < 2; ++e)
+#pragma unroll
+          for (int k = 0; k < 4; ++k) dr[e][k] = rng_lds[buf][t][e][k][lane];
         uint64_t mask = 0;
         // electron-local parts of both moves first (trial position, distances to
         // the nuclei, orbital values: none depends on the other electron's
@@ -433,10 +446,10 @@ __global__ void __launch_bounds__(256) sweep2_ws_kernel(Sweep2Args a) {
         double nx[2][3], nr[2][NNUC], ne[2][NNUC], nphi[2], uu[2];
 #pragma unroll
         for (int e = 0; e < 2; ++e) {
-          nx[e][0] = x[3 * e] + rng_lds[buf][t][e][0][lane];
-          nx[e][1] = x[3 * e + 1] + rng_lds[buf][t][e][1][lane];
-          nx[e][2] = x[3 * e + 2] + rng_lds[buf][t][e][2][lane];
-          uu[e] = rng_lds[buf][t][e][3][lane];
+          nx[e][0] = x[3 * e] + dr[e][0];
+          nx[e][1] = x[3 * e + 1] + dr[e][1];
+          nx[e][2] = x[3 * e + 2] + dr[e][2];
+          uu[e] = dr[e][3];
           nphi[e] = 0.0;
 #pragma unroll
           for (int I = 0; I < NNUC; ++I) {
