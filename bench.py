@@ -123,23 +123,25 @@ def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
     def optimize(ctx, n):
         return ctx.optimize(n, SWEEPS, BLOCK, M.OPT_SR, SR_STEP, M.SR_TEXTBOOK, SR_EPS)
 
-    # warmup on its own context: the timed run starts from the initial parameters
-    if warmup:
-        with M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed + 1, step=BOX, block_size=BLOCK) as w:
-            w.set_launch_config(VARIANTS[variant])
-            w.init_walkers()
-            if host_loop:
-                p = np.array(PARAMS0)
-                for _ in range(warmup):
-                    p, *_ = one_iteration(w, p)
-            else:
-                optimize(w, warmup)
-            w.synchronize()
-
+    # the timed context is set up first; the warmup runs on its own context
+    # (the timed run starts from the initial parameters) right before the
+    # timed loop, so the device is not idle between the two
     ctx = M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed, step=BOX, block_size=BLOCK)
     ctx.set_launch_config(VARIANTS[variant])
     ctx.init_walkers()
     ctx.synchronize()
+    w = None
+    if warmup:  # closed after the timed loop: freeing it would pause the device
+        w = M.Context([M.h2(WALKERS, *PARAMS0)], seed=seed + 1, step=BOX, block_size=BLOCK)
+        w.set_launch_config(VARIANTS[variant])
+        w.init_walkers()
+        if host_loop:
+            p = np.array(PARAMS0)
+            for _ in range(warmup):
+                p, *_ = one_iteration(w, p)
+        else:
+            optimize(w, warmup)
+        w.synchronize()
     energies = []
     launches_total = 0
     t0 = time.perf_counter()
@@ -153,6 +155,8 @@ def gpu_run(iters, warmup, seed=0, variant="auto", host_loop=False):
         p, log = optimize(ctx, iters)
     ctx.synchronize()
     t1 = time.perf_counter()
+    if w is not None:
+        w.close()
     if not host_loop:
         _, launches_total = ctx.last_timing()
         nb = WALKERS * (SWEEPS // BLOCK - 1)
